@@ -1,0 +1,139 @@
+/*
+ * txa.h -- C ABI of the MI355X finite-element assembly engine.
+ *
+ * The engine replaces, for one GPU, Tianxin/Panzer's volume residual +
+ * Jacobian path: workset geometry (IntegrationValues2, BasisValues2), gather,
+ * DOF/DOFGradient, the integrator evaluators, the Sacado-FAD element Jacobian
+ * (here an explicit tangent) and ScatterResidual_Tpetra into a Tpetra-layout
+ * CRS matrix addressed through GlobalIndexer local IDs.
+ *
+ * Conventions
+ *  - Every entry point returns TXA_OK (0) or an error code; nothing throws
+ *    across the ABI.  txa_last_error() gives the message of the last failure on
+ *    a context.  (The reference throws std::logic_error / std::invalid_argument
+ *    via TEUCHOS_TEST_FOR_EXCEPTION; the C++ shim in txa_panzer.hpp rethrows.)
+ *  - Borrow-only: pointers passed in are read during the call (setup calls) or
+ *    until the stream reaches the launched work (assembly calls).  Mesh,
+ *    DOF-map and graph arrays are owned by the context and live on the device
+ *    until txa_mesh_destroy / txa_destroy.
+ *  - "host" pointers are CPU memory, "device" pointers are HIP device memory of
+ *    the context's device.  `stream` is a hipStream_t (NULL = default stream).
+ *  - A context is not thread-safe; calls on it are stream-ordered.
+ */
+#ifndef TXA_H
+#define TXA_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define TXA_OK 0
+#define TXA_ERR_INVALID 1     /* bad argument (std::invalid_argument in the reference) */
+#define TXA_ERR_LOGIC 2       /* inconsistent state (std::logic_error in the reference) */
+#define TXA_ERR_DEVICE 3      /* HIP runtime failure */
+#define TXA_ERR_NOMEM 4       /* device allocation failed */
+#define TXA_ERR_UNSUPPORTED 5 /* combination not implemented */
+
+/* element topologies (shards: Hexahedron<8>, Tetrahedron<4> from a 6-tet split
+ * of each hex, HGRAD_HEX_C2 on Hexahedron<8> geometry) */
+#define TXA_HEX8 1
+#define TXA_TET4 2
+#define TXA_HEX27 3
+
+/* physics */
+#define TXA_DIFFUSION 0       /* k grad(u).grad(v) + rho_c udot v - f v   (scalar) */
+#define TXA_ELASTICITY 1      /* sigma(u):eps(v) + rho (a + a_R v).v + b_R sigma(v):eps(v) - b.v */
+
+/* element-integration precision (geometry and scatter-accumulation are FP64 in all) */
+#define TXA_FP64 0
+#define TXA_FP32 1
+#define TXA_SPLIT_BF16 2
+
+typedef struct txa_context *txa_ctx;
+
+/* Context on one HIP device.
+ * Replaces the per-rank Kokkos execution space + Tpetra node that
+ * panzer::AssemblyEngine runs on (disc-fe/src/Panzer_AssemblyEngine_impl.hpp:67-129). */
+int txa_create(int device, txa_ctx *out);
+int txa_destroy(txa_ctx ctx);
+const char *txa_last_error(txa_ctx ctx);
+int txa_sync(txa_ctx ctx);
+/* Library version string (static). */
+const char *txa_version(void);
+
+/* Structured brick mesh: nx*ny*nz hex cells (x fastest), vertex id
+ * i + (nx+1)(j + (ny+1)k), element order and local vertex order of
+ * CubeHexMeshFactory::buildBlock
+ * (adapters-stk/src/stk_interface/Panzer_STK_CubeHexMeshFactory.cpp:401-460) and,
+ * for TXA_TET4 / TXA_HEX27, the entity numbering of CartesianConnManager
+ * (dof-mgr/test/cartesian_topology/CartesianConnManager.cpp:220-235, 744-890).
+ * vertex_coords: host [(nx+1)(ny+1)(nz+1)][3] FP64 (any geometry with that
+ * topology).  Replaces STK mesh construction + STKConnManager::buildConnectivity
+ * (adapters-stk/src/Panzer_STKConnManager.cpp:243-300). */
+int txa_mesh_brick(txa_ctx ctx, int topo, int nx, int ny, int nz, const double *vertex_coords,
+                   int *mesh_id);
+int txa_mesh_destroy(txa_ctx ctx, int mesh_id);
+/* out[0]=elements out[1]=connectivity entries/element out[2]=vertices */
+int txa_mesh_info(txa_ctx ctx, int mesh_id, int64_t *out);
+
+/* DOF map for `nfields` fields on every connectivity entity (1 = scalar,
+ * 3 = displacement).  GIDs by the GUN rule on one rank, owned_ by first touch
+ * over elements, LIDs = position in owned_.  Built on the device.
+ * Replaces DOFManager::buildGlobalUnknowns (dof-mgr/src/Panzer_DOFManager.cpp:449-715)
+ * and GlobalIndexer::buildLocalIds (dof-mgr/src/Panzer_GlobalIndexer.hpp:548-640). */
+int txa_dofmap_build(txa_ctx ctx, int mesh_id, int nfields, int64_t *ndofs);
+/* Copies (host, nullable): owned GIDs [ndofs]; element LIDs [elements][ndof_per_elem]
+ * (GlobalIndexer::getOwnedIndices / getElementLIDs, Panzer_GlobalIndexer.hpp:254-320). */
+int txa_dofmap_copy(txa_ctx ctx, int mesh_id, int64_t *owned_gids, int32_t *elem_lids);
+/* LID of every DOF-carrying lattice point and field, host [points][nfields]:
+ * vertex i + (nx+1)(j + (ny+1)k) for HEX8/TET4, Q2 node I + (2nx+1)(J + (2ny+1)K)
+ * for HEX27.  The nodal form of getElementLIDs + getGIDFieldOffsets
+ * (dof-mgr/src/Panzer_GlobalIndexer.hpp:254-320, dof-mgr/src/Panzer_DOFManager.cpp:376-389),
+ * used to address nodesets (Dirichlet rows, TianXin_PointEvaluator_impl.hpp:44-190). */
+int txa_dofmap_point_lids(txa_ctx ctx, int mesh_id, int32_t *point_lids, int64_t *npoints);
+
+/* CRS graph in LID space: rows = LIDs, columns = union of element closures,
+ * sorted ascending, duplicates merged.  Built on the device.
+ * Replaces TpetraLinearObjFactory::buildGhostedGraph + fillComplete
+ * (disc-fe/src/lof/Panzer_TpetraLinearObjFactory_impl.hpp:559-650). */
+int txa_graph_build(txa_ctx ctx, int mesh_id, int64_t *nnz);
+int txa_graph_copy(txa_ctx ctx, int mesh_id, int64_t *rowptr, int32_t *col);   /* host outputs */
+int txa_graph_device(txa_ctx ctx, int mesh_id, const int64_t **rowptr, const int32_t **col);
+
+/* One residual + Jacobian evaluation.  All arrays are device memory.
+ * f (ndofs) and vals (nnz) are overwritten with the assembled sums, Dirichlet
+ * rows applied last (f_i = x_i - g_i, identity matrix row).
+ * Jacobian = beta dR/dx + alpha dR/dxdot + gamma2 dR/dxddot, the seeds of
+ * GatherSolution_Tpetra<Jacobian>
+ * (disc-fe/src/evaluators/Panzer_GatherSolution_Tpetra_impl.hpp:549-612).
+ * Replaces AssemblyEngine<Jacobian>::evaluate volume + Dirichlet fill
+ * (disc-fe/src/Panzer_AssemblyEngine_impl.hpp:67-177, 305-321) running
+ * Gather -> DOF/DOFGradient -> Integrator_* -> ScatterResidual_Tpetra
+ * (disc-fe/src/evaluators/Panzer_ScatterResidual_Tpetra_impl.hpp:375-520). */
+typedef struct {
+  int physics;                 /* TXA_DIFFUSION or TXA_ELASTICITY */
+  int precision;               /* TXA_FP64, TXA_FP32, TXA_SPLIT_BF16 */
+  const double *coef;          /* [elements] k (diffusion) or Young's modulus (elasticity) */
+  double nu;                   /* Poisson ratio */
+  double rho;                  /* rho*c (diffusion) or density (elasticity) */
+  double src[3];               /* source f (diffusion) / body force per volume (elasticity) */
+  double rayleigh_a, rayleigh_b;
+  const double *x, *xdot, *xddot;   /* [ndofs] by LID; xdot/xddot may be NULL */
+  double alpha, beta, gamma2;
+  int transient;               /* workset.evaluate_transient_terms */
+  double *f;                   /* [ndofs] residual, or NULL */
+  double *vals;                /* [nnz] CRS values, or NULL */
+  const int32_t *bc_lids;      /* [n_bc] Dirichlet LIDs, or NULL */
+  const double *bc_vals;       /* [n_bc] prescribed values, or NULL (= 0) */
+  int64_t n_bc;
+} txa_assembly_args;
+
+int txa_assemble(txa_ctx ctx, int mesh_id, const txa_assembly_args *args, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TXA_H */

@@ -1,0 +1,211 @@
+"""ctypes wrapper of the CPU oracle (``oracle/oracle.c``).
+
+TEST INFRASTRUCTURE ONLY.  The oracle restates the reference's volume
+residual + Jacobian path (mesh numbering, GUN GIDs, first-touch LIDs, CRS
+graph, cached workset geometry, forward-AD Jacobian, scatter, Dirichlet rows);
+each C function cites the reference file:line it follows.  Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg use it, and only
+as the checker or the timed CPU baseline -- the product package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+QUAD4, HEX8, TET4, HEX27 = 0, 1, 2, 3
+DIFFUSION, ELASTICITY = 0, 1
+MODE_FAD, MODE_EXPLICIT = 0, 1
+
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with the committed Makefile (gcc, OpenMP)."""
+    root = os.path.dirname(HERE)
+    if force or not os.path.exists(LIB_PATH) or \
+            os.path.getmtime(LIB_PATH) < os.path.getmtime(os.path.join(HERE, "oracle.c")):
+        subprocess.run(["make", "-s", "-B" if force else "-s", "-f", "oracle/Makefile"],
+                       cwd=root, check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+P = ctypes.c_void_p
+I64 = ctypes.c_int64
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [
+        ("topo", ctypes.c_int), ("physics", ctypes.c_int),
+        ("nelem", I64), ("nvert", ctypes.c_int),
+        ("elem_vert", P), ("coords", P),
+        ("ndof", ctypes.c_int), ("elem_lid", P), ("coef", P),
+        ("nu", ctypes.c_double), ("rho", ctypes.c_double), ("src", ctypes.c_double * 3),
+        ("rayleigh_a", ctypes.c_double), ("rayleigh_b", ctypes.c_double),
+        ("workset_size", ctypes.c_int), ("nthreads", ctypes.c_int),
+        ("rowptr", P), ("col", P),
+    ]
+
+
+class State(ctypes.Structure):
+    _fields_ = [
+        ("x", P), ("xdot", P), ("xddot", P),
+        ("alpha", ctypes.c_double), ("beta", ctypes.c_double), ("gamma2", ctypes.c_double),
+        ("transient", ctypes.c_int), ("mode", ctypes.c_int), ("want_jac", ctypes.c_int),
+    ]
+
+
+def _declare(L):
+    L.orc_mesh_sizes.argtypes = [ctypes.c_int] * 4 + [P]
+    L.orc_mesh_build.argtypes = [ctypes.c_int] * 4 + [P, P]
+    L.orc_dofmap.argtypes = [I64, ctypes.c_int, P, ctypes.c_int, P, P, P]
+    L.orc_dofmap.restype = I64
+    L.orc_graph_rowptr.argtypes = [I64, ctypes.c_int, P, I64, P]
+    L.orc_graph_rowptr.restype = I64
+    L.orc_graph_cols.argtypes = [I64, ctypes.c_int, P, I64, P, P]
+    L.orc_setup.argtypes = [ctypes.POINTER(Problem)]
+    L.orc_setup.restype = P
+    L.orc_free.argtypes = [P]
+    L.orc_evaluate.argtypes = [P, ctypes.POINTER(State), P, P]
+    L.orc_dirichlet.argtypes = [I64, P, P, P, P, P, P, P]
+    L.orc_cell_geometry.argtypes = [P, I64, P, P]
+    L.orc_refelem_info.argtypes = [ctypes.c_int, P]
+    L.orc_refelem_tables.argtypes = [ctypes.c_int, P, P, P, P]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(P)
+
+
+def mesh_sizes(topo, nx, ny, nz=1):
+    out = np.zeros(6, np.int64)
+    if lib().orc_mesh_sizes(topo, nx, ny, nz, _p(out)):
+        raise ValueError("bad mesh size")
+    keys = ("nelem", "nconn", "nvert", "nnodes", "nentities", "dim")
+    return dict(zip(keys, (int(v) for v in out)))
+
+
+def brick_mesh(topo, nx, ny, nz=1):
+    """Connectivity entity ids [E][nconn] and geometry vertex ids [E][nvert]."""
+    s = mesh_sizes(topo, nx, ny, nz)
+    conn = np.zeros((s["nelem"], s["nconn"]), np.int64)
+    ev = np.zeros((s["nelem"], s["nvert"]), np.int64)
+    lib().orc_mesh_build(topo, nx, ny, nz, _p(conn), _p(ev))
+    return conn, ev, s
+
+
+def dofmap(conn, nfields):
+    """GUN GIDs + first-touch owned_ + LIDs: (elem_gid, owned, elem_lid)."""
+    conn = np.ascontiguousarray(conn, np.int64)
+    E, nc = conn.shape
+    cap = int(np.unique(conn).size) * nfields
+    eg = np.zeros((E, nc * nfields), np.int64)
+    owned = np.zeros(cap, np.int64)
+    el = np.zeros((E, nc * nfields), np.int32)
+    n = lib().orc_dofmap(E, nc, _p(conn), nfields, _p(eg), _p(owned), _p(el))
+    if n < 0:
+        raise RuntimeError("oracle dofmap failed")
+    return eg, owned[:n], el
+
+
+def graph(elem_lid, nrows):
+    """CRS graph (rowptr int64, col int32): union of element closures, sorted rows."""
+    el = np.ascontiguousarray(elem_lid, np.int32)
+    E, nd = el.shape
+    rowptr = np.zeros(nrows + 1, np.int64)
+    nnz = lib().orc_graph_rowptr(E, nd, _p(el), nrows, _p(rowptr))
+    col = np.zeros(max(nnz, 1), np.int32)
+    if lib().orc_graph_cols(E, nd, _p(el), nrows, _p(rowptr), _p(col)):
+        raise RuntimeError("oracle graph failed")
+    return rowptr, col[:nnz]
+
+
+def refelem(topo):
+    info = np.zeros(4, np.int32)
+    lib().orc_refelem_info(topo, _p(info))
+    dim, nb, nv, nq = (int(v) for v in info)
+    N = np.zeros((nb, nq)); dN = np.zeros((nb, nq, 3)); qp = np.zeros((nq, 3)); qw = np.zeros(nq)
+    lib().orc_refelem_tables(topo, _p(N), _p(dN), _p(qp), _p(qw))
+    return dict(dim=dim, nb=nb, nv=nv, nq=nq, N=N, dN=dN[:, :, :dim], qp=qp[:, :dim], qw=qw)
+
+
+class Assembler:
+    """Cached-workset assembly (orc_setup = worksets, orc_evaluate = one pass)."""
+
+    def __init__(self, topo, physics, elem_vert, coords, elem_lid, coef, rowptr, col, *,
+                 nu=0.3, rho=1.0, src=(0.0, 0.0, 0.0), rayleigh=(0.0, 0.0), workset_size=1000,
+                 nthreads=1):
+        self._keep = [np.ascontiguousarray(elem_vert, np.int64), np.ascontiguousarray(coords, np.float64),
+                      np.ascontiguousarray(elem_lid, np.int32), np.ascontiguousarray(coef, np.float64),
+                      np.ascontiguousarray(rowptr, np.int64), np.ascontiguousarray(col, np.int32)]
+        ev, xyz, el, cf, rp, cl = self._keep
+        self.nrows = rp.size - 1
+        self.nnz = int(rp[-1])
+        pr = Problem()
+        pr.topo, pr.physics = topo, physics
+        pr.nelem, pr.nvert = ev.shape[0], ev.shape[1]
+        pr.elem_vert, pr.coords = _p(ev), _p(xyz)
+        pr.ndof, pr.elem_lid, pr.coef = el.shape[1], _p(el), _p(cf)
+        pr.nu, pr.rho = nu, rho
+        s = list(src) + [0.0] * (3 - len(src))
+        pr.src = (ctypes.c_double * 3)(*s[:3])
+        pr.rayleigh_a, pr.rayleigh_b = rayleigh
+        pr.workset_size, pr.nthreads = workset_size, nthreads
+        pr.rowptr, pr.col = _p(rp), _p(cl)
+        self._pr = pr
+        self._ctx = lib().orc_setup(ctypes.byref(pr))
+        if not self._ctx:
+            raise RuntimeError("oracle setup failed (element/physics mismatch?)")
+
+    def evaluate(self, x, xdot=None, xddot=None, *, alpha=0.0, beta=1.0, gamma2=0.0, transient=False,
+                 mode=MODE_FAD, want_jac=True, f=None, vals=None):
+        x = np.ascontiguousarray(x, np.float64)
+        xd = None if xdot is None else np.ascontiguousarray(xdot, np.float64)
+        xdd = None if xddot is None else np.ascontiguousarray(xddot, np.float64)
+        st = State()
+        st.x, st.xdot, st.xddot = _p(x), _p(xd), _p(xdd)
+        st.alpha, st.beta, st.gamma2 = alpha, beta, gamma2
+        st.transient, st.mode, st.want_jac = int(transient), mode, int(want_jac)
+        if f is None:
+            f = np.zeros(self.nrows)
+        if vals is None and want_jac:
+            vals = np.zeros(max(self.nnz, 1))
+        if lib().orc_evaluate(self._ctx, ctypes.byref(st), _p(f), _p(vals)):
+            raise RuntimeError("oracle evaluate failed (column missing from graph?)")
+        return f, (vals[: self.nnz] if want_jac else None)
+
+    def cell_geometry(self, e, nq, nb, dim):
+        wm = np.zeros(nq); g = np.zeros((nb, nq, dim))
+        lib().orc_cell_geometry(self._ctx, e, _p(wm), _p(g))
+        return wm, g
+
+    def __del__(self):
+        try:
+            if self._ctx:
+                lib().orc_free(self._ctx)
+                self._ctx = None
+        except Exception:
+            pass
+
+
+def dirichlet(lids, g, x, rowptr, col, f, vals):
+    lids = np.ascontiguousarray(lids, np.int32)
+    g = None if g is None else np.ascontiguousarray(g, np.float64)
+    lib().orc_dirichlet(lids.size, _p(lids), _p(g), _p(np.ascontiguousarray(x)), _p(rowptr), _p(col),
+                        _p(f), _p(vals))
+    return f, vals

@@ -1,0 +1,145 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Integer outputs (GIDs, LIDs, CRS row pointers and columns) must match
+bit-exactly.  FP64 values: normwise max|d| / max|ref| <= 1e-12 (summation order
+differs from the oracle's workset loop).  FP32 element integration:
+<= 1e-6 normwise and entrywise over |ref| > 1e-12 max|ref|.
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import tianxin_amd as txa
+from tianxin_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+TOPO = {txa.HEX8: O.HEX8, txa.TET4: O.TET4, txa.HEX27: O.HEX27}
+TOL64 = 1e-12
+TOL32 = 1e-6
+
+
+def rel_err(got, ref):
+    return float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+def entry_err(got, ref, tau=1e-12):
+    m = np.abs(ref) > tau * np.max(np.abs(ref))
+    return float(np.max(np.abs(got[m] - ref[m]) / np.abs(ref[m])))
+
+
+class Case:
+    def __init__(self, engine, topo, nx, ny, nz, nf, jitter=0.0, seed=1):
+        print(f"[case] topo={topo} mesh={nx}x{ny}x{nz} nf={nf} jitter={jitter}", flush=True)
+        self.topo, self.nf = topo, nf
+        self.X = synth.brick_coords(nx, ny, nz, jitter=jitter, seed=seed)
+        self.m = engine.brick(topo, nx, ny, nz, self.X)
+        self.ndofs = self.m.build_dofmap(nf)
+        self.nnz = self.m.build_graph()
+        conn, self.ev, s = O.brick_mesh(TOPO[topo], nx, ny, nz)
+        self.conn = conn
+        self.eg, self.owned, self.el = O.dofmap(conn, nf)
+        self.rp, self.col = O.graph(self.el, self.owned.size)
+
+    def oracle(self, physics, coef, **kw):
+        return O.Assembler(TOPO[self.topo], physics, self.ev, self.X, self.el, coef, self.rp, self.col, **kw)
+
+
+def dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda", dtype=dtype)
+
+
+@pytest.mark.parametrize("topo,n,nf", [(txa.HEX8, 3, 1), (txa.HEX8, 4, 3), (txa.TET4, 3, 1),
+                                       (txa.HEX27, 2, 3), (txa.HEX27, 3, 1)])
+def test_dofmap_and_graph_bit_exact(engine, topo, n, nf):
+    c = Case(engine, topo, n, n + 1, n + 2, nf)
+    owned, el = c.m.dofmap()
+    assert np.array_equal(owned, c.owned)
+    assert np.array_equal(el, c.el)
+    rp, col = c.m.graph()
+    assert np.array_equal(rp, c.rp)
+    assert np.array_equal(col, c.col)
+    c.m.destroy()
+
+
+def test_graph_bit_exact_32_cube(engine):
+    c = Case(engine, txa.HEX8, 32, 32, 32, 1)
+    rp, col = c.m.graph()
+    assert rp[-1] == 912673 and c.ndofs == 35937
+    assert np.array_equal(rp, c.rp) and np.array_equal(col, c.col)
+    c.m.destroy()
+
+
+def run_both(engine, c, physics, precision=txa.FP64, transient=False, bc=False, seed=5, nu=0.3, rho=1.0,
+             src=(1.0, 0.0, 0.0), rayleigh=(0.0, 0.0), alpha=0.0, beta=1.0, gamma2=0.0):
+    E = c.el.shape[0]
+    coef = synth.lognormal(seed, E, 0.5)
+    x = synth.uniform(seed + 1, c.ndofs, -1, 1)
+    xd = synth.uniform(seed + 2, c.ndofs, -1, 1) if transient else None
+    xdd = synth.uniform(seed + 3, c.ndofs, -1, 1) if (transient and physics == txa.ELASTICITY) else None
+    A = c.oracle(physics, coef, nu=nu, rho=rho, src=src, rayleigh=rayleigh)
+    fr, vr = A.evaluate(x, xd, xdd, alpha=alpha, beta=beta, gamma2=gamma2, transient=transient,
+                        mode=O.MODE_EXPLICIT)
+    bcl = None
+    if bc:
+        lid_pt = np.zeros(c.ndofs, np.int64)
+        lid_pt[c.el.ravel()] = np.repeat(c.conn.ravel(), c.nf)
+        bcl = np.nonzero((lid_pt < c.X.shape[0]) & (np.abs(c.X[np.minimum(lid_pt, c.X.shape[0] - 1), 0]) < 1e-12))[0]
+        bcl = bcl.astype(np.int32)
+        O.dirichlet(bcl, None, x, c.rp, c.col, fr, vr)
+    f = torch.empty(c.ndofs, device="cuda", dtype=torch.float64)
+    v = torch.empty(c.nnz, device="cuda", dtype=torch.float64)
+    c.m.assemble(physics=physics, precision=precision, coef=dev(coef), x=dev(x),
+                 xdot=None if xd is None else dev(xd), xddot=None if xdd is None else dev(xdd),
+                 f=f, vals=v, alpha=alpha, beta=beta, gamma2=gamma2, transient=transient, nu=nu, rho=rho,
+                 src=src, rayleigh=rayleigh, bc_lids=None if bcl is None else dev(bcl, torch.int32))
+    torch.cuda.synchronize()
+    return f.cpu().numpy(), v.cpu().numpy(), fr, vr
+
+
+@pytest.mark.parametrize("n,jitter", [(1, 0.0), (3, 0.0), (5, 0.15), (33, 0.1)])
+def test_hex8_diffusion_fp64(engine, n, jitter):
+    c = Case(engine, txa.HEX8, n, n + 2, n + 1, 1, jitter=jitter)
+    f, v, fr, vr = run_both(engine, c, txa.DIFFUSION)
+    assert rel_err(f, fr) <= TOL64
+    assert rel_err(v, vr) <= TOL64
+    c.m.destroy()
+
+
+def test_hex8_diffusion_transient_and_dirichlet_fp64(engine):
+    c = Case(engine, txa.HEX8, 7, 6, 5, 1, jitter=0.1)
+    f, v, fr, vr = run_both(engine, c, txa.DIFFUSION, transient=True, bc=True, alpha=3.0, beta=0.5, rho=2.0)
+    assert rel_err(f, fr) <= TOL64
+    assert rel_err(v, vr) <= TOL64
+    c.m.destroy()
+
+
+@pytest.mark.parametrize("transient", [False, True])
+def test_hex8_diffusion_fp32(engine, transient):
+    c = Case(engine, txa.HEX8, 20, 18, 16, 1, jitter=0.1)
+    f, v, fr, vr = run_both(engine, c, txa.DIFFUSION, precision=txa.FP32, transient=transient, alpha=2.0)
+    assert rel_err(f, fr) <= TOL32
+    assert rel_err(v, vr) <= TOL32
+    assert entry_err(v, vr) <= 1e-5
+    c.m.destroy()
+
+
+@pytest.mark.parametrize("topo,physics,nf,n", [(txa.TET4, txa.DIFFUSION, 1, 4), (txa.HEX27, txa.DIFFUSION, 1, 3),
+                                               (txa.HEX8, txa.ELASTICITY, 3, 3), (txa.HEX27, txa.ELASTICITY, 3, 2)])
+def test_generic_path_fp64(engine, topo, physics, nf, n):
+    c = Case(engine, topo, n, n, n + 1, nf, jitter=0.1)
+    trans = physics == txa.ELASTICITY
+    f, v, fr, vr = run_both(engine, c, physics, transient=trans, src=(0.5, -1.0, 0.25), rho=1.5,
+                            rayleigh=(0.01, 0.001), alpha=4.0, gamma2=16.0)
+    assert rel_err(f, fr) <= TOL64
+    assert rel_err(v, vr) <= TOL64
+    c.m.destroy()
+
+
+def test_deterministic_repeat(engine):
+    c = Case(engine, txa.HEX8, 16, 16, 16, 1, jitter=0.1)
+    a = run_both(engine, c, txa.DIFFUSION)
+    b = run_both(engine, c, txa.DIFFUSION)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    c.m.destroy()

@@ -1,0 +1,214 @@
+"""MI355X-native finite-element assembly engine (Tianxin/Panzer volume residual + Jacobian path).
+
+The compute path is ``libtxa.so`` (hand-written HIP kernels for gfx950 behind the
+C ABI declared in ``include/txa.h``).  This module is the Python binding used by
+the tests and the benchmark: device memory and streams come from PyTorch, every
+entry point goes straight to the C ABI.  There is no CPU fallback: when the
+library is missing or no GPU is present, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libtxa.so")
+HEADER = os.path.join(ROOT, "include", "txa.h")
+
+TXA_OK, TXA_ERR_INVALID, TXA_ERR_LOGIC, TXA_ERR_DEVICE, TXA_ERR_NOMEM, TXA_ERR_UNSUPPORTED = range(6)
+HEX8, TET4, HEX27 = 1, 2, 3
+DIFFUSION, ELASTICITY = 0, 1
+FP64, FP32, SPLIT_BF16 = 0, 1, 2
+
+_lib = None
+
+
+class TxaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[txa error {code}] {msg}")
+        self.code = code
+
+
+P = ctypes.c_void_p
+I64 = ctypes.c_int64
+
+
+class AssemblyArgs(ctypes.Structure):
+    _fields_ = [
+        ("physics", ctypes.c_int), ("precision", ctypes.c_int),
+        ("coef", P), ("nu", ctypes.c_double), ("rho", ctypes.c_double),
+        ("src", ctypes.c_double * 3), ("rayleigh_a", ctypes.c_double), ("rayleigh_b", ctypes.c_double),
+        ("x", P), ("xdot", P), ("xddot", P),
+        ("alpha", ctypes.c_double), ("beta", ctypes.c_double), ("gamma2", ctypes.c_double),
+        ("transient", ctypes.c_int),
+        ("f", P), ("vals", P),
+        ("bc_lids", P), ("bc_vals", P), ("n_bc", I64),
+    ]
+
+
+def header_symbols(path: str = HEADER):
+    """Names of the functions declared in include/txa.h."""
+    text = open(path).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*)\s*(txa_\w+)\s*\(", text, re.M)))
+
+
+def load_library(path: str = LIB_PATH):
+    """Load libtxa.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with tianxin_amd/build.py (or __graft_entry__.build())")
+    L = ctypes.CDLL(path)
+    L.txa_create.argtypes = [ctypes.c_int, ctypes.POINTER(P)]
+    L.txa_destroy.argtypes = [P]
+    L.txa_last_error.argtypes = [P]
+    L.txa_last_error.restype = ctypes.c_char_p
+    L.txa_version.restype = ctypes.c_char_p
+    L.txa_sync.argtypes = [P]
+    L.txa_mesh_brick.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P,
+                                 ctypes.POINTER(ctypes.c_int)]
+    L.txa_mesh_destroy.argtypes = [P, ctypes.c_int]
+    L.txa_mesh_info.argtypes = [P, ctypes.c_int, P]
+    L.txa_dofmap_build.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(I64)]
+    L.txa_dofmap_copy.argtypes = [P, ctypes.c_int, P, P]
+    L.txa_dofmap_point_lids.argtypes = [P, ctypes.c_int, P, ctypes.POINTER(I64)]
+    L.txa_graph_build.argtypes = [P, ctypes.c_int, ctypes.POINTER(I64)]
+    L.txa_graph_copy.argtypes = [P, ctypes.c_int, P, P]
+    L.txa_graph_device.argtypes = [P, ctypes.c_int, ctypes.POINTER(P), ctypes.POINTER(P)]
+    L.txa_assemble.argtypes = [P, ctypes.c_int, ctypes.POINTER(AssemblyArgs), P]
+    _lib = L
+    return L
+
+
+def _np(a):
+    return a.ctypes.data_as(P)
+
+
+def _dptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    if t is None:
+        return None
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Engine:
+    """One context on one GPU (txa_create / txa_destroy)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load_library()
+        h = P()
+        rc = self.lib.txa_create(device, ctypes.byref(h))
+        if rc != TXA_OK:
+            raise TxaError(rc, self.lib.txa_last_error(None).decode())
+        self.h = h
+        self.device = device
+
+    def check(self, rc):
+        if rc != TXA_OK:
+            raise TxaError(rc, self.lib.txa_last_error(self.h).decode())
+
+    def sync(self):
+        self.check(self.lib.txa_sync(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.txa_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def brick(self, topo, nx, ny, nz, coords):
+        return BrickMesh(self, topo, nx, ny, nz, coords)
+
+
+class BrickMesh:
+    """Structured brick mesh resident on the device, with its DOF map and CRS graph."""
+
+    def __init__(self, eng: Engine, topo, nx, ny, nz, coords):
+        self.eng = eng
+        self.topo, self.nx, self.ny, self.nz = topo, nx, ny, nz
+        c = np.ascontiguousarray(coords, np.float64)
+        if c.shape != ((nx + 1) * (ny + 1) * (nz + 1), 3):
+            raise ValueError("coords must be [(nx+1)(ny+1)(nz+1)][3]")
+        mid = ctypes.c_int()
+        eng.check(eng.lib.txa_mesh_brick(eng.h, topo, nx, ny, nz, _np(c), ctypes.byref(mid)))
+        self.id = mid.value
+        info = np.zeros(3, np.int64)
+        eng.check(eng.lib.txa_mesh_info(eng.h, self.id, _np(info)))
+        self.nelem, self.nconn, self.nverts = (int(v) for v in info)
+        self.ndofs = None
+        self.nnz = None
+        self.nfields = None
+
+    def build_dofmap(self, nfields):
+        n = I64()
+        self.eng.check(self.eng.lib.txa_dofmap_build(self.eng.h, self.id, nfields, ctypes.byref(n)))
+        self.ndofs = int(n.value)
+        self.nfields = nfields
+        return self.ndofs
+
+    def dofmap(self):
+        owned = np.zeros(self.ndofs, np.int64)
+        el = np.zeros((self.nelem, self.nconn * self.nfields), np.int32)
+        self.eng.check(self.eng.lib.txa_dofmap_copy(self.eng.h, self.id, _np(owned), _np(el)))
+        return owned, el
+
+    def point_lids(self):
+        """LID of each lattice point and field, [points][nfields] (int32)."""
+        n = I64()
+        self.eng.check(self.eng.lib.txa_dofmap_point_lids(self.eng.h, self.id, None, ctypes.byref(n)))
+        out = np.zeros((int(n.value), self.nfields), np.int32)
+        self.eng.check(self.eng.lib.txa_dofmap_point_lids(self.eng.h, self.id, _np(out), None))
+        return out
+
+    def build_graph(self):
+        n = I64()
+        self.eng.check(self.eng.lib.txa_graph_build(self.eng.h, self.id, ctypes.byref(n)))
+        self.nnz = int(n.value)
+        return self.nnz
+
+    def graph(self, with_cols=True):
+        rp = np.zeros(self.ndofs + 1, np.int64)
+        col = np.zeros(max(self.nnz, 1), np.int32) if with_cols else None
+        self.eng.check(self.eng.lib.txa_graph_copy(self.eng.h, self.id, _np(rp), None if col is None else _np(col)))
+        return rp, (col[: self.nnz] if with_cols else None)
+
+    def assemble(self, *, physics, coef, x, f=None, vals=None, precision=FP64, xdot=None, xddot=None,
+                 alpha=0.0, beta=1.0, gamma2=0.0, transient=False, nu=0.3, rho=1.0, src=(0.0, 0.0, 0.0),
+                 rayleigh=(0.0, 0.0), bc_lids=None, bc_vals=None, stream=None):
+        """One residual + Jacobian evaluation into device tensors f [ndofs] and vals [nnz]."""
+        a = AssemblyArgs()
+        a.physics, a.precision = physics, precision
+        a.coef = _dptr(coef)
+        a.nu, a.rho = nu, rho
+        s = list(src) + [0.0] * 3
+        a.src = (ctypes.c_double * 3)(*s[:3])
+        a.rayleigh_a, a.rayleigh_b = rayleigh
+        a.x, a.xdot, a.xddot = _dptr(x), _dptr(xdot), _dptr(xddot)
+        a.alpha, a.beta, a.gamma2 = alpha, beta, gamma2
+        a.transient = int(transient)
+        a.f, a.vals = _dptr(f), _dptr(vals)
+        a.bc_lids, a.bc_vals = _dptr(bc_lids), _dptr(bc_vals)
+        a.n_bc = 0 if bc_lids is None else int(bc_lids.numel())
+        if stream is None:
+            import torch
+            stream = torch.cuda.current_stream().cuda_stream
+        self.eng.check(self.eng.lib.txa_assemble(self.eng.h, self.id, ctypes.byref(a), ctypes.c_void_p(stream)))
+
+    def destroy(self):
+        if self.id is not None:
+            self.eng.check(self.eng.lib.txa_mesh_destroy(self.eng.h, self.id))
+            self.id = None

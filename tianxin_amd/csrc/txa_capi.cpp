@@ -1,0 +1,176 @@
+// C ABI of the assembly engine: status codes + per-context last-error string,
+// no exception crosses the boundary.
+#include <cstring>
+#include <new>
+
+#include "txa_internal.hpp"
+
+using txa::Context;
+using txa::Error;
+using txa::Mesh;
+
+struct txa_context : Context {};
+
+namespace {
+
+thread_local std::string g_create_error;
+
+template <class F>
+int guarded(txa_ctx ctx, F &&f) {
+  if (!ctx) return TXA_ERR_INVALID;
+  try {
+    TXA_HIP(hipSetDevice(ctx->device));
+    f();
+    ctx->last_error.clear();
+    return TXA_OK;
+  } catch (const Error &e) {
+    ctx->last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    ctx->last_error = "host allocation failed";
+    return TXA_ERR_NOMEM;
+  } catch (const std::exception &e) {
+    ctx->last_error = e.what();
+    return TXA_ERR_LOGIC;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *txa_version(void) { return "tianxin_amd 0.1 (gfx950)"; }
+
+int txa_create(int device, txa_ctx *out) {
+  if (!out) return TXA_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    g_create_error = "no HIP device";
+    return TXA_ERR_DEVICE;
+  }
+  if (device < 0 || device >= n) {
+    g_create_error = "device index out of range";
+    return TXA_ERR_INVALID;
+  }
+  if (hipSetDevice(device) != hipSuccess) {
+    g_create_error = "hipSetDevice failed";
+    return TXA_ERR_DEVICE;
+  }
+  txa_context *c = new (std::nothrow) txa_context();
+  if (!c) return TXA_ERR_NOMEM;
+  c->device = device;
+  *out = c;
+  return TXA_OK;
+}
+
+int txa_destroy(txa_ctx ctx) {
+  if (!ctx) return TXA_ERR_INVALID;
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  delete ctx;
+  return TXA_OK;
+}
+
+const char *txa_last_error(txa_ctx ctx) { return ctx ? ctx->last_error.c_str() : g_create_error.c_str(); }
+
+int txa_sync(txa_ctx ctx) {
+  return guarded(ctx, [&] { TXA_HIP(hipDeviceSynchronize()); });
+}
+
+int txa_mesh_brick(txa_ctx ctx, int topo, int nx, int ny, int nz, const double *vertex_coords, int *mesh_id) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(mesh_id && vertex_coords, TXA_ERR_INVALID, "null argument");
+    TXA_REQUIRE(topo == TXA_HEX8 || topo == TXA_TET4 || topo == TXA_HEX27, TXA_ERR_INVALID, "unknown topology");
+    TXA_REQUIRE(nx >= 1 && ny >= 1 && nz >= 1, TXA_ERR_INVALID, "brick dimensions must be >= 1");
+    TXA_REQUIRE(static_cast<int64_t>(nx + 1) * (ny + 1) < (1LL << 31), TXA_ERR_UNSUPPORTED, "brick too wide");
+    auto m = std::make_unique<Mesh>();
+    m->topo = topo; m->nx = nx; m->ny = ny; m->nz = nz;
+    txa::mesh_generate(*m, vertex_coords, nullptr);
+    txa::upload_refelem(*m, m->ref, nullptr);
+    int id = ctx->next_id++;
+    ctx->meshes[id] = std::move(m);
+    *mesh_id = id;
+  });
+}
+
+int txa_mesh_destroy(txa_ctx ctx, int mesh_id) {
+  return guarded(ctx, [&] {
+    auto it = ctx->meshes.find(mesh_id);
+    TXA_REQUIRE(it != ctx->meshes.end(), TXA_ERR_INVALID, "unknown mesh id");
+    TXA_HIP(hipDeviceSynchronize());
+    ctx->meshes.erase(it);
+  });
+}
+
+int txa_mesh_info(txa_ctx ctx, int mesh_id, int64_t *out) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(out, TXA_ERR_INVALID, "null argument");
+    Mesh &m = ctx->mesh(mesh_id);
+    out[0] = m.nelem; out[1] = m.nconn; out[2] = m.nverts;
+  });
+}
+
+int txa_dofmap_build(txa_ctx ctx, int mesh_id, int nfields, int64_t *ndofs) {
+  return guarded(ctx, [&] {
+    Mesh &m = ctx->mesh(mesh_id);
+    txa::dofmap_build(m, nfields, nullptr);
+    if (ndofs) *ndofs = m.ndofs;
+  });
+}
+
+int txa_dofmap_copy(txa_ctx ctx, int mesh_id, int64_t *owned_gids, int32_t *elem_lids) {
+  return guarded(ctx, [&] {
+    Mesh &m = ctx->mesh(mesh_id);
+    TXA_REQUIRE(m.ndofs > 0, TXA_ERR_LOGIC, "DOF map not built");
+    if (owned_gids) TXA_HIP(hipMemcpy(owned_gids, m.owned.p, m.owned.bytes(), hipMemcpyDeviceToHost));
+    if (elem_lids)
+      TXA_HIP(hipMemcpy(elem_lids, m.elem_lid.p, sizeof(int32_t) * m.nelem * m.ndof_elem, hipMemcpyDeviceToHost));
+  });
+}
+
+int txa_dofmap_point_lids(txa_ctx ctx, int mesh_id, int32_t *point_lids, int64_t *npoints) {
+  return guarded(ctx, [&] {
+    Mesh &m = ctx->mesh(mesh_id);
+    TXA_REQUIRE(m.ndofs > 0, TXA_ERR_LOGIC, "DOF map not built");
+    if (npoints) *npoints = m.npoints;
+    if (point_lids)
+      TXA_HIP(hipMemcpy(point_lids, m.point_lid.p, sizeof(int32_t) * m.npoints * m.nfields, hipMemcpyDeviceToHost));
+  });
+}
+
+int txa_graph_build(txa_ctx ctx, int mesh_id, int64_t *nnz) {
+  return guarded(ctx, [&] {
+    Mesh &m = ctx->mesh(mesh_id);
+    txa::graph_build(m, nullptr);
+    if (nnz) *nnz = m.nnz;
+  });
+}
+
+int txa_graph_copy(txa_ctx ctx, int mesh_id, int64_t *rowptr, int32_t *col) {
+  return guarded(ctx, [&] {
+    Mesh &m = ctx->mesh(mesh_id);
+    TXA_REQUIRE(m.has_graph, TXA_ERR_LOGIC, "graph not built");
+    if (rowptr) TXA_HIP(hipMemcpy(rowptr, m.rowptr.p, sizeof(int64_t) * (m.ndofs + 1), hipMemcpyDeviceToHost));
+    if (col) TXA_HIP(hipMemcpy(col, m.col.p, sizeof(int32_t) * m.nnz, hipMemcpyDeviceToHost));
+  });
+}
+
+int txa_graph_device(txa_ctx ctx, int mesh_id, const int64_t **rowptr, const int32_t **col) {
+  return guarded(ctx, [&] {
+    Mesh &m = ctx->mesh(mesh_id);
+    TXA_REQUIRE(m.has_graph, TXA_ERR_LOGIC, "graph not built");
+    if (rowptr) *rowptr = m.rowptr.p;
+    if (col) *col = m.col.p;
+  });
+}
+
+int txa_assemble(txa_ctx ctx, int mesh_id, const txa_assembly_args *args, void *stream) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(args, TXA_ERR_INVALID, "null args");
+    Mesh &m = ctx->mesh(mesh_id);
+    txa::assemble(m, *args, static_cast<hipStream_t>(stream));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,112 @@
+// Internal declarations of the MI355X assembly engine (not part of the ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/txa.h"
+
+namespace txa {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define TXA_HIP(call)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      throw ::txa::Error(TXA_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define TXA_REQUIRE(cond, code, msg)                 \
+  do {                                               \
+    if (!(cond)) throw ::txa::Error((code), (msg));  \
+  } while (0)
+
+// Owning device buffer.
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error(e == hipErrorOutOfMemory ? TXA_ERR_NOMEM : TXA_ERR_DEVICE,
+                  std::string("hipMalloc of ") + std::to_string(count * sizeof(T)) + " bytes: " +
+                      hipGetErrorString(e));
+    }
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+// A structured brick mesh with its DOF map and CRS graph, resident on the device.
+struct Mesh {
+  int topo = 0, nx = 0, ny = 0, nz = 0;
+  int64_t nelem = 0;     // elements (6 per cell for TET4)
+  int nconn = 0;         // connectivity entities per element
+  int nvert_elem = 0;    // geometry vertices per element
+  int64_t nverts = 0;    // vertex nodes
+  int64_t nentities = 0; // entity-id bound
+  DevBuf<double> xyz;    // SoA vertex coordinates: x[nverts], y[nverts], z[nverts]
+  DevBuf<int64_t> conn;  // [nelem][nconn] entity ids
+  // DOF map
+  int nfields = 0;
+  int ndof_elem = 0;
+  int64_t ndofs = 0;
+  DevBuf<int64_t> owned;      // [ndofs] GID of each LID
+  DevBuf<int32_t> elem_lid;   // [nelem][ndof_elem]
+  int64_t npoints = 0;        // lattice points carrying DOFs (vertices; (2n+1)^3 for HEX27)
+  DevBuf<int32_t> point_lid;  // [npoints][nfields] LID of each lattice point and field
+  // graph
+  bool has_graph = false;
+  int64_t nnz = 0;
+  DevBuf<int64_t> rowptr;
+  DevBuf<int32_t> col;
+  int perm_width = 0;         // bytes per row of the stencil slot map (27*nfields rounded up to 16)
+  DevBuf<uint8_t> perm;       // [ndofs][perm_width] position of each stencil neighbour in the row, 255 = absent
+  DevBuf<double> ref;         // reference-element tables of the topology
+};
+
+struct Context {
+  int device = 0;
+  std::string last_error;
+  std::map<int, std::unique_ptr<Mesh>> meshes;
+  int next_id = 1;
+  Mesh &mesh(int id) {
+    auto it = meshes.find(id);
+    TXA_REQUIRE(it != meshes.end(), TXA_ERR_INVALID, "unknown mesh id " + std::to_string(id));
+    return *it->second;
+  }
+};
+
+// setup kernels (txa_setup.hip)
+void mesh_generate(Mesh &m, const double *host_coords, hipStream_t s);
+void dofmap_build(Mesh &m, int nfields, hipStream_t s);
+void graph_build(Mesh &m, hipStream_t s);
+void exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s);
+
+// assembly (txa_assemble.hip)
+void upload_refelem(Mesh &m, DevBuf<double> &tab, hipStream_t s);
+void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s);
+
+}  // namespace txa

@@ -1,0 +1,73 @@
+"""Seeded synthetic inputs for the assembly workloads.
+
+One counter-based generator (SplitMix64 -> uniform double in [0,1); Box-Muller
+-> normal) so every run regenerates the same mesh coordinates, coefficients and
+solution vectors from a seed.  The same arrays are handed to the GPU path and to
+the CPU checker, so no generator needs to be reproduced in a second language.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(seed: int, n: int, offset: int = 0) -> np.ndarray:
+    """n outputs of SplitMix64 for counters offset..offset+n-1 (uint64)."""
+    with np.errstate(over="ignore"):
+        i = np.arange(offset + 1, offset + n + 1, dtype=np.uint64)
+        z = np.uint64(seed & 0xFFFFFFFFFFFFFFFF) + i * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed: int, n: int, lo: float = 0.0, hi: float = 1.0) -> np.ndarray:
+    u = (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return lo + (hi - lo) * u
+
+
+def normal(seed: int, n: int, sigma: float = 1.0) -> np.ndarray:
+    m = (n + 1) // 2
+    u = uniform(seed, 2 * m)
+    u1 = 1.0 - u[0::2]          # (0, 1]
+    u2 = u[1::2]
+    r = np.sqrt(-2.0 * np.log(u1))
+    z = np.empty(2 * m)
+    z[0::2] = r * np.cos(2.0 * math.pi * u2)
+    z[1::2] = r * np.sin(2.0 * math.pi * u2)
+    return sigma * z[:n]
+
+
+def brick_coords(nx: int, ny: int, nz: int, *, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0),
+                 jitter: float = 0.0, seed: int = 0) -> np.ndarray:
+    """Vertex coordinates [(nx+1)(ny+1)(nz+1)][3], node id = i + (nx+1)(j + (ny+1)k).
+
+    Coordinates follow CubeHexMeshFactory's lattice (x0 + i*dx).  With
+    ``jitter`` > 0 every interior vertex moves by a seeded uniform offset in
+    [-jitter*h, +jitter*h] per axis; boundary vertices stay on the box faces.
+    """
+    h = [(hi[d] - lo[d]) / n for d, n in enumerate((nx, ny, nz))]
+    i = np.arange(nx + 1, dtype=np.float64)
+    j = np.arange(ny + 1, dtype=np.float64)
+    k = np.arange(nz + 1, dtype=np.float64)
+    X = np.empty(((nz + 1), (ny + 1), (nx + 1), 3))
+    X[..., 0] = lo[0] + i[None, None, :] * h[0]
+    X[..., 1] = lo[1] + j[None, :, None] * h[1]
+    X[..., 2] = lo[2] + k[:, None, None] * h[2]
+    if jitter > 0.0:
+        nn = X.shape[0] * X.shape[1] * X.shape[2]
+        off = uniform(seed, 3 * nn, -1.0, 1.0).reshape(X.shape)
+        inner = np.zeros(X.shape[:3], bool)
+        inner[1:-1, 1:-1, 1:-1] = True
+        for d in range(3):
+            X[..., d] += np.where(inner, off[..., d] * jitter * h[d], 0.0)
+    return X.reshape(-1, 3)
+
+
+def lognormal(seed: int, n: int, sigma: float) -> np.ndarray:
+    """exp(g), g ~ N(0, sigma^2) iid."""
+    return np.exp(normal(seed, n, sigma))
