@@ -398,7 +398,7 @@ __global__ __launch_bounds__(GB) void k_generic(GenericParams p) {
       atomicAdd(&sR[I], rcontrib);
       if (p.vals) {
         const int64_t slot = find_slot(p.rowptr, p.col, lids[I], lids[Jd]);
-        atomicAdd(p.vals + slot, jv);
+        if (slot < p.rowptr[lids[I] + 1] && p.col[slot] == lids[Jd]) atomicAdd(p.vals + slot, jv);
       }
     }
     __syncthreads();
@@ -416,13 +416,14 @@ __global__ __launch_bounds__(GB) void k_generic(GenericParams p) {
 // Dirichlet rows: f_i = x_i - g_i; matrix row -> identity row
 // (disc-fe/src/lof/Panzer_TpetraLinearObjContainer.hpp:228-237, 300-318)
 // ============================================================================
-__global__ void k_dirichlet(int64_t n, const int32_t *__restrict__ lids, const double *__restrict__ g,
+__global__ void k_dirichlet(int64_t n, int64_t nrows, const int32_t *__restrict__ lids, const double *__restrict__ g,
                             const double *__restrict__ x, const int64_t *__restrict__ rowptr,
                             const int32_t *__restrict__ col, double *__restrict__ f, double *__restrict__ vals) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x % 64;
   if (i >= n) return;
   const int32_t r = lids[i];
+  if (r < 0 || r >= nrows) return;  // LIDs outside the map are ignored
   if (lane == 0 && f) f[r] = x[r] - (g ? g[i] : 0.0);
   if (vals)
     for (int64_t k = rowptr[r] + lane; k < rowptr[r + 1]; k += 64) vals[k] = col[k] == r ? 1.0 : 0.0;
@@ -500,7 +501,7 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s) {
   if (a.n_bc > 0) {
     const int64_t per = 4;  // rows per 256-thread block
     hipLaunchKernelGGL(k_dirichlet, dim3(static_cast<unsigned>((a.n_bc + per - 1) / per)), dim3(256), 0, s, a.n_bc,
-                       a.bc_lids, a.bc_vals, a.x, m.rowptr.p, m.col.p, a.f, a.vals);
+                       m.ndofs, a.bc_lids, a.bc_vals, a.x, m.rowptr.p, m.col.p, a.f, a.vals);
     TXA_HIP(hipGetLastError());
   }
 }
