@@ -39,9 +39,7 @@
 #include <omp.h>
 #endif
 
-enum { ORC_QUAD4 = 0, ORC_HEX8 = 1, ORC_TET4 = 2, ORC_HEX27 = 3 };
-enum { ORC_DIFFUSION = 0, ORC_ELASTICITY = 1 };
-enum { ORC_MODE_FAD = 0, ORC_MODE_EXPLICIT = 1 };
+#include "oracle.h"
 
 #define MAXB 27   /* basis functions per element */
 #define MAXQ 27   /* integration points per element */
@@ -462,32 +460,7 @@ static void fad_axpy(fad *y, double s, const fad *x, int n)   /* y += s*x */
 /* Problem, cached worksets, evaluation                                      */
 /* ------------------------------------------------------------------------ */
 
-typedef struct {
-  int topo, physics;
-  int64_t nelem;
-  int nvert;
-  const int64_t *elem_vert;   /* [nelem][nvert] */
-  const double *coords;       /* [nnodes][dim] */
-  int ndof;
-  const int32_t *elem_lid;    /* [nelem][ndof] */
-  const double *coef;         /* per element: k (diffusion) or Young's modulus */
-  double nu, rho, src[3];
-  double rayleigh_a, rayleigh_b;
-  int workset_size;
-  int nthreads;
-  const int64_t *rowptr;
-  const int32_t *col;
-} orc_problem;
-
-typedef struct {
-  const double *x, *xdot, *xddot;   /* solution vectors by LID (xdot/xddot may be NULL) */
-  double alpha, beta, gamma2;       /* seeds for xdot, x, xddot */
-  int transient;                    /* workset.evaluate_transient_terms */
-  int mode;                         /* ORC_MODE_FAD or ORC_MODE_EXPLICIT */
-  int want_jac;
-} orc_state;
-
-typedef struct {
+struct orc_ctx {
   orc_problem p;
   refelem re;
   int nf;                /* fields per basis function */
@@ -497,7 +470,7 @@ typedef struct {
   double *gradN;         /* [nelem][nb][nq][dim]  grad_basis */
   double *wgradN;        /* [nelem][nb][nq][dim]  weighted_grad_basis */
   double *wN;            /* [nelem][nb][nq]       weighted_basis_scalar */
-} orc_ctx;
+};
 
 static void inv_det(int dim, const double J[3][3], double Ji[3][3], double *det)
 {

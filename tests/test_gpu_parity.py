@@ -147,3 +147,23 @@ def test_deterministic_repeat(engine):
     b = run_both(engine, c, txa.DIFFUSION)
     assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
     c.m.destroy()
+
+
+def test_hex8_diffusion_fp64_many_blocks_and_slabs(engine):
+    # several workgroups in x and y and several z slabs, checked against the oracle
+    c = Case(engine, txa.HEX8, 64, 48, 40, 1, jitter=0.1, seed=9)
+    f, v, fr, vr = run_both(engine, c, txa.DIFFUSION, bc=True)
+    assert rel_err(f, fr) <= TOL64
+    assert rel_err(v, vr) <= TOL64
+    c.m.destroy()
+
+
+def test_panzer_shim_driver():
+    # C++ evaluator layer (include/txa_panzer.hpp) driven like AssemblyEngine, checked against the oracle
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "_bin", "panzer_shim_test")
+    assert os.path.exists(exe), "build the driver with __graft_entry__.build()"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0 and "TEST PASSED" in r.stdout

@@ -13,267 +13,17 @@
 // disc-fe/src/Panzer_BasisValues2_impl.hpp:1379-1460) and the Sacado-FAD
 // Jacobian replaced by the explicit tangent beta*K + alpha*C + gamma2*M.
 //
-// Two kernel families:
-//  * k_q1_hex8_diffusion: hex8 scalar diffusion on the structured brick.
-//    Row-owner, z-marching: a workgroup owns a 31x7 patch of vertex columns,
-//    computes the 32x8 element columns around it one z-layer at a time (one
-//    element per thread), hands each element's rows to the owners of its
-//    vertices through LDS, and every owner writes its complete CRS row once
-//    with plain stores -- no atomics, no zero-fill, deterministic sums.
-//  * k_generic: any topology / physics, one workgroup per element, FP64 atomic
-//    scatter into a zeroed CRS (the unfused baseline and the coverage path).
+// Dispatch: hex8 diffusion goes to the row-owner kernel (txa_q1_hex8.hip);
+// every other topology / physics combination runs k_generic here: one
+// workgroup per element, FP64 atomic scatter into a zeroed CRS.
 #include "txa_internal.hpp"
 #include "txa_refelem.hpp"
 
+#include <algorithm>
+#include <vector>
+
 namespace txa {
 namespace {
-
-// ============================================================================
-// Fast path: hex8 diffusion, row-owner z-marching
-// ============================================================================
-constexpr int TX = 32, TY = 8, NT = TX * TY;
-constexpr double kG = 0.57735026918962576451;  // 1/sqrt(3)
-
-struct Q1DiffParams {
-  int nx, ny, nz;
-  const double *X, *Y, *Z;     // SoA vertex coordinates
-  const int32_t *lid;          // LID of each vertex (one field)
-  const double *coef;          // k per element
-  const double *x, *xdot;      // by LID
-  double alpha, beta, rhoc, src;
-  double *f, *vals;
-  const int64_t *rowptr;
-  const uint8_t *perm;         // [rows][32] stencil slot of the 27 neighbours, 255 = absent
-};
-
-__device__ __forceinline__ constexpr int sym8(int a, int b) {
-  return a <= b ? a * 8 - (a * (a - 1)) / 2 + (b - a) : b * 8 - (b * (b - 1)) / 2 + (a - b);
-}
-__device__ __forceinline__ constexpr double sgnbit(int v, int d) { return ((v >> d) & 1) ? 1.0 : -1.0; }
-
-// Element stiffness (and mass when TRANS) of a trilinear hex in binary vertex
-// order v = bx + 2 by + 4 bz, 2x2x2 Gauss.  Geometry (J, cofactors, det) in
-// FP64; the basis-gradient transform and the contraction in Real.
-// Outputs: J_e = beta K (+ alpha M) packed symmetric, R_e = K u (+ M udot) - F.
-template <typename Real, bool TRANS>
-__device__ __forceinline__ void hex8_diffusion_element(const double (&Xc)[3][8], const double (&u)[8],
-                                                       const double (&ud)[8], double k, double rhoc, double src,
-                                                       double alpha, double beta, Real (&Je)[36], Real (&Re)[8]) {
-  // x(xi) = c0 + c1 xi + c2 eta + c3 zeta + c4 xi eta + c5 eta zeta + c6 xi zeta + c7 xi eta zeta
-  double c[8][3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-#pragma unroll
-    for (int m = 1; m < 8; ++m) c[m][d] = 0.0;
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      const double x = Xc[d][v];
-      const double sx = sgnbit(v, 0), sy = sgnbit(v, 1), sz = sgnbit(v, 2);
-      c[1][d] += sx * x; c[2][d] += sy * x; c[3][d] += sz * x;
-      c[4][d] += sx * sy * x; c[5][d] += sy * sz * x; c[6][d] += sx * sz * x; c[7][d] += sx * sy * sz * x;
-    }
-#pragma unroll
-    for (int m = 1; m < 8; ++m) c[m][d] *= 0.125;
-  }
-  Real K[36];
-  Real M[TRANS ? 36 : 1];
-  double F[8];
-#pragma unroll
-  for (int i = 0; i < 36; ++i) { K[i] = Real(0); if (TRANS) M[i] = Real(0); }
-#pragma unroll
-  for (int v = 0; v < 8; ++v) F[v] = 0.0;
-  double sgn = 1.0;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const double xi = (q & 1) ? kG : -kG, eta = (q & 2) ? kG : -kG, zeta = (q & 4) ? kG : -kG;
-    double J[3][3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      J[d][0] = c[1][d] + c[4][d] * eta + c[6][d] * zeta + c[7][d] * (eta * zeta);
-      J[d][1] = c[2][d] + c[4][d] * xi + c[5][d] * zeta + c[7][d] * (xi * zeta);
-      J[d][2] = c[3][d] + c[5][d] * eta + c[6][d] * xi + c[7][d] * (xi * eta);
-    }
-    // cofactor matrix: (J^-1)_{ji} = cof_{ij} / det
-    double cof[3][3];
-    cof[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-    cof[0][1] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-    cof[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-    cof[1][0] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
-    cof[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
-    cof[1][2] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
-    cof[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
-    cof[2][1] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
-    cof[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    const double det = J[0][0] * cof[0][0] + J[0][1] * cof[0][1] + J[0][2] * cof[0][2];
-    if (q == 0) sgn = det < 0.0 ? -1.0 : 1.0;   // computeCellMeasure sign rule
-    const double wm = det * sgn;                 // Gauss weights are 1
-    const double idet = 1.0 / det;
-    const Real cw = Real(k * wm * idet * idet);  // grad N = cof . dN / det
-    Real cr[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) cr[i][j] = Real(cof[i][j]);
-    Real g[8][3];
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      const double sx = sgnbit(v, 0), sy = sgnbit(v, 1), sz = sgnbit(v, 2);
-      const Real dx = Real(0.125 * sx * (1.0 + sy * eta) * (1.0 + sz * zeta));
-      const Real dy = Real(0.125 * sy * (1.0 + sx * xi) * (1.0 + sz * zeta));
-      const Real dz = Real(0.125 * sz * (1.0 + sx * xi) * (1.0 + sy * eta));
-#pragma unroll
-      for (int i = 0; i < 3; ++i) g[v][i] = cr[i][0] * dx + cr[i][1] * dy + cr[i][2] * dz;
-      const double Nv = 0.125 * (1.0 + sx * xi) * (1.0 + sy * eta) * (1.0 + sz * zeta);
-      F[v] += (wm * src) * Nv;
-    }
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      Real ha[3] = {cw * g[a][0], cw * g[a][1], cw * g[a][2]};
-#pragma unroll
-      for (int b = a; b < 8; ++b) K[sym8(a, b)] += ha[0] * g[b][0] + ha[1] * g[b][1] + ha[2] * g[b][2];
-    }
-    if (TRANS) {
-      const Real cm = Real(rhoc * wm);
-#pragma unroll
-      for (int a = 0; a < 8; ++a) {
-        const double Na = 0.125 * (1.0 + sgnbit(a, 0) * xi) * (1.0 + sgnbit(a, 1) * eta) * (1.0 + sgnbit(a, 2) * zeta);
-#pragma unroll
-        for (int b = a; b < 8; ++b) {
-          const double Nb =
-              0.125 * (1.0 + sgnbit(b, 0) * xi) * (1.0 + sgnbit(b, 1) * eta) * (1.0 + sgnbit(b, 2) * zeta);
-          M[sym8(a, b)] += cm * Real(Na * Nb);
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    Real r = Real(0);
-#pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      r += K[sym8(a, b)] * Real(u[b]);
-      if (TRANS) r += M[sym8(a, b)] * Real(ud[b]);
-    }
-    Re[a] = Real(double(r) - F[a]);
-  }
-#pragma unroll
-  for (int i = 0; i < 36; ++i) Je[i] = TRANS ? Real(beta) * K[i] + Real(alpha) * M[i] : Real(beta) * K[i];
-}
-
-// Stage rows a_lo = cx + 2cy and a_hi = a_lo + 4 (binary order) of one element.
-template <typename Real>
-__device__ __forceinline__ void stage_corner(Real *s, int slot0, int t, const Real (&Je)[36], const Real (&Re)[8],
-                                             int alo) {
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    s[(slot0 + b) * NT + t] = Je[sym8(alo, b)];
-    s[(slot0 + 8 + b) * NT + t] = Je[sym8(alo + 4, b)];
-  }
-  s[(slot0 + 16) * NT + t] = Re[alo];
-  s[(slot0 + 17) * NT + t] = Re[alo + 4];
-}
-
-// Add the rows of the element at corner (cx, cy) of the owner's vertex column.
-template <int CX, int CY, typename Get>
-__device__ __forceinline__ void take_corner(double (&acc)[27], double (&accn)[18], double &rc, double &rn, Get get) {
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
-    const int s = (bx - CX + 1) + 3 * (by - CY + 1);
-    acc[s + 9 * (bz + 1)] += double(get(b));
-    accn[s + 9 * bz] += double(get(8 + b));
-  }
-  rc += double(get(16));
-  rn += double(get(17));
-}
-
-__device__ __forceinline__ void write_row(const Q1DiffParams &p, int64_t node, const double (&acc)[27], double r) {
-  const int32_t L = p.lid[node];
-  if (p.f) p.f[L] = r;
-  if (p.vals) {
-    const int64_t base = p.rowptr[L];
-    const uint4 *pp = reinterpret_cast<const uint4 *>(p.perm + static_cast<int64_t>(L) * 32);
-    const uint4 w0 = pp[0], w1 = pp[1];
-    const uint32_t w[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
-#pragma unroll
-    for (int s = 0; s < 27; ++s) {
-      const uint32_t o = (w[s >> 2] >> (8 * (s & 3))) & 0xffu;
-      if (o != 0xffu) p.vals[base + o] = acc[s];
-    }
-  }
-}
-
-template <typename Real, bool TRANS>
-__global__ __launch_bounds__(NT) void k_q1_hex8_diffusion(Q1DiffParams p) {
-  __shared__ Real stage[36 * NT];
-  const int t = threadIdx.x, tx = t % TX, ty = t / TX;
-  const int x0 = -1 + static_cast<int>(blockIdx.x) * (TX - 1);
-  const int y0 = -1 + static_cast<int>(blockIdx.y) * (TY - 1);
-  const int ex = x0 + tx, ey = y0 + ty;
-  const bool evalid = ex >= 0 && ex < p.nx && ey >= 0 && ey < p.ny;
-  const int oi = ex + 1, oj = ey + 1;  // owned vertex column: corner (1,1) of this thread's element
-  const bool owner = tx < TX - 1 && ty < TY - 1 && oi <= p.nx && oj <= p.ny;
-  const int64_t sx = p.nx + 1, sxy = static_cast<int64_t>(p.nx + 1) * (p.ny + 1);
-
-  double acc[27], accn[18];
-  double rc = 0.0, rn = 0.0;
-#pragma unroll
-  for (int s = 0; s < 27; ++s) acc[s] = 0.0;
-#pragma unroll
-  for (int s = 0; s < 18; ++s) accn[s] = 0.0;
-
-  for (int ez = 0; ez < p.nz; ++ez) {
-    Real Je[36], Re[8];
-    if (evalid) {
-      double Xc[3][8], u[8], ud[8];
-#pragma unroll
-      for (int v = 0; v < 8; ++v) {
-        const int64_t n = (ex + (v & 1)) + sx * (ey + ((v >> 1) & 1)) + sxy * (ez + ((v >> 2) & 1));
-        Xc[0][v] = p.X[n]; Xc[1][v] = p.Y[n]; Xc[2][v] = p.Z[n];
-        const int32_t L = p.lid[n];
-        u[v] = p.x[L];
-        ud[v] = TRANS ? p.xdot[L] : 0.0;
-      }
-      const double k = p.coef[ex + static_cast<int64_t>(p.nx) * (ey + static_cast<int64_t>(p.ny) * ez)];
-      hex8_diffusion_element<Real, TRANS>(Xc, u, ud, k, p.rhoc, p.src, p.alpha, p.beta, Je, Re);
-    } else {
-#pragma unroll
-      for (int i = 0; i < 36; ++i) Je[i] = Real(0);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) Re[i] = Real(0);
-    }
-    // round A: corners (0,1) [owner tx-1] and (1,0) [owner ty-1]
-    stage_corner(stage, 0, t, Je, Re, 2);
-    stage_corner(stage, 18, t, Je, Re, 1);
-    __syncthreads();
-    if (owner) {
-      take_corner<0, 1>(acc, accn, rc, rn, [&](int v) { return stage[v * NT + t + 1]; });
-      take_corner<1, 0>(acc, accn, rc, rn, [&](int v) { return stage[(18 + v) * NT + t + TX]; });
-    }
-    __syncthreads();
-    // round B: corner (0,0) [owner (tx-1, ty-1)]
-    stage_corner(stage, 0, t, Je, Re, 0);
-    __syncthreads();
-    if (owner) {
-      take_corner<0, 0>(acc, accn, rc, rn, [&](int v) { return stage[v * NT + t + TX + 1]; });
-      // own element: corner (1,1)
-      take_corner<1, 1>(acc, accn, rc, rn, [&](int v) -> Real {
-        if (v < 8) return Je[sym8(3, v)];
-        if (v < 16) return Je[sym8(7, v - 8)];
-        return v == 16 ? Re[3] : Re[7];
-      });
-      write_row(p, oi + sx * oj + sxy * ez, acc, rc);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < 18; ++s) { acc[s] = accn[s]; accn[s] = 0.0; }
-#pragma unroll
-    for (int s = 18; s < 27; ++s) acc[s] = 0.0;
-    rc = rn;
-    rn = 0.0;
-  }
-  if (owner) write_row(p, oi + sx * oj + sxy * p.nz, acc, rc);
-}
 
 // ============================================================================
 // Generic path: one workgroup per element, atomic scatter
@@ -459,23 +209,7 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s) {
   const bool trans = a.transient != 0;
 
   if (m.topo == TXA_HEX8 && a.physics == TXA_DIFFUSION && a.precision != TXA_SPLIT_BF16) {
-    Q1DiffParams p;
-    p.nx = m.nx; p.ny = m.ny; p.nz = m.nz;
-    p.X = m.xyz.p; p.Y = m.xyz.p + m.nverts; p.Z = m.xyz.p + 2 * m.nverts;
-    p.lid = m.point_lid.p;
-    p.coef = a.coef; p.x = a.x; p.xdot = a.xdot;
-    p.alpha = a.alpha; p.beta = a.beta; p.rhoc = a.rho; p.src = a.src[0];
-    p.f = a.f; p.vals = a.vals;
-    p.rowptr = m.rowptr.p; p.perm = m.perm.p;
-    dim3 grid((m.nx + 1 + TX - 2) / (TX - 1), (m.ny + 1 + TY - 2) / (TY - 1));
-    if (a.precision == TXA_FP64) {
-      if (trans) hipLaunchKernelGGL((k_q1_hex8_diffusion<double, true>), grid, dim3(NT), 0, s, p);
-      else hipLaunchKernelGGL((k_q1_hex8_diffusion<double, false>), grid, dim3(NT), 0, s, p);
-    } else {
-      if (trans) hipLaunchKernelGGL((k_q1_hex8_diffusion<float, true>), grid, dim3(NT), 0, s, p);
-      else hipLaunchKernelGGL((k_q1_hex8_diffusion<float, false>), grid, dim3(NT), 0, s, p);
-    }
-    TXA_HIP(hipGetLastError());
+    launch_q1_hex8_diffusion(m, a, s);
   } else {
     TXA_REQUIRE(a.precision == TXA_FP64, TXA_ERR_UNSUPPORTED,
                 "reduced-precision integration is implemented for hex8 diffusion only");

@@ -108,5 +108,7 @@ void exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t 
 // assembly (txa_assemble.hip)
 void upload_refelem(Mesh &m, DevBuf<double> &tab, hipStream_t s);
 void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s);
+// hex8 diffusion fast path (txa_q1_hex8.hip)
+void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s);
 
 }  // namespace txa

@@ -1,0 +1,594 @@
+// Hex8 (Q1) scalar diffusion on the structured brick: the row-owner,
+// z-marching assembly kernel.
+//
+// Replaces Gather -> DOF/DOFGradient -> Integrator_GradBasisDotVector (k grad u),
+// Integrator_BasisTimesScalar (source, transient rho_c udot) -> ScatterResidual_Tpetra
+// (disc-fe/src/evaluators/Panzer_GatherSolution_Tpetra_impl.hpp:549-650,
+//  Panzer_DOFGradient_impl.hpp:60-160, Panzer_Integrator_GradBasisDotVector_impl.hpp:225-294,
+//  Panzer_Integrator_BasisTimesScalar_impl.hpp:215-270, Panzer_ScatterResidual_Tpetra_impl.hpp:375-520)
+// for HGRAD_HEX_C1 with 2x2x2 Gauss, with IntegrationValues2 / BasisValues2
+// (disc-fe/src/Panzer_IntegrationValues2.cpp:949-1221, Panzer_BasisValues2_impl.hpp:1379-1460)
+// recomputed per element in registers.
+//
+// Element stiffness by sum factorisation.  At the 2x2x2 Gauss points the
+// reference gradient of vertex a factors into 1D terms,
+//   dN_a/dxi = (s_ax / 2) P(a_y, q_y) P(a_z, q_z),   P(bit, qbit) = (1 +- 1/sqrt3)/2,
+// so K_ab = sum_q sum_jk A_q[jk] D_j(a,q) D_k(b,q) with the metric tensor
+// A_q = k w |det J| J^-1 J^-T = k sgn / det * cof^T cof (cof = cofactor matrix of J)
+// contracts axis by axis: three diagonal tables S_jj (3x3 classes each) and
+// three cross tables X_jk (12 entries each) built from the 8x6 metric values,
+// then each of the 36 symmetric entries is a signed sum of 9 table values.
+// About 650 operations instead of ~1600 for the direct product.
+//
+// Geometry (trilinear-map coefficients, J, cofactors, det, 1/det) is FP64 in
+// every precision; Real is the precision of the metric tensor and the
+// contraction; row sums over elements are FP64.
+#include "txa_internal.hpp"
+
+#include <algorithm>
+
+namespace txa {
+namespace {
+
+constexpr int TX = 32, TY = 8, NE = TX * TY;   // element tile: one element per thread of waves 0-3
+constexpr int NT = 2 * NE;                     // block: 4 element waves + 4 owner waves
+constexpr double kG = 0.57735026918962576451;   // 1/sqrt(3)
+constexpr double kP = 0.5 * (1.0 + kG);         // P(bit == qbit)
+constexpr double kM = 0.5 * (1.0 - kG);         // P(bit != qbit)
+constexpr int NSTAGE = 44;                      // 36 packed K + 8 R per element
+
+#ifndef TXA_Q1_NT_STORES
+#define TXA_Q1_NT_STORES 0    // non-temporal stores for the CRS values
+#endif
+
+struct Q1DiffParams {
+  int nx, ny, nz;
+  int zc;                      // vertex layers owned per workgroup (blockIdx.z selects the slab)
+  const double *X, *Y, *Z;     // SoA vertex coordinates
+  const int32_t *lid;          // LID of each vertex (one field)
+  const double *coef;          // k per element
+  const double *x, *xdot;      // by LID
+  double alpha, beta, rhoc, src;
+  double *f, *vals;
+  const int64_t *rowptr;
+  const uint8_t *perm;         // [rows][32] stencil slot of the 27 neighbours, 255 = absent
+};
+
+__device__ __forceinline__ constexpr int sym8(int a, int b) {
+  return a <= b ? a * 8 - (a * (a - 1)) / 2 + (b - a) : b * 8 - (b * (b - 1)) / 2 + (a - b);
+}
+__device__ __forceinline__ constexpr int bit(int v, int d) { return (v >> d) & 1; }
+__device__ __forceinline__ constexpr double sgn(int v, int d) { return bit(v, d) ? 1.0 : -1.0; }
+__device__ __forceinline__ constexpr double Pq(int b, int qb) { return b == qb ? kP : kM; }
+// pair class of two bits: 0 = (0,0), 1 = (1,1), 2 = mixed
+__device__ __forceinline__ constexpr int cls(int a, int b) { return a == b ? a : 2; }
+__device__ __forceinline__ constexpr int cls_lo(int c) { return c == 1 ? 1 : 0; }
+__device__ __forceinline__ constexpr int cls_hi(int c) { return c == 0 ? 0 : 1; }
+__device__ __forceinline__ constexpr double Wq(int c, int qb) { return Pq(cls_lo(c), qb) * Pq(cls_hi(c), qb); }
+
+template <typename Real, bool TRANS>
+struct Hex8Out {
+  Real K[36];   // beta K + alpha M, packed symmetric, binary vertex order
+  Real R[8];    // K u + M udot - F
+};
+
+// One element.  X: vertex coordinates in binary order v = bx + 2by + 4bz.
+template <typename Real, bool TRANS>
+__device__ __forceinline__ void hex8_element(const double (&X)[3][8], const double (&u)[8], const double (&ud)[8],
+                                             double k, double rhoc, double src, double alpha, double beta,
+                                             Hex8Out<Real, TRANS> &o) {
+  // Walsh transform of the vertex coordinates: h[s] = sum_v prod_{d in s} sgn(v,d) X_v,
+  // the coefficients (times 8) of 1, xi, eta, xi eta, zeta, xi zeta, eta zeta, xi eta zeta
+  double h[3][8];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) h[d][v] = X[d][v];
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int v = 0; v < 8; ++v)
+        if (!bit(v, b)) {
+          const double lo = h[d][v], hi = h[d][v | (1 << b)];
+          h[d][v] = lo + hi;
+          h[d][v | (1 << b)] = hi - lo;
+        }
+  }
+  // metric tensor at the 8 points, scale 1/4 of the gradient products folded in
+  Real A[8][6];     // xx yy zz xy yz xz
+  double wm[8];
+  double sg = 1.0;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const double xi = bit(q, 0) ? kG : -kG, eta = bit(q, 1) ? kG : -kG, zeta = bit(q, 2) ? kG : -kG;
+    double J[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      J[d][0] = 0.125 * (h[d][1] + h[d][3] * eta + h[d][5] * zeta + h[d][7] * (eta * zeta));
+      J[d][1] = 0.125 * (h[d][2] + h[d][3] * xi + h[d][6] * zeta + h[d][7] * (xi * zeta));
+      J[d][2] = 0.125 * (h[d][4] + h[d][5] * xi + h[d][6] * eta + h[d][7] * (xi * eta));
+    }
+    double c[3][3];
+    c[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    c[0][1] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    c[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    c[1][0] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+    c[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+    c[1][2] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+    c[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    c[2][1] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+    c[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const double det = J[0][0] * c[0][0] + J[0][1] * c[0][1] + J[0][2] * c[0][2];
+    if (q == 0) sg = det < 0.0 ? -1.0 : 1.0;   // computeCellMeasure: sign of det at the first point
+    wm[q] = det * sg;                          // Gauss weights are 1
+    const double cw = 0.25 * k * sg / det;     // k wm / det^2, times the 1/4 of (s/2)(s/2)
+    Real cr[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) cr[i][j] = Real(c[i][j]);
+    const Real cwr = Real(cw);
+    A[q][0] = cwr * (cr[0][0] * cr[0][0] + cr[1][0] * cr[1][0] + cr[2][0] * cr[2][0]);
+    A[q][1] = cwr * (cr[0][1] * cr[0][1] + cr[1][1] * cr[1][1] + cr[2][1] * cr[2][1]);
+    A[q][2] = cwr * (cr[0][2] * cr[0][2] + cr[1][2] * cr[1][2] + cr[2][2] * cr[2][2]);
+    A[q][3] = cwr * (cr[0][0] * cr[0][1] + cr[1][0] * cr[1][1] + cr[2][0] * cr[2][1]);
+    A[q][4] = cwr * (cr[0][1] * cr[0][2] + cr[1][1] * cr[1][2] + cr[2][1] * cr[2][2]);
+    A[q][5] = cwr * (cr[0][0] * cr[0][2] + cr[1][0] * cr[1][2] + cr[2][0] * cr[2][2]);
+  }
+  // ---- diagonal tables S_dd[c1][c2]: sum over the two other axes after summing axis d
+  Real S[3][3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;   // the two other axes, ascending
+    Real Ab[2][2];
+#pragma unroll
+    for (int i1 = 0; i1 < 2; ++i1)
+#pragma unroll
+      for (int i2 = 0; i2 < 2; ++i2) {
+        const int q0 = (i1 << d1) | (i2 << d2);
+        Ab[i1][i2] = A[q0][d] + A[q0 | (1 << d)][d];
+      }
+    Real T[2][3];
+#pragma unroll
+    for (int i1 = 0; i1 < 2; ++i1)
+#pragma unroll
+      for (int c2 = 0; c2 < 3; ++c2) T[i1][c2] = Real(Wq(c2, 0)) * Ab[i1][0] + Real(Wq(c2, 1)) * Ab[i1][1];
+#pragma unroll
+    for (int c1 = 0; c1 < 3; ++c1)
+#pragma unroll
+      for (int c2 = 0; c2 < 3; ++c2) S[d][c1][c2] = Real(Wq(c1, 0)) * T[0][c2] + Real(Wq(c1, 1)) * T[1][c2];
+  }
+  // ---- cross tables
+  // Xxy[al][be][cz] = sum_q A_xy P(al,qx) P(be,qy) W(cz,qz)
+  // Xyz[cx][be][ga] = sum_q A_yz W(cx,qx) P(be,qy) P(ga,qz)
+  // Xxz[al][cy][ga] = sum_q A_xz P(al,qx) W(cy,qy) P(ga,qz)
+  Real Xxy[2][2][3], Xyz[3][2][2], Xxz[2][3][2];
+  {
+    Real U[2][2][2];   // [al][qy][qz]
+#pragma unroll
+    for (int al = 0; al < 2; ++al)
+#pragma unroll
+      for (int qy = 0; qy < 2; ++qy)
+#pragma unroll
+        for (int qz = 0; qz < 2; ++qz)
+          U[al][qy][qz] = Real(Pq(al, 0)) * A[0 | (qy << 1) | (qz << 2)][3] +
+                          Real(Pq(al, 1)) * A[1 | (qy << 1) | (qz << 2)][3];
+#pragma unroll
+    for (int al = 0; al < 2; ++al)
+#pragma unroll
+      for (int be = 0; be < 2; ++be) {
+        const Real V0 = Real(Pq(be, 0)) * U[al][0][0] + Real(Pq(be, 1)) * U[al][1][0];
+        const Real V1 = Real(Pq(be, 0)) * U[al][0][1] + Real(Pq(be, 1)) * U[al][1][1];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Xxy[al][be][c] = Real(Wq(c, 0)) * V0 + Real(Wq(c, 1)) * V1;
+      }
+  }
+  {
+    Real U[2][2][2];   // [ga][qx][qy]: sum over qz with P(ga, qz)
+#pragma unroll
+    for (int ga = 0; ga < 2; ++ga)
+#pragma unroll
+      for (int qx = 0; qx < 2; ++qx)
+#pragma unroll
+        for (int qy = 0; qy < 2; ++qy)
+          U[ga][qx][qy] = Real(Pq(ga, 0)) * A[qx | (qy << 1)][4] + Real(Pq(ga, 1)) * A[qx | (qy << 1) | 4][4];
+#pragma unroll
+    for (int ga = 0; ga < 2; ++ga)
+#pragma unroll
+      for (int be = 0; be < 2; ++be) {
+        const Real V0 = Real(Pq(be, 0)) * U[ga][0][0] + Real(Pq(be, 1)) * U[ga][0][1];   // qx = 0
+        const Real V1 = Real(Pq(be, 0)) * U[ga][1][0] + Real(Pq(be, 1)) * U[ga][1][1];   // qx = 1
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Xyz[c][be][ga] = Real(Wq(c, 0)) * V0 + Real(Wq(c, 1)) * V1;
+      }
+  }
+  {
+    Real U[2][2][2];   // [al][qy][qz]: sum over qx with P(al, qx)
+#pragma unroll
+    for (int al = 0; al < 2; ++al)
+#pragma unroll
+      for (int qy = 0; qy < 2; ++qy)
+#pragma unroll
+        for (int qz = 0; qz < 2; ++qz)
+          U[al][qy][qz] = Real(Pq(al, 0)) * A[0 | (qy << 1) | (qz << 2)][5] +
+                          Real(Pq(al, 1)) * A[1 | (qy << 1) | (qz << 2)][5];
+#pragma unroll
+    for (int al = 0; al < 2; ++al)
+#pragma unroll
+      for (int ga = 0; ga < 2; ++ga) {
+        const Real V0 = Real(Pq(ga, 0)) * U[al][0][0] + Real(Pq(ga, 1)) * U[al][0][1];   // qy = 0
+        const Real V1 = Real(Pq(ga, 0)) * U[al][1][0] + Real(Pq(ga, 1)) * U[al][1][1];   // qy = 1
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Xxz[al][c][ga] = Real(Wq(c, 0)) * V0 + Real(Wq(c, 1)) * V1;
+      }
+  }
+  // ---- entries
+  Real K[36];
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = a; b < 8; ++b) {
+      const int ax = bit(a, 0), ay = bit(a, 1), az = bit(a, 2);
+      const int bx = bit(b, 0), by = bit(b, 1), bz = bit(b, 2);
+      const int cx = cls(ax, bx), cy = cls(ay, by), cz = cls(az, bz);
+      Real v = Real(sgn(a, 0) * sgn(b, 0)) * S[0][cy][cz] + Real(sgn(a, 1) * sgn(b, 1)) * S[1][cx][cz] +
+               Real(sgn(a, 2) * sgn(b, 2)) * S[2][cx][cy];
+      v += Real(sgn(a, 0) * sgn(b, 1)) * Xxy[bx][ay][cz] + Real(sgn(a, 1) * sgn(b, 0)) * Xxy[ax][by][cz];
+      v += Real(sgn(a, 1) * sgn(b, 2)) * Xyz[cx][by][az] + Real(sgn(a, 2) * sgn(b, 1)) * Xyz[cx][ay][bz];
+      v += Real(sgn(a, 0) * sgn(b, 2)) * Xxz[bx][cy][az] + Real(sgn(a, 2) * sgn(b, 0)) * Xxz[ax][cy][bz];
+      K[sym8(a, b)] = v;
+    }
+  // ---- source load F_a = src sum_q wm_q N_a(q) and mass classes (FP64), axis by axis
+  double F[8];
+  {
+    double t1[2][2][2];   // [ax][qy][qz]
+#pragma unroll
+    for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+      for (int qy = 0; qy < 2; ++qy)
+#pragma unroll
+        for (int qz = 0; qz < 2; ++qz)
+          t1[ax][qy][qz] = Pq(ax, 0) * wm[(qy << 1) | (qz << 2)] + Pq(ax, 1) * wm[1 | (qy << 1) | (qz << 2)];
+#pragma unroll
+    for (int ax = 0; ax < 2; ++ax)
+#pragma unroll
+      for (int ay = 0; ay < 2; ++ay) {
+        const double t20 = Pq(ay, 0) * t1[ax][0][0] + Pq(ay, 1) * t1[ax][1][0];
+        const double t21 = Pq(ay, 0) * t1[ax][0][1] + Pq(ay, 1) * t1[ax][1][1];
+#pragma unroll
+        for (int az = 0; az < 2; ++az) F[ax | (ay << 1) | (az << 2)] = src * (Pq(az, 0) * t20 + Pq(az, 1) * t21);
+      }
+  }
+  Real M[TRANS ? 27 : 1];   // M_ab = rho_c sum_q wm N_a N_b, indexed by the three pair classes
+  if (TRANS) {
+    double m1[3][2][2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int qy = 0; qy < 2; ++qy)
+#pragma unroll
+        for (int qz = 0; qz < 2; ++qz)
+          m1[c][qy][qz] = Wq(c, 0) * wm[(qy << 1) | (qz << 2)] + Wq(c, 1) * wm[1 | (qy << 1) | (qz << 2)];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int cy = 0; cy < 3; ++cy) {
+        const double m20 = Wq(cy, 0) * m1[c][0][0] + Wq(cy, 1) * m1[c][1][0];
+        const double m21 = Wq(cy, 0) * m1[c][0][1] + Wq(cy, 1) * m1[c][1][1];
+#pragma unroll
+        for (int cz = 0; cz < 3; ++cz) M[c + 3 * (cy + 3 * cz)] = Real(rhoc * (Wq(cz, 0) * m20 + Wq(cz, 1) * m21));
+      }
+  }
+  // ---- residual and tangent
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    Real r = Real(0);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      r += K[sym8(a, b)] * Real(u[b]);
+      if (TRANS) {
+        const int mc = cls(bit(a, 0), bit(b, 0)) + 3 * (cls(bit(a, 1), bit(b, 1)) + 3 * cls(bit(a, 2), bit(b, 2)));
+        r += M[mc] * Real(ud[b]);
+      }
+    }
+    o.R[a] = Real(double(r) - F[a]);
+  }
+#pragma unroll
+  for (int a = 0; a < 8; ++a)
+#pragma unroll
+    for (int b = a; b < 8; ++b) {
+      Real j = Real(beta) * K[sym8(a, b)];
+      if (TRANS) {
+        const int mc = cls(bit(a, 0), bit(b, 0)) + 3 * (cls(bit(a, 1), bit(b, 1)) + 3 * cls(bit(a, 2), bit(b, 2)));
+        j += Real(alpha) * M[mc];
+      }
+      o.K[sym8(a, b)] = j;
+    }
+}
+
+__device__ __forceinline__ void scatter_row(const Q1DiffParams &p, int32_t L, int64_t base, const uint32_t (&w)[8],
+                                            const double (&acc)[27]) {
+#pragma unroll
+  for (int s = 0; s < 27; ++s) {
+    const uint32_t o = (w[s >> 2] >> (8 * (s & 3))) & 0xffu;
+    if (o != 0xffu) p.vals[base + o] = acc[s];
+  }
+}
+
+constexpr int ROWBUF = 27 * 32;   // doubles per 32-lane group: 31 rows of at most 27 entries
+
+// Orders LDS writes and reads between the lanes of one wavefront.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Writes the finished rows of one vertex layer; called by the owner waves.
+// Rows are written group by group (the 32 lanes of one x line of owners): when
+// the owners of a group hold consecutive LIDs their rows are one contiguous CSR
+// range, assembled in LDS in column order and stored with coalesced writes;
+// otherwise each owner stores its own row entry by entry.
+struct RowMeta {
+  int32_t L = -1;           // LID of the row (-1: no row on this lane)
+  int64_t base = 0, end = 0;
+  uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // stencil slot map, 27 bytes used
+};
+
+// Row pointers and slot map of row L, issued ahead of use so the loads overlap the element work.
+__device__ __forceinline__ RowMeta load_meta(const Q1DiffParams &p, int32_t L) {
+  RowMeta m;
+  m.L = L;
+  if (L >= 0 && p.vals) {
+    m.base = p.rowptr[L];
+    m.end = p.rowptr[L + 1];
+    const uint4 *pp = reinterpret_cast<const uint4 *>(p.perm + static_cast<int64_t>(L) * 32);
+    const uint4 w0 = pp[0], w1 = pp[1];
+    m.w[0] = w0.x; m.w[1] = w0.y; m.w[2] = w0.z; m.w[3] = w0.w;
+    m.w[4] = w1.x; m.w[5] = w1.y; m.w[6] = w1.z; m.w[7] = w1.w;
+  }
+  return m;
+}
+
+__device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &m, const double (&acc)[27], double r,
+                                          double *rowbuf) {
+  const int lane = threadIdx.x & 63, gl = lane & 31, half = lane >> 5;
+  const int grp = (threadIdx.x - NE) >> 5;
+  const bool act = m.L >= 0;
+  const int32_t L = m.L;
+  const int64_t base = m.base, end = m.end;
+  if (act && p.f) p.f[L] = r;
+  if (!p.vals) return;   // uniform over the block
+  // owners of a group are a prefix of its lanes; contiguous when successive LIDs differ by one
+  const int32_t Lnext = __shfl_down(L, 1, 32);
+  const bool next_act = __shfl_down(static_cast<int>(act), 1, 32) != 0 && gl < 31;
+  const bool ok = !act || !next_act || Lnext == L + 1;
+  const uint64_t okm = __ballot(ok), actm = __ballot(act);
+  const uint32_t gok = static_cast<uint32_t>(okm >> (32 * half)), gact = static_cast<uint32_t>(actm >> (32 * half));
+  const int nact = __popc(gact);
+  const bool contiguous = gok == 0xffffffffu && nact > 0;
+  const int64_t base0 = __shfl(base, 0, 32);
+  const int64_t end_last = __shfl(end, nact > 0 ? nact - 1 : 0, 32);
+  double *buf = rowbuf + grp * ROWBUF;
+  if (contiguous && act) {
+    const int64_t off = base - base0;
+#pragma unroll
+    for (int s = 0; s < 27; ++s) {
+      const uint32_t o = (m.w[s >> 2] >> (8 * (s & 3))) & 0xffu;
+      if (o != 0xffu) buf[off + o] = acc[s];
+    }
+  }
+  wave_lds_sync();   // the group's lanes live in one wavefront
+  if (contiguous) {
+    const int total = static_cast<int>(end_last - base0);
+#if TXA_Q1_NT_STORES
+    for (int j = gl; j < total; j += 32) __builtin_nontemporal_store(buf[j], p.vals + base0 + j);
+#else
+    for (int j = gl; j < total; j += 32) p.vals[base0 + j] = buf[j];
+#endif
+  } else if (act) {
+    scatter_row(p, L, base, m.w, acc);
+  }
+}
+
+// Add rows a_lo = CX + 2CY (bottom) and a_lo + 4 (top) of the element at
+// corner (CX, CY) of the owner's vertex column, read from the stage.
+template <int CX, int CY, typename Real>
+__device__ __forceinline__ void take_corner(const Real *s, int col, double (&acc)[27], double (&accn)[18], double &rc,
+                                            double &rn) {
+  constexpr int alo = CX + 2 * CY, ahi = alo + 4;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
+    const int st = (bx - CX + 1) + 3 * (by - CY + 1);
+    acc[st + 9 * (bz + 1)] += double(s[sym8(alo, b) * NE + col]);
+    accn[st + 9 * bz] += double(s[sym8(ahi, b) * NE + col]);
+  }
+  rc += double(s[(36 + alo) * NE + col]);
+  rn += double(s[(36 + ahi) * NE + col]);
+}
+
+// Vertex data of one face (4 vertices) of an element column.
+template <bool TRANS>
+struct Face {
+  double x[3][4], u[4], ud[4];
+};
+
+template <bool TRANS>
+__device__ __forceinline__ void load_face(const Q1DiffParams &p, int64_t n0, int64_t sx, const int32_t (&L)[4],
+                                          Face<TRANS> &f) {
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int64_t n = n0 + (v & 1) + sx * (v >> 1);
+    f.x[0][v] = p.X[n]; f.x[1][v] = p.Y[n]; f.x[2][v] = p.Z[n];
+    f.u[v] = p.x[L[v]];
+    f.ud[v] = TRANS ? p.xdot[L[v]] : 0.0;
+  }
+}
+
+__device__ __forceinline__ void load_face_lids(const Q1DiffParams &p, int64_t n0, int64_t sx, int32_t (&L)[4]) {
+#pragma unroll
+  for (int v = 0; v < 4; ++v) L[v] = p.lid[n0 + (v & 1) + sx * (v >> 1)];
+}
+
+// Block = 4 element waves + 4 owner waves.  Per element layer ez the element
+// waves compute layer ez into registers while the owner waves sum layer ez-1
+// from the stage and store the finished rows; then the element waves publish
+// layer ez in the stage.  Loads (element waves) and stores (owner waves) sit in
+// different waves, so neither waits for the other's memory counter.
+template <typename Real, bool TRANS>
+__global__ __launch_bounds__(NT) void k_q1_hex8_diffusion(Q1DiffParams p) {
+  __shared__ Real stage[NSTAGE * NE];
+  __shared__ double rowbuf[8 * ROWBUF];
+  const bool is_elem = threadIdx.x < NE;
+  const int t = is_elem ? threadIdx.x : threadIdx.x - NE;
+  const int tx = t % TX, ty = t / TX;
+  const int x0 = -1 + static_cast<int>(blockIdx.x) * (TX - 1);
+  const int y0 = -1 + static_cast<int>(blockIdx.y) * (TY - 1);
+  const int64_t sx = p.nx + 1, sxy = static_cast<int64_t>(p.nx + 1) * (p.ny + 1);
+  // this workgroup owns vertex layers [k0, k1); element layer k0-1 is recomputed
+  // (it also belongs to the slab below) because vertex layer k0 needs it
+  const int k0 = static_cast<int>(blockIdx.z) * p.zc;
+  const int k1 = min(k0 + p.zc, p.nz + 1);
+  const int ez0 = max(k0 - 1, 0), ez1 = min(k1 - 1, p.nz - 1);
+
+  if (is_elem) {
+    // ---------------- element waves
+    constexpr bool kPrefetch = sizeof(Real) == 4;   // FP32 integration leaves registers for it
+    const int ex = x0 + tx, ey = y0 + ty;
+    const bool evalid = ex >= 0 && ex < p.nx && ey >= 0 && ey < p.ny;
+    const int64_t col0 = evalid ? ex + sx * ey : 0;   // vertex (ex, ey, 0)
+    Face<TRANS> fb, ft, fn;        // bottom, top and prefetched next-top faces
+    int32_t Ln[4] = {0, 0, 0, 0};  // LIDs of the face after the prefetched one
+    if (evalid) {
+      int32_t L[4];
+      load_face_lids(p, col0 + sxy * ez0, sx, L);
+      load_face(p, col0 + sxy * ez0, sx, L, fb);
+      load_face_lids(p, col0 + sxy * (ez0 + 1), sx, L);
+      load_face(p, col0 + sxy * (ez0 + 1), sx, L, ft);
+      if (kPrefetch && ez0 + 2 <= p.nz) load_face_lids(p, col0 + sxy * (ez0 + 2), sx, Ln);
+    }
+    for (int ez = ez0; ez <= ez1 + 1; ++ez) {
+      Hex8Out<Real, TRANS> o;
+      if (ez <= ez1) {
+        if (evalid) {
+          if (kPrefetch) {
+            // prefetch the face above the top one (layer ez+2) and the LIDs beyond it
+            if (ez + 2 <= p.nz) {
+              load_face(p, col0 + sxy * (ez + 2), sx, Ln, fn);
+              if (ez + 3 <= p.nz) load_face_lids(p, col0 + sxy * (ez + 3), sx, Ln);
+            }
+          } else if (ez > ez0) {
+            // no register room for a prefetch (FP64): load the top face now
+            int32_t L[4];
+            load_face_lids(p, col0 + sxy * (ez + 1), sx, L);
+            load_face(p, col0 + sxy * (ez + 1), sx, L, ft);
+          }
+          double X[3][8], u[8], ud[8];
+#pragma unroll
+          for (int v = 0; v < 4; ++v) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) { X[d][v] = fb.x[d][v]; X[d][v + 4] = ft.x[d][v]; }
+            u[v] = fb.u[v]; u[v + 4] = ft.u[v];
+            ud[v] = fb.ud[v]; ud[v + 4] = ft.ud[v];
+          }
+          const double k = p.coef[ex + static_cast<int64_t>(p.nx) * (ey + static_cast<int64_t>(p.ny) * ez)];
+          hex8_element<Real, TRANS>(X, u, ud, k, p.rhoc, p.src, p.alpha, p.beta, o);
+          fb = ft;
+          if (kPrefetch) ft = fn;
+        } else {
+#pragma unroll
+          for (int i = 0; i < 36; ++i) o.K[i] = Real(0);
+#pragma unroll
+          for (int i = 0; i < 8; ++i) o.R[i] = Real(0);
+        }
+      }
+      __syncthreads();   // the owners are done with the stage
+      if (ez <= ez1) {
+#pragma unroll
+        for (int i = 0; i < 36; ++i) stage[i * NE + t] = o.K[i];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) stage[(36 + i) * NE + t] = o.R[i];
+      }
+      __syncthreads();   // layer ez is in the stage
+    }
+  } else {
+    // ---------------- owner waves: vertex column (oi, oj) is corner (1,1) of element
+    // column (tx, ty), corner (0,1) of (tx+1, ty), (1,0) of (tx, ty+1), (0,0) of (tx+1, ty+1)
+    const int oi = x0 + 1 + tx, oj = y0 + 1 + ty;
+    const bool owner = tx < TX - 1 && ty < TY - 1 && oi <= p.nx && oj <= p.ny;
+    const int64_t ocol = owner ? oi + sx * oj : 0;
+    double acc[27], accn[18];
+    double rc = 0.0, rn = 0.0;
+#pragma unroll
+    for (int s = 0; s < 27; ++s) acc[s] = 0.0;
+#pragma unroll
+    for (int s = 0; s < 18; ++s) accn[s] = 0.0;
+    RowMeta meta;        // rows of vertex layer e, consumed at step e
+    int32_t Lq = -1;     // LID of the owned vertex in layer e+1
+    if (owner) {
+      const int32_t L0 = p.lid[ocol + sxy * ez0];
+      meta = load_meta(p, ez0 >= k0 ? L0 : -1);
+      Lq = p.lid[ocol + sxy * (ez0 + 1)];
+    }
+    for (int ez = ez0; ez <= ez1 + 1; ++ez) {
+      if (ez > ez0) {
+        // sum element layer e = ez-1 from the stage
+        const int e = ez - 1;
+        RowMeta next;
+        if (owner) {
+          next = load_meta(p, e + 1 >= k0 ? Lq : -1);
+          Lq = e + 2 <= p.nz ? p.lid[ocol + sxy * (e + 2)] : -1;
+          take_corner<1, 1>(stage, t, acc, accn, rc, rn);
+          take_corner<0, 1>(stage, t + 1, acc, accn, rc, rn);
+          take_corner<1, 0>(stage, t + TX, acc, accn, rc, rn);
+          take_corner<0, 0>(stage, t + TX + 1, acc, accn, rc, rn);
+        }
+        if (e >= k0) emit_rows(p, meta, acc, rc, rowbuf);
+        meta = next;
+#pragma unroll
+        for (int s = 0; s < 18; ++s) { acc[s] = accn[s]; accn[s] = 0.0; }
+#pragma unroll
+        for (int s = 18; s < 27; ++s) acc[s] = 0.0;
+        rc = rn;
+        rn = 0.0;
+      }
+      __syncthreads();   // done with the stage
+      __syncthreads();   // next layer is in the stage
+    }
+    // the top vertex layer of the mesh has no element layer above it
+    if (k1 == p.nz + 1) emit_rows(p, meta, acc, rc, rowbuf);
+  }
+}
+
+}  // namespace
+
+void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s) {
+  Q1DiffParams p;
+  p.nx = m.nx; p.ny = m.ny; p.nz = m.nz;
+  p.X = m.xyz.p; p.Y = m.xyz.p + m.nverts; p.Z = m.xyz.p + 2 * m.nverts;
+  p.lid = m.point_lid.p;
+  p.coef = a.coef; p.x = a.x; p.xdot = a.xdot;
+  p.alpha = a.alpha; p.beta = a.beta; p.rhoc = a.rho; p.src = a.src[0];
+  p.f = a.f; p.vals = a.vals;
+  p.rowptr = m.rowptr.p; p.perm = m.perm.p;
+  // z slabs: enough workgroups to cover the 256 CUs several times over
+  const int gx = (m.nx + 1 + TX - 2) / (TX - 1), gy = (m.ny + 1 + TY - 2) / (TY - 1);
+  static_assert(NT == 512, "block shape");
+  int nchunk = (2048 + gx * gy - 1) / (gx * gy);
+  nchunk = std::max(1, std::min(nchunk, (m.nz + 1 + 7) / 8));
+  p.zc = (m.nz + 1 + nchunk - 1) / nchunk;
+  nchunk = (m.nz + 1 + p.zc - 1) / p.zc;
+  dim3 grid(gx, gy, nchunk);
+  const bool trans = a.transient != 0;
+  if (a.precision == TXA_FP64) {
+    if (trans) hipLaunchKernelGGL((k_q1_hex8_diffusion<double, true>), grid, dim3(NT), 0, s, p);
+    else hipLaunchKernelGGL((k_q1_hex8_diffusion<double, false>), grid, dim3(NT), 0, s, p);
+  } else {
+    if (trans) hipLaunchKernelGGL((k_q1_hex8_diffusion<float, true>), grid, dim3(NT), 0, s, p);
+    else hipLaunchKernelGGL((k_q1_hex8_diffusion<float, false>), grid, dim3(NT), 0, s, p);
+  }
+  TXA_HIP(hipGetLastError());
+}
+
+}  // namespace txa
