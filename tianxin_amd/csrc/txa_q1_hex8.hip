@@ -37,8 +37,14 @@ constexpr double kP = 0.5 * (1.0 + kG);         // P(bit == qbit)
 constexpr double kM = 0.5 * (1.0 - kG);         // P(bit != qbit)
 constexpr int NSTAGE = 44;                      // 36 packed K + 8 R per element
 
+#ifndef TXA_Q1_FP32_RING
+#define TXA_Q1_FP32_RING 1    // FP32 integration uses the one-role LDS-ring kernel
+#endif
+#ifndef TXA_Q1_TARGET_BLOCKS
+#define TXA_Q1_TARGET_BLOCKS 2048   // z slabs are sized to give about this many workgroups
+#endif
 #ifndef TXA_Q1_NT_STORES
-#define TXA_Q1_NT_STORES 0    // non-temporal stores for the CRS values
+#define TXA_Q1_NT_STORES 1    // non-temporal stores for the CRS values (written once, not re-read)
 #endif
 
 struct Q1DiffParams {
@@ -65,6 +71,15 @@ __device__ __forceinline__ constexpr int cls(int a, int b) { return a == b ? a :
 __device__ __forceinline__ constexpr int cls_lo(int c) { return c == 1 ? 1 : 0; }
 __device__ __forceinline__ constexpr int cls_hi(int c) { return c == 0 ? 0 : 1; }
 __device__ __forceinline__ constexpr double Wq(int c, int qb) { return Pq(cls_lo(c), qb) * Pq(cls_hi(c), qb); }
+
+// 1/x: hardware estimate refined by two Newton steps (correctly rounded to within an ulp
+// for the normal range; det of a valid element is far from the denormal range).
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+  r = __builtin_fma(__builtin_fma(-x, r, 1.0), r, r);
+  return r;
+}
 
 template <typename Real, bool TRANS>
 struct Hex8Out {
@@ -121,7 +136,7 @@ __device__ __forceinline__ void hex8_element(const double (&X)[3][8], const doub
     const double det = J[0][0] * c[0][0] + J[0][1] * c[0][1] + J[0][2] * c[0][2];
     if (q == 0) sg = det < 0.0 ? -1.0 : 1.0;   // computeCellMeasure: sign of det at the first point
     wm[q] = det * sg;                          // Gauss weights are 1
-    const double cw = 0.25 * k * sg / det;     // k wm / det^2, times the 1/4 of (s/2)(s/2)
+    const double cw = 0.25 * k * sg * recip(det);   // k wm / det^2, times the 1/4 of (s/2)(s/2)
     Real cr[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -315,7 +330,8 @@ __device__ __forceinline__ void scatter_row(const Q1DiffParams &p, int32_t L, in
   }
 }
 
-constexpr int ROWBUF = 27 * 32;   // doubles per 32-lane group: 31 rows of at most 27 entries
+constexpr int ROWBUF = 27 * 32;   // doubles per 32-lane group: 31 rows of at most 27 entries (837),
+constexpr int ROWBUF_SPARE = 27 * 31;   // then a spare slot for the entries a row does not have
 
 // Orders LDS writes and reads between the lanes of one wavefront.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -351,9 +367,8 @@ __device__ __forceinline__ RowMeta load_meta(const Q1DiffParams &p, int32_t L) {
 }
 
 __device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &m, const double (&acc)[27], double r,
-                                          double *rowbuf) {
+                                          double *rowbuf, int grp) {
   const int lane = threadIdx.x & 63, gl = lane & 31, half = lane >> 5;
-  const int grp = (threadIdx.x - NE) >> 5;
   const bool act = m.L >= 0;
   const int32_t L = m.L;
   const int64_t base = m.base, end = m.end;
@@ -372,10 +387,11 @@ __device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &
   double *buf = rowbuf + grp * ROWBUF;
   if (contiguous && act) {
     const int64_t off = base - base0;
+    // branch-free: absent stencil positions go to a spare slot past the group's rows
 #pragma unroll
     for (int s = 0; s < 27; ++s) {
       const uint32_t o = (m.w[s >> 2] >> (8 * (s & 3))) & 0xffu;
-      if (o != 0xffu) buf[off + o] = acc[s];
+      buf[o == 0xffu ? ROWBUF_SPARE : off + o] = acc[s];
     }
   }
   wave_lds_sync();   // the group's lanes live in one wavefront
@@ -544,7 +560,7 @@ __global__ __launch_bounds__(NT) void k_q1_hex8_diffusion(Q1DiffParams p) {
           take_corner<1, 0>(stage, t + TX, acc, accn, rc, rn);
           take_corner<0, 0>(stage, t + TX + 1, acc, accn, rc, rn);
         }
-        if (e >= k0) emit_rows(p, meta, acc, rc, rowbuf);
+        if (e >= k0) emit_rows(p, meta, acc, rc, rowbuf, t >> 5);
         meta = next;
 #pragma unroll
         for (int s = 0; s < 18; ++s) { acc[s] = accn[s]; accn[s] = 0.0; }
@@ -557,8 +573,146 @@ __global__ __launch_bounds__(NT) void k_q1_hex8_diffusion(Q1DiffParams p) {
       __syncthreads();   // next layer is in the stage
     }
     // the top vertex layer of the mesh has no element layer above it
-    if (k1 == p.nz + 1) emit_rows(p, meta, acc, rc, rowbuf);
+    if (k1 == p.nz + 1) emit_rows(p, meta, acc, rc, rowbuf, t >> 5);
   }
+}
+
+
+// One-role variant: every thread computes an element and owns a vertex column.
+// The vertex data of the tile's faces (33 x 9 columns) is staged in an LDS ring
+// of two z faces; the face two layers up is loaded into registers at the start
+// of a layer and stored into the ring after the element work, so every global
+// load of a layer is issued before that layer's row stores and no wait on a
+// load also waits for fresh stores.
+template <typename Real, bool TRANS>
+__global__ __launch_bounds__(NE) void k_q1_hex8_ring(Q1DiffParams p) {
+  constexpr int FW = TX + 1, FC = FW * (TY + 1);   // vertex columns of the tile
+  constexpr int NVAL = TRANS ? 5 : 4;              // x, y, z, u (, udot)
+  constexpr int kStageD = (NSTAGE * NE * static_cast<int>(sizeof(Real)) + 7) / 8;
+  constexpr int kUnionD = kStageD > 8 * ROWBUF ? kStageD : 8 * ROWBUF;
+  __shared__ double lds[kUnionD];                  // element stage, then the row buffers
+  __shared__ double face[2][NVAL][FC];
+  Real *stage = reinterpret_cast<Real *>(lds);
+  const int t = threadIdx.x, tx = t % TX, ty = t / TX;
+  const int x0 = -1 + static_cast<int>(blockIdx.x) * (TX - 1);
+  const int y0 = -1 + static_cast<int>(blockIdx.y) * (TY - 1);
+  const int64_t sx = p.nx + 1, sxy = static_cast<int64_t>(p.nx + 1) * (p.ny + 1);
+  const int k0 = static_cast<int>(blockIdx.z) * p.zc;
+  const int k1 = min(k0 + p.zc, p.nz + 1);
+  const int ez0 = max(k0 - 1, 0), ez1 = min(k1 - 1, p.nz - 1);
+
+  const int ex = x0 + tx, ey = y0 + ty;
+  const bool evalid = ex >= 0 && ex < p.nx && ey >= 0 && ey < p.ny;
+  const int oi = ex + 1, oj = ey + 1;
+  const bool owner = tx < TX - 1 && ty < TY - 1 && oi <= p.nx && oj <= p.ny;
+  const int64_t ocol = owner ? oi + sx * oj : 0;
+
+  // face columns loaded by this thread: c = t and c = t + NE
+  int64_t vcol[2];
+  bool vval[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int c = t + r * NE;
+    const int vx = x0 + c % FW, vy = y0 + c / FW;
+    vval[r] = c < FC && vx >= 0 && vx <= p.nx && vy >= 0 && vy <= p.ny;
+    vcol[r] = vval[r] ? vx + sx * vy : 0;
+  }
+  auto load_col = [&](int r, int z, int32_t L, double (&v)[NVAL]) {
+    const int64_t n = vcol[r] + sxy * z;
+    v[0] = p.X[n]; v[1] = p.Y[n]; v[2] = p.Z[n];
+    v[3] = p.x[L];
+    if (TRANS) v[NVAL - 1] = p.xdot[L];
+  };
+  auto store_col = [&](int r, int slot, const double (&v)[NVAL]) {
+    const int c = t + r * NE;
+    if (c < FC)
+#pragma unroll
+      for (int i = 0; i < NVAL; ++i) face[slot][i][c] = vval[r] ? v[i] : 0.0;
+  };
+  int32_t lq[2] = {0, 0};   // LIDs of the face two layers up
+#pragma unroll
+  for (int z = 0; z < 2; ++z)
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      double v[NVAL] = {};
+      if (vval[r]) load_col(r, ez0 + z, p.lid[vcol[r] + sxy * (ez0 + z)], v);
+      store_col(r, (ez0 + z) & 1, v);
+    }
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+    if (vval[r] && ez0 + 2 <= p.nz) lq[r] = p.lid[vcol[r] + sxy * (ez0 + 2)];
+
+  double acc[27], accn[18];
+  double rc = 0.0, rn = 0.0;
+#pragma unroll
+  for (int s = 0; s < 27; ++s) acc[s] = 0.0;
+#pragma unroll
+  for (int s = 0; s < 18; ++s) accn[s] = 0.0;
+  int32_t Lcur = owner ? p.lid[ocol + sxy * ez0] : -1;
+  __syncthreads();
+
+  for (int ez = ez0; ez <= ez1; ++ez) {
+    // loads of this layer, all before its stores: the face two layers up, the LIDs
+    // beyond it, the owner's row pointers and slot map, the owner's next LID
+    const bool more = ez + 2 <= p.nz;
+    double fv[2][NVAL];
+    if (more) {
+#pragma unroll
+      for (int r = 0; r < 2; ++r)
+        if (vval[r]) {
+          load_col(r, ez + 2, lq[r], fv[r]);
+          if (ez + 3 <= p.nz) lq[r] = p.lid[vcol[r] + sxy * (ez + 3)];
+        }
+    }
+    const RowMeta meta = load_meta(p, ez >= k0 ? Lcur : -1);
+    const int32_t Lnext = owner ? p.lid[ocol + sxy * (ez + 1)] : -1;
+
+    Hex8Out<Real, TRANS> o;
+    if (evalid) {
+      double X[3][8], u[8], ud[8];
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const int c = (tx + (v & 1)) + FW * (ty + ((v >> 1) & 1));
+        const int slot = (ez + (v >> 2)) & 1;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) X[d][v] = face[slot][d][c];
+        u[v] = face[slot][3][c];
+        ud[v] = TRANS ? face[slot][NVAL - 1][c] : 0.0;
+      }
+      const double k = p.coef[ex + static_cast<int64_t>(p.nx) * (ey + static_cast<int64_t>(p.ny) * ez)];
+      hex8_element<Real, TRANS>(X, u, ud, k, p.rhoc, p.src, p.alpha, p.beta, o);
+    } else {
+#pragma unroll
+      for (int i = 0; i < 36; ++i) o.K[i] = Real(0);
+#pragma unroll
+      for (int i = 0; i < 8; ++i) o.R[i] = Real(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 36; ++i) stage[i * NE + t] = o.K[i];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) stage[(36 + i) * NE + t] = o.R[i];
+    __syncthreads();
+    if (owner) {
+      take_corner<1, 1>(stage, t, acc, accn, rc, rn);
+      take_corner<0, 1>(stage, t + 1, acc, accn, rc, rn);
+      take_corner<1, 0>(stage, t + TX, acc, accn, rc, rn);
+      take_corner<0, 0>(stage, t + TX + 1, acc, accn, rc, rn);
+    }
+    __syncthreads();   // the stage is free: row buffers and the next face may be written
+    if (more)
+#pragma unroll
+      for (int r = 0; r < 2; ++r) store_col(r, ez & 1, fv[r]);
+    if (ez >= k0) emit_rows(p, meta, acc, rc, lds, t >> 5);
+#pragma unroll
+    for (int s = 0; s < 18; ++s) { acc[s] = accn[s]; accn[s] = 0.0; }
+#pragma unroll
+    for (int s = 18; s < 27; ++s) acc[s] = 0.0;
+    rc = rn;
+    rn = 0.0;
+    Lcur = Lnext;
+    __syncthreads();   // row buffers copied out, next face visible
+  }
+  if (k1 == p.nz + 1) emit_rows(p, load_meta(p, Lcur), acc, rc, lds, t >> 5);
 }
 
 }  // namespace
@@ -575,18 +729,25 @@ void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s
   // z slabs: enough workgroups to cover the 256 CUs several times over
   const int gx = (m.nx + 1 + TX - 2) / (TX - 1), gy = (m.ny + 1 + TY - 2) / (TY - 1);
   static_assert(NT == 512, "block shape");
-  int nchunk = (2048 + gx * gy - 1) / (gx * gy);
+  int nchunk = (TXA_Q1_TARGET_BLOCKS + gx * gy - 1) / (gx * gy);
   nchunk = std::max(1, std::min(nchunk, (m.nz + 1 + 7) / 8));
   p.zc = (m.nz + 1 + nchunk - 1) / nchunk;
   nchunk = (m.nz + 1 + p.zc - 1) / p.zc;
   dim3 grid(gx, gy, nchunk);
   const bool trans = a.transient != 0;
+  // FP32 integration: one-role kernel with the LDS face ring (2 workgroups per CU);
+  // FP64: element and owner waves split (the FP64 element state leaves no room for the ring)
   if (a.precision == TXA_FP64) {
     if (trans) hipLaunchKernelGGL((k_q1_hex8_diffusion<double, true>), grid, dim3(NT), 0, s, p);
     else hipLaunchKernelGGL((k_q1_hex8_diffusion<double, false>), grid, dim3(NT), 0, s, p);
   } else {
+#if TXA_Q1_FP32_RING
+    if (trans) hipLaunchKernelGGL((k_q1_hex8_ring<float, true>), grid, dim3(NE), 0, s, p);
+    else hipLaunchKernelGGL((k_q1_hex8_ring<float, false>), grid, dim3(NE), 0, s, p);
+#else
     if (trans) hipLaunchKernelGGL((k_q1_hex8_diffusion<float, true>), grid, dim3(NT), 0, s, p);
     else hipLaunchKernelGGL((k_q1_hex8_diffusion<float, false>), grid, dim3(NT), 0, s, p);
+#endif
   }
   TXA_HIP(hipGetLastError());
 }
