@@ -167,3 +167,20 @@ def test_panzer_shim_driver():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr)
     assert r.returncode == 0 and "TEST PASSED" in r.stdout
+
+
+@pytest.mark.parametrize("precision,tol", [(txa.FP64, TOL64), (txa.FP32, TOL32)])
+def test_tet4_fast_path(engine, precision, tol):
+    c = Case(engine, txa.TET4, 13, 9, 11, 1, jitter=0.1, seed=4)
+    f, v, fr, vr = run_both(engine, c, txa.DIFFUSION, precision=precision)
+    assert rel_err(f, fr) <= tol
+    assert rel_err(v, vr) <= tol
+    c.m.destroy()
+
+
+def test_tet4_transient_dirichlet_fp64(engine):
+    c = Case(engine, txa.TET4, 40, 17, 9, 1, jitter=0.1, seed=6)
+    f, v, fr, vr = run_both(engine, c, txa.DIFFUSION, transient=True, bc=True, alpha=2.5, rho=1.7)
+    assert rel_err(f, fr) <= TOL64
+    assert rel_err(v, vr) <= TOL64
+    c.m.destroy()

@@ -1,5 +1,5 @@
-// Hex8 (Q1) scalar diffusion on the structured brick: the row-owner,
-// z-marching assembly kernel.
+// Q1 scalar diffusion on the structured brick (hex8 cells, or the 6-tet split of
+// each cell): the row-owner, z-marching assembly kernels.
 //
 // Replaces Gather -> DOF/DOFGradient -> Integrator_GradBasisDotVector (k grad u),
 // Integrator_BasisTimesScalar (source, transient rho_c udot) -> ScatterResidual_Tpetra
@@ -321,6 +321,96 @@ __device__ __forceinline__ void hex8_element(const double (&X)[3][8], const doub
     }
 }
 
+
+// The six tets of a cell (CartesianConnManager's 6-tet split,
+// dof-mgr/test/cartesian_topology/CartesianConnManager.cpp:228-235), vertices
+// renumbered from shards hex order to the binary order v = bx + 2by + 4bz.
+__device__ __forceinline__ constexpr int tet_v(int t, int j) {
+  constexpr int T[6][4] = {{0, 3, 2, 7}, {0, 2, 6, 7}, {0, 6, 4, 7}, {0, 5, 7, 4}, {1, 5, 7, 0}, {1, 7, 3, 0}};
+  return T[t][j];
+}
+
+// The six linear tets of one cell, assembled into the cell's 8x8 matrix (pairs
+// not joined by a tet edge stay zero).  Constant gradients: grad N_j = cof[:,j]/det
+// (j = 1..3), grad N_0 = -(sum); measure w |det| with w = 1/6; the degree-2
+// rule integrates the source (|T|/4 per vertex) and mass (|T|/20 (1 + delta)) exactly.
+// k: the six tet coefficients, in element order 6*cell + t.
+template <typename Real, bool TRANS>
+__device__ __forceinline__ void tet6_cell(const double (&X)[3][8], const double (&u)[8], const double (&ud)[8],
+                                          const double (&k)[6], double rhoc, double src, double alpha, double beta,
+                                          Hex8Out<Real, TRANS> &o) {
+  Real K[36], M[TRANS ? 36 : 1];
+  double F[8];
+#pragma unroll
+  for (int i = 0; i < 36; ++i) { K[i] = Real(0); if (TRANS) M[i] = Real(0); }
+#pragma unroll
+  for (int v = 0; v < 8; ++v) F[v] = 0.0;
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    double J[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) J[d][j] = X[d][tet_v(t, j + 1)] - X[d][tet_v(t, 0)];
+    double c[3][3];
+    c[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    c[0][1] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    c[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    c[1][0] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+    c[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+    c[1][2] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+    c[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    c[2][1] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+    c[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const double det = J[0][0] * c[0][0] + J[0][1] * c[0][1] + J[0][2] * c[0][2];
+    const double sg = det < 0.0 ? -1.0 : 1.0;
+    const double vol = det * sg / 6.0;                 // sum of the weighted measure
+    const Real cw = Real(k[t] * sg * recip(det) / 6.0);
+    Real g[4][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      g[1][i] = Real(c[i][0]); g[2][i] = Real(c[i][1]); g[3][i] = Real(c[i][2]);
+      g[0][i] = -(g[1][i] + g[2][i] + g[3][i]);
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      F[tet_v(t, a)] += src * vol * 0.25;
+#pragma unroll
+      for (int b = a; b < 4; ++b) {
+        const Real gg = g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2];
+        K[sym8(tet_v(t, a), tet_v(t, b))] += cw * gg;
+        if (TRANS) M[sym8(tet_v(t, a), tet_v(t, b))] += Real(rhoc * vol * (a == b ? 0.1 : 0.05));
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    Real r = Real(0);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      r += K[sym8(a, b)] * Real(u[b]);
+      if (TRANS) r += M[sym8(a, b)] * Real(ud[b]);
+    }
+    o.R[a] = Real(double(r) - F[a]);
+  }
+#pragma unroll
+  for (int i = 0; i < 36; ++i) o.K[i] = TRANS ? Real(beta) * K[i] + Real(alpha) * M[i] : Real(beta) * K[i];
+}
+
+// Cell routine of the element kind: one hex8 element, or the six tets of the cell.
+template <int KIND, typename Real, bool TRANS>
+__device__ __forceinline__ void cell_element(const Q1DiffParams &p, int64_t cell, const double (&X)[3][8],
+                                             const double (&u)[8], const double (&ud)[8], Hex8Out<Real, TRANS> &o) {
+  if (KIND == TXA_TET4) {
+    double k[6];
+#pragma unroll
+    for (int t = 0; t < 6; ++t) k[t] = p.coef[6 * cell + t];
+    tet6_cell<Real, TRANS>(X, u, ud, k, p.rhoc, p.src, p.alpha, p.beta, o);
+  } else {
+    hex8_element<Real, TRANS>(X, u, ud, p.coef[cell], p.rhoc, p.src, p.alpha, p.beta, o);
+  }
+}
+
 __device__ __forceinline__ void scatter_row(const Q1DiffParams &p, int32_t L, int64_t base, const uint32_t (&w)[8],
                                             const double (&acc)[27]) {
 #pragma unroll
@@ -452,7 +542,7 @@ __device__ __forceinline__ void load_face_lids(const Q1DiffParams &p, int64_t n0
 // from the stage and store the finished rows; then the element waves publish
 // layer ez in the stage.  Loads (element waves) and stores (owner waves) sit in
 // different waves, so neither waits for the other's memory counter.
-template <typename Real, bool TRANS>
+template <int KIND, typename Real, bool TRANS>
 __global__ __launch_bounds__(NT) void k_q1_hex8_diffusion(Q1DiffParams p) {
   __shared__ Real stage[NSTAGE * NE];
   __shared__ double rowbuf[8 * ROWBUF];
@@ -508,8 +598,8 @@ __global__ __launch_bounds__(NT) void k_q1_hex8_diffusion(Q1DiffParams p) {
             u[v] = fb.u[v]; u[v + 4] = ft.u[v];
             ud[v] = fb.ud[v]; ud[v + 4] = ft.ud[v];
           }
-          const double k = p.coef[ex + static_cast<int64_t>(p.nx) * (ey + static_cast<int64_t>(p.ny) * ez)];
-          hex8_element<Real, TRANS>(X, u, ud, k, p.rhoc, p.src, p.alpha, p.beta, o);
+          cell_element<KIND, Real, TRANS>(p, ex + static_cast<int64_t>(p.nx) * (ey + static_cast<int64_t>(p.ny) * ez),
+                                          X, u, ud, o);
           fb = ft;
           if (kPrefetch) ft = fn;
         } else {
@@ -584,7 +674,7 @@ __global__ __launch_bounds__(NT) void k_q1_hex8_diffusion(Q1DiffParams p) {
 // of a layer and stored into the ring after the element work, so every global
 // load of a layer is issued before that layer's row stores and no wait on a
 // load also waits for fresh stores.
-template <typename Real, bool TRANS>
+template <int KIND, typename Real, bool TRANS>
 __global__ __launch_bounds__(NE) void k_q1_hex8_ring(Q1DiffParams p) {
   constexpr int FW = TX + 1, FC = FW * (TY + 1);   // vertex columns of the tile
   constexpr int NVAL = TRANS ? 5 : 4;              // x, y, z, u (, udot)
@@ -679,8 +769,8 @@ __global__ __launch_bounds__(NE) void k_q1_hex8_ring(Q1DiffParams p) {
         u[v] = face[slot][3][c];
         ud[v] = TRANS ? face[slot][NVAL - 1][c] : 0.0;
       }
-      const double k = p.coef[ex + static_cast<int64_t>(p.nx) * (ey + static_cast<int64_t>(p.ny) * ez)];
-      hex8_element<Real, TRANS>(X, u, ud, k, p.rhoc, p.src, p.alpha, p.beta, o);
+      cell_element<KIND, Real, TRANS>(p, ex + static_cast<int64_t>(p.nx) * (ey + static_cast<int64_t>(p.ny) * ez), X,
+                                      u, ud, o);
     } else {
 #pragma unroll
       for (int i = 0; i < 36; ++i) o.K[i] = Real(0);
@@ -715,6 +805,25 @@ __global__ __launch_bounds__(NE) void k_q1_hex8_ring(Q1DiffParams p) {
   if (k1 == p.nz + 1) emit_rows(p, load_meta(p, Lcur), acc, rc, lds, t >> 5);
 }
 
+
+// FP32 integration: one-role kernel with the LDS face ring (2 workgroups per CU);
+// FP64: element and owner waves split (the FP64 element state leaves no room for the ring)
+template <int KIND>
+void launch_kind(const Q1DiffParams &p, dim3 grid, int precision, bool trans, hipStream_t s) {
+  if (precision == TXA_FP64) {
+    if (trans) hipLaunchKernelGGL((k_q1_hex8_diffusion<KIND, double, true>), grid, dim3(NT), 0, s, p);
+    else hipLaunchKernelGGL((k_q1_hex8_diffusion<KIND, double, false>), grid, dim3(NT), 0, s, p);
+  } else {
+#if TXA_Q1_FP32_RING
+    if (trans) hipLaunchKernelGGL((k_q1_hex8_ring<KIND, float, true>), grid, dim3(NE), 0, s, p);
+    else hipLaunchKernelGGL((k_q1_hex8_ring<KIND, float, false>), grid, dim3(NE), 0, s, p);
+#else
+    if (trans) hipLaunchKernelGGL((k_q1_hex8_diffusion<KIND, float, true>), grid, dim3(NT), 0, s, p);
+    else hipLaunchKernelGGL((k_q1_hex8_diffusion<KIND, float, false>), grid, dim3(NT), 0, s, p);
+#endif
+  }
+}
+
 }  // namespace
 
 void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s) {
@@ -735,20 +844,8 @@ void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s
   nchunk = (m.nz + 1 + p.zc - 1) / p.zc;
   dim3 grid(gx, gy, nchunk);
   const bool trans = a.transient != 0;
-  // FP32 integration: one-role kernel with the LDS face ring (2 workgroups per CU);
-  // FP64: element and owner waves split (the FP64 element state leaves no room for the ring)
-  if (a.precision == TXA_FP64) {
-    if (trans) hipLaunchKernelGGL((k_q1_hex8_diffusion<double, true>), grid, dim3(NT), 0, s, p);
-    else hipLaunchKernelGGL((k_q1_hex8_diffusion<double, false>), grid, dim3(NT), 0, s, p);
-  } else {
-#if TXA_Q1_FP32_RING
-    if (trans) hipLaunchKernelGGL((k_q1_hex8_ring<float, true>), grid, dim3(NE), 0, s, p);
-    else hipLaunchKernelGGL((k_q1_hex8_ring<float, false>), grid, dim3(NE), 0, s, p);
-#else
-    if (trans) hipLaunchKernelGGL((k_q1_hex8_diffusion<float, true>), grid, dim3(NT), 0, s, p);
-    else hipLaunchKernelGGL((k_q1_hex8_diffusion<float, false>), grid, dim3(NT), 0, s, p);
-#endif
-  }
+  if (m.topo == TXA_TET4) launch_kind<TXA_TET4>(p, grid, a.precision, trans, s);
+  else launch_kind<TXA_HEX8>(p, grid, a.precision, trans, s);
   TXA_HIP(hipGetLastError());
 }
 
