@@ -133,6 +133,34 @@ typedef struct {
 
 int txa_assemble(txa_ctx ctx, int mesh_id, const txa_assembly_args *args, void *stream);
 
+/* Batched assembly of many resident meshes (HEX8 and TET4, Q1 diffusion) in one
+ * persistent launch: every mesh contributes its tiles to one work list that the
+ * launch drains, then one Dirichlet launch covers all meshes.  With use_graph
+ * the two launches (and the upload of the per-mesh argument blocks) are
+ * captured once into a HIP graph and replayed.  Replaces the element-block x
+ * workset double loop of AssemblyEngine::evaluateVolume
+ * (disc-fe/src/Panzer_AssemblyEngine_impl.hpp:134-180): one batch = every block
+ * of every model, one launch instead of one evaluateFields per workset.
+ * Meshes of a batch need a scalar DOF map and a graph and cannot be destroyed
+ * while the batch exists.  All items share `precision` (TXA_FP64 or TXA_FP32)
+ * and the transient flag. */
+int txa_batch_create(txa_ctx ctx, const int *mesh_ids, int n, int precision, int transient, int *batch_id);
+int txa_batch_destroy(txa_ctx ctx, int batch_id);
+/* args: host array [n], one per mesh in creation order (device pointers inside,
+ * as for txa_assemble).  Host-blocking only while a previous launch of the
+ * same batch still reads its argument blocks. */
+int txa_batch_assemble(txa_ctx ctx, int batch_id, const txa_assembly_args *args, int use_graph, void *stream);
+/* Autotuner: times `reps` launches for each slab height (vertex layers per
+ * work item: 8, 16, 32, 64, whole column) and persistent grid (1, 2, 4
+ * workgroups per CU), keeps the fastest and reports its time per launch in
+ * best_ms.  The counterpart of the workset-size sweep in
+ * adapters-stk/example/MixedPoissonExample/analyze_performance.py:57-61; it
+ * writes f / vals as an assembly does. */
+int txa_batch_tune(txa_ctx ctx, int batch_id, const txa_assembly_args *args, int reps, void *stream,
+                   double *best_ms);
+/* out[0]=meshes out[1]=work items out[2]=slab height (0 = whole) out[3]=persistent workgroups */
+int txa_batch_info(txa_ctx ctx, int batch_id, int64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,6 +81,11 @@ def load_library(path: str = LIB_PATH):
     L.txa_graph_copy.argtypes = [P, ctypes.c_int, P, P]
     L.txa_graph_device.argtypes = [P, ctypes.c_int, ctypes.POINTER(P), ctypes.POINTER(P)]
     L.txa_assemble.argtypes = [P, ctypes.c_int, ctypes.POINTER(AssemblyArgs), P]
+    L.txa_batch_create.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.txa_batch_destroy.argtypes = [P, ctypes.c_int]
+    L.txa_batch_assemble.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P]
+    L.txa_batch_tune.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P, ctypes.POINTER(ctypes.c_double)]
+    L.txa_batch_info.argtypes = [P, ctypes.c_int, P]
     _lib = L
     return L
 
@@ -98,6 +103,33 @@ def _dptr(t):
     if not t.is_contiguous():
         raise ValueError("expected a contiguous tensor")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream().cuda_stream
+    return ctypes.c_void_p(stream)
+
+
+def make_args(*, physics, coef, x, f=None, vals=None, precision=FP64, xdot=None, xddot=None,
+              alpha=0.0, beta=1.0, gamma2=0.0, transient=False, nu=0.3, rho=1.0, src=(0.0, 0.0, 0.0),
+              rayleigh=(0.0, 0.0), bc_lids=None, bc_vals=None):
+    """txa_assembly_args from device tensors (see include/txa.h)."""
+    a = AssemblyArgs()
+    a.physics, a.precision = physics, precision
+    a.coef = _dptr(coef)
+    a.nu, a.rho = nu, rho
+    s = list(src) + [0.0] * 3
+    a.src = (ctypes.c_double * 3)(*s[:3])
+    a.rayleigh_a, a.rayleigh_b = rayleigh
+    a.x, a.xdot, a.xddot = _dptr(x), _dptr(xdot), _dptr(xddot)
+    a.alpha, a.beta, a.gamma2 = alpha, beta, gamma2
+    a.transient = int(transient)
+    a.f, a.vals = _dptr(f), _dptr(vals)
+    a.bc_lids, a.bc_vals = _dptr(bc_lids), _dptr(bc_vals)
+    a.n_bc = 0 if bc_lids is None else int(bc_lids.numel())
+    return a
 
 
 class Engine:
@@ -186,29 +218,63 @@ class BrickMesh:
         self.eng.check(self.eng.lib.txa_graph_copy(self.eng.h, self.id, _np(rp), None if col is None else _np(col)))
         return rp, (col[: self.nnz] if with_cols else None)
 
-    def assemble(self, *, physics, coef, x, f=None, vals=None, precision=FP64, xdot=None, xddot=None,
-                 alpha=0.0, beta=1.0, gamma2=0.0, transient=False, nu=0.3, rho=1.0, src=(0.0, 0.0, 0.0),
-                 rayleigh=(0.0, 0.0), bc_lids=None, bc_vals=None, stream=None):
-        """One residual + Jacobian evaluation into device tensors f [ndofs] and vals [nnz]."""
-        a = AssemblyArgs()
-        a.physics, a.precision = physics, precision
-        a.coef = _dptr(coef)
-        a.nu, a.rho = nu, rho
-        s = list(src) + [0.0] * 3
-        a.src = (ctypes.c_double * 3)(*s[:3])
-        a.rayleigh_a, a.rayleigh_b = rayleigh
-        a.x, a.xdot, a.xddot = _dptr(x), _dptr(xdot), _dptr(xddot)
-        a.alpha, a.beta, a.gamma2 = alpha, beta, gamma2
-        a.transient = int(transient)
-        a.f, a.vals = _dptr(f), _dptr(vals)
-        a.bc_lids, a.bc_vals = _dptr(bc_lids), _dptr(bc_vals)
-        a.n_bc = 0 if bc_lids is None else int(bc_lids.numel())
-        if stream is None:
-            import torch
-            stream = torch.cuda.current_stream().cuda_stream
-        self.eng.check(self.eng.lib.txa_assemble(self.eng.h, self.id, ctypes.byref(a), ctypes.c_void_p(stream)))
+    def assemble(self, *, stream=None, **kw):
+        """One residual + Jacobian evaluation into device tensors f [ndofs] and vals [nnz].
+
+        Keywords as make_args."""
+        a = make_args(**kw)
+        self.eng.check(self.eng.lib.txa_assemble(self.eng.h, self.id, ctypes.byref(a), _stream(stream)))
 
     def destroy(self):
         if self.id is not None:
             self.eng.check(self.eng.lib.txa_mesh_destroy(self.eng.h, self.id))
+            self.id = None
+
+
+class Batch:
+    """Many resident meshes assembled by one persistent launch (txa_batch_*)."""
+
+    def __init__(self, eng: Engine, meshes, precision=FP64, transient=False):
+        self.eng = eng
+        self.meshes = list(meshes)
+        ids = np.array([m.id for m in self.meshes], np.int32)
+        bid = ctypes.c_int()
+        eng.check(eng.lib.txa_batch_create(eng.h, _np(ids), len(ids), precision, int(transient), ctypes.byref(bid)))
+        self.id = bid.value
+        self.precision = precision
+        self.transient = transient
+
+    def _pack(self, per_mesh):
+        if len(per_mesh) != len(self.meshes):
+            raise ValueError("one argument set per mesh")
+        arr = (AssemblyArgs * len(per_mesh))()
+        for i, kw in enumerate(per_mesh):
+            kw = dict(kw)
+            kw.setdefault("precision", self.precision)
+            kw.setdefault("transient", self.transient)
+            arr[i] = make_args(**kw)
+        return arr
+
+    def assemble(self, per_mesh, use_graph=True, stream=None):
+        """per_mesh: one make_args keyword dict per mesh, in batch order."""
+        arr = self._pack(per_mesh)
+        self.eng.check(self.eng.lib.txa_batch_assemble(self.eng.h, self.id, ctypes.cast(arr, P), int(use_graph),
+                                                       _stream(stream)))
+
+    def tune(self, per_mesh, reps=3, stream=None):
+        """Autotune slab height and persistent grid; returns the best time per launch (ms)."""
+        arr = self._pack(per_mesh)
+        ms = ctypes.c_double()
+        self.eng.check(self.eng.lib.txa_batch_tune(self.eng.h, self.id, ctypes.cast(arr, P), reps, _stream(stream),
+                                                   ctypes.byref(ms)))
+        return ms.value
+
+    def info(self):
+        out = np.zeros(4, np.int64)
+        self.eng.check(self.eng.lib.txa_batch_info(self.eng.h, self.id, _np(out)))
+        return dict(meshes=int(out[0]), items=int(out[1]), slab=int(out[2]), grid=int(out[3]))
+
+    def destroy(self):
+        if self.id is not None:
+            self.eng.check(self.eng.lib.txa_batch_destroy(self.eng.h, self.id))
             self.id = None

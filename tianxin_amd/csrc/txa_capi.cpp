@@ -98,6 +98,9 @@ int txa_mesh_destroy(txa_ctx ctx, int mesh_id) {
   return guarded(ctx, [&] {
     auto it = ctx->meshes.find(mesh_id);
     TXA_REQUIRE(it != ctx->meshes.end(), TXA_ERR_INVALID, "unknown mesh id");
+    for (auto &kv : ctx->batches)
+      for (int id : txa::batch_mesh_ids(*kv.second))
+        TXA_REQUIRE(id != mesh_id, TXA_ERR_LOGIC, "mesh is part of batch " + std::to_string(kv.first));
     TXA_HIP(hipDeviceSynchronize());
     ctx->meshes.erase(it);
   });
@@ -170,6 +173,52 @@ int txa_assemble(txa_ctx ctx, int mesh_id, const txa_assembly_args *args, void *
     TXA_REQUIRE(args, TXA_ERR_INVALID, "null args");
     Mesh &m = ctx->mesh(mesh_id);
     txa::assemble(m, *args, static_cast<hipStream_t>(stream));
+  });
+}
+
+int txa_batch_create(txa_ctx ctx, const int *mesh_ids, int n, int precision, int transient, int *batch_id) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(mesh_ids && batch_id && n > 0, TXA_ERR_INVALID, "null argument or empty batch");
+    std::vector<Mesh *> ms;
+    std::vector<int> ids(mesh_ids, mesh_ids + n);
+    for (int id : ids) ms.push_back(&ctx->mesh(id));
+    txa::Batch *b = txa::batch_create(ms, ids, precision, transient != 0);
+    const int id = ctx->next_id++;
+    ctx->batches[id] = b;
+    *batch_id = id;
+  });
+}
+
+int txa_batch_destroy(txa_ctx ctx, int batch_id) {
+  return guarded(ctx, [&] {
+    auto it = ctx->batches.find(batch_id);
+    TXA_REQUIRE(it != ctx->batches.end(), TXA_ERR_INVALID, "unknown batch id");
+    TXA_HIP(hipDeviceSynchronize());
+    txa::batch_free(it->second);
+    ctx->batches.erase(it);
+  });
+}
+
+int txa_batch_assemble(txa_ctx ctx, int batch_id, const txa_assembly_args *args, int use_graph, void *stream) {
+  return guarded(ctx, [&] {
+    txa::batch_assemble(ctx->batch(batch_id), args, use_graph != 0, static_cast<hipStream_t>(stream));
+  });
+}
+
+int txa_batch_tune(txa_ctx ctx, int batch_id, const txa_assembly_args *args, int reps, void *stream,
+                   double *best_ms) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(args, TXA_ERR_INVALID, "null args");
+    TXA_REQUIRE(reps >= 1 && reps <= 1000, TXA_ERR_INVALID, "reps must be in [1, 1000]");
+    const double ms = txa::batch_tune(ctx->batch(batch_id), args, static_cast<hipStream_t>(stream), reps);
+    if (best_ms) *best_ms = ms;
+  });
+}
+
+int txa_batch_info(txa_ctx ctx, int batch_id, int64_t *out) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(out, TXA_ERR_INVALID, "null argument");
+    txa::batch_info(ctx->batch(batch_id), out);
   });
 }
 

@@ -8,6 +8,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/txa.h"
 
@@ -87,11 +88,23 @@ struct Mesh {
   DevBuf<double> ref;         // reference-element tables of the topology
 };
 
+struct Batch;
+void batch_free(Batch *b);
+
 struct Context {
   int device = 0;
   std::string last_error;
   std::map<int, std::unique_ptr<Mesh>> meshes;
+  std::map<int, Batch *> batches;
   int next_id = 1;
+  ~Context() {
+    for (auto &kv : batches) batch_free(kv.second);
+  }
+  Batch &batch(int id) {
+    auto it = batches.find(id);
+    TXA_REQUIRE(it != batches.end(), TXA_ERR_INVALID, "unknown batch id " + std::to_string(id));
+    return *it->second;
+  }
   Mesh &mesh(int id) {
     auto it = meshes.find(id);
     TXA_REQUIRE(it != meshes.end(), TXA_ERR_INVALID, "unknown mesh id " + std::to_string(id));
@@ -108,7 +121,12 @@ void exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t 
 // assembly (txa_assemble.hip)
 void upload_refelem(Mesh &m, DevBuf<double> &tab, hipStream_t s);
 void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s);
-// hex8 diffusion fast path (txa_q1_hex8.hip)
+// Q1 diffusion fast path and batches (txa_q1_hex8.hip)
 void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s);
+Batch *batch_create(const std::vector<Mesh *> &meshes, const std::vector<int> &ids, int precision, bool transient);
+void batch_assemble(Batch &b, const txa_assembly_args *args, bool use_graph, hipStream_t s);
+double batch_tune(Batch &b, const txa_assembly_args *args, hipStream_t s, int reps);
+void batch_info(const Batch &b, int64_t *out);
+const std::vector<int> &batch_mesh_ids(const Batch &b);
 
 }  // namespace txa

@@ -26,6 +26,7 @@
 #include "txa_internal.hpp"
 
 #include <algorithm>
+#include <cstring>
 
 namespace txa {
 namespace {
@@ -543,18 +544,17 @@ __device__ __forceinline__ void load_face_lids(const Q1DiffParams &p, int64_t n0
 // layer ez in the stage.  Loads (element waves) and stores (owner waves) sit in
 // different waves, so neither waits for the other's memory counter.
 template <int KIND, typename Real, bool TRANS>
-__global__ __launch_bounds__(NT) void k_q1_hex8_diffusion(Q1DiffParams p) {
-  __shared__ Real stage[NSTAGE * NE];
-  __shared__ double rowbuf[8 * ROWBUF];
+__device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by, int bz, Real *stage,
+                                           double *rowbuf) {
   const bool is_elem = threadIdx.x < NE;
   const int t = is_elem ? threadIdx.x : threadIdx.x - NE;
   const int tx = t % TX, ty = t / TX;
-  const int x0 = -1 + static_cast<int>(blockIdx.x) * (TX - 1);
-  const int y0 = -1 + static_cast<int>(blockIdx.y) * (TY - 1);
+  const int x0 = -1 + bx * (TX - 1);
+  const int y0 = -1 + by * (TY - 1);
   const int64_t sx = p.nx + 1, sxy = static_cast<int64_t>(p.nx + 1) * (p.ny + 1);
   // this workgroup owns vertex layers [k0, k1); element layer k0-1 is recomputed
   // (it also belongs to the slab below) because vertex layer k0 needs it
-  const int k0 = static_cast<int>(blockIdx.z) * p.zc;
+  const int k0 = bz * p.zc;
   const int k1 = min(k0 + p.zc, p.nz + 1);
   const int ez0 = max(k0 - 1, 0), ez1 = min(k1 - 1, p.nz - 1);
 
@@ -674,20 +674,28 @@ __global__ __launch_bounds__(NT) void k_q1_hex8_diffusion(Q1DiffParams p) {
 // of a layer and stored into the ring after the element work, so every global
 // load of a layer is issued before that layer's row stores and no wait on a
 // load also waits for fresh stores.
+constexpr int FW = TX + 1, FC = FW * (TY + 1);     // vertex columns of the tile
+template <bool TRANS>
+constexpr int ring_nval() { return TRANS ? 5 : 4; }  // x, y, z, u (, udot)
+template <typename Real>
+constexpr int ring_union_doubles() {
+  return (NSTAGE * NE * static_cast<int>(sizeof(Real)) + 7) / 8 > 8 * ROWBUF
+             ? (NSTAGE * NE * static_cast<int>(sizeof(Real)) + 7) / 8
+             : 8 * ROWBUF;
+}
+
 template <int KIND, typename Real, bool TRANS>
-__global__ __launch_bounds__(NE) void k_q1_hex8_ring(Q1DiffParams p) {
-  constexpr int FW = TX + 1, FC = FW * (TY + 1);   // vertex columns of the tile
-  constexpr int NVAL = TRANS ? 5 : 4;              // x, y, z, u (, udot)
-  constexpr int kStageD = (NSTAGE * NE * static_cast<int>(sizeof(Real)) + 7) / 8;
-  constexpr int kUnionD = kStageD > 8 * ROWBUF ? kStageD : 8 * ROWBUF;
-  __shared__ double lds[kUnionD];                  // element stage, then the row buffers
-  __shared__ double face[2][NVAL][FC];
+__device__ __forceinline__ void ring_block(const Q1DiffParams &p, int bx, int by, int bz, double *lds,
+                                           double *facebuf) {
+  constexpr int NVAL = ring_nval<TRANS>();
+  // element stage, then the row buffers, in `lds`; faces [2][NVAL][FC] in `facebuf`
   Real *stage = reinterpret_cast<Real *>(lds);
+  auto face = [&](int slot, int i, int c) -> double & { return facebuf[(slot * NVAL + i) * FC + c]; };
   const int t = threadIdx.x, tx = t % TX, ty = t / TX;
-  const int x0 = -1 + static_cast<int>(blockIdx.x) * (TX - 1);
-  const int y0 = -1 + static_cast<int>(blockIdx.y) * (TY - 1);
+  const int x0 = -1 + bx * (TX - 1);
+  const int y0 = -1 + by * (TY - 1);
   const int64_t sx = p.nx + 1, sxy = static_cast<int64_t>(p.nx + 1) * (p.ny + 1);
-  const int k0 = static_cast<int>(blockIdx.z) * p.zc;
+  const int k0 = bz * p.zc;
   const int k1 = min(k0 + p.zc, p.nz + 1);
   const int ez0 = max(k0 - 1, 0), ez1 = min(k1 - 1, p.nz - 1);
 
@@ -717,7 +725,7 @@ __global__ __launch_bounds__(NE) void k_q1_hex8_ring(Q1DiffParams p) {
     const int c = t + r * NE;
     if (c < FC)
 #pragma unroll
-      for (int i = 0; i < NVAL; ++i) face[slot][i][c] = vval[r] ? v[i] : 0.0;
+      for (int i = 0; i < NVAL; ++i) face(slot, i, c) = vval[r] ? v[i] : 0.0;
   };
   int32_t lq[2] = {0, 0};   // LIDs of the face two layers up
 #pragma unroll
@@ -765,9 +773,9 @@ __global__ __launch_bounds__(NE) void k_q1_hex8_ring(Q1DiffParams p) {
         const int c = (tx + (v & 1)) + FW * (ty + ((v >> 1) & 1));
         const int slot = (ez + (v >> 2)) & 1;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) X[d][v] = face[slot][d][c];
-        u[v] = face[slot][3][c];
-        ud[v] = TRANS ? face[slot][NVAL - 1][c] : 0.0;
+        for (int d = 0; d < 3; ++d) X[d][v] = face(slot, d, c);
+        u[v] = face(slot, 3, c);
+        ud[v] = TRANS ? face(slot, NVAL - 1, c) : 0.0;
       }
       cell_element<KIND, Real, TRANS>(p, ex + static_cast<int64_t>(p.nx) * (ey + static_cast<int64_t>(p.ny) * ez), X,
                                       u, ud, o);
@@ -806,6 +814,114 @@ __global__ __launch_bounds__(NE) void k_q1_hex8_ring(Q1DiffParams p) {
 }
 
 
+template <int KIND, typename Real, bool TRANS>
+__global__ __launch_bounds__(NT) void k_q1_hex8_diffusion(Q1DiffParams p) {
+  __shared__ Real stage[NSTAGE * NE];
+  __shared__ double rowbuf[8 * ROWBUF];
+  spec_block<KIND, Real, TRANS>(p, blockIdx.x, blockIdx.y, blockIdx.z, stage, rowbuf);
+}
+
+template <int KIND, typename Real, bool TRANS>
+__global__ __launch_bounds__(NE) void k_q1_hex8_ring(Q1DiffParams p) {
+  __shared__ double lds[ring_union_doubles<Real>()];
+  __shared__ double facebuf[2 * ring_nval<TRANS>() * FC];
+  ring_block<KIND, Real, TRANS>(p, blockIdx.x, blockIdx.y, blockIdx.z, lds, facebuf);
+}
+
+
+// ============================================================================
+// Batched assembly: many meshes in one persistent launch
+// ============================================================================
+// Work item = one tile (workgroup's worth: a 31x7 patch of vertex columns and a
+// z slab) of one mesh.  Workgroups pull items from a counter until the list is
+// exhausted; the last workgroup to leave resets the counter, so the launch can
+// be replayed (as a graph node) without a separate clear.
+struct BatchQueue {
+  unsigned next;
+  unsigned done;
+};
+
+__device__ __forceinline__ int fetch_item(BatchQueue *q, int *slot) {
+  if (threadIdx.x == 0) *slot = static_cast<int>(atomicAdd(&q->next, 1u));
+  __syncthreads();
+  const int it = *slot;
+  __syncthreads();
+  return it;
+}
+
+__device__ __forceinline__ void leave_queue(BatchQueue *q) {
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&q->done, 1u) == gridDim.x - 1) {
+      atomicExch(&q->next, 0u);
+      atomicExch(&q->done, 0u);
+    }
+  }
+}
+
+// One element kind per launch: a kernel holding both bodies needs the register
+// budget of the larger one plus spills.
+template <int KIND, typename Real, bool TRANS>
+__global__ __launch_bounds__(NE) void k_q1_batch_ring(const Q1DiffParams *__restrict__ prm,
+                                                      const int4 *__restrict__ items, int nitems, BatchQueue *q) {
+  __shared__ double lds[ring_union_doubles<Real>()];
+  __shared__ double facebuf[2 * ring_nval<TRANS>() * FC];
+  __shared__ int slot;
+  for (;;) {
+    const int it = fetch_item(q, &slot);
+    if (it >= nitems) break;
+    const int4 w = items[it];
+    const Q1DiffParams &p = prm[w.x];
+    if (w.w * p.zc > p.nz) continue;   // slab past the mesh: nothing to do (uniform over the block)
+    ring_block<KIND, Real, TRANS>(p, w.y, w.z, w.w, lds, facebuf);
+    __syncthreads();
+  }
+  leave_queue(q);
+}
+
+template <int KIND, typename Real, bool TRANS>
+__global__ __launch_bounds__(NT) void k_q1_batch_spec(const Q1DiffParams *__restrict__ prm,
+                                                      const int4 *__restrict__ items, int nitems, BatchQueue *q) {
+  __shared__ Real stage[NSTAGE * NE];
+  __shared__ double rowbuf[8 * ROWBUF];
+  __shared__ int slot;
+  for (;;) {
+    const int it = fetch_item(q, &slot);
+    if (it >= nitems) break;
+    const int4 w = items[it];
+    const Q1DiffParams &p = prm[w.x];
+    if (w.w * p.zc > p.nz) continue;   // slab past the mesh: nothing to do (uniform over the block)
+    spec_block<KIND, Real, TRANS>(p, w.y, w.z, w.w, stage, rowbuf);
+    __syncthreads();
+  }
+  leave_queue(q);
+}
+
+struct BcItem {
+  const int32_t *lids;
+  const double *g;
+  const double *x;
+  const int64_t *rowptr;
+  const int32_t *col;
+  double *f;
+  double *vals;
+  int64_t n, nrows;
+};
+
+// Dirichlet rows of every mesh of a batch: blockIdx.x = mesh, blockIdx.y strides its rows.
+__global__ void k_dirichlet_batch(const BcItem *__restrict__ items) {
+  const BcItem b = items[blockIdx.x];
+  const int lane = threadIdx.x % 64;
+  for (int64_t i = static_cast<int64_t>(blockIdx.y) * (blockDim.x / 64) + threadIdx.x / 64; i < b.n;
+       i += static_cast<int64_t>(gridDim.y) * (blockDim.x / 64)) {
+    const int32_t r = b.lids[i];
+    if (r < 0 || r >= b.nrows) continue;
+    if (lane == 0 && b.f) b.f[r] = b.x[r] - (b.g ? b.g[i] : 0.0);
+    if (b.vals)
+      for (int64_t k = b.rowptr[r] + lane; k < b.rowptr[r + 1]; k += 64) b.vals[k] = b.col[k] == r ? 1.0 : 0.0;
+  }
+}
+
 // FP32 integration: one-role kernel with the LDS face ring (2 workgroups per CU);
 // FP64: element and owner waves split (the FP64 element state leaves no room for the ring)
 template <int KIND>
@@ -826,27 +942,272 @@ void launch_kind(const Q1DiffParams &p, dim3 grid, int precision, bool trans, hi
 
 }  // namespace
 
-void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s) {
+namespace {
+
+Q1DiffParams make_params(const Mesh &m, const txa_assembly_args &a, int zc) {
   Q1DiffParams p;
   p.nx = m.nx; p.ny = m.ny; p.nz = m.nz;
+  p.zc = zc;
   p.X = m.xyz.p; p.Y = m.xyz.p + m.nverts; p.Z = m.xyz.p + 2 * m.nverts;
   p.lid = m.point_lid.p;
   p.coef = a.coef; p.x = a.x; p.xdot = a.xdot;
   p.alpha = a.alpha; p.beta = a.beta; p.rhoc = a.rho; p.src = a.src[0];
   p.f = a.f; p.vals = a.vals;
   p.rowptr = m.rowptr.p; p.perm = m.perm.p;
+  return p;
+}
+
+inline int tiles_x(const Mesh &m) { return (m.nx + 1 + TX - 2) / (TX - 1); }
+inline int tiles_y(const Mesh &m) { return (m.ny + 1 + TY - 2) / (TY - 1); }
+
+}  // namespace
+
+void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s) {
   // z slabs: enough workgroups to cover the 256 CUs several times over
-  const int gx = (m.nx + 1 + TX - 2) / (TX - 1), gy = (m.ny + 1 + TY - 2) / (TY - 1);
+  const int gx = tiles_x(m), gy = tiles_y(m);
   static_assert(NT == 512, "block shape");
   int nchunk = (TXA_Q1_TARGET_BLOCKS + gx * gy - 1) / (gx * gy);
   nchunk = std::max(1, std::min(nchunk, (m.nz + 1 + 7) / 8));
-  p.zc = (m.nz + 1 + nchunk - 1) / nchunk;
-  nchunk = (m.nz + 1 + p.zc - 1) / p.zc;
+  int zc = (m.nz + 1 + nchunk - 1) / nchunk;
+  nchunk = (m.nz + 1 + zc - 1) / zc;
+  const Q1DiffParams p = make_params(m, a, zc);
   dim3 grid(gx, gy, nchunk);
   const bool trans = a.transient != 0;
   if (m.topo == TXA_TET4) launch_kind<TXA_TET4>(p, grid, a.precision, trans, s);
   else launch_kind<TXA_HEX8>(p, grid, a.precision, trans, s);
   TXA_HIP(hipGetLastError());
 }
+
+// ---------------------------------------------------------------------------- batches
+struct Batch {
+  std::vector<int> ids;
+  std::vector<Mesh *> meshes;
+  int precision = TXA_FP64;
+  bool transient = false;
+  int slab = 0;   // vertex layers per work item (0: whole column)
+  int grid = 0;   // persistent workgroups
+  std::vector<int> zc;
+  std::vector<int4> items_h;   // hex8 items, then tet4 items
+  size_t n_hex = 0;            // items of hex8 meshes (the first n_hex of items_h)
+  DevBuf<int4> items;
+  DevBuf<Q1DiffParams> prm;
+  DevBuf<BcItem> bc;
+  DevBuf<BatchQueue> q;        // [2]: hex8 queue, tet4 queue
+  Q1DiffParams *prm_h = nullptr;   // pinned: a graph's copy node reads it at replay
+  BcItem *bc_h = nullptr;
+  hipStream_t cap = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipEvent_t done = nullptr;
+  bool launched = false;
+  ~Batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (done) (void)hipEventDestroy(done);
+    if (cap) (void)hipStreamDestroy(cap);
+    if (prm_h) (void)hipHostFree(prm_h);
+    if (bc_h) (void)hipHostFree(bc_h);
+  }
+};
+
+namespace {
+
+void batch_drop_graph(Batch &b) {
+  if (b.exec) (void)hipGraphExecDestroy(b.exec);
+  if (b.graph) (void)hipGraphDestroy(b.graph);
+  b.exec = nullptr;
+  b.graph = nullptr;
+}
+
+// Work lists for a slab height, one per element kind: items of the tallest
+// slabs first, so the queue hands out the longest items early and the short
+// ones fill the tail.  The slab height is also written into the pinned
+// argument blocks (their zc must match the items): no launch of the batch may
+// be in flight.
+void batch_build_items(Batch &b, int slab) {
+  b.slab = slab;
+  b.zc.resize(b.meshes.size());
+  std::vector<int4> lists[2];
+  for (size_t i = 0; i < b.meshes.size(); ++i) {
+    const Mesh &m = *b.meshes[i];
+    const int layers = m.nz + 1;
+    const int zc = slab > 0 ? std::min(slab, layers) : layers;
+    b.zc[i] = zc;
+    b.prm_h[i].zc = zc;
+    const int nch = (layers + zc - 1) / zc;
+    std::vector<int4> &out = lists[m.topo == TXA_TET4 ? 1 : 0];
+    for (int z = 0; z < nch; ++z)
+      for (int y = 0; y < tiles_y(m); ++y)
+        for (int x = 0; x < tiles_x(m); ++x) out.push_back(make_int4(static_cast<int>(i), x, y, z));
+  }
+  auto cost = [&](const int4 &w) {
+    const Mesh &m = *b.meshes[w.x];
+    const int k0 = w.w * b.zc[w.x];
+    return std::min(k0 + b.zc[w.x], m.nz + 1) - std::max(k0 - 1, 0);
+  };
+  for (auto &l : lists)
+    std::stable_sort(l.begin(), l.end(), [&](const int4 &u, const int4 &v) { return cost(u) > cost(v); });
+  b.n_hex = lists[0].size();
+  b.items_h = lists[0];
+  b.items_h.insert(b.items_h.end(), lists[1].begin(), lists[1].end());
+  b.items.alloc(b.items_h.size());
+  TXA_HIP(hipMemcpy(b.items.p, b.items_h.data(), sizeof(int4) * b.items_h.size(), hipMemcpyHostToDevice));
+  batch_drop_graph(b);
+}
+
+void batch_fill(Batch &b, const txa_assembly_args *args) {
+  for (size_t i = 0; i < b.meshes.size(); ++i) {
+    const Mesh &m = *b.meshes[i];
+    const txa_assembly_args &a = args[i];
+    TXA_REQUIRE(a.physics == TXA_DIFFUSION, TXA_ERR_UNSUPPORTED, "batched assembly covers Q1 diffusion");
+    TXA_REQUIRE(a.precision == b.precision, TXA_ERR_INVALID, "every batch item must use the batch precision");
+    TXA_REQUIRE((a.transient != 0) == b.transient, TXA_ERR_INVALID, "every batch item must match the batch transient flag");
+    TXA_REQUIRE(a.x && a.coef, TXA_ERR_INVALID, "x and coef are required");
+    TXA_REQUIRE(!b.transient || a.xdot, TXA_ERR_INVALID, "transient assembly needs xdot");
+    TXA_REQUIRE(a.n_bc == 0 || a.bc_lids, TXA_ERR_INVALID, "bc_lids missing");
+    b.prm_h[i] = make_params(m, a, b.zc[i]);
+    BcItem &c = b.bc_h[i];
+    c.lids = a.bc_lids; c.g = a.bc_vals; c.x = a.x; c.rowptr = m.rowptr.p; c.col = m.col.p;
+    c.f = a.f; c.vals = a.vals; c.n = a.n_bc; c.nrows = m.ndofs;
+  }
+}
+
+template <int KIND>
+void launch_batch(const Batch &b, dim3 g, const int4 *items, int n, BatchQueue *q, hipStream_t s) {
+  if (b.precision == TXA_FP64) {
+    if (b.transient) hipLaunchKernelGGL((k_q1_batch_spec<KIND, double, true>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
+    else hipLaunchKernelGGL((k_q1_batch_spec<KIND, double, false>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
+  } else {
+    if (b.transient) hipLaunchKernelGGL((k_q1_batch_ring<KIND, float, true>), g, dim3(NE), 0, s, b.prm.p, items, n, q);
+    else hipLaunchKernelGGL((k_q1_batch_ring<KIND, float, false>), g, dim3(NE), 0, s, b.prm.p, items, n, q);
+  }
+  TXA_HIP(hipGetLastError());
+}
+
+void batch_enqueue(Batch &b, hipStream_t s) {
+  const size_t n = b.meshes.size();
+  for (size_t i = 0; i < n; ++i) {
+    const Q1DiffParams &p = b.prm_h[i];
+    const Mesh &m = *b.meshes[i];
+    TXA_REQUIRE(p.zc == b.zc[i] && p.nx == m.nx && p.ny == m.ny && p.nz == m.nz, TXA_ERR_LOGIC,
+                "batch argument block out of step with its work list");
+  }
+  TXA_HIP(hipMemcpyAsync(b.prm.p, b.prm_h, sizeof(Q1DiffParams) * n, hipMemcpyHostToDevice, s));
+  TXA_HIP(hipMemcpyAsync(b.bc.p, b.bc_h, sizeof(BcItem) * n, hipMemcpyHostToDevice, s));
+  const int nk[2] = {static_cast<int>(b.n_hex), static_cast<int>(b.items_h.size() - b.n_hex)};
+  const int4 *it[2] = {b.items.p, b.items.p + b.n_hex};
+  for (int k = 0; k < 2; ++k) {
+    if (nk[k] == 0) continue;
+    const dim3 g(std::min(b.grid, nk[k]));
+    BatchQueue *q = b.q.p + k;
+    if (k == 0) launch_batch<TXA_HEX8>(b, g, it[k], nk[k], q, s);
+    else launch_batch<TXA_TET4>(b, g, it[k], nk[k], q, s);
+  }
+  hipLaunchKernelGGL(k_dirichlet_batch, dim3(static_cast<unsigned>(n), 8), dim3(256), 0, s, b.bc.p);
+  TXA_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+Batch *batch_create(const std::vector<Mesh *> &meshes, const std::vector<int> &ids, int precision, bool transient) {
+  TXA_REQUIRE(!meshes.empty(), TXA_ERR_INVALID, "empty batch");
+  TXA_REQUIRE(precision == TXA_FP64 || precision == TXA_FP32, TXA_ERR_UNSUPPORTED,
+              "batched assembly runs in FP64 or FP32 integration");
+  std::unique_ptr<Batch> b(new Batch());
+  b->ids = ids;
+  b->meshes = meshes;
+  b->precision = precision;
+  b->transient = transient;
+  for (Mesh *m : meshes) {
+    TXA_REQUIRE(m->topo == TXA_HEX8 || m->topo == TXA_TET4, TXA_ERR_UNSUPPORTED, "batched assembly covers hex8 and tet4 meshes");
+    TXA_REQUIRE(m->has_graph && m->nfields == 1, TXA_ERR_LOGIC, "batch meshes need a scalar DOF map and a graph");
+  }
+  b->prm.alloc(meshes.size());
+  b->bc.alloc(meshes.size());
+  b->q.alloc(2);
+  TXA_HIP(hipMemset(b->q.p, 0, 2 * sizeof(BatchQueue)));
+  TXA_HIP(hipHostMalloc(reinterpret_cast<void **>(&b->prm_h), sizeof(Q1DiffParams) * meshes.size(), hipHostMallocDefault));
+  TXA_HIP(hipHostMalloc(reinterpret_cast<void **>(&b->bc_h), sizeof(BcItem) * meshes.size(), hipHostMallocDefault));
+  std::memset(b->prm_h, 0, sizeof(Q1DiffParams) * meshes.size());
+  std::memset(b->bc_h, 0, sizeof(BcItem) * meshes.size());
+  TXA_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+  TXA_HIP(hipEventCreateWithFlags(&b->done, hipEventDisableTiming));
+  int dev = 0, ncu = 0;
+  TXA_HIP(hipGetDevice(&dev));
+  TXA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  b->grid = 2 * ncu;
+  batch_build_items(*b, 32);
+  return b.release();
+}
+
+void batch_free(Batch *b) { delete b; }
+
+void batch_assemble(Batch &b, const txa_assembly_args *args, bool use_graph, hipStream_t s) {
+  TXA_REQUIRE(args, TXA_ERR_INVALID, "null args");
+  // the previous launch must be done reading the pinned parameter blocks
+  if (b.launched) TXA_HIP(hipEventSynchronize(b.done));
+  batch_fill(b, args);
+  if (use_graph) {
+    if (!b.exec) {
+      TXA_HIP(hipStreamBeginCapture(b.cap, hipStreamCaptureModeThreadLocal));
+      batch_enqueue(b, b.cap);
+      TXA_HIP(hipStreamEndCapture(b.cap, &b.graph));
+      TXA_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+    }
+    TXA_HIP(hipGraphLaunch(b.exec, s));
+  } else {
+    batch_enqueue(b, s);
+  }
+  TXA_HIP(hipEventRecord(b.done, s));
+  b.launched = true;
+}
+
+// Autotuner: times the batch over slab heights (the analogue of Panzer's workset
+// size sweep) and persistent grid sizes, keeps the fastest.  out: best ms.
+double batch_tune(Batch &b, const txa_assembly_args *args, hipStream_t s, int reps) {
+  int dev = 0, ncu = 0;
+  TXA_HIP(hipGetDevice(&dev));
+  TXA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const int slabs[] = {8, 16, 32, 64, 0};
+  const int grids[] = {ncu, 2 * ncu, 4 * ncu};
+  hipEvent_t e0, e1;
+  TXA_HIP(hipEventCreate(&e0));
+  TXA_HIP(hipEventCreate(&e1));
+  double best = 1e30;
+  int best_slab = b.slab, best_grid = b.grid;
+  if (b.launched) TXA_HIP(hipEventSynchronize(b.done));
+  batch_fill(b, args);   // the argument blocks stay as they are for every timed launch
+  for (int sl : slabs) {
+    TXA_HIP(hipStreamSynchronize(s));
+    batch_build_items(b, sl);
+    for (int g : grids) {
+      b.grid = g;
+      batch_enqueue(b, s);   // warm-up
+      TXA_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < reps; ++r) batch_enqueue(b, s);
+      TXA_HIP(hipEventRecord(e1, s));
+      TXA_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      TXA_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms / reps < best) { best = ms / reps; best_slab = sl; best_grid = g; }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  batch_build_items(b, best_slab);
+  b.grid = best_grid;
+  TXA_HIP(hipEventRecord(b.done, s));
+  b.launched = true;
+  return best;
+}
+
+void batch_info(const Batch &b, int64_t *out) {
+  out[0] = static_cast<int64_t>(b.meshes.size());
+  out[1] = static_cast<int64_t>(b.items_h.size());
+  out[2] = b.slab;
+  out[3] = b.grid;
+}
+
+const std::vector<int> &batch_mesh_ids(const Batch &b) { return b.ids; }
 
 }  // namespace txa
