@@ -184,3 +184,31 @@ def test_tet4_transient_dirichlet_fp64(engine):
     assert rel_err(f, fr) <= TOL64
     assert rel_err(v, vr) <= TOL64
     c.m.destroy()
+
+
+def test_cfg1_poisson_32_cube(engine):
+    """Q1 Poisson on the 32^3 unit cube: k lognormal (sigma 0.5), f = 1, u ~ U[-1, 1],
+    u = 0 on x = 0 (the nodeset 'left', 33^2 = 1089 DOFs), FP64.  Oracle with
+    workset 1000 and the forward-AD Jacobian, as the reference runs it."""
+    n = 32
+    c = Case(engine, txa.HEX8, n, n, n, 1)
+    assert c.ndofs == 35937 and c.nnz == 912673
+    k = synth.lognormal(1001, n ** 3, 0.5)
+    x = synth.uniform(2002, c.ndofs, -1, 1)
+    pl = c.m.point_lids()[:, 0]
+    bc = np.unique(pl[np.arange(pl.size) % (n + 1) == 0]).astype(np.int32)
+    assert bc.size == 1089
+    A = O.Assembler(O.HEX8, O.DIFFUSION, c.ev, c.X, c.el, k, c.rp, c.col, src=(1.0,), workset_size=1000)
+    fr, vr = A.evaluate(x, mode=O.MODE_FAD)
+    O.dirichlet(bc, None, x, c.rp, c.col, fr, vr)
+    f = torch.empty(c.ndofs, device="cuda", dtype=torch.float64)
+    v = torch.empty(c.nnz, device="cuda", dtype=torch.float64)
+    c.m.assemble(physics=txa.DIFFUSION, coef=dev(k), x=dev(x), f=f, vals=v, src=(1.0,),
+                 bc_lids=dev(bc, torch.int32))
+    torch.cuda.synchronize()
+    f, v = f.cpu().numpy(), v.cpu().numpy()
+    assert rel_err(f, fr) <= TOL64 and rel_err(v, vr) <= TOL64
+    assert np.all(f[bc] == x[bc])   # g = 0
+    rp, col = c.m.graph()
+    assert np.array_equal(rp, c.rp) and np.array_equal(col, c.col)
+    c.m.destroy()

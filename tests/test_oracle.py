@@ -325,22 +325,31 @@ def test_patch_test_linear_field_interior_rows_zero():
     assert np.max(np.abs(f[interior])) < 1e-13
 
 
+# Q2 nodes of HGRAD_HEX_C2 in element-local order (vertices, edge midpoints, face
+# centres, cell centre) on the reference cube [0,1]^3.
+Q2_LATTICE = np.array([[0, 0, 0], [2, 0, 0], [2, 2, 0], [0, 2, 0], [0, 0, 2], [2, 0, 2], [2, 2, 2], [0, 2, 2],
+                       [1, 0, 0], [2, 1, 0], [1, 2, 0], [0, 1, 0], [1, 0, 2], [2, 1, 2], [1, 2, 2], [0, 1, 2],
+                       [0, 0, 1], [2, 0, 1], [2, 2, 1], [0, 2, 1], [1, 0, 1], [2, 1, 1], [1, 2, 1], [0, 1, 1],
+                       [1, 1, 0], [1, 1, 2], [1, 1, 1]]) / 2.0
+HEXV = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+
+
+def element_nodes(topo, X):
+    """Physical positions of the element's nodes, element-local order (one-cell brick X)."""
+    Xv = X[[int(v[0] + 2 * v[1] + 4 * v[2]) for v in HEXV]]   # shards vertex order
+    if topo == O.HEX8:
+        return Xv
+    w = np.array([[np.prod([(1 - l[d]) if hv[d] == 0 else l[d] for d in range(3)]) for hv in HEXV]
+                  for l in Q2_LATTICE])
+    return w @ Xv
+
+
 def test_elasticity_rigid_body_modes_in_kernel():
     X = synth.brick_coords(1, 1, 1, jitter=0.0)
     X = X + 0.07 * synth.uniform(3, X.size, -1, 1).reshape(X.shape)   # distorted single hex
     for topo in (O.HEX8, O.HEX27):
         K, _, el = single_element_matrix(topo, X, physics=O.ELASTICITY, nf=3, coef=2.0, nu=0.3)
-        if topo == O.HEX8:
-            P = X[[0, 1, 3, 2, 4, 5, 7, 6]]   # node ids in shards vertex order
-        else:  # Q2 node positions: trilinear map of the reference lattice
-            lat = np.array([[0, 0, 0], [2, 0, 0], [2, 2, 0], [0, 2, 0], [0, 0, 2], [2, 0, 2], [2, 2, 2], [0, 2, 2],
-                            [1, 0, 0], [2, 1, 0], [1, 2, 0], [0, 1, 0], [1, 0, 2], [2, 1, 2], [1, 2, 2], [0, 1, 2],
-                            [0, 0, 1], [2, 0, 1], [2, 2, 1], [0, 2, 1], [1, 0, 1], [2, 1, 1], [1, 2, 1], [0, 1, 1],
-                            [1, 1, 0], [1, 1, 2], [1, 1, 1]]) / 2.0
-            hexv = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
-            Xv = X[[int(v[0] + 2 * v[1] + 4 * v[2]) for v in hexv]]
-            w = np.array([[np.prod([(1 - l[d]) if hv[d] == 0 else l[d] for d in range(3)]) for hv in hexv] for l in lat])
-            P = w @ Xv
+        P = element_nodes(topo, X)
         modes = []
         for d in range(3):
             m = np.zeros_like(P); m[:, d] = 1.0; modes.append(m.ravel())
@@ -351,6 +360,39 @@ def test_elasticity_rigid_body_modes_in_kernel():
         ev = np.linalg.eigvalsh(0.5 * (K + K.T))
         assert ev.min() > -1e-12 * ev.max()
         assert np.sum(ev < 1e-10 * ev.max()) == 6
+
+
+@pytest.mark.parametrize("topo", [O.HEX8, O.HEX27])
+def test_elasticity_uniform_strain_energy_and_inertia(topo):
+    """Closed forms that fix the constitutive law and the inertia term.  On a
+    parallelepiped X = A xi + b (volume |det A|) a linear displacement u = G x is
+    reproduced exactly, its strain eps = sym(G) is uniform, and the element energy
+    is u^T K u = V (lambda tr(eps)^2 + 2 mu eps:eps) with the Lame constants of
+    (E, nu).  A rigid unit acceleration along d gives inertial forces summing to
+    rho V along d and nothing across."""
+    Amap = np.array([[1.2, 0.1, -0.05], [0.08, 0.9, 0.12], [-0.1, 0.07, 1.1]])
+    X = synth.brick_coords(1, 1, 1) @ Amap.T + np.array([0.3, -0.2, 0.1])
+    V = abs(np.linalg.det(Amap))
+    E, nu, rho = 2.0, 0.3, 1.7
+    lam, mu = E * nu / ((1 + nu) * (1 - 2 * nu)), E / (2 * (1 + nu))
+    K, _, _ = single_element_matrix(topo, X, physics=O.ELASTICITY, nf=3, coef=E, nu=nu)
+    P = element_nodes(topo, X)
+    G = np.array([[0.01, 0.004, -0.002], [0.003, -0.006, 0.005], [0.001, 0.002, 0.008]])
+    u = (P @ G.T).ravel()                    # element DOF order: node-major, field-minor
+    eps = 0.5 * (G + G.T)
+    energy = V * (lam * np.trace(eps) ** 2 + 2 * mu * np.sum(eps * eps))
+    assert abs(u @ K @ u - energy) <= 1e-12 * energy
+    M, _, _ = single_element_matrix(topo, X, physics=O.ELASTICITY, nf=3, coef=E, nu=nu, rho=rho, alpha=0.0,
+                                    beta=0.0, gamma2=1.0, transient=True)
+    for d in range(3):
+        t = np.zeros_like(P); t[:, d] = 1.0
+        force = (M @ t.ravel()).reshape(P.shape)
+        assert abs(force[:, d].sum() - rho * V) <= 1e-12 * rho * V
+        assert np.max(np.abs(np.delete(force.sum(axis=0), d))) <= 1e-12 * rho * V
+    # Rayleigh damping enters the xdot seed as a_R M + b_R K
+    C, _, _ = single_element_matrix(topo, X, physics=O.ELASTICITY, nf=3, coef=E, nu=nu, rho=rho, alpha=1.0,
+                                    beta=0.0, transient=True, rayleigh=(0.01, 0.001))
+    assert np.max(np.abs(C - (0.01 * M + 0.001 * K))) <= 1e-12 * np.max(np.abs(C))
 
 
 def test_dirichlet_rows_identity():
