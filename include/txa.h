@@ -161,6 +161,52 @@ int txa_batch_tune(txa_ctx ctx, int batch_id, const txa_assembly_args *args, int
 /* out[0]=meshes out[1]=work items out[2]=slab height (0 = whole) out[3]=persistent workgroups */
 int txa_batch_info(txa_ctx ctx, int batch_id, int64_t *out);
 
+/* Serving: a worker thread inside the library admits assembly requests for
+ * resident meshes by HBM footprint and runs admitted requests concurrently,
+ * one HIP stream per slot; requests that arrive while others run are admitted
+ * as slots and memory free up (continuous batching).  Each request is one
+ * implicit time step's residual + Jacobian, the evaluate() call of
+ * AssemblyEngine with the time-integration coefficients of
+ * AssemblyEngineInArgs (disc-fe/src/Panzer_AssemblyEngine_impl.hpp:67-129,
+ * Panzer_AssemblyEngine_InArgs.hpp:95-106).  Second-order problems use
+ * Newmark: R = M a + C v + K u - f with C = rayleigh_a M + rayleigh_b K and
+ * J = K + newmark_gamma/(newmark_beta dt) C + 1/(newmark_beta dt^2) M;
+ * first-order problems use J = K + (1/dt) M. */
+typedef struct {
+  int mesh_id;                 /* a mesh the server was created with */
+  int physics;                 /* TXA_ELASTICITY (Newmark) or TXA_DIFFUSION */
+  int precision;               /* as txa_assembly_args */
+  const double *coef;          /* device [elements] */
+  double nu, rho;
+  double src[3];
+  double rayleigh_a, rayleigh_b;
+  double dt, newmark_beta, newmark_gamma;
+  const double *u, *v, *a;     /* device [ndofs] state, read until the request is done; a may be NULL for diffusion */
+  const int32_t *bc_lids;      /* device [n_bc] or NULL */
+  const double *bc_vals;       /* device [n_bc] or NULL (= 0) */
+  int64_t n_bc;
+  int want_jacobian;           /* 0: residual only */
+} txa_serve_request;
+
+/* hbm_budget: bytes of the output arena (f and CRS values of admitted requests
+ * until released); slots: requests in flight at once (1..32). */
+int txa_serve_create(txa_ctx ctx, const int *mesh_ids, int n, int64_t hbm_budget, int slots, int *server_id);
+/* Stops the worker after the running requests; queued ones fail. */
+int txa_serve_destroy(txa_ctx ctx, int server_id);
+/* Non-blocking: queues the request (TXA_ERR_NOMEM if it can never fit the budget). */
+int txa_serve_submit(txa_ctx ctx, int server_id, const txa_serve_request *req, int64_t *request_id);
+/* status: 0 queued, 1 running, 2 done (f / vals point at the result, device memory),
+ * 3 failed (returns TXA_ERR_DEVICE with the message).  timeout_s > 0 waits up to
+ * that long for the request to finish. */
+int txa_serve_poll(txa_ctx ctx, int server_id, int64_t request_id, double timeout_s, int *status, const double **f,
+                   const double **vals);
+/* Returns the request's memory to the arena (done or failed requests only). */
+int txa_serve_release(txa_ctx ctx, int server_id, int64_t request_id);
+/* out[11]: completed, failed, p50 ms, p99 ms, mean ms, max ms (submit -> completion),
+ * elements/s, requests/s (first submit -> last completion), peak requests in
+ * flight, peak arena bytes, arena bytes in use */
+int txa_serve_stats(txa_ctx ctx, int server_id, double *out);
+
 #ifdef __cplusplus
 }
 #endif

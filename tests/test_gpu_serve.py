@@ -1,0 +1,175 @@
+"""GPU: the serving loop (txa_serve_*).
+
+Requests are Newmark steps of transient hex8 elasticity (beta_N = 0.25,
+gamma_N = 0.5, Rayleigh a = 0.01, b = 0.001, dt = 1e-3, state ~ U[-1e-3, 1e-3])
+plus first-order diffusion steps.  Every result must match the oracle at the
+FP64 tolerance (1e-12 normwise), and the same request must give the same
+result alone and in a mix of concurrent requests: within 1e-13 normwise (the
+elasticity kernel sums with FP64 atomics, whose order varies between runs).
+"""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import tianxin_amd as txa
+from tianxin_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+DT, NB, NG, RAY, RHO, NU = 1e-3, 0.25, 0.5, (0.01, 0.001), 2.0, 0.3
+TOPO = {txa.HEX8: O.HEX8}
+
+
+def rel_err(got, ref):
+    return float(np.max(np.abs(got - ref)) / max(np.max(np.abs(ref)), 1e-300))
+
+
+def dev(a, dtype=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(a), device="cuda", dtype=dtype)
+
+
+class Model:
+    def __init__(self, engine, physics, nx, ny, nz, seed):
+        print(f"[serve model] physics={physics} mesh={nx}x{ny}x{nz}", flush=True)
+        self.physics = physics
+        self.nf = 3 if physics == txa.ELASTICITY else 1
+        self.X = synth.brick_coords(nx, ny, nz, jitter=0.1, seed=seed)
+        self.m = engine.brick(txa.HEX8, nx, ny, nz, self.X)
+        self.ndofs = self.m.build_dofmap(self.nf)
+        self.nnz = self.m.build_graph()
+        self.coef = synth.lognormal(seed + 1, self.m.nelem, 0.3)
+        pl = self.m.point_lids()
+        self.bc = np.unique(pl[np.arange(pl.shape[0]) % (nx + 1) == 0].ravel()).astype(np.int32)
+        conn, ev, _ = O.brick_mesh(O.HEX8, nx, ny, nz)
+        _, owned, el = O.dofmap(conn, self.nf)
+        self.rp, self.col = O.graph(el, owned.size)
+        self.A = O.Assembler(O.HEX8, physics, ev, self.X, el, self.coef, self.rp, self.col, nu=NU, rho=RHO,
+                             src=(1.0, 0.0, -0.5), rayleigh=RAY if physics == txa.ELASTICITY else (0.0, 0.0))
+        self.d_coef, self.d_bc = dev(self.coef), dev(self.bc, torch.int32)
+
+    def state(self, seed):
+        u, v, a = (synth.uniform(seed + i, self.ndofs, -1e-3, 1e-3) for i in range(3))
+        return u, v, (a if self.physics == txa.ELASTICITY else None)
+
+    def upload(self, st):
+        u, v, a = st
+        return dev(u), dev(v), None if a is None else dev(a)
+
+    def submit(self, srv, st, d=None):
+        d = self.upload(st) if d is None else d
+        rid = srv.submit(self.m, physics=self.physics, coef=self.d_coef, u=d[0], v=d[1], a=d[2], dt=DT,
+                         newmark=(NB, NG), nu=NU, rho=RHO, src=(1.0, 0.0, -0.5),
+                         rayleigh=RAY if self.physics == txa.ELASTICITY else (0.0, 0.0), bc_lids=self.d_bc)
+        return rid, d   # keep the state tensors alive until the request is done
+
+    def oracle(self, st):
+        u, v, a = st
+        if self.physics == txa.ELASTICITY:
+            kw = dict(alpha=NG / (NB * DT), beta=1.0, gamma2=1.0 / (NB * DT * DT))
+        else:
+            kw = dict(alpha=1.0 / DT, beta=1.0, gamma2=0.0)
+        fr, vr = self.A.evaluate(u, v, a, transient=True, mode=O.MODE_EXPLICIT, **kw)
+        O.dirichlet(self.bc, None, u, self.rp, self.col, fr, vr)
+        return fr, vr
+
+    def footprint(self):
+        return 8 * (self.ndofs + self.nnz) + 512
+
+
+@pytest.fixture(scope="module")
+def models(engine):
+    ms = [Model(engine, txa.ELASTICITY, 3, 4, 2, 11), Model(engine, txa.ELASTICITY, 9, 7, 8, 12),
+          Model(engine, txa.DIFFUSION, 12, 10, 9, 13), Model(engine, txa.ELASTICITY, 14, 12, 10, 14),
+          Model(engine, txa.ELASTICITY, 24, 24, 24, 15)]
+    yield ms
+    for m in ms:
+        m.m.destroy()
+
+
+def test_serve_alone_and_in_the_mix(engine, models):
+    srv = txa.Server(engine, [m.m for m in models], hbm_budget=1 << 30, slots=4)
+    try:
+        probe = models[3]
+        st0 = probe.state(500)
+        rid, keep = probe.submit(srv, st0)
+        f_alone, v_alone = srv.result(rid, probe.m)
+        srv.release(rid)
+        fr, vr = probe.oracle(st0)
+        assert rel_err(f_alone, fr) <= 1e-12 and rel_err(v_alone, vr) <= 1e-12
+        # a mix of 16 requests over all models, states uploaded first and then submitted
+        # back to back; the probe state sits in the middle
+        order = [4, 3, 1, 0, 2, 4, 1, 2, 3, 2, 4, 0, 3, 1, 4, 0]
+        states = [st0 if i == 8 else models[k].state(600 + 10 * i) for i, k in enumerate(order)]
+        dstates = [models[k].upload(st) for k, st in zip(order, states)]
+        torch.cuda.synchronize()
+        jobs = []
+        for i, k in enumerate(order):
+            rid, keep = models[k].submit(srv, states[i], dstates[i])
+            jobs.append((k, states[i], rid, keep))
+        for i, (k, st, rid, keep) in enumerate(jobs):
+            f, v = srv.result(rid, models[k].m)
+            srv.release(rid)
+            fr, vr = models[k].oracle(st)
+            e = (rel_err(f, fr), rel_err(v, vr))
+            print(f"request {i} model {k}: oracle R {e[0]:.2e} J {e[1]:.2e}", flush=True)
+            assert e[0] <= 1e-12 and e[1] <= 1e-12
+            if i == 8:
+                assert rel_err(f, f_alone) <= 1e-13 and rel_err(v, v_alone) <= 1e-13
+        s = srv.stats()
+        print(s)
+        assert s["completed"] == 17 and s["failed"] == 0
+        assert 0.0 < s["p50_ms"] <= s["p99_ms"] <= s["max_ms"]
+        assert s["peak_inflight"] >= 2          # the mix overlapped on the device
+        assert s["bytes_in_use"] == 0
+    finally:
+        srv.destroy()
+
+
+def test_serve_admission_by_footprint(engine, models):
+    big = models[3]
+    budget = int(1.5 * big.footprint())      # room for one such request at a time
+    srv = txa.Server(engine, [m.m for m in models], hbm_budget=budget, slots=4)
+    try:
+        st = [big.state(700 + 3 * i) for i in range(2)]
+        r0, k0 = big.submit(srv, st[0])
+        r1, k1 = big.submit(srv, st[1])
+        f0, v0 = srv.result(r0, big.m)
+        status, _, _ = srv.poll(r1)
+        assert status == txa.QUEUED              # waits for memory, not for a slot
+        with pytest.raises(txa.TxaError) as e:
+            srv.release(r1)
+        assert e.value.code == txa.TXA_ERR_LOGIC
+        srv.release(r0)                          # frees the arena: r1 is admitted
+        f1, v1 = srv.result(r1, big.m)
+        srv.release(r1)
+        for (f, v), s_ in zip(((f0, v0), (f1, v1)), st):
+            fr, vr = big.oracle(s_)
+            assert rel_err(f, fr) <= 1e-12 and rel_err(v, vr) <= 1e-12
+        s = srv.stats()
+        assert s["peak_inflight"] == 1 and s["peak_bytes"] <= budget
+        # a request that can never fit is refused at submission
+        small = txa.Server(engine, [big.m], hbm_budget=big.footprint() // 2, slots=1)
+        try:
+            with pytest.raises(txa.TxaError) as e:
+                big.submit(small, st[0])
+            assert e.value.code == txa.TXA_ERR_NOMEM
+        finally:
+            small.destroy()
+    finally:
+        srv.destroy()
+
+
+def test_serve_rejections(engine, models):
+    srv = txa.Server(engine, [models[0].m], hbm_budget=1 << 24, slots=2)
+    try:
+        with pytest.raises(txa.TxaError) as e:
+            models[1].submit(srv, models[1].state(1))      # mesh not served here
+        assert e.value.code == txa.TXA_ERR_INVALID
+        with pytest.raises(txa.TxaError) as e:
+            models[0].m.destroy()                            # resident while the server lives
+        assert e.value.code == txa.TXA_ERR_LOGIC
+        with pytest.raises(txa.TxaError):
+            srv.poll(12345)
+    finally:
+        srv.destroy()

@@ -50,6 +50,22 @@ class AssemblyArgs(ctypes.Structure):
     ]
 
 
+class ServeRequest(ctypes.Structure):
+    _fields_ = [
+        ("mesh_id", ctypes.c_int), ("physics", ctypes.c_int), ("precision", ctypes.c_int),
+        ("coef", P), ("nu", ctypes.c_double), ("rho", ctypes.c_double), ("src", ctypes.c_double * 3),
+        ("rayleigh_a", ctypes.c_double), ("rayleigh_b", ctypes.c_double),
+        ("dt", ctypes.c_double), ("newmark_beta", ctypes.c_double), ("newmark_gamma", ctypes.c_double),
+        ("u", P), ("v", P), ("a", P),
+        ("bc_lids", P), ("bc_vals", P), ("n_bc", I64), ("want_jacobian", ctypes.c_int),
+    ]
+
+
+QUEUED, RUNNING, DONE, FAILED = range(4)
+SERVE_STATS = ("completed", "failed", "p50_ms", "p99_ms", "mean_ms", "max_ms", "elements_per_s", "requests_per_s",
+               "peak_inflight", "peak_bytes", "bytes_in_use")
+
+
 def header_symbols(path: str = HEADER):
     """Names of the functions declared in include/txa.h."""
     text = open(path).read()
@@ -86,6 +102,13 @@ def load_library(path: str = LIB_PATH):
     L.txa_batch_assemble.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P]
     L.txa_batch_tune.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P, ctypes.POINTER(ctypes.c_double)]
     L.txa_batch_info.argtypes = [P, ctypes.c_int, P]
+    L.txa_serve_create.argtypes = [P, P, ctypes.c_int, I64, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    L.txa_serve_destroy.argtypes = [P, ctypes.c_int]
+    L.txa_serve_submit.argtypes = [P, ctypes.c_int, ctypes.POINTER(ServeRequest), ctypes.POINTER(I64)]
+    L.txa_serve_poll.argtypes = [P, ctypes.c_int, I64, ctypes.c_double, ctypes.POINTER(ctypes.c_int),
+                                 ctypes.POINTER(P), ctypes.POINTER(P)]
+    L.txa_serve_release.argtypes = [P, ctypes.c_int, I64]
+    L.txa_serve_stats.argtypes = [P, ctypes.c_int, P]
     _lib = L
     return L
 
@@ -278,3 +301,88 @@ class Batch:
         if self.id is not None:
             self.eng.check(self.eng.lib.txa_batch_destroy(self.eng.h, self.id))
             self.id = None
+
+
+class Server:
+    """Serving loop over resident meshes (txa_serve_*): admission by HBM footprint,
+    concurrent slots, p50/p99 latency."""
+
+    def __init__(self, eng: Engine, meshes, hbm_budget: int, slots: int = 4):
+        self.eng = eng
+        self.meshes = {m.id: m for m in meshes}
+        ids = np.array([m.id for m in meshes], np.int32)
+        sid = ctypes.c_int()
+        eng.check(eng.lib.txa_serve_create(eng.h, _np(ids), len(ids), int(hbm_budget), slots, ctypes.byref(sid)))
+        self.id = sid.value
+
+    def submit(self, mesh, *, physics, coef, u, v, a=None, dt=1e-3, newmark=(0.25, 0.5), nu=0.3, rho=1.0,
+               src=(0.0, 0.0, 0.0), rayleigh=(0.0, 0.0), bc_lids=None, bc_vals=None, want_jacobian=True,
+               precision=FP64):
+        """Queues one request; the device arrays must stay alive until it is done.  Returns its id."""
+        r = ServeRequest()
+        r.mesh_id, r.physics, r.precision = mesh.id, physics, precision
+        r.coef = _dptr(coef)
+        r.nu, r.rho = nu, rho
+        s = list(src) + [0.0] * 3
+        r.src = (ctypes.c_double * 3)(*s[:3])
+        r.rayleigh_a, r.rayleigh_b = rayleigh
+        r.dt, (r.newmark_beta, r.newmark_gamma) = dt, newmark
+        r.u, r.v, r.a = _dptr(u), _dptr(v), _dptr(a)
+        r.bc_lids, r.bc_vals = _dptr(bc_lids), _dptr(bc_vals)
+        r.n_bc = 0 if bc_lids is None else int(bc_lids.numel())
+        r.want_jacobian = int(want_jacobian)
+        rid = I64()
+        self.eng.check(self.eng.lib.txa_serve_submit(self.eng.h, self.id, ctypes.byref(r), ctypes.byref(rid)))
+        return int(rid.value)
+
+    def poll(self, rid, timeout=0.0):
+        """(status, f_ptr, vals_ptr); waits up to `timeout` seconds for completion."""
+        st, f, v = ctypes.c_int(), P(), P()
+        self.eng.check(self.eng.lib.txa_serve_poll(self.eng.h, self.id, rid, float(timeout), ctypes.byref(st),
+                                                   ctypes.byref(f), ctypes.byref(v)))
+        return st.value, f.value, v.value
+
+    def result(self, rid, mesh, timeout=120.0):
+        """Waits for request rid and copies its f (and CRS values) to host numpy arrays."""
+        st, f, v = self.poll(rid, timeout)
+        if st != DONE:
+            raise TxaError(TXA_ERR_LOGIC, f"request {rid} not done after {timeout} s (status {st})")
+        out_f = np.empty(mesh.ndofs, np.float64)
+        _copy_device_to_host(out_f, f)
+        out_v = None
+        if v:
+            out_v = np.empty(mesh.nnz, np.float64)
+            _copy_device_to_host(out_v, v)
+        return out_f, out_v
+
+    def release(self, rid):
+        self.eng.check(self.eng.lib.txa_serve_release(self.eng.h, self.id, rid))
+
+    def stats(self):
+        out = np.zeros(len(SERVE_STATS), np.float64)
+        self.eng.check(self.eng.lib.txa_serve_stats(self.eng.h, self.id, _np(out)))
+        return dict(zip(SERVE_STATS, (float(x) for x in out)))
+
+    def destroy(self):
+        if self.id is not None:
+            self.eng.check(self.eng.lib.txa_serve_destroy(self.eng.h, self.id))
+            self.id = None
+
+
+def _copy_device_to_host(dst, src_ptr):
+    """hipMemcpy (device to host) of dst.size doubles from device address src_ptr into numpy array dst."""
+    hip = _hip()
+    rc = hip.hipMemcpy(ctypes.c_void_p(dst.ctypes.data), ctypes.c_void_p(src_ptr), ctypes.c_size_t(8 * dst.size), 2)
+    if rc != 0:
+        raise TxaError(TXA_ERR_DEVICE, f"hipMemcpy device->host failed ({rc})")
+
+
+_hip_lib = None
+
+
+def _hip():
+    global _hip_lib
+    if _hip_lib is None:
+        _hip_lib = ctypes.CDLL("libamdhip64.so")
+        _hip_lib.hipMemcpy.argtypes = [P, P, ctypes.c_size_t, ctypes.c_int]
+    return _hip_lib

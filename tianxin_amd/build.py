@@ -11,7 +11,7 @@ LIB = os.path.join(HERE, "libtxa.so")
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["txa_capi.cpp", "txa_setup.hip", "txa_assemble.hip", "txa_q1_hex8.hip"]
+SOURCES = ["txa_capi.cpp", "txa_setup.hip", "txa_assemble.hip", "txa_q1_hex8.hip", "txa_serve.cpp"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          # packed FP32 instructions stay off: ordinary float code only
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -42,7 +42,7 @@ def build(verbose: bool = False, jobs: int = 8) -> str:
             sys.stderr.write(f"--- {src}\n{out}\n")
         raise RuntimeError("hipcc failed: " + ", ".join(s for s, _ in failed))
     tmp = LIB + ".tmp"
-    subprocess.run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-o", tmp, *objs], check=True)
+    subprocess.run([HIPCC, "-shared", "-pthread", f"--offload-arch={ARCH}", "-o", tmp, *objs], check=True)
     os.replace(tmp, LIB)
     return LIB
 

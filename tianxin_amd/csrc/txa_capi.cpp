@@ -101,6 +101,9 @@ int txa_mesh_destroy(txa_ctx ctx, int mesh_id) {
     for (auto &kv : ctx->batches)
       for (int id : txa::batch_mesh_ids(*kv.second))
         TXA_REQUIRE(id != mesh_id, TXA_ERR_LOGIC, "mesh is part of batch " + std::to_string(kv.first));
+    for (auto &kv : ctx->servers)
+      for (int id : txa::serve_mesh_ids(*kv.second))
+        TXA_REQUIRE(id != mesh_id, TXA_ERR_LOGIC, "mesh is served by server " + std::to_string(kv.first));
     TXA_HIP(hipDeviceSynchronize());
     ctx->meshes.erase(it);
   });
@@ -219,6 +222,52 @@ int txa_batch_info(txa_ctx ctx, int batch_id, int64_t *out) {
   return guarded(ctx, [&] {
     TXA_REQUIRE(out, TXA_ERR_INVALID, "null argument");
     txa::batch_info(ctx->batch(batch_id), out);
+  });
+}
+
+int txa_serve_create(txa_ctx ctx, const int *mesh_ids, int n, int64_t hbm_budget, int slots, int *server_id) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(mesh_ids && server_id && n > 0, TXA_ERR_INVALID, "null argument or no meshes");
+    std::vector<int> ids(mesh_ids, mesh_ids + n);
+    std::vector<Mesh *> ms;
+    for (int id : ids) ms.push_back(&ctx->mesh(id));
+    txa::Server *s = txa::serve_create(ctx->device, ms, ids, hbm_budget, slots);
+    const int id = ctx->next_id++;
+    ctx->servers[id] = s;
+    *server_id = id;
+  });
+}
+
+int txa_serve_destroy(txa_ctx ctx, int server_id) {
+  return guarded(ctx, [&] {
+    auto it = ctx->servers.find(server_id);
+    TXA_REQUIRE(it != ctx->servers.end(), TXA_ERR_INVALID, "unknown server id");
+    txa::serve_free(it->second);
+    ctx->servers.erase(it);
+    TXA_HIP(hipDeviceSynchronize());
+  });
+}
+
+int txa_serve_submit(txa_ctx ctx, int server_id, const txa_serve_request *req, int64_t *request_id) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(req && request_id, TXA_ERR_INVALID, "null argument");
+    *request_id = txa::serve_submit(ctx->server(server_id), *req);
+  });
+}
+
+int txa_serve_poll(txa_ctx ctx, int server_id, int64_t request_id, double timeout_s, int *status, const double **f,
+                   const double **vals) {
+  return guarded(ctx, [&] { txa::serve_poll(ctx->server(server_id), request_id, timeout_s, status, f, vals); });
+}
+
+int txa_serve_release(txa_ctx ctx, int server_id, int64_t request_id) {
+  return guarded(ctx, [&] { txa::serve_release(ctx->server(server_id), request_id); });
+}
+
+int txa_serve_stats(txa_ctx ctx, int server_id, double *out) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(out, TXA_ERR_INVALID, "null argument");
+    txa::serve_stats(ctx->server(server_id), out);
   });
 }
 

@@ -90,15 +90,24 @@ struct Mesh {
 
 struct Batch;
 void batch_free(Batch *b);
+struct Server;
+void serve_free(Server *s);
 
 struct Context {
   int device = 0;
   std::string last_error;
   std::map<int, std::unique_ptr<Mesh>> meshes;
   std::map<int, Batch *> batches;
+  std::map<int, Server *> servers;
   int next_id = 1;
   ~Context() {
+    for (auto &kv : servers) serve_free(kv.second);   // joins the worker threads first
     for (auto &kv : batches) batch_free(kv.second);
+  }
+  Server &server(int id) {
+    auto it = servers.find(id);
+    TXA_REQUIRE(it != servers.end(), TXA_ERR_INVALID, "unknown server id " + std::to_string(id));
+    return *it->second;
   }
   Batch &batch(int id) {
     auto it = batches.find(id);
@@ -128,5 +137,13 @@ void batch_assemble(Batch &b, const txa_assembly_args *args, bool use_graph, hip
 double batch_tune(Batch &b, const txa_assembly_args *args, hipStream_t s, int reps);
 void batch_info(const Batch &b, int64_t *out);
 const std::vector<int> &batch_mesh_ids(const Batch &b);
+// serving loop (txa_serve.cpp)
+Server *serve_create(int device, const std::vector<Mesh *> &meshes, const std::vector<int> &ids, int64_t budget,
+                     int slots);
+const std::vector<int> &serve_mesh_ids(const Server &s);
+int64_t serve_submit(Server &s, const txa_serve_request &r);
+void serve_poll(Server &s, int64_t id, double timeout_s, int *status, const double **f, const double **vals);
+void serve_release(Server &s, int64_t id);
+void serve_stats(Server &s, double *out);
 
 }  // namespace txa
