@@ -160,6 +160,182 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir):
     print(out, TRAFFIC_FILE)
 
 
+def log_uniform_sizes(seed, count, lo, hi):
+    from tianxin_amd import synth
+    import numpy as np
+    u = synth.uniform(seed, count, np.log(lo), np.log(hi))
+    return np.rint(np.exp(u)).astype(int)
+
+
+def bench_batch(args):
+    """Many meshes in one persistent launch: 512 meshes, n^3 cells with n log-uniform in
+    [8, 64], hex8 or tet4 (6-tet split) 50/50 by seed, Poisson with lognormal k
+    (sigma 0.5), u ~ U[-1, 1], u = 0 on x = 0.  Times the autotuned FP32 batch (graph
+    replay), the FP64 batch and the plain FP64 path (one txa_assemble per mesh)."""
+    import numpy as np
+    import torch
+
+    import tianxin_amd as txa
+    from tianxin_amd import synth
+
+    nm = args.meshes
+    sizes = log_uniform_sizes(31, nm, 8, 64)
+    kinds = np.where(synth.uniform(32, nm, 0, 1) < 0.5, txa.HEX8, txa.TET4)
+    eng = txa.Engine(0)
+    t0 = time.time()
+    ms, per = [], []
+    for i in range(nm):
+        n = int(sizes[i])
+        m = eng.brick(int(kinds[i]), n, n, n, synth.brick_coords(n, n, n))
+        nd = m.build_dofmap(1)
+        nz = m.build_graph()
+        pl = m.point_lids()[:, 0]
+        bc = torch.as_tensor(np.unique(pl[np.arange(pl.size) % (n + 1) == 0]).astype(np.int32), device="cuda")
+        k = torch.as_tensor(synth.lognormal(1000 + i, m.nelem, 0.5), device="cuda")
+        x = torch.as_tensor(synth.uniform(5000 + i, nd, -1, 1), device="cuda")
+        bufs = {p: (torch.empty(nd, device="cuda", dtype=torch.float64),
+                    torch.empty(nz, device="cuda", dtype=torch.float64)) for p in (txa.FP32, txa.FP64)}
+        ms.append(m)
+        per.append(dict(coef=k, x=x, bc=bc, bufs=bufs))
+    nel = sum(m.nelem for m in ms)
+    cells = int(np.sum(sizes.astype(np.int64) ** 3))
+    log(f"setup {time.time() - t0:.1f}s: {nm} meshes, {cells} cells, {nel} elements")
+
+    def kw(i, prec):
+        f, v = per[i]["bufs"][prec]
+        return dict(physics=txa.DIFFUSION, coef=per[i]["coef"], x=per[i]["x"], f=f, vals=v, src=(1.0,),
+                    bc_lids=per[i]["bc"])
+
+    def time_it(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    out = {}
+    for prec, name in ((txa.FP32, "fp32_batch"), (txa.FP64, "fp64_batch")):
+        b = txa.Batch(eng, ms, precision=prec)
+        args_list = [kw(i, prec) for i in range(nm)]
+        tuned_ms = b.tune(args_list, reps=3)
+        info = b.info()
+        t = time_it(lambda: b.assemble(args_list, use_graph=True), args.steps, 2)
+        out[name] = {"ms_per_step": t, "elements_per_s": nel / (t / 1e3), "tuner_best_ms": tuned_ms,
+                     "slab": info["slab"], "grid": info["grid"], "work_items": info["items"]}
+        log(f"{name}: {t:.3f} ms (slab {info['slab']}, grid {info['grid']})")
+        b.destroy()
+
+    def plain():
+        for i, m in enumerate(ms):
+            m.assemble(precision=txa.FP64, **kw(i, txa.FP64))
+    t = time_it(plain, max(2, args.steps // 4), 1)
+    out["fp64_per_mesh"] = {"ms_per_step": t, "elements_per_s": nel / (t / 1e3)}
+    log(f"fp64 one launch per mesh: {t:.3f} ms")
+    # FP32 batch vs FP64, per mesh
+    worst = {"R": 0.0, "J": 0.0}
+    for i in range(nm):
+        (f32, v32), (f64, v64) = per[i]["bufs"][txa.FP32], per[i]["bufs"][txa.FP64]
+        worst["R"] = max(worst["R"], float((f32 - f64).abs().max() / f64.abs().max()))
+        worst["J"] = max(worst["J"], float((v32 - v64).abs().max() / v64.abs().max()))
+    value = out["fp32_batch"]["elements_per_s"]
+    line = {"metric": "elements/s (residual+Jacobian), batched", "value": value, "unit": "elements/s",
+            "n_gpus": 1, "steps": args.steps, "ms_per_step": out["fp32_batch"]["ms_per_step"],
+            "higher_is_better": True, "dtype": "f32 element integration, f64 geometry and scatter",
+            "data": "synthetic",
+            "config": {"workload": f"{nm} Poisson meshes, n^3 cells with n log-uniform in [8, 64], hex8/tet4 50/50",
+                       "cells": cells, "elements": nel, "hex8_meshes": int(np.sum(kinds == txa.HEX8))},
+            "forms": out, "max_rel_err_fp32_vs_fp64_per_mesh": worst,
+            "speedup_vs_fp64_per_mesh": value / out["fp64_per_mesh"]["elements_per_s"]}
+    print(json.dumps(line), flush=True)
+
+
+def bench_serve(args):
+    """Serving: Newmark steps of transient hex8 elasticity (beta 0.25, gamma 0.5,
+    Rayleigh a 0.01, b 0.001, dt 1e-3) on a pool of resident meshes with n^3 cells,
+    n log-uniform in [16, 128]; Poisson arrivals at --qps; each request carries a
+    seeded state U[-1e-3, 1e-3] (generated on the device).  Reports p50/p99
+    submit-to-completion latency and throughput."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    import tianxin_amd as txa
+    from tianxin_amd import synth
+
+    eng = txa.Engine(0)
+    sizes = log_uniform_sizes(41, args.pool, 16, 128)
+    t0 = time.time()
+    pool = []
+    for i, n in enumerate(sizes):
+        n = int(n)
+        m = eng.brick(txa.HEX8, n, n, n, synth.brick_coords(n, n, n))
+        nd = m.build_dofmap(3)
+        m.build_graph()
+        pl = m.point_lids()
+        bc = torch.as_tensor(np.unique(pl[np.arange(pl.shape[0]) % (n + 1) == 0].ravel()).astype(np.int32),
+                             device="cuda")
+        E = torch.as_tensor(synth.lognormal(2000 + i, m.nelem, 0.3), device="cuda")
+        pool.append((m, E, bc))
+    log(f"setup {time.time() - t0:.1f}s: pool n = {list(map(int, sizes))}")
+    pick = np.minimum((synth.uniform(43, args.requests, 0, 1) * len(pool)).astype(int), len(pool) - 1)
+    gaps = -np.log(np.maximum(synth.uniform(44, args.requests, 0, 1), 1e-12)) / args.qps
+    states = []
+    g = torch.Generator(device="cuda")
+    for r in range(args.requests):
+        m = pool[pick[r]][0]
+        g.manual_seed(7000 + r)
+        states.append([torch.rand(m.ndofs, generator=g, device="cuda", dtype=torch.float64) * 2e-3 - 1e-3
+                       for _ in range(3)])
+    torch.cuda.synchronize()
+    srv = txa.Server(eng, [p[0] for p in pool], hbm_budget=int(args.budget_gb * 2 ** 30), slots=args.slots)
+    rids = [None] * args.requests
+    ready = threading.Semaphore(0)
+
+    def collect():
+        for r in range(args.requests):
+            ready.acquire()
+            st = txa.QUEUED
+            while st != txa.DONE:
+                st, _, _ = srv.poll(rids[r], 60.0)
+            srv.release(rids[r])
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t_start = time.perf_counter()
+    t_next = t_start
+    for r in range(args.requests):
+        t_next += gaps[r]
+        delay = t_next - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        m, E, bc = pool[pick[r]]
+        u, v, a = states[r]
+        rids[r] = srv.submit(m, physics=txa.ELASTICITY, coef=E, u=u, v=v, a=a, dt=1e-3, newmark=(0.25, 0.5),
+                             nu=0.3, rho=1.0, rayleigh=(0.01, 0.001), bc_lids=bc)
+        ready.release()
+    th.join()
+    wall = time.perf_counter() - t_start
+    s = srv.stats()
+    srv.destroy()
+    nel = sum(int(pool[i][0].nelem) for i in pick)
+    line = {"metric": "serving: p50/p99 latency and elements/s (transient elasticity residual+Jacobian)",
+            "value": s["elements_per_s"], "unit": "elements/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f64", "data": "synthetic",
+            "config": {"workload": "hex8 transient elasticity (Newmark), n^3 cells, n log-uniform in [16, 128]",
+                       "pool_n": list(map(int, sizes)), "requests": args.requests, "qps": args.qps,
+                       "slots": args.slots, "hbm_budget_gb": args.budget_gb, "dt": 1e-3},
+            "latency_ms": {"p50": s["p50_ms"], "p99": s["p99_ms"], "mean": s["mean_ms"], "max": s["max_ms"]},
+            "requests_per_s": s["requests_per_s"], "elements_served": nel, "wall_s": wall,
+            "peak_inflight": s["peak_inflight"], "peak_arena_bytes": s["peak_bytes"]}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,6 +350,14 @@ def main():
     ap.add_argument("--summarize-profiles", nargs=3, metavar=("STATS_DIR", "FETCH_DIR", "WRITE_DIR"),
                     help="CPU only: turn rocprofv3 csv output (kernel stats, FETCH_SIZE pass, WRITE_SIZE pass) "
                          "into the summaries under profiles/")
+    ap.add_argument("--workload", choices=("single", "batch", "serve"), default="single",
+                    help="single: the 256^3 Q1 line (default); batch / serve: the other two forms")
+    ap.add_argument("--meshes", type=int, default=512)
+    ap.add_argument("--pool", type=int, default=8)
+    ap.add_argument("--requests", type=int, default=48)
+    ap.add_argument("--qps", type=float, default=20.0)
+    ap.add_argument("--slots", type=int, default=4)
+    ap.add_argument("--budget-gb", type=float, default=64.0)
     args = ap.parse_args()
     if args.summarize_profiles:
         summarize_profiles(*args.summarize_profiles)
@@ -182,6 +366,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                           "(multi-GPU sharding is a design note, not implemented)"}))
         sys.exit(2)
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     import numpy as np
     import torch

@@ -12,6 +12,7 @@ import torch
 import oracle as O
 import tianxin_amd as txa
 from tianxin_amd import synth
+from tests_support_q2 import q2_positions
 
 pytestmark = pytest.mark.gpu
 
@@ -211,4 +212,49 @@ def test_cfg1_poisson_32_cube(engine):
     assert np.all(f[bc] == x[bc])   # g = 0
     rp, col = c.m.graph()
     assert np.array_equal(rp, c.rp) and np.array_equal(col, c.col)
+    c.m.destroy()
+
+
+@pytest.mark.parametrize("topo,n,precision,tol", [(txa.HEX8, (7, 5, 6), txa.FP64, TOL64),
+                                                  (txa.HEX8, (7, 5, 6), txa.FP32, TOL32),
+                                                  (txa.HEX27, (3, 4, 2), txa.FP64, TOL64),
+                                                  (txa.HEX27, (3, 4, 2), txa.FP32, TOL32),
+                                                  (txa.HEX27, (3, 4, 2), txa.SPLIT_BF16, TOL32)])
+def test_elasticity_kernel_transient(engine, topo, n, precision, tol):
+    """Element-centric elasticity kernel: jittered mesh, Newmark-type seeds, Rayleigh
+    damping, body force and Dirichlet rows, in every integration precision."""
+    c = Case(engine, topo, *n, 3, jitter=0.1, seed=8)
+    f, v, fr, vr = run_both(engine, c, txa.ELASTICITY, precision=precision, transient=True, bc=True, nu=0.3,
+                            rho=2.0, src=(0.5, -1.0, 0.25), rayleigh=(0.01, 0.001), alpha=2000.0, beta=1.0,
+                            gamma2=4.0e6)
+    ef, ev = rel_err(f, fr), rel_err(v, vr)
+    print(f"topo {topo} precision {precision}: R {ef:.2e} J {ev:.2e}")
+    assert ef <= tol and ev <= tol
+    c.m.destroy()
+
+
+@pytest.mark.parametrize("precision", [txa.FP64, txa.SPLIT_BF16])
+def test_hex27_rigid_motion_is_stress_free(engine, precision):
+    """K (rigid translation + rotation) = 0 on every row: the Q2 space holds linear
+    fields exactly on the trilinear geometry, so R = K u vanishes (no load, no BCs)."""
+    c = Case(engine, txa.HEX27, 4, 3, 5, 3, jitter=0.1, seed=12)
+    pl = c.m.point_lids()            # [points][3]
+    n = (2 * 4 + 1, 2 * 3 + 1, 2 * 5 + 1)
+    P = q2_positions(c.X, 4, 3, 5)   # Q2 nodes: trilinear map of the half-cell lattice
+    w = np.array([0.2, -0.3, 0.1])
+    u_pt = np.array([1.0, -2.0, 0.5]) + np.cross(w, P)
+    x = np.zeros(c.ndofs)
+    for k in range(3):
+        x[pl[:, k]] = u_pt[:, k]
+    E = synth.lognormal(3, c.m.nelem, 0.3)
+    f = torch.empty(c.ndofs, device="cuda", dtype=torch.float64)
+    v = torch.empty(c.nnz, device="cuda", dtype=torch.float64)
+    c.m.assemble(physics=txa.ELASTICITY, precision=precision, coef=dev(E), x=dev(x), f=f, vals=v, nu=0.3)
+    torch.cuda.synchronize()
+    v = v.cpu().numpy()
+    f = f.cpu().numpy()
+    scale = np.max(np.abs(v)) * np.max(np.abs(x))
+    tol = 1e-13 if precision == txa.FP64 else 1e-5   # 81 products per row, each good to ~1e-7
+    assert np.max(np.abs(f)) <= tol * scale
+    assert n[0] * n[1] * n[2] == pl.shape[0]
     c.m.destroy()

@@ -210,9 +210,11 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s) {
 
   if ((m.topo == TXA_HEX8 || m.topo == TXA_TET4) && a.physics == TXA_DIFFUSION && a.precision != TXA_SPLIT_BF16) {
     launch_q1_hex8_diffusion(m, a, s);
+  } else if (elastic_supported(m, a)) {
+    launch_elastic(m, a, s);
   } else {
     TXA_REQUIRE(a.precision == TXA_FP64, TXA_ERR_UNSUPPORTED,
-                "reduced-precision integration is implemented for hex8 / tet4 diffusion");
+                "reduced-precision integration is implemented for hex8 / tet4 diffusion and 3-field elasticity");
     TXA_REQUIRE(m.ref.p != nullptr, TXA_ERR_LOGIC, "reference tables missing");
     RefElem r;
     refelem_build(r, m.topo);

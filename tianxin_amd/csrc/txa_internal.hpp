@@ -85,6 +85,8 @@ struct Mesh {
   DevBuf<int32_t> col;
   int perm_width = 0;         // bytes per row of the stencil slot map (27*nfields rounded up to 16)
   DevBuf<uint8_t> perm;       // [ndofs][perm_width] position of each stencil neighbour in the row, 255 = absent
+  int nperm_w = 0;            // entries per node of nperm
+  DevBuf<uint16_t> nperm;     // 3-field HEX8 / HEX27: [nodes][nperm_w] position of (neighbour, field 0) in the node's rows
   DevBuf<double> ref;         // reference-element tables of the topology
 };
 
@@ -130,6 +132,10 @@ void exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t 
 // assembly (txa_assemble.hip)
 void upload_refelem(Mesh &m, DevBuf<double> &tab, hipStream_t s);
 void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s);
+// element-centric elasticity (txa_elastic.hip)
+void node_perm_build(Mesh &m, hipStream_t s);
+bool elastic_supported(const Mesh &m, const txa_assembly_args &a);
+void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s);
 // Q1 diffusion fast path and batches (txa_q1_hex8.hip)
 void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s);
 Batch *batch_create(const std::vector<Mesh *> &meshes, const std::vector<int> &ids, int precision, bool transient);

@@ -462,6 +462,7 @@ void graph_build(Mesh &m, hipStream_t s) {
                        m.rowptr.p, m.col.p, m.perm.p);
     TXA_HIP(hipGetLastError());
   }
+  node_perm_build(m, s);   // neighbour positions for the 3-field elasticity kernels
   TXA_HIP(hipStreamSynchronize(s));
   m.has_graph = true;
 }
