@@ -254,6 +254,101 @@ def bench_batch(args):
     print(json.dumps(line), flush=True)
 
 
+Q2_FLOP_PER_ELEM = 2 * 81 * 162 * 81 + 2 * 162 * 6 * 81   # dense B^T (W B), B in R^{162 x 81}
+Q2_BYTES_PER_ELEM = 55.7e3                                 # compulsory: CRS values + column indices per element
+
+
+def bench_q2(args):
+    """Q2 hex27 linear elasticity, 3 DOFs/node, 81x81 element matrices, 3x3x3 Gauss:
+    n^3 cells (default 64), E lognormal (sigma 0.3) per element, nu = 0.3, interior
+    vertices jittered by U[-0.1h, 0.1h], u ~ U[-1, 1], u = 0 on x = 0.  Times the
+    split-BF16 MFMA form, the FP32 form and the plain FP64 path; reports their errors
+    against FP64 and a full-size rigid-motion check (R(rigid u) = 0)."""
+    import numpy as np
+    import torch
+
+    import tianxin_amd as txa
+    from tianxin_amd import synth
+
+    n = args.n if args.n != 256 else 64
+    eng = txa.Engine(0)
+    t0 = time.time()
+    X = synth.brick_coords(n, n, n, jitter=0.1, seed=61)
+    m = eng.brick(txa.HEX27, n, n, n, X)
+    nd = m.build_dofmap(3)
+    nnz = m.build_graph()
+    pl = m.point_lids()
+    px = 2 * n + 1
+    bc = torch.as_tensor(np.unique(pl[np.arange(pl.shape[0]) % px == 0].ravel()).astype(np.int32), device="cuda")
+    E = torch.as_tensor(synth.lognormal(62, m.nelem, 0.3), device="cuda")
+    x = torch.as_tensor(synth.uniform(63, nd, -1, 1), device="cuda")
+    nel = m.nelem
+    log(f"setup {time.time() - t0:.1f}s: {nel} elements, {nd} dofs, {nnz} nnz")
+    out, res = {}, {}
+    for prec, name in ((txa.SPLIT_BF16, "split_bf16"), (txa.FP32, "fp32"), (txa.FP64, "fp64")):
+        f = torch.empty(nd, device="cuda", dtype=torch.float64)
+        v = torch.empty(nnz, device="cuda", dtype=torch.float64)
+
+        def step():
+            m.assemble(physics=txa.ELASTICITY, precision=prec, coef=E, x=x, f=f, vals=v, nu=0.3, bc_lids=bc)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+        ev[0].record()
+        for i in range(args.steps):
+            step()
+            ev[i + 1].record()
+        torch.cuda.synchronize()
+        per = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)])
+        ms = float(per.mean())
+        out[name] = {"ms_per_step": ms, "elements_per_s": nel / (ms / 1e3),
+                     "dense_tflops": Q2_FLOP_PER_ELEM * nel / (ms / 1e3) / 1e12,
+                     "hbm_frac": Q2_BYTES_PER_ELEM * nel / (ms / 1e3) / HBM_PEAK,
+                     "p50": float(np.percentile(per, 50)), "p99": float(np.percentile(per, 99))}
+        res[name] = (f, v)
+        log(f"{name}: {ms:.3f} ms/step")
+    f64, v64 = res["fp64"]
+
+    def rel(a, b):
+        return float((a - b).abs().max() / b.abs().max())
+
+    def ent(a, b, tau=1e-3):
+        mask = b.abs() > tau * b.abs().max()
+        return float(((a - b).abs()[mask] / b.abs()[mask]).max())
+    errs = {k: {"R_normwise": rel(res[k][0], f64), "J_normwise": rel(res[k][1], v64),
+                "R_entrywise_over_1e-3_max": ent(res[k][0], f64), "J_entrywise_over_1e-3_max": ent(res[k][1], v64)}
+            for k in ("split_bf16", "fp32")}
+    del res
+    # size-independent property: a rigid motion is stress free (no load, no Dirichlet rows)
+    P = synth.q2_positions(X, n, n, n)
+    u = np.array([1.0, -2.0, 0.5]) + np.cross(np.array([0.2, -0.3, 0.1]), P)
+    xr = np.zeros(nd)
+    for k in range(3):
+        xr[pl[:, k]] = u[:, k]
+    xr = torch.as_tensor(xr, device="cuda")
+    rigid = {}
+    for prec, name in ((txa.SPLIT_BF16, "split_bf16"), (txa.FP64, "fp64")):
+        f = torch.empty(nd, device="cuda", dtype=torch.float64)
+        v = torch.empty(nnz, device="cuda", dtype=torch.float64)
+        m.assemble(physics=txa.ELASTICITY, precision=prec, coef=E, x=xr, f=f, vals=v, nu=0.3)
+        torch.cuda.synchronize()
+        rigid[name] = float(f.abs().max() / (v.abs().max() * xr.abs().max()))
+        del f, v
+    best = out["split_bf16"]
+    line = {"metric": "elements/s (residual+Jacobian), Q2 elasticity", "value": best["elements_per_s"],
+            "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": best["ms_per_step"], "higher_is_better": True,
+            "dtype": "split-bf16 MFMA (fp32 accumulate) element integration, f64 geometry and scatter",
+            "data": "synthetic",
+            "config": {"workload": f"Q2 hex27 linear elasticity {n}^3, 3 dofs/node, jitter 0.1h interior vertices",
+                       "elements": nel, "dofs": nd, "nnz": nnz, "quadrature": "3x3x3 Gauss", "E": "lognormal sigma=0.3",
+                       "nu": 0.3, "dirichlet": "u=0 on x=0"},
+            "forms": out, "max_rel_err_vs_fp64": errs, "rigid_motion_residual_rel": rigid,
+            "speedup_split_vs_fp64": best["elements_per_s"] / out["fp64"]["elements_per_s"]}
+    print(json.dumps(line), flush=True)
+
+
 def bench_serve(args):
     """Serving: Newmark steps of transient hex8 elasticity (beta 0.25, gamma 0.5,
     Rayleigh a 0.01, b 0.001, dt 1e-3) on a pool of resident meshes with n^3 cells,
@@ -350,7 +445,7 @@ def main():
     ap.add_argument("--summarize-profiles", nargs=3, metavar=("STATS_DIR", "FETCH_DIR", "WRITE_DIR"),
                     help="CPU only: turn rocprofv3 csv output (kernel stats, FETCH_SIZE pass, WRITE_SIZE pass) "
                          "into the summaries under profiles/")
-    ap.add_argument("--workload", choices=("single", "batch", "serve"), default="single",
+    ap.add_argument("--workload", choices=("single", "batch", "serve", "q2"), default="single",
                     help="single: the 256^3 Q1 line (default); batch / serve: the other two forms")
     ap.add_argument("--meshes", type=int, default=512)
     ap.add_argument("--pool", type=int, default=8)
@@ -370,6 +465,8 @@ def main():
         return bench_batch(args)
     if args.workload == "serve":
         return bench_serve(args)
+    if args.workload == "q2":
+        return bench_q2(args)
 
     import numpy as np
     import torch

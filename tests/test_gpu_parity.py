@@ -12,7 +12,6 @@ import torch
 import oracle as O
 import tianxin_amd as txa
 from tianxin_amd import synth
-from tests_support_q2 import q2_positions
 
 pytestmark = pytest.mark.gpu
 
@@ -240,7 +239,7 @@ def test_hex27_rigid_motion_is_stress_free(engine, precision):
     c = Case(engine, txa.HEX27, 4, 3, 5, 3, jitter=0.1, seed=12)
     pl = c.m.point_lids()            # [points][3]
     n = (2 * 4 + 1, 2 * 3 + 1, 2 * 5 + 1)
-    P = q2_positions(c.X, 4, 3, 5)   # Q2 nodes: trilinear map of the half-cell lattice
+    P = synth.q2_positions(c.X, 4, 3, 5)   # Q2 nodes: trilinear map of the half-cell lattice
     w = np.array([0.2, -0.3, 0.1])
     u_pt = np.array([1.0, -2.0, 0.5]) + np.cross(w, P)
     x = np.zeros(c.ndofs)

@@ -10,15 +10,17 @@
 // Jacobian replaced by the explicit tangent
 //   J_e = beta K + alpha (a_R M + b_R K) + gamma2 M,   R_e = K u + C v + M a - F.
 //
-// One workgroup per element (grid-stride).  Geometry (J, det J, J^-1) is FP64 at
+// One workgroup per element (grid-stride).  The reference element is the
+// tensor product of 1D Gauss rules and Lagrange bases (tables in LDS), so no
+// per-point tables are read from memory.  Geometry (J, det J, J^-1) is FP64 at
 // every quadrature point.  The element stiffness is assembled from the
 // gradient Gram matrix
 //   T[(k,a),(l,b)] = sum_q w_q |det J_q| dN_a/dx_k dN_b/dx_l        (3 NB x 3 NB),
 //   K[(a,i),(b,j)] = lambda T[(i,a),(j,b)] + mu T[(j,a),(i,b)] + delta_ij mu sum_k T[(k,a),(k,b)],
 // i.e. B^T D B for isotropic D written as one symmetric product G^T W G.  T is
-// computed in FP64 or FP32 on the VALU, or with bf16 MFMA on a three-way split
-// of each FP32 operand (x = x0 + x1 + x2, every x_s in bf16) keeping the six
-// cross products of order <= 2^-16, accumulated in FP32.  The scatter adds FP64
+// computed in FP64 or FP32 on the VALU, or as T = O^T O with O = sqrt(w |det J|) G
+// by bf16 MFMA on a three-way split of each FP32 operand (x = x0 + x1 + x2, every
+// x_s in bf16) keeping the six cross products of order <= 2^-16, accumulated in FP32.  The scatter adds FP64
 // values into the CRS with hardware FP64 atomics; the row position of every
 // (node, neighbour) pair comes from a per-node table built with the graph.
 #include <type_traits>
@@ -39,7 +41,6 @@ struct ElasticParams {
   const int32_t *elem_lid;      // [E][3 NB]: node-major, field-minor
   const uint16_t *nperm;        // [nodes][nperm_w]: position of (neighbour, field 0) in the node's rows
   int nperm_w;
-  const double *tab;            // N[nb][nq] | dN[nb][nq][3] | gdN[8][nq][3] | qw[nq]
   const double *coef;           // Young's modulus per element
   double nu, rho, src[3], ra, rb;
   const double *x, *xdot, *xddot;
@@ -93,19 +94,30 @@ template <int NB, int NQ, int MODE>
 struct Smem {
   using D = Dims<NB>;
   using TReal = typename std::conditional<MODE == MODE_F64, double, float>::type;
+  // 1D tables on [-1, 1]: Gauss rule, field basis (2 or 3 nodes) and the linear
+  // geometry basis, values and derivatives at each Gauss point
+  double bL[3][3], bdL[3][3], gL[2][3], gdL[2][3], w1[3];
+  int8_t off[NB][3];         // lattice offset of each element node
   double X[8][3];
   double Ji[NQ][9];
   double det[NQ];
-  double wm[NQ];
+  double wm[NQ];             // w det J (the sign of the first point is applied where used)
+  double sw[NQ];             // sqrt(w |det J|)
   double u[D::ND], v[D::ND], a[D::ND], r[D::ND];
   int32_t L[D::ND];
   int64_t row[D::ND];
-  uint16_t pos[NB][NB];
+  uint16_t pos[NB][NB];      // row position of (node b, field 0) in node a's rows
   TReal M[NB][NB];
-  TReal T[D::ND][D::ND + 1];
-  // gradients (VALU modes) or the bf16 operand splits (MFMA mode)
-  typename std::conditional<MODE == MODE_SPLIT, short, TReal>::type
-      G[MODE == MODE_SPLIT ? 2 * 3 * D::MP * D::KP : NQ * 3 * NB];
+  // VALU modes: gradients G and the Gram matrix T.  MFMA mode: the bf16 operand
+  // parts, then T over the same bytes once the products are in registers.
+  union {
+    struct {
+      TReal T[D::ND][D::ND + 1];
+      TReal G[NQ * 3 * NB];
+    } v;
+    short ops[3 * D::MP * D::KP];
+    float T32[D::ND][D::ND + 1];
+  } w;
 };
 
 template <int NB, int NQ, int BS, int MODE>
@@ -113,19 +125,49 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
   using D = Dims<NB>;
   using TReal = typename Smem<NB, NQ, MODE>::TReal;
   constexpr int ND = D::ND;
+  constexpr int Q1 = NQ == 27 ? 3 : 2;        // Gauss points per axis
+  // final phase: lane (a, b) forms the 3x3 block of node pair (a, b); a group of GW
+  // lanes shares a and runs over b
+  constexpr int GW = NB == 27 ? 32 : 8;
+  constexpr int GPW = 64 / GW;
+  constexpr int AP = (BS / 64) * GPW;          // nodes a per pass
+  constexpr int NPASS = (NB + AP - 1) / AP;
   __shared__ Smem<NB, NQ, MODE> sm;
-  const double *tN = p.tab;
-  const double *tdN = tN + NB * NQ;
-  const double *tgdN = tdN + NB * NQ * 3;
-  const double *tqw = tgdN + 8 * NQ * 3;
   const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int gb = lane % GW, ga = wave * GPW + lane / GW;
   const bool tr = p.transient && p.xdot;
   const bool tra = tr && p.xddot;
+  const bool has_src = p.src[0] != 0.0 || p.src[1] != 0.0 || p.src[2] != 0.0;
 
-  if constexpr (MODE == MODE_SPLIT) {
-    // zero the operand padding once: rows >= ND and quadrature columns >= NQ stay zero
-    for (int i = t; i < 2 * 3 * D::MP * D::KP; i += BS) sm.G[i] = 0;
+  if (t == 0) {
+    // Gauss-Legendre on [-1, 1]; Lagrange bases on the nodes {-1, 1} or {-1, 0, 1}
+    const double g3 = 0.77459666924148337704;   // sqrt(3/5)
+    const double g2 = 0.57735026918962576451;   // 1/sqrt(3)
+    for (int q = 0; q < Q1; ++q) {
+      const double xi = Q1 == 3 ? (q - 1) * g3 : (2 * q - 1) * g2;
+      sm.w1[q] = Q1 == 3 ? (q == 1 ? 8.0 / 9.0 : 5.0 / 9.0) : 1.0;
+      sm.gL[0][q] = 0.5 * (1.0 - xi);
+      sm.gL[1][q] = 0.5 * (1.0 + xi);
+      sm.gdL[0][q] = -0.5;
+      sm.gdL[1][q] = 0.5;
+      if (NB == 27) {
+        sm.bL[0][q] = 0.5 * xi * (xi - 1.0);
+        sm.bL[1][q] = 1.0 - xi * xi;
+        sm.bL[2][q] = 0.5 * xi * (xi + 1.0);
+        sm.bdL[0][q] = xi - 0.5;
+        sm.bdL[1][q] = -2.0 * xi;
+        sm.bdL[2][q] = xi + 0.5;
+      } else {
+        sm.bL[0][q] = sm.gL[0][q];
+        sm.bL[1][q] = sm.gL[1][q];
+        sm.bdL[0][q] = -0.5;
+        sm.bdL[1][q] = 0.5;
+      }
+    }
   }
+  for (int i = t; i < NB * 3; i += BS) sm.off[i / 3][i % 3] = NB == 8 ? kHexV[i / 3][i % 3] : kQ2[i / 3][i % 3];
+  __syncthreads();
 
   for (int64_t e = blockIdx.x; e < p.nelem; e += gridDim.x) {
     // ---- gather: vertices, LIDs, row starts, state
@@ -141,21 +183,34 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
       sm.u[i] = p.x[L];
       sm.v[i] = tr ? p.xdot[L] : 0.0;
       sm.a[i] = tra ? p.xddot[L] : 0.0;
-      sm.r[i] = 0.0;
     }
     __syncthreads();
-    // ---- geometry (FP64) and the row positions of the element's node pairs
+    // ---- row positions of this lane's node pairs (loads in flight until the last phase)
+    uint32_t posr[NPASS];
+#pragma unroll
+    for (int k = 0; k < NPASS; ++k) {
+      const int a = ga + k * AP;
+      posr[k] = 0xffffu;
+      if (a < NB && gb < NB) {
+        const int s = (sm.off[gb][0] - sm.off[a][0] + D::R) + D::S * (sm.off[gb][1] - sm.off[a][1] + D::R) +
+                      D::S * D::S * (sm.off[gb][2] - sm.off[a][2] + D::R);
+        posr[k] = p.nperm[static_cast<int64_t>(sm.L[3 * a] / 3) * p.nperm_w + s];
+      }
+    }
+    // ---- geometry (FP64)
     if (t < NQ) {
-      double J[3][3];
+      const int qx = t % Q1, qy = (t / Q1) % Q1, qz = t / (Q1 * Q1);
+      double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 #pragma unroll
-      for (int i = 0; i < 3; ++i)
+      for (int vtx = 0; vtx < 8; ++vtx) {
+        const int ox = kHexV[vtx][0], oy = kHexV[vtx][1], oz = kHexV[vtx][2];
+        const double gx = sm.gL[ox][qx], gy = sm.gL[oy][qy], gz = sm.gL[oz][qz];
+        const double d[3] = {sm.gdL[ox][qx] * gy * gz, gx * sm.gdL[oy][qy] * gz, gx * gy * sm.gdL[oz][qz]};
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
-          double s = 0.0;
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-          for (int vtx = 0; vtx < 8; ++vtx) s += sm.X[vtx][i] * tgdN[(vtx * NQ + t) * 3 + j];
-          J[i][j] = s;
-        }
+          for (int j = 0; j < 3; ++j) J[i][j] += sm.X[vtx][i] * d[j];
+      }
       const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
       const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
       const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
@@ -172,61 +227,82 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
       Ji[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
       Ji[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
       sm.det[t] = det;
+      sm.wm[t] = sm.w1[qx] * sm.w1[qy] * sm.w1[qz] * det;
     }
-    for (int i = t; i < NB * NB; i += BS) {
-      const int a = i / NB, b = i % NB;
-      const int8_t(*off)[3] = NB == 8 ? kHexV : kQ2;
-      const int s = (off[b][0] - off[a][0] + D::R) + D::S * (off[b][1] - off[a][1] + D::R) +
-                    D::S * D::S * (off[b][2] - off[a][2] + D::R);
-      sm.pos[a][b] = p.nperm[static_cast<int64_t>(sm.L[3 * a] / 3) * p.nperm_w + s];
+    if constexpr (MODE == MODE_SPLIT) {
+      // operand rows >= ND are zero (T covered them in the previous element)
+      constexpr int PADV = (D::MP - ND) * D::KP / 8;   // 16-byte vectors per part
+      for (int i = t; i < 3 * PADV; i += BS) {
+        const int s = i / PADV, o = i % PADV;
+        reinterpret_cast<uint4 *>(sm.w.ops + (s * D::MP + ND) * D::KP)[o] = make_uint4(0, 0, 0, 0);
+      }
     }
     __syncthreads();
-    if (t < NQ) sm.wm[t] = tqw[t] * sm.det[t] * (sm.det[0] < 0.0 ? -1.0 : 1.0);
+    // measure w |det J| with the sign of the first point (Intrepid2 computeCellMeasure)
+    const double sg = sm.det[0] < 0.0 ? -1.0 : 1.0;
+    if (t < NQ) sm.sw[t] = sqrt(sg * sm.wm[t]);
+    // mass matrix (inertia and Rayleigh terms only)
+    if (tr)
+      for (int i = t; i < NB * NB; i += BS) {
+        const int a = i / NB, b = i % NB;
+        double s = 0.0;
+        for (int q = 0; q < NQ; ++q) {
+          const int qx = q % Q1, qy = (q / Q1) % Q1, qz = q / (Q1 * Q1);
+          s += sm.wm[q] * (sm.bL[sm.off[a][0]][qx] * sm.bL[sm.off[a][1]][qy] * sm.bL[sm.off[a][2]][qz]) *
+               (sm.bL[sm.off[b][0]][qx] * sm.bL[sm.off[b][1]][qy] * sm.bL[sm.off[b][2]][qz]);
+        }
+        sm.M[a][b] = static_cast<TReal>(sg * s);
+      }
     __syncthreads();
-    // ---- physical gradients at the quadrature points, mass matrix
-    for (int i = t; i < NQ * NB; i += BS) {
+    // ---- physical gradients at the quadrature points (MFMA mode: also the zero columns q >= NQ)
+    constexpr int QG = MODE == MODE_SPLIT ? D::KP : NQ;
+    for (int i = t; i < QG * NB; i += BS) {
       const int q = i / NB, a = i % NB;
+      if (MODE == MODE_SPLIT && q >= NQ) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+          for (int s = 0; s < 3; ++s) sm.w.ops[(s * D::MP + k * NB + a) * D::KP + q] = 0;
+        continue;
+      }
+      const int qx = q % Q1, qy = (q / Q1) % Q1, qz = q / (Q1 * Q1);
+      const int ax = sm.off[a][0], ay = sm.off[a][1], az = sm.off[a][2];
+      const double lx = sm.bL[ax][qx], ly = sm.bL[ay][qy], lz = sm.bL[az][qz];
+      const double dn[3] = {sm.bdL[ax][qx] * ly * lz, lx * sm.bdL[ay][qy] * lz, lx * ly * sm.bdL[az][qz]};
       const double *Ji = sm.Ji[q];
-      const double *dn = tdN + (a * NQ + q) * 3;
+      const double sq = MODE == MODE_SPLIT ? sm.sw[q] : 1.0;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const double g = Ji[0 * 3 + k] * dn[0] + Ji[1 * 3 + k] * dn[1] + Ji[2 * 3 + k] * dn[2];
         if constexpr (MODE == MODE_SPLIT) {
-          // A operand [m = (k,a)][q] = g; B operand [n = (k,a)][q] = w_q g; three bf16 parts each
+          // operand [m = (k,a)][q] = sqrt(w |det J|) g in three bf16 parts: T = O^T O
           short s0, s1, s2;
+          split3(static_cast<float>(sq * g), s0, s1, s2);
           const int m = k * NB + a;
-          short *A = sm.G, *B = sm.G + 3 * D::MP * D::KP;
-          split3(static_cast<float>(g), s0, s1, s2);
-          A[(0 * D::MP + m) * D::KP + q] = s0;
-          A[(1 * D::MP + m) * D::KP + q] = s1;
-          A[(2 * D::MP + m) * D::KP + q] = s2;
-          split3(static_cast<float>(sm.wm[q] * g), s0, s1, s2);
-          B[(0 * D::MP + m) * D::KP + q] = s0;
-          B[(1 * D::MP + m) * D::KP + q] = s1;
-          B[(2 * D::MP + m) * D::KP + q] = s2;
+          sm.w.ops[(0 * D::MP + m) * D::KP + q] = s0;
+          sm.w.ops[(1 * D::MP + m) * D::KP + q] = s1;
+          sm.w.ops[(2 * D::MP + m) * D::KP + q] = s2;
         } else {
-          sm.G[(q * 3 + k) * NB + a] = static_cast<TReal>(g);
+          sm.w.v.G[(q * 3 + k) * NB + a] = static_cast<TReal>(g);
         }
       }
-    }
-    for (int i = t; i < NB * NB; i += BS) {
-      const int a = i / NB, b = i % NB;
-      double s = 0.0;
-      for (int q = 0; q < NQ; ++q) s += sm.wm[q] * tN[a * NQ + q] * tN[b * NQ + q];
-      sm.M[a][b] = static_cast<TReal>(s);
     }
     __syncthreads();
     // ---- gradient Gram matrix T
     if constexpr (MODE == MODE_SPLIT) {
       constexpr int NT = D::MP / 16;                 // 16x16 tiles per side
       constexpr int NTILE = NT * (NT + 1) / 2;       // upper triangle
-      const int lane = t & 63, wave = t >> 6;
-      const short *A = sm.G, *B = sm.G + 3 * D::MP * D::KP;
-      for (int ti = wave; ti < NTILE; ti += BS / 64) {
+      constexpr int TPW = (NTILE + BS / 64 - 1) / (BS / 64);
+      const short *O = sm.w.ops;
+      v4f acc[TPW];
+#pragma unroll
+      for (int k = 0; k < TPW; ++k) {
+        acc[k] = v4f{0.f, 0.f, 0.f, 0.f};
+        const int ti = wave + k * (BS / 64);
+        if (ti >= NTILE) continue;
         int tm = 0, rem = ti;
         while (rem >= NT - tm) { rem -= NT - tm; ++tm; }
         const int tn = tm + rem;
-        v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int ks = 0; ks < D::KP / 16; ++ks) {
           const int k0 = ks * 16 + 4 * (lane >> 4);
@@ -234,26 +310,35 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
           v4s a_[3], b_[3];
 #pragma unroll
           for (int s = 0; s < 3; ++s) {
-            a_[s] = *reinterpret_cast<const v4s *>(A + (s * D::MP + ma) * D::KP + k0);
-            b_[s] = *reinterpret_cast<const v4s *>(B + (s * D::MP + nb) * D::KP + k0);
+            a_[s] = *reinterpret_cast<const v4s *>(O + (s * D::MP + ma) * D::KP + k0);
+            b_[s] = *reinterpret_cast<const v4s *>(O + (s * D::MP + nb) * D::KP + k0);
           }
-          // products of order 1, 2^-8 and 2^-16
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[2], b_[0], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[1], b_[1], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[0], b_[2], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[1], b_[0], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[0], b_[1], acc, 0, 0, 0);
-          acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[0], b_[0], acc, 0, 0, 0);
+          // products of order 2^-16, 2^-8 and 1 (smallest first)
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[2], b_[0], acc[k], 0, 0, 0);
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[1], b_[1], acc[k], 0, 0, 0);
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[0], b_[2], acc[k], 0, 0, 0);
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[1], b_[0], acc[k], 0, 0, 0);
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[0], b_[1], acc[k], 0, 0, 0);
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[0], b_[0], acc[k], 0, 0, 0);
         }
+      }
+      __syncthreads();   // every wave has read the operands: T goes over them
+#pragma unroll
+      for (int k = 0; k < TPW; ++k) {
+        const int ti = wave + k * (BS / 64);
+        if (ti >= NTILE) continue;
+        int tm = 0, rem = ti;
+        while (rem >= NT - tm) { rem -= NT - tm; ++tm; }
+        const int tn = tm + rem;
         // D[4 (lane/16) + i][lane % 16] of the tile; off-diagonal tiles also fill the
-        // mirrored tile (the split products are not exactly symmetric, so a diagonal
-        // tile keeps its own entries on both sides)
+        // mirrored tile (the summation order of the split products differs between
+        // (m, n) and (n, m), so a diagonal tile keeps its own entries on both sides)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int rr = tm * 16 + 4 * (lane >> 4) + i, cc = tn * 16 + (lane & 15);
           if (rr < ND && cc < ND) {
-            sm.T[rr][cc] = acc[i];
-            if (tm != tn) sm.T[cc][rr] = acc[i];
+            sm.w.T32[rr][cc] = acc[k][i];
+            if (tm != tn) sm.w.T32[cc][rr] = acc[k][i];
           }
         }
       }
@@ -263,50 +348,87 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
         const int kr = rr / NB, ar = rr % NB, kc = cc / NB, ac = cc % NB;
         TReal s = 0;
         for (int q = 0; q < NQ; ++q)
-          s += static_cast<TReal>(sm.wm[q]) * sm.G[(q * 3 + kr) * NB + ar] * sm.G[(q * 3 + kc) * NB + ac];
-        sm.T[rr][cc] = s;
+          s += static_cast<TReal>(sm.wm[q]) * sm.w.v.G[(q * 3 + kr) * NB + ar] * sm.w.v.G[(q * 3 + kc) * NB + ac];
+        sm.w.v.T[rr][cc] = static_cast<TReal>(sg) * s;
       }
     }
     __syncthreads();
-    // ---- element matrix entries: residual and CRS scatter
+    // ---- 3x3 blocks of node pairs: residual rows, Jacobian values written over the
+    // pair's own T entries (the block of (a, b) depends only on those nine values)
+    //   K_ij = lambda T[(i,a),(j,b)] + mu T[(j,a),(i,b)] + delta_ij mu sum_k T[(k,a),(k,b)]
     const double E = p.coef[e];
     const double lam = E * p.nu / ((1.0 + p.nu) * (1.0 - 2.0 * p.nu));
     const double mu = E / (2.0 * (1.0 + p.nu));
-    for (int idx = t; idx < ND * ND; idx += BS) {
-      const int P = idx / ND, Q = idx % ND;
-      const int a = P / 3, i = P % 3, b = Q / 3, j = Q % 3;
-      double Kv = lam * static_cast<double>(sm.T[i * NB + a][j * NB + b]) +
-                  mu * static_cast<double>(sm.T[j * NB + a][i * NB + b]);
-      double Mv = 0.0;
-      if (i == j) {
-        Kv += mu * (static_cast<double>(sm.T[a][b]) + static_cast<double>(sm.T[NB + a][NB + b]) +
-                    static_cast<double>(sm.T[2 * NB + a][2 * NB + b]));
-        Mv = p.rho * static_cast<double>(sm.M[a][b]);
+    auto Tref = [&](int r, int c) -> TReal & {
+      if constexpr (MODE == MODE_SPLIT) return sm.w.T32[r][c];
+      else return sm.w.v.T[r][c];
+    };
+#pragma unroll
+    for (int k = 0; k < NPASS; ++k) {
+      const int a = ga + k * AP, b = gb;
+      double rr[3] = {0.0, 0.0, 0.0};
+      if (a < NB && b < NB) {
+        double t9[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) t9[i][j] = static_cast<double>(Tref(i * NB + a, j * NB + b));
+        const double trs = t9[0][0] + t9[1][1] + t9[2][2];
+        const double Mab = tr ? p.rho * static_cast<double>(sm.M[a][b]) : 0.0;
+        sm.pos[a][b] = static_cast<uint16_t>(posr[k]);
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) {
+            const double Kv = lam * t9[i][j] + mu * t9[j][i] + (i == j ? mu * trs : 0.0);
+            const double Mv = i == j ? Mab : 0.0;
+            double jv = p.beta * Kv;
+            rr[i] += Kv * sm.u[3 * b + j];
+            if (tr) {
+              const double Cv = p.ra * Mv + p.rb * Kv;
+              rr[i] += Cv * sm.v[3 * b + j];
+              jv += p.alpha * Cv;
+              if (tra) {
+                rr[i] += Mv * sm.a[3 * b + j];
+                jv += p.gamma2 * Mv;
+              }
+            }
+            Tref(i * NB + a, j * NB + b) = static_cast<TReal>(jv);
+          }
       }
-      double rc = Kv * sm.u[Q];
-      double jv = p.beta * Kv;
-      if (tr) {
-        const double Cv = p.ra * Mv + p.rb * Kv;
-        rc += Cv * sm.v[Q];
-        jv += p.alpha * Cv;
-        if (tra) {
-          rc += Mv * sm.a[Q];
-          jv += p.gamma2 * Mv;
-        }
-      }
-      unsafeAtomicAdd(&sm.r[P], rc);
-      if (p.vals) {
-        const uint16_t ps = sm.pos[a][b];
-        if (ps != 0xffffu) unsafeAtomicAdd(p.vals + sm.row[P] + ps + j, jv);
-      }
+      // row residuals: fixed-order reduction over the group's b
+#pragma unroll
+      for (int off = GW / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) rr[i] += __shfl_xor(rr[i], off, 64);
+      if (a < NB && b == 0)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) sm.r[3 * a + i] = rr[i];
     }
+    __syncthreads();
+    // ---- CRS scatter, row-major: consecutive lanes on consecutive entries of a row
+    if (p.vals)
+      for (int idx = t; idx < ND * ND; idx += BS) {
+        const int P = idx / ND, Q = idx % ND;
+        const int a = P / 3, i = P % 3, b = Q / 3, j = Q % 3;
+        const uint16_t ps = sm.pos[a][b];
+        if (ps != 0xffffu)
+          unsafeAtomicAdd(p.vals + sm.row[P] + ps + j, static_cast<double>(Tref(i * NB + a, j * NB + b)));
+      }
     __syncthreads();
     if (p.f)
       for (int i = t; i < ND; i += BS) {
-        const int a = i / 3, k = i % 3;
         double F = 0.0;
-        for (int q = 0; q < NQ; ++q) F += sm.wm[q] * tN[a * NQ + q];
-        unsafeAtomicAdd(p.f + sm.L[i], sm.r[i] - F * p.src[k]);
+        if (has_src) {
+          const int a = i / 3;
+          const int ax = sm.off[a][0], ay = sm.off[a][1], az = sm.off[a][2];
+          for (int q = 0; q < NQ; ++q) {
+            const int qx = q % Q1, qy = (q / Q1) % Q1, qz = q / (Q1 * Q1);
+            F += sm.wm[q] * sm.bL[ax][qx] * sm.bL[ay][qy] * sm.bL[az][qz];
+          }
+          F *= sg * p.src[i % 3];
+        }
+        unsafeAtomicAdd(p.f + sm.L[i], sm.r[i] - F);
       }
     __syncthreads();
   }
@@ -347,7 +469,11 @@ void launch(const ElasticParams &p, hipStream_t s) {
   int dev = 0, ncu = 0;
   TXA_HIP(hipGetDevice(&dev));
   TXA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  const int64_t want = static_cast<int64_t>(ncu) * (BS == 64 ? 16 : 4);
+  // grid-stride over elements with exactly the resident workgroups: a workgroup that
+  // only starts when another ends would stretch the tail by its whole share
+  int per_cu = 0;
+  TXA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_elastic<NB, NQ, BS, MODE>, BS, 0));
+  const int64_t want = static_cast<int64_t>(ncu) * std::max(per_cu, 1);
   const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(p.nelem, want)));
   hipLaunchKernelGGL((k_elastic<NB, NQ, BS, MODE>), dim3(grid), dim3(BS), 0, s, p);
   TXA_HIP(hipGetLastError());
@@ -386,7 +512,6 @@ void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s) {
   p.elem_lid = m.elem_lid.p;
   p.nperm = m.nperm.p;
   p.nperm_w = m.nperm_w;
-  p.tab = m.ref.p;
   p.coef = a.coef;
   p.nu = a.nu; p.rho = a.rho;
   p.src[0] = a.src[0]; p.src[1] = a.src[1]; p.src[2] = a.src[2];
