@@ -60,8 +60,8 @@ def cpu_baseline(n=128, reps=8):
 
 
 COPY_BYTES = 1 << 30
-RING_KERNEL = "k_q1_hex8_ring<1, float, false>"      # HEX8, FP32 integration, steady
-SPEC_KERNEL = "k_q1_hex8_diffusion<1, double, false>"  # HEX8, FP64
+FP32_KERNEL = "k_q1_hex8_diffusion<1, float, false>"   # HEX8, FP32 integration, steady
+FP64_KERNEL = "k_q1_hex8_diffusion<1, double, false>"  # HEX8, FP64
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "q1_hex8_diffusion_fp32_256_traffic.json")
 
 
@@ -124,17 +124,17 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir):
     for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:12]:
         lines.append(f"| `{r['Name'][:110]}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
                      f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} |")
-    ring = [r for r in stats if _match(r["Name"], RING_KERNEL)]
+    ring = [r for r in stats if _match(r["Name"], FP32_KERNEL)]
     # torch.mul(src, 2, out=dst), the last three dispatches of the run: reads and writes COPY_BYTES
     copy_pred = lambda n: "MulFunctor<double>" in n
-    kf, nf = _counter_per_dispatch(fetch_dir, "FETCH_SIZE", lambda n: _match(n, RING_KERNEL))
-    kw, nw = _counter_per_dispatch(write_dir, "WRITE_SIZE", lambda n: _match(n, RING_KERNEL))
+    kf, nf = _counter_per_dispatch(fetch_dir, "FETCH_SIZE", lambda n: _match(n, FP32_KERNEL))
+    kw, nw = _counter_per_dispatch(write_dir, "WRITE_SIZE", lambda n: _match(n, FP32_KERNEL))
     cf, _ = _counter_per_dispatch(fetch_dir, "FETCH_SIZE", copy_pred, last=3)
     cw, _ = _counter_per_dispatch(write_dir, "WRITE_SIZE", copy_pred, last=3)
     ff, fw = COPY_BYTES / (cf * 1024.0), COPY_BYTES / (cw * 1024.0)
     fetch_b, write_b = kf * 1024.0 * ff, kw * 1024.0 * fw
     nel = 256 ** 3
-    tr = {"kernel": RING_KERNEL, "workload": "Q1 Poisson hex8 256^3, FP32 integration",
+    tr = {"kernel": FP32_KERNEL, "workload": "Q1 Poisson hex8 256^3, FP32 integration",
           "fetch_kib_per_dispatch": kf, "write_kib_per_dispatch": kw, "dispatches": [nf, nw],
           "copy_fetch_kib": cf, "copy_write_kib": cw, "fetch_factor": ff, "write_factor": fw,
           "fetch_bytes": fetch_b, "write_bytes": write_b, "bytes_per_launch": fetch_b + write_b,
@@ -145,7 +145,7 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir):
     json.dump(tr, open(TRAFFIC_FILE, "w"), indent=1)
     lines += ["", "## HBM traffic of the FP32 kernel (rocprofv3 --pmc, one pass per counter)", ""]
     lines += ["    " + c for c in tr["commands"]]
-    lines += ["", f"- `{RING_KERNEL}` average over {nf} / {nw} dispatches: FETCH_SIZE {kf:.0f} KiB, "
+    lines += ["", f"- `{FP32_KERNEL}` average over {nf} / {nw} dispatches: FETCH_SIZE {kf:.0f} KiB, "
                   f"WRITE_SIZE {kw:.0f} KiB",
               f"- 1 GiB elementwise calibration kernel (dst = 2 src): FETCH_SIZE {cf:.0f} KiB, WRITE_SIZE {cw:.0f} KiB -> scale factors "
               f"{ff:.3f} (fetch), {fw:.3f} (write)",
@@ -153,7 +153,7 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir):
               f"{(fetch_b + write_b) / 1e9:.3f} GB; algorithmic {BYTES_PER_ELEM * nel / 1e9:.3f} GB "
               f"({BYTES_PER_ELEM} B/element x {nel} elements)"]
     if ring:
-        lines += [f"- `{RING_KERNEL}` average duration in the stats pass: "
+        lines += [f"- `{FP32_KERNEL}` average duration in the stats pass: "
                   f"{float(ring[0]['AverageNs']) / 1e3:.1f} us over {ring[0]['Calls']} calls"]
     out = os.path.join(ROOT, "profiles", "q1_hex8_diffusion_fp32_256_kernel_stats.md")
     open(out, "w").write("\n".join(lines) + "\n")
@@ -572,7 +572,7 @@ def main():
         "wall_s_timed": wall32,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / (HBM_PEAK / 1e9), "traffic": traffic,
-                     "kernel": RING_KERNEL, "kernel_ms": kt32,
+                     "kernel": FP32_KERNEL, "kernel_ms": kt32,
                      "bytes_per_element": BYTES_PER_ELEM},
     }
     if args.copy_calibration:

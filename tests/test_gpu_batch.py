@@ -2,9 +2,9 @@
 
 The batch runs the same per-tile kernel bodies as the single-mesh path, so its
 values must match the oracle's to the usual tolerances (FP64 1e-12, FP32 1e-6
-normwise) and txa_assemble's to 1e-14 normwise (the batch kernels are separate
-compiled instances of those bodies, and the compiler may fuse multiply-adds
-differently in each).  Graph replays are checked one by
+normwise) and txa_assemble's to 1e-14 (FP64) or 1e-6 (FP32) normwise: the batch
+kernels are separate compiled instances of those bodies, and the compiler may
+fuse multiply-adds differently in each, which shows at FP32 rounding level.  Graph replays are checked one by
 one against eager launches, with new argument pointers and seeds each time.
 """
 import numpy as np
@@ -25,7 +25,7 @@ SHAPES = [(txa.HEX8, 1, 1, 1, 0.0), (txa.TET4, 2, 3, 1, 0.1), (txa.HEX8, 40, 9, 
           (txa.HEX8, 31, 7, 3, 0.0), (txa.TET4, 33, 15, 6, 0.1)]
 
 
-SAME = 1e-14
+SAME = {txa.FP64: 1e-14, txa.FP32: 1e-6}
 
 
 def rel_err(got, ref):
@@ -112,7 +112,7 @@ def test_batch_matches_single_and_oracle(engine, items, precision, transient, to
             e = (rel_err(f, fr), rel_err(v, vr), rel_err(f, fs), rel_err(v, vs))
             print(f"item {i}: oracle R {e[0]:.2e} J {e[1]:.2e}  vs txa_assemble R {e[2]:.2e} J {e[3]:.2e}", flush=True)
             assert e[0] <= tol and e[1] <= tol, f"item {i} differs from the oracle"
-            assert e[2] <= SAME and e[3] <= SAME, f"item {i} differs from txa_assemble"
+            assert e[2] <= SAME[precision] and e[3] <= SAME[precision], f"item {i} differs from txa_assemble"
     finally:
         b.destroy()
 
@@ -153,7 +153,7 @@ def test_batch_autotune(engine, items):
         got = run_batch(b, items, xs, False, 0.0, use_graph=True)
         single = run_single(items, xs, txa.FP32, False, 0.0)
         for (f, v), (fs, vs) in zip(got, single):
-            assert rel_err(f, fs) <= SAME and rel_err(v, vs) <= SAME
+            assert rel_err(f, fs) <= SAME[txa.FP32] and rel_err(v, vs) <= SAME[txa.FP32]
     finally:
         b.destroy()
 

@@ -39,10 +39,16 @@ constexpr double kM = 0.5 * (1.0 - kG);         // P(bit != qbit)
 constexpr int NSTAGE = 44;                      // 36 packed K + 8 R per element
 
 #ifndef TXA_Q1_FP32_RING
-#define TXA_Q1_FP32_RING 1    // FP32 integration uses the one-role LDS-ring kernel
+#define TXA_Q1_FP32_RING 0    // 1: FP32 integration uses the one-role LDS-ring kernel (0: warp-specialised)
 #endif
 #ifndef TXA_Q1_TARGET_BLOCKS
 #define TXA_Q1_TARGET_BLOCKS 2048   // z slabs are sized to give about this many workgroups
+#endif
+#ifndef TXA_Q1_IDENT_FAST
+#define TXA_Q1_IDENT_FAST 1   // rows whose slot map is the identity skip the per-entry decode
+#endif
+#ifndef TXA_Q1_UNROLL2
+#define TXA_Q1_UNROLL2 1      // layer loop unrolled by two: the accumulator hand-over becomes renaming
 #endif
 #ifndef TXA_Q1_NT_STORES
 #define TXA_Q1_NT_STORES 1    // non-temporal stores for the CRS values (written once, not re-read)
@@ -110,6 +116,11 @@ __device__ __forceinline__ void hex8_element(const double (&X)[3][8], const doub
           h[d][v | (1 << b)] = hi - lo;
         }
   }
+  // 1/8 of the trilinear map folded into the coefficients
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int v = 1; v < 8; ++v) h[d][v] *= 0.125;
   // metric tensor at the 8 points, scale 1/4 of the gradient products folded in
   Real A[8][6];     // xx yy zz xy yz xz
   double wm[8];
@@ -120,9 +131,9 @@ __device__ __forceinline__ void hex8_element(const double (&X)[3][8], const doub
     double J[3][3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-      J[d][0] = 0.125 * (h[d][1] + h[d][3] * eta + h[d][5] * zeta + h[d][7] * (eta * zeta));
-      J[d][1] = 0.125 * (h[d][2] + h[d][3] * xi + h[d][6] * zeta + h[d][7] * (xi * zeta));
-      J[d][2] = 0.125 * (h[d][4] + h[d][5] * xi + h[d][6] * eta + h[d][7] * (xi * eta));
+      J[d][0] = h[d][1] + h[d][3] * eta + h[d][5] * zeta + h[d][7] * (eta * zeta);
+      J[d][1] = h[d][2] + h[d][3] * xi + h[d][6] * zeta + h[d][7] * (xi * zeta);
+      J[d][2] = h[d][4] + h[d][5] * xi + h[d][6] * eta + h[d][7] * (xi * eta);
     }
     double c[3][3];
     c[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
@@ -476,7 +487,22 @@ __device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &
   const int64_t base0 = __shfl(base, 0, 32);
   const int64_t end_last = __shfl(end, nact > 0 ? nact - 1 : 0, 32);
   double *buf = rowbuf + grp * ROWBUF;
-  if (contiguous && act) {
+#if TXA_Q1_IDENT_FAST
+  // interior rows: the 27 neighbours sit in the row in lattice order (slot s at s)
+  const bool ident = m.w[0] == 0x03020100u && m.w[1] == 0x07060504u && m.w[2] == 0x0b0a0908u &&
+                     m.w[3] == 0x0f0e0d0cu && m.w[4] == 0x13121110u && m.w[5] == 0x17161514u &&
+                     (m.w[6] & 0x00ffffffu) == 0x001a1918u;
+  const bool fast = __all(!act || (ident && contiguous));   // uniform over the wave
+#else
+  const bool fast = false;
+#endif
+  if (fast) {
+    if (act) {
+      const int64_t off = base - base0;
+#pragma unroll
+      for (int s = 0; s < 27; ++s) buf[off + s] = acc[s];
+    }
+  } else if (contiguous && act) {
     const int64_t off = base - base0;
     // branch-free: absent stencil positions go to a spare slot past the group's rows
 #pragma unroll
@@ -749,6 +775,9 @@ __device__ __forceinline__ void ring_block(const Q1DiffParams &p, int bx, int by
   int32_t Lcur = owner ? p.lid[ocol + sxy * ez0] : -1;
   __syncthreads();
 
+#if TXA_Q1_UNROLL2
+#pragma unroll 2
+#endif
   for (int ez = ez0; ez <= ez1; ++ez) {
     // loads of this layer, all before its stores: the face two layers up, the LIDs
     // beyond it, the owner's row pointers and slot map, the owner's next LID
@@ -1078,8 +1107,13 @@ void launch_batch(const Batch &b, dim3 g, const int4 *items, int n, BatchQueue *
     if (b.transient) hipLaunchKernelGGL((k_q1_batch_spec<KIND, double, true>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
     else hipLaunchKernelGGL((k_q1_batch_spec<KIND, double, false>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
   } else {
+#if TXA_Q1_FP32_RING
     if (b.transient) hipLaunchKernelGGL((k_q1_batch_ring<KIND, float, true>), g, dim3(NE), 0, s, b.prm.p, items, n, q);
     else hipLaunchKernelGGL((k_q1_batch_ring<KIND, float, false>), g, dim3(NE), 0, s, b.prm.p, items, n, q);
+#else
+    if (b.transient) hipLaunchKernelGGL((k_q1_batch_spec<KIND, float, true>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
+    else hipLaunchKernelGGL((k_q1_batch_spec<KIND, float, false>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
+#endif
   }
   TXA_HIP(hipGetLastError());
 }
