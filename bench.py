@@ -352,9 +352,11 @@ def bench_q2(args):
 def bench_serve(args):
     """Serving: Newmark steps of transient hex8 elasticity (beta 0.25, gamma 0.5,
     Rayleigh a 0.01, b 0.001, dt 1e-3) on a pool of resident meshes with n^3 cells,
-    n log-uniform in [16, 128]; Poisson arrivals at --qps; each request carries a
-    seeded state U[-1e-3, 1e-3] (generated on the device).  Reports p50/p99
-    submit-to-completion latency and throughput."""
+    n log-uniform in [16, 128].  Phase 1: Poisson arrivals at --qps, p50/p99
+    submit-to-completion latency.  Phase 2: the same requests queued at once
+    (throughput at saturation), beside the plain FP64 path (the same requests one
+    txa_assemble after another on one stream).  Each request carries a seeded state
+    U[-1e-3, 1e-3] generated on the device."""
     import threading
 
     import numpy as np
@@ -363,6 +365,7 @@ def bench_serve(args):
     import tianxin_amd as txa
     from tianxin_amd import synth
 
+    prec = {"fp64": txa.FP64, "fp32": txa.FP32}[args.serve_precision]
     eng = txa.Engine(0)
     sizes = log_uniform_sizes(41, args.pool, 16, 128)
     t0 = time.time()
@@ -370,7 +373,7 @@ def bench_serve(args):
     for i, n in enumerate(sizes):
         n = int(n)
         m = eng.brick(txa.HEX8, n, n, n, synth.brick_coords(n, n, n))
-        nd = m.build_dofmap(3)
+        m.build_dofmap(3)
         m.build_graph()
         pl = m.point_lids()
         bc = torch.as_tensor(np.unique(pl[np.arange(pl.shape[0]) % (n + 1) == 0].ravel()).astype(np.int32),
@@ -388,46 +391,78 @@ def bench_serve(args):
         states.append([torch.rand(m.ndofs, generator=g, device="cuda", dtype=torch.float64) * 2e-3 - 1e-3
                        for _ in range(3)])
     torch.cuda.synchronize()
-    srv = txa.Server(eng, [p[0] for p in pool], hbm_budget=int(args.budget_gb * 2 ** 30), slots=args.slots)
-    rids = [None] * args.requests
-    ready = threading.Semaphore(0)
+    nel = sum(int(pool[i][0].nelem) for i in pick)
+    kw = dict(physics=txa.ELASTICITY, dt=1e-3, newmark=(0.25, 0.5), nu=0.3, rho=1.0, rayleigh=(0.01, 0.001),
+              precision=prec)
 
-    def collect():
+    def serve(arrivals):
+        srv = txa.Server(eng, [p_[0] for p_ in pool], hbm_budget=int(args.budget_gb * 2 ** 30), slots=args.slots)
+        rids = [None] * args.requests
+        ready = threading.Semaphore(0)
+
+        def collect():
+            for r in range(args.requests):
+                ready.acquire()
+                st = txa.QUEUED
+                while st != txa.DONE:
+                    st, _, _ = srv.poll(rids[r], 60.0)
+                srv.release(rids[r])
+
+        th = threading.Thread(target=collect)
+        th.start()
+        t_start = time.perf_counter()
+        t_next = t_start
         for r in range(args.requests):
-            ready.acquire()
-            st = txa.QUEUED
-            while st != txa.DONE:
-                st, _, _ = srv.poll(rids[r], 60.0)
-            srv.release(rids[r])
+            if arrivals:
+                t_next += gaps[r]
+                delay = t_next - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+            m, E, bc = pool[pick[r]]
+            u, v, a = states[r]
+            rids[r] = srv.submit(m, coef=E, u=u, v=v, a=a, bc_lids=bc, **kw)
+            ready.release()
+        th.join()
+        wall = time.perf_counter() - t_start
+        st = srv.stats()
+        srv.destroy()
+        return st, wall
 
-    th = threading.Thread(target=collect)
-    th.start()
-    t_start = time.perf_counter()
-    t_next = t_start
+    lat, wall_lat = serve(True)
+    sat, wall_sat = serve(False)
+    log(f"latency phase: p50 {lat['p50_ms']:.2f} ms p99 {lat['p99_ms']:.2f} ms; saturated: "
+        f"{nel / wall_sat:.3g} elements/s")
+    # plain FP64 path: the same requests back to back on one stream
+    outs = {}
+    for i, (m, E, bc) in enumerate(pool):
+        outs[i] = (torch.empty(m.ndofs, device="cuda", dtype=torch.float64),
+                   torch.empty(m.nnz, device="cuda", dtype=torch.float64))
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
     for r in range(args.requests):
-        t_next += gaps[r]
-        delay = t_next - time.perf_counter()
-        if delay > 0:
-            time.sleep(delay)
         m, E, bc = pool[pick[r]]
         u, v, a = states[r]
-        rids[r] = srv.submit(m, physics=txa.ELASTICITY, coef=E, u=u, v=v, a=a, dt=1e-3, newmark=(0.25, 0.5),
-                             nu=0.3, rho=1.0, rayleigh=(0.01, 0.001), bc_lids=bc)
-        ready.release()
-    th.join()
-    wall = time.perf_counter() - t_start
-    s = srv.stats()
-    srv.destroy()
-    nel = sum(int(pool[i][0].nelem) for i in pick)
+        f, vals = outs[pick[r]]
+        m.assemble(physics=txa.ELASTICITY, precision=txa.FP64, coef=E, x=u, xdot=v, xddot=a, f=f, vals=vals,
+                   alpha=0.5 / (0.25 * 1e-3), beta=1.0, gamma2=1.0 / (0.25 * 1e-6), transient=True, nu=0.3, rho=1.0,
+                   rayleigh=(0.01, 0.001), bc_lids=bc)
+    torch.cuda.synchronize()
+    wall_plain = time.perf_counter() - tp
     line = {"metric": "serving: p50/p99 latency and elements/s (transient elasticity residual+Jacobian)",
-            "value": s["elements_per_s"], "unit": "elements/s", "n_gpus": 1, "higher_is_better": True,
-            "dtype": "f64", "data": "synthetic",
+            "value": nel / wall_sat, "unit": "elements/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f32 element integration, f64 geometry and scatter" if prec == txa.FP32 else "f64",
+            "data": "synthetic",
             "config": {"workload": "hex8 transient elasticity (Newmark), n^3 cells, n log-uniform in [16, 128]",
                        "pool_n": list(map(int, sizes)), "requests": args.requests, "qps": args.qps,
                        "slots": args.slots, "hbm_budget_gb": args.budget_gb, "dt": 1e-3},
-            "latency_ms": {"p50": s["p50_ms"], "p99": s["p99_ms"], "mean": s["mean_ms"], "max": s["max_ms"]},
-            "requests_per_s": s["requests_per_s"], "elements_served": nel, "wall_s": wall,
-            "peak_inflight": s["peak_inflight"], "peak_arena_bytes": s["peak_bytes"]}
+            "latency_ms": {"p50": lat["p50_ms"], "p99": lat["p99_ms"], "mean": lat["mean_ms"], "max": lat["max_ms"],
+                           "at_qps": args.qps, "requests_per_s": lat["requests_per_s"],
+                           "peak_inflight": lat["peak_inflight"]},
+            "saturated": {"elements_per_s": nel / wall_sat, "wall_s": wall_sat, "p50_ms": sat["p50_ms"],
+                          "p99_ms": sat["p99_ms"], "peak_inflight": sat["peak_inflight"],
+                          "peak_arena_bytes": sat["peak_bytes"]},
+            "plain_fp64_sequential": {"elements_per_s": nel / wall_plain, "wall_s": wall_plain},
+            "speedup_vs_plain_fp64": wall_plain / wall_sat, "elements_served": nel}
     print(json.dumps(line), flush=True)
 
 
@@ -512,6 +547,7 @@ def main():
     ap.add_argument("--qps", type=float, default=20.0)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--budget-gb", type=float, default=64.0)
+    ap.add_argument("--serve-precision", choices=("fp64", "fp32"), default="fp32")
     ap.add_argument("--summarize-q2", nargs=2, metavar=("STATS_DIR", "PMC_DIR"),
                     help="CPU only: summarise rocprofv3 csv output of --workload q2 into profiles/")
     args = ap.parse_args()
