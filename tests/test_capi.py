@@ -38,3 +38,14 @@ def test_null_context_is_rejected(lib):
     assert lib.txa_sync(None) == txa.TXA_ERR_INVALID
     lib.txa_assemble.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
     assert lib.txa_assemble(None, 1, None, None) == txa.TXA_ERR_INVALID
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import json
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]

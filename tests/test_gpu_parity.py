@@ -130,7 +130,8 @@ def test_hex8_diffusion_fp32(engine, transient):
 
 
 @pytest.mark.parametrize("topo,physics,nf,n", [(txa.TET4, txa.DIFFUSION, 1, 4), (txa.HEX27, txa.DIFFUSION, 1, 3),
-                                               (txa.HEX8, txa.ELASTICITY, 3, 3), (txa.HEX27, txa.ELASTICITY, 3, 2)])
+                                               (txa.HEX8, txa.ELASTICITY, 3, 3), (txa.HEX27, txa.ELASTICITY, 3, 2),
+                                               (txa.TET4, txa.ELASTICITY, 3, 3)])
 def test_generic_path_fp64(engine, topo, physics, nf, n):
     c = Case(engine, topo, n, n, n + 1, nf, jitter=0.1)
     trans = physics == txa.ELASTICITY
