@@ -30,134 +30,41 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n=128, reps=8):
+def cpu_baseline(n=128, reps=8, n1=48, reps1=3):
     """Oracle (reference algorithm: cached worksets, forward-AD Jacobian, row-search
-    scatter) on host cores, on an n^3 sample of the same workload."""
+    scatter) on host cores, on an n^3 sample of the same workload with all threads
+    (at most 16), and on an n1^3 sample with one thread."""
     import numpy as np
 
     import oracle as O
     from tianxin_amd import synth
 
     cores = min(16, os.cpu_count() or 1)
-    conn, ev, _ = O.brick_mesh(O.HEX8, n, n, n)
-    _, owned, el = O.dofmap(conn, 1)
-    rp, col = O.graph(el, owned.size)
-    X = synth.brick_coords(n, n, n)
-    k = synth.lognormal(SEED_K, n ** 3, 0.5)
-    x = synth.uniform(SEED_U, owned.size, -1, 1)
-    A = O.Assembler(O.HEX8, O.DIFFUSION, ev, X, el, k, rp, col, src=(1.0,), workset_size=1000, nthreads=cores)
-    for _ in range(2):
-        A.evaluate(x, mode=O.MODE_FAD)
-    ts = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        A.evaluate(x, mode=O.MODE_FAD)
-        ts.append(time.perf_counter() - t0)
-    med = float(np.median(ts))
-    return {"value": n ** 3 / med, "unit": "elements/s", "cores": cores, "kind": "port",
+
+    def run(nn, threads, rr):
+        conn, ev, _ = O.brick_mesh(O.HEX8, nn, nn, nn)
+        _, owned, el = O.dofmap(conn, 1)
+        rp, col = O.graph(el, owned.size)
+        X = synth.brick_coords(nn, nn, nn)
+        k = synth.lognormal(SEED_K, nn ** 3, 0.5)
+        x = synth.uniform(SEED_U, owned.size, -1, 1)
+        A = O.Assembler(O.HEX8, O.DIFFUSION, ev, X, el, k, rp, col, src=(1.0,), workset_size=1000, nthreads=threads)
+        for _ in range(2):
+            A.evaluate(x, mode=O.MODE_FAD)
+        ts = []
+        for _ in range(rr):
+            t0 = time.perf_counter()
+            A.evaluate(x, mode=O.MODE_FAD)
+            ts.append(time.perf_counter() - t0)
+        return nn ** 3 / float(np.median(ts))
+
+    v = run(n, cores, reps)
+    v1 = run(n1, 1, reps1)
+    return {"value": v, "unit": "elements/s", "cores": cores, "kind": "port",
             "sample": f"Q1 Poisson {n}^3 hex8 ({n ** 3} elements), FP64, forward-AD Jacobian, workset 1000, "
-                      f"median of {reps} after 2 warm-ups, {cores} OpenMP threads"}
-
-
-COPY_BYTES = 1 << 30
-FP32_KERNEL = "k_q1_hex8_diffusion<1, float, false>"   # HEX8, FP32 integration, steady
-FP64_KERNEL = "k_q1_hex8_diffusion<1, double, false>"  # HEX8, FP64
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "q1_hex8_diffusion_fp32_256_traffic.json")
-
-
-def _csvs(d, suffix):
-    import glob
-    out = sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True))
-    if not out:
-        raise SystemExit(f"no *{suffix} under {d}")
-    return out
-
-
-def _rows(d, suffix):
-    import csv
-    rows = []
-    for fn in _csvs(d, suffix):
-        with open(fn) as fh:
-            rows += list(csv.DictReader(fh))
-    return rows
-
-
-def _match(name, kernel):
-    """Demangled-name match of a kernel instance: 'k_q1_hex8_ring<1, float, false>'."""
-    base, targs = kernel.split("<", 1)
-    targs = [t.strip() for t in targs.rstrip(">").split(",")]
-    if base not in name:
-        return False
-    inside = name.split(base, 1)[1]
-    return all(t in inside.split("(")[0] for t in targs) and inside.split("(")[0].replace(" ", "").startswith(
-        "<" + ",".join(targs))
-
-
-def _counter_per_dispatch(d, counter, pred, last=None):
-    """Mean per-dispatch value of `counter` over dispatches whose kernel name satisfies
-    pred (only the `last` such dispatches when given)."""
-    per = {}
-    for r in _rows(d, "counter_collection.csv"):
-        if r.get("Counter_Name") != counter or not pred(r.get("Kernel_Name", "")):
-            continue
-        key = int(r.get("Dispatch_Id") or r.get("Correlation_Id"))
-        per[key] = per.get(key, 0.0) + float(r["Counter_Value"])
-    if not per:
-        raise SystemExit(f"no {counter} samples for the requested kernel in {d}")
-    keys = sorted(per)[-last:] if last else sorted(per)
-    return sum(per[k] for k in keys) / len(keys), len(keys)
-
-
-def summarize_profiles(stats_dir, fetch_dir, write_dir):
-    """Writes profiles/q1_hex8_diffusion_fp32_256_kernel_stats.md and the traffic JSON.
-
-    FETCH_SIZE and WRITE_SIZE are in KiB per dispatch; each is scaled by what the
-    same counter reads for a 1 GiB elementwise kernel (dst = 2 src: COPY_BYTES read,
-    COPY_BYTES written)."""
-    os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
-    stats = _rows(stats_dir, "kernel_stats.csv")
-    cmd_stats = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_stats -- "
-                 "python3 bench.py --steps 5 --warmup 1 --no-cpu")
-    lines = ["# Kernel statistics: Q1 Poisson hex8 256^3 (bench.py workload)", "",
-             "Command (from the repository root, on one MI355X):", "", "    " + cmd_stats, "",
-             "| kernel | calls | average us | min us | max us | share % |", "|---|---:|---:|---:|---:|---:|"]
-    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:12]:
-        lines.append(f"| `{r['Name'][:110]}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
-                     f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} |")
-    ring = [r for r in stats if _match(r["Name"], FP32_KERNEL)]
-    # torch.mul(src, 2, out=dst), the last three dispatches of the run: reads and writes COPY_BYTES
-    copy_pred = lambda n: "MulFunctor<double>" in n
-    kf, nf = _counter_per_dispatch(fetch_dir, "FETCH_SIZE", lambda n: _match(n, FP32_KERNEL))
-    kw, nw = _counter_per_dispatch(write_dir, "WRITE_SIZE", lambda n: _match(n, FP32_KERNEL))
-    cf, _ = _counter_per_dispatch(fetch_dir, "FETCH_SIZE", copy_pred, last=3)
-    cw, _ = _counter_per_dispatch(write_dir, "WRITE_SIZE", copy_pred, last=3)
-    ff, fw = COPY_BYTES / (cf * 1024.0), COPY_BYTES / (cw * 1024.0)
-    fetch_b, write_b = kf * 1024.0 * ff, kw * 1024.0 * fw
-    nel = 256 ** 3
-    tr = {"kernel": FP32_KERNEL, "workload": "Q1 Poisson hex8 256^3, FP32 integration",
-          "fetch_kib_per_dispatch": kf, "write_kib_per_dispatch": kw, "dispatches": [nf, nw],
-          "copy_fetch_kib": cf, "copy_write_kib": cw, "fetch_factor": ff, "write_factor": fw,
-          "fetch_bytes": fetch_b, "write_bytes": write_b, "bytes_per_launch": fetch_b + write_b,
-          "algorithmic_bytes": BYTES_PER_ELEM * nel,
-          "commands": [f"rocprofv3 --pmc {c} --output-format csv -d gpurun_out/pmc_{c.lower()} -- "
-                       "python3 bench.py --steps 2 --warmup 1 --no-cpu --copy-calibration"
-                       for c in ("FETCH_SIZE", "WRITE_SIZE")]}
-    json.dump(tr, open(TRAFFIC_FILE, "w"), indent=1)
-    lines += ["", "## HBM traffic of the FP32 kernel (rocprofv3 --pmc, one pass per counter)", ""]
-    lines += ["    " + c for c in tr["commands"]]
-    lines += ["", f"- `{FP32_KERNEL}` average over {nf} / {nw} dispatches: FETCH_SIZE {kf:.0f} KiB, "
-                  f"WRITE_SIZE {kw:.0f} KiB",
-              f"- 1 GiB elementwise calibration kernel (dst = 2 src): FETCH_SIZE {cf:.0f} KiB, WRITE_SIZE {cw:.0f} KiB -> scale factors "
-              f"{ff:.3f} (fetch), {fw:.3f} (write)",
-              f"- traffic per launch: {fetch_b / 1e9:.3f} GB read + {write_b / 1e9:.3f} GB written = "
-              f"{(fetch_b + write_b) / 1e9:.3f} GB; algorithmic {BYTES_PER_ELEM * nel / 1e9:.3f} GB "
-              f"({BYTES_PER_ELEM} B/element x {nel} elements)"]
-    if ring:
-        lines += [f"- `{FP32_KERNEL}` average duration in the stats pass: "
-                  f"{float(ring[0]['AverageNs']) / 1e3:.1f} us over {ring[0]['Calls']} calls"]
-    out = os.path.join(ROOT, "profiles", "q1_hex8_diffusion_fp32_256_kernel_stats.md")
-    open(out, "w").write("\n".join(lines) + "\n")
-    print(out, TRAFFIC_FILE)
+                      f"median of {reps} after 2 warm-ups, {cores} OpenMP threads",
+            "single_core": {"value": v1, "unit": "elements/s", "cores": 1,
+                            "sample": f"{n1}^3 hex8, median of {reps1} after 2 warm-ups, 1 thread"}}
 
 
 def log_uniform_sizes(seed, count, lo, hi):
