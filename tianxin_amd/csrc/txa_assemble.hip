@@ -194,7 +194,19 @@ void upload_refelem(Mesh &m, DevBuf<double> &tab, hipStream_t s) {
   TXA_HIP(hipStreamSynchronize(s));
 }
 
-void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s) {
+static bool uses_q1_path(const Mesh &m, int physics, int precision) {
+  return (m.topo == TXA_HEX8 || m.topo == TXA_TET4) && physics == TXA_DIFFUSION && precision != TXA_SPLIT_BF16;
+}
+
+size_t assemble_workspace_bytes(const Mesh &m, int physics, int precision, bool want_f, bool want_vals) {
+  txa_assembly_args a{};
+  a.physics = physics;
+  a.precision = precision;
+  if (uses_q1_path(m, physics, precision) || !elastic_supported(m, a)) return 0;
+  return elastic_workspace_bytes(m, precision, want_f, want_vals);
+}
+
+void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
   TXA_REQUIRE(m.has_graph, TXA_ERR_LOGIC, "assemble before graph_build");
   TXA_REQUIRE(a.x != nullptr, TXA_ERR_INVALID, "x is required");
   TXA_REQUIRE(a.coef != nullptr, TXA_ERR_INVALID, "coef is required");
@@ -208,10 +220,10 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s) {
   TXA_REQUIRE(a.n_bc == 0 || a.bc_lids != nullptr, TXA_ERR_INVALID, "bc_lids missing");
   const bool trans = a.transient != 0;
 
-  if ((m.topo == TXA_HEX8 || m.topo == TXA_TET4) && a.physics == TXA_DIFFUSION && a.precision != TXA_SPLIT_BF16) {
+  if (uses_q1_path(m, a.physics, a.precision)) {
     launch_q1_hex8_diffusion(m, a, s);
   } else if (elastic_supported(m, a)) {
-    launch_elastic(m, a, s);
+    launch_elastic(m, a, s, ws);
   } else {
     TXA_REQUIRE(a.precision == TXA_FP64, TXA_ERR_UNSUPPORTED,
                 "reduced-precision integration is implemented for hex8 / tet4 diffusion and 3-field elasticity");

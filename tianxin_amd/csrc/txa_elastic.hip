@@ -20,9 +20,15 @@
 // i.e. B^T D B for isotropic D written as one symmetric product G^T W G.  T is
 // computed in FP64 or FP32 on the VALU, or as T = O^T O with O = sqrt(w |det J|) G
 // by bf16 MFMA on a three-way split of each FP32 operand (x = x0 + x1 + x2, every
-// x_s in bf16) keeping the six cross products of order <= 2^-16, accumulated in FP32.  The scatter adds FP64
-// values into the CRS with hardware FP64 atomics; the row position of every
-// (node, neighbour) pair comes from a per-node table built with the graph.
+// x_s in bf16) keeping the six cross products of order <= 2^-16, accumulated in FP32.
+//
+// Assembly into the CRS is two-phase.  The element kernel writes each element's
+// Jacobian block (FP32, FP64 in FP64 mode) and residual (FP64) to a workspace,
+// coalesced; a node-centric gather then sums, for every node, the blocks of the
+// elements around it into the node's three CRS rows in FP64, in a fixed element
+// order, and writes each row once.  No atomics: results are bitwise reproducible.
+// The row position of every (node, neighbour) pair comes from a per-node table
+// built with the graph.
 #include <type_traits>
 
 #include "txa_internal.hpp"
@@ -39,15 +45,13 @@ struct ElasticParams {
   const int64_t *conn;          // the first 8 entries of an element are its vertices
   const double *X, *Y, *Z;
   const int32_t *elem_lid;      // [E][3 NB]: node-major, field-minor
-  const uint16_t *nperm;        // [nodes][nperm_w]: position of (neighbour, field 0) in the node's rows
-  int nperm_w;
   const double *coef;           // Young's modulus per element
   double nu, rho, src[3], ra, rb;
   const double *x, *xdot, *xddot;
   double alpha, beta, gamma2;
   int transient;
-  double *f, *vals;
-  const int64_t *rowptr;
+  void *jbuf;        // element Jacobian blocks [E][ND][ND], row-major (FP32, FP64 in FP64 mode); null: none
+  double *rbuf;      // element residuals [E][ND]; null: none
 };
 
 // lattice offsets of the element's nodes: shards vertex order (hex8), Q2 order (hex27)
@@ -104,15 +108,17 @@ struct Smem {
   double wm[NQ];             // w det J (the sign of the first point is applied where used)
   double sw[NQ];             // sqrt(w |det J|)
   double u[D::ND], v[D::ND], a[D::ND], r[D::ND];
-  int32_t L[D::ND];
-  int64_t row[D::ND];
-  uint16_t pos[NB][NB];      // row position of (node b, field 0) in node a's rows
   TReal M[NB][NB];
   // VALU modes: gradients G and the Gram matrix T.  MFMA mode: the bf16 operand
-  // parts, then T over the same bytes once the products are in registers.
-  union {
+  // parts, then T over the same bytes once the products are in registers.  The
+  // operand rows are read as 8-byte vectors: the union is 16-byte aligned whatever
+  // the fields before it add up to.
+  // FP64 rows are not padded (81 doubles is already an odd bank stride), which keeps
+  // the FP64 form under half of a CU's LDS: two workgroups per CU.
+  static constexpr int TP = MODE == MODE_F64 ? D::ND : D::ND + 1;
+  union alignas(16) {
     struct {
-      TReal T[D::ND][D::ND + 1];
+      TReal T[D::ND][TP];
       TReal G[NQ * 3 * NB];
     } v;
     short ops[3 * D::MP * D::KP];
@@ -170,7 +176,7 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
   __syncthreads();
 
   for (int64_t e = blockIdx.x; e < p.nelem; e += gridDim.x) {
-    // ---- gather: vertices, LIDs, row starts, state
+    // ---- gather: vertices, state
     if (t < 24) {
       const int vtx = t / 3, d = t % 3;
       const int64_t n = p.conn[e * p.nconn + vtx];
@@ -178,25 +184,11 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
     }
     for (int i = t; i < ND; i += BS) {
       const int32_t L = p.elem_lid[e * ND + i];
-      sm.L[i] = L;
-      sm.row[i] = p.rowptr[L];
       sm.u[i] = p.x[L];
       sm.v[i] = tr ? p.xdot[L] : 0.0;
       sm.a[i] = tra ? p.xddot[L] : 0.0;
     }
     __syncthreads();
-    // ---- row positions of this lane's node pairs (loads in flight until the last phase)
-    uint32_t posr[NPASS];
-#pragma unroll
-    for (int k = 0; k < NPASS; ++k) {
-      const int a = ga + k * AP;
-      posr[k] = 0xffffu;
-      if (a < NB && gb < NB) {
-        const int s = (sm.off[gb][0] - sm.off[a][0] + D::R) + D::S * (sm.off[gb][1] - sm.off[a][1] + D::R) +
-                      D::S * D::S * (sm.off[gb][2] - sm.off[a][2] + D::R);
-        posr[k] = p.nperm[static_cast<int64_t>(sm.L[3 * a] / 3) * p.nperm_w + s];
-      }
-    }
     // ---- geometry (FP64)
     if (t < NQ) {
       const int qx = t % Q1, qy = (t / Q1) % Q1, qz = t / (Q1 * Q1);
@@ -375,7 +367,6 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
           for (int j = 0; j < 3; ++j) t9[i][j] = static_cast<double>(Tref(i * NB + a, j * NB + b));
         const double trs = t9[0][0] + t9[1][1] + t9[2][2];
         const double Mab = tr ? p.rho * static_cast<double>(sm.M[a][b]) : 0.0;
-        sm.pos[a][b] = static_cast<uint16_t>(posr[k]);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -406,17 +397,16 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
         for (int i = 0; i < 3; ++i) sm.r[3 * a + i] = rr[i];
     }
     __syncthreads();
-    // ---- CRS scatter, row-major: consecutive lanes on consecutive entries of a row
-    if (p.vals)
+    // ---- element block to the workspace, row-major (DOF 3a + i), consecutive lanes on
+    // consecutive entries
+    if (p.jbuf)
       for (int idx = t; idx < ND * ND; idx += BS) {
         const int P = idx / ND, Q = idx % ND;
         const int a = P / 3, i = P % 3, b = Q / 3, j = Q % 3;
-        const uint16_t ps = sm.pos[a][b];
-        if (ps != 0xffffu)
-          unsafeAtomicAdd(p.vals + sm.row[P] + ps + j, static_cast<double>(Tref(i * NB + a, j * NB + b)));
+        static_cast<TReal *>(p.jbuf)[e * (ND * ND) + idx] = Tref(i * NB + a, j * NB + b);
       }
     __syncthreads();
-    if (p.f)
+    if (p.rbuf)
       for (int i = t; i < ND; i += BS) {
         double F = 0.0;
         if (has_src) {
@@ -428,9 +418,131 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
           }
           F *= sg * p.src[i % 3];
         }
-        unsafeAtomicAdd(p.f + sm.L[i], sm.r[i] - F);
+        p.rbuf[e * ND + i] = sm.r[i] - F;
       }
     __syncthreads();
+  }
+}
+
+// Second phase: node-centric gather.  One wave per lattice node assembles the
+// node's three CRS rows (consecutive LIDs with one column structure) in LDS in
+// FP64 from the blocks of the elements around it, in a fixed element order, then
+// writes them once, coalesced, with the residual.
+struct GatherParams {
+  int nx, ny, nz;
+  int64_t px, py, pz, npoints;
+  const int32_t *point_lid;   // [points][3]
+  const int64_t *rowptr;
+  const uint16_t *nperm;
+  int nperm_w;
+  const void *jbuf;
+  const double *rbuf;
+  double *f, *vals;
+};
+
+__device__ __forceinline__ void gather_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int NB, typename JT>
+__global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
+  constexpr int ND = 3 * NB;
+  constexpr int R = NB == 8 ? 1 : 2, S = 2 * R + 1;
+  constexpr int STEP = NB == 8 ? 1 : 2;        // lattice points per cell edge
+  constexpr int MAXLEN = 3 * S * S * S;        // entries per row
+  __shared__ double buf[4][3 * MAXLEN];
+  __shared__ int8_t inv[3][3][3];
+  // the wave index is uniform: everything derived from the node stays in scalar registers
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  constexpr int NR = (3 * ND + 63) / 64;   // block entries per lane and element
+  // per lane and r: lattice offset index of the entry's column node, plus R in each axis
+  int lterm[NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r) {
+    const int bn = ((lane + 64 * r) % ND) / 3;
+    const int8_t *o = NB == 8 ? kHexV[bn] : kQ2[bn];
+    lterm[r] = (o[0] + R) + S * (o[1] + R) + S * S * (o[2] + R);
+  }
+  for (int i = threadIdx.x; i < NB; i += 256) {
+    const int8_t *o = NB == 8 ? kHexV[i] : kQ2[i];
+    inv[o[0]][o[1]][o[2]] = static_cast<int8_t>(i);
+  }
+  __syncthreads();
+  double *b = buf[w];
+  const JT *J = static_cast<const JT *>(g.jbuf);
+  for (int64_t pt = static_cast<int64_t>(blockIdx.x) * 4 + w; pt < g.npoints; pt += static_cast<int64_t>(gridDim.x) * 4) {
+    const int64_t I = pt % g.px, Jy = (pt / g.px) % g.py, K = pt / (g.px * g.py);
+    const int32_t L0 = g.point_lid[pt * 3];
+    const int64_t base = g.rowptr[L0];
+    // row length (at most S^3 neighbours x 3 fields; clamped so the LDS rows never overflow)
+    const int len = min(static_cast<int>(g.rowptr[L0 + 1] - base), MAXLEN);
+    const int64_t node = L0 / 3;
+    const uint16_t *np = g.nperm + node * g.nperm_w;
+    for (int k = lane; k < 3 * len; k += 64) b[k] = 0.0;
+    gather_lds_sync();
+    // cells containing the node (at most 8), per axis [c_lo, c_hi]
+    const int64_t cx_lo = I % STEP == 0 ? I / STEP - 1 : I / STEP, cx_hi = I / STEP;
+    const int64_t cy_lo = Jy % STEP == 0 ? Jy / STEP - 1 : Jy / STEP, cy_hi = Jy / STEP;
+    const int64_t cz_lo = K % STEP == 0 ? K / STEP - 1 : K / STEP, cz_hi = K / STEP;
+    int64_t ce[8];
+    int ca[8], cs[8];
+    int ne = 0;
+    for (int64_t cz = cz_lo; cz <= cz_hi; ++cz)
+      for (int64_t cy = cy_lo; cy <= cy_hi; ++cy)
+        for (int64_t cx = cx_lo; cx <= cx_hi; ++cx)
+          if (cx >= 0 && cx < g.nx && cy >= 0 && cy < g.ny && cz >= 0 && cz < g.nz) {
+            const int ox = static_cast<int>(I - STEP * cx), oy = static_cast<int>(Jy - STEP * cy),
+                      oz = static_cast<int>(K - STEP * cz);   // lattice offset of the node in the cell
+            ce[ne] = cx + g.nx * (cy + g.ny * cz);
+            ca[ne] = inv[ox][oy][oz];
+            cs[ne] = ox + S * oy + S * S * oz;
+            ++ne;
+          }
+    // every load first (one round trip per node), then the additions in element order
+    JT val[8][NR];
+    uint16_t pos[8][NR];
+    double rr[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      rr[c] = 0.0;
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        val[c][r] = JT(0);
+        pos[c][r] = 0xffffu;
+        const int idx = lane + 64 * r;
+        if (c < ne && idx < 3 * ND && J) {
+          val[c][r] = J[(ce[c] * ND + 3 * ca[c]) * ND + idx];   // rows 3a .. 3a+2 of the element block
+          pos[c][r] = np[lterm[r] - cs[c]];
+        }
+      }
+      if (c < ne && lane < 3 && g.rbuf) rr[c] = g.rbuf[ce[c] * ND + 3 * ca[c] + lane];
+    }
+    // the values stay opaque until here, so no conversion pulls a wait in among the loads
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+#pragma unroll
+      for (int r = 0; r < NR; ++r) asm volatile("" : "+v"(val[c][r]));
+    double racc = 0.0;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      if (c < ne) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const int idx = lane + 64 * r;
+          const int i = idx / ND, j = (idx % ND) % 3;
+          const uint16_t ps = pos[c][r];
+          if (idx < 3 * ND && ps != 0xffffu && ps + j < len) b[i * len + ps + j] += static_cast<double>(val[c][r]);
+        }
+        racc += rr[c];
+      }
+      gather_lds_sync();   // the next element may add to the same entries
+    }
+    if (g.vals)
+      for (int k = lane; k < 3 * len; k += 64) g.vals[base + k] = b[k];
+    if (g.f && lane < 3) g.f[L0 + lane] = racc;
+    gather_lds_sync();   // the buffer is reused by the next node
   }
 }
 
@@ -479,12 +591,29 @@ void launch(const ElasticParams &p, hipStream_t s) {
   TXA_HIP(hipGetLastError());
 }
 
+constexpr size_t kWsAlign = 256;
+
+size_t ws_round(size_t b) { return (b + kWsAlign - 1) / kWsAlign * kWsAlign; }
+
+// workspace bytes: residuals [E][ND] FP64 when f is wanted, blocks [E][ND][ND] when vals are
+void ws_parts(const Mesh &m, int precision, bool want_f, bool want_vals, size_t *rbytes, size_t *jbytes) {
+  const size_t nd = static_cast<size_t>(m.ndof_elem), ne = static_cast<size_t>(m.nelem);
+  *rbytes = want_f ? ws_round(ne * nd * sizeof(double)) : 0;
+  *jbytes = want_vals ? ws_round(ne * nd * nd * (precision == TXA_FP64 ? sizeof(double) : sizeof(float))) : 0;
+}
+
 }  // namespace
 
 bool elastic_supported(const Mesh &m, const txa_assembly_args &a) {
   if (a.physics != TXA_ELASTICITY || m.nfields != 3 || !m.nperm.p) return false;
   if (m.topo == TXA_HEX8) return a.precision == TXA_FP64 || a.precision == TXA_FP32;
   return m.topo == TXA_HEX27;
+}
+
+size_t elastic_workspace_bytes(const Mesh &m, int precision, bool want_f, bool want_vals) {
+  size_t r = 0, j = 0;
+  ws_parts(m, precision, want_f, want_vals, &r, &j);
+  return r + j;
 }
 
 void node_perm_build(Mesh &m, hipStream_t s) {
@@ -502,7 +631,14 @@ void node_perm_build(Mesh &m, hipStream_t s) {
   TXA_HIP(hipGetLastError());
 }
 
-void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s) {
+void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
+  if (!a.f && !a.vals) return;
+  size_t rbytes = 0, jbytes = 0;
+  ws_parts(m, a.precision, a.f != nullptr, a.vals != nullptr, &rbytes, &jbytes);
+  // the caller's workspace, or a stream-ordered allocation released after the gather
+  char *w = static_cast<char *>(ws);
+  const bool own = w == nullptr;
+  if (own) TXA_HIP(hipMallocAsync(reinterpret_cast<void **>(&w), rbytes + jbytes, s));
   ElasticParams p;
   p.nq = m.topo == TXA_HEX8 ? 8 : 27;
   p.nelem = m.nelem;
@@ -510,8 +646,6 @@ void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s) {
   p.conn = m.conn.p;
   p.X = m.xyz.p; p.Y = m.xyz.p + m.nverts; p.Z = m.xyz.p + 2 * m.nverts;
   p.elem_lid = m.elem_lid.p;
-  p.nperm = m.nperm.p;
-  p.nperm_w = m.nperm_w;
   p.coef = a.coef;
   p.nu = a.nu; p.rho = a.rho;
   p.src[0] = a.src[0]; p.src[1] = a.src[1]; p.src[2] = a.src[2];
@@ -519,9 +653,8 @@ void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s) {
   p.x = a.x; p.xdot = a.xdot; p.xddot = a.xddot;
   p.alpha = a.alpha; p.beta = a.beta; p.gamma2 = a.gamma2;
   p.transient = a.transient;
-  p.f = a.f; p.vals = a.vals; p.rowptr = m.rowptr.p;
-  if (a.f) TXA_HIP(hipMemsetAsync(a.f, 0, sizeof(double) * m.ndofs, s));
-  if (a.vals) TXA_HIP(hipMemsetAsync(a.vals, 0, sizeof(double) * m.nnz, s));
+  p.rbuf = rbytes ? reinterpret_cast<double *>(w) : nullptr;
+  p.jbuf = jbytes ? static_cast<void *>(w + rbytes) : nullptr;
   if (m.topo == TXA_HEX8) {
     if (a.precision == TXA_FP32) launch<8, 8, 64, MODE_F32>(p, s);
     else launch<8, 8, 64, MODE_F64>(p, s);
@@ -530,6 +663,33 @@ void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s) {
     else if (a.precision == TXA_FP32) launch<27, 27, 256, MODE_F32>(p, s);
     else launch<27, 27, 256, MODE_F64>(p, s);
   }
+  GatherParams g;
+  g.nx = m.nx; g.ny = m.ny; g.nz = m.nz;
+  const int64_t st = m.topo == TXA_HEX8 ? 1 : 2;
+  g.px = st * m.nx + 1; g.py = st * m.ny + 1; g.pz = st * m.nz + 1;
+  g.npoints = m.npoints;
+  g.point_lid = m.point_lid.p;
+  g.rowptr = m.rowptr.p;
+  g.nperm = m.nperm.p;
+  g.nperm_w = m.nperm_w;
+  g.jbuf = p.jbuf;
+  g.rbuf = p.rbuf;
+  g.f = a.f;
+  g.vals = a.vals;
+  int dev = 0, ncu = 0;
+  TXA_HIP(hipGetDevice(&dev));
+  TXA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const unsigned grid = static_cast<unsigned>(std::min<int64_t>((m.npoints + 3) / 4, static_cast<int64_t>(ncu) * 16));
+  const bool wide = a.precision == TXA_FP64;
+  if (m.topo == TXA_HEX8) {
+    if (wide) hipLaunchKernelGGL((k_elastic_gather<8, double>), dim3(grid), dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((k_elastic_gather<8, float>), dim3(grid), dim3(256), 0, s, g);
+  } else {
+    if (wide) hipLaunchKernelGGL((k_elastic_gather<27, double>), dim3(grid), dim3(256), 0, s, g);
+    else hipLaunchKernelGGL((k_elastic_gather<27, float>), dim3(grid), dim3(256), 0, s, g);
+  }
+  TXA_HIP(hipGetLastError());
+  if (own) TXA_HIP(hipFreeAsync(w, s));
 }
 
 }  // namespace txa

@@ -131,11 +131,15 @@ void exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t 
 
 // assembly (txa_assemble.hip)
 void upload_refelem(Mesh &m, DevBuf<double> &tab, hipStream_t s);
-void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s);
+// ws: device workspace of assemble_workspace_bytes(m, a) bytes (256-byte aligned),
+// or null to allocate it stream-ordered on s for the call
+void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws = nullptr);
+size_t assemble_workspace_bytes(const Mesh &m, int physics, int precision, bool want_f, bool want_vals);
 // element-centric elasticity (txa_elastic.hip)
 void node_perm_build(Mesh &m, hipStream_t s);
 bool elastic_supported(const Mesh &m, const txa_assembly_args &a);
-void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s);
+size_t elastic_workspace_bytes(const Mesh &m, int precision, bool want_f, bool want_vals);
+void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws);
 // Q1 diffusion fast path and batches (txa_q1_hex8.hip)
 void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s);
 Batch *batch_create(const std::vector<Mesh *> &meshes, const std::vector<int> &ids, int precision, bool transient);

@@ -14,8 +14,10 @@
 //
 // Memory: one arena of the configured budget is allocated at creation; each
 // admitted request takes f [ndofs] and vals [nnz] from it (first fit, 256 B
-// aligned) until the caller releases the request.  A request is admitted when a
-// slot is free and its footprint fits; the queue is first come, first served.
+// aligned) until the caller releases the request, and the assembly workspace
+// (element blocks and residuals of the two-phase elasticity kernels) until its
+// kernels complete.  A request is admitted when a slot is free and its whole
+// footprint fits; the queue is first come, first served.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -86,8 +88,8 @@ struct Request {
   int64_t id = 0;
   Mesh *mesh = nullptr;
   txa_serve_request r{};
-  size_t off_f = 0, off_v = 0, bytes_f = 0, bytes_v = 0;
-  bool has_mem = false;
+  size_t off_f = 0, off_v = 0, off_w = 0, bytes_f = 0, bytes_v = 0, bytes_w = 0;
+  bool has_mem = false, has_ws = false;
   int slot = -1;
   int status = QUEUED;
   std::string error;
@@ -133,7 +135,14 @@ struct Server {
   double *fptr(const Request &q) { return reinterpret_cast<double *>(arena.mem.p + q.off_f); }
   double *vptr(const Request &q) { return q.bytes_v ? reinterpret_cast<double *>(arena.mem.p + q.off_v) : nullptr; }
 
+  void release_ws(Request &q) {
+    if (!q.has_ws) return;
+    arena.give(q.off_w, q.bytes_w);
+    q.has_ws = false;
+  }
+
   void release_mem(Request &q) {
+    release_ws(q);
     if (!q.has_mem) return;
     arena.give(q.off_f, q.bytes_f);
     if (q.bytes_v) arena.give(q.off_v, q.bytes_v);
@@ -172,7 +181,7 @@ struct Server {
     q.slot = s;
     q.t_start = Clock::now();
     try {
-      assemble(*q.mesh, a, streams[s]);
+      assemble(*q.mesh, a, streams[s], q.has_ws ? arena.mem.p + q.off_w : nullptr);
       TXA_HIP(hipEventRecord(q.ev, streams[s]));
       q.status = RUNNING;
       slot_req[s] = q.id;
@@ -199,6 +208,7 @@ struct Server {
         const hipError_t e = hipEventQuery(q.ev);
         if (e == hipErrorNotReady) continue;
         q.t_done = Clock::now();
+        release_ws(q);
         if (e == hipSuccess) {
           q.status = DONE;
           lat_ms.push_back(std::chrono::duration<double, std::milli>(q.t_done - q.t_submit).count());
@@ -219,15 +229,22 @@ struct Server {
         auto fs = std::find(slot_req.begin(), slot_req.end(), int64_t(-1));
         if (fs == slot_req.end()) break;
         Request &q = *reqs[queue.front()];
-        size_t of = 0, ov = 0;
+        size_t of = 0, ov = 0, ow = 0;
         if (!arena.take(q.bytes_f, &of)) break;
         if (q.bytes_v && !arena.take(q.bytes_v, &ov)) {
           arena.give(of, q.bytes_f);
           break;
         }
+        if (q.bytes_w && !arena.take(q.bytes_w, &ow)) {
+          arena.give(of, q.bytes_f);
+          if (q.bytes_v) arena.give(ov, q.bytes_v);
+          break;
+        }
         q.off_f = of;
         q.off_v = ov;
+        q.off_w = ow;
         q.has_mem = true;
+        q.has_ws = q.bytes_w != 0;
         queue.pop_front();
         launch(q, static_cast<int>(fs - slot_req.begin()));
       }
@@ -290,7 +307,8 @@ int64_t serve_submit(Server &s, const txa_serve_request &r) {
   q->r = r;
   q->bytes_f = sizeof(double) * static_cast<size_t>(m.ndofs);
   q->bytes_v = r.want_jacobian ? sizeof(double) * static_cast<size_t>(m.nnz) : 0;
-  TXA_REQUIRE(round_up(q->bytes_f) + round_up(q->bytes_v) <= s.arena.cap, TXA_ERR_NOMEM,
+  q->bytes_w = assemble_workspace_bytes(m, r.physics, r.precision, true, r.want_jacobian != 0);
+  TXA_REQUIRE(round_up(q->bytes_f) + round_up(q->bytes_v) + round_up(q->bytes_w) <= s.arena.cap, TXA_ERR_NOMEM,
               "request footprint exceeds the server's HBM budget");
   TXA_HIP(hipEventCreateWithFlags(&q->ev, hipEventDisableTiming));
   std::lock_guard<std::mutex> lk(s.mu);
