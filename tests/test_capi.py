@@ -49,3 +49,33 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def _unbound_names(path):
+    """Names a Python file reads that nothing in it binds (and that are not builtins)."""
+    import ast
+    import builtins
+    tree = ast.parse(open(path).read())
+    bound, loaded = set(dir(builtins)) | {"__file__", "__name__"}, {}
+    for node in ast.walk(tree):
+        if isinstance(node, ast.Name):
+            if isinstance(node.ctx, ast.Load):
+                loaded.setdefault(node.id, node.lineno)
+            else:
+                bound.add(node.id)
+        elif isinstance(node, (ast.FunctionDef, ast.AsyncFunctionDef, ast.ClassDef)):
+            bound.add(node.name)
+        elif isinstance(node, ast.arg):
+            bound.add(node.arg)
+        elif isinstance(node, (ast.Import, ast.ImportFrom)):
+            bound.update((a.asname or a.name).split(".")[0] for a in node.names)
+        elif isinstance(node, ast.ExceptHandler) and node.name:
+            bound.add(node.name)
+    return {n: ln for n, ln in loaded.items() if n not in bound}
+
+
+def test_bench_reads_no_undefined_name():
+    # every code path of bench.py (the GPU ones included) only reads names it defines or imports
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for f in ("bench.py", "__graft_entry__.py"):
+        assert _unbound_names(os.path.join(root, f)) == {}, f

@@ -175,7 +175,17 @@ int txa_assemble(txa_ctx ctx, int mesh_id, const txa_assembly_args *args, void *
   return guarded(ctx, [&] {
     TXA_REQUIRE(args, TXA_ERR_INVALID, "null args");
     Mesh &m = ctx->mesh(mesh_id);
-    txa::assemble(m, *args, static_cast<hipStream_t>(stream));
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t wsb = txa::assemble_workspace_bytes(m, args->physics, args->precision, args->f != nullptr,
+                                                     args->vals != nullptr);
+    void *ws = wsb ? ctx->ws_alloc(wsb, s) : nullptr;
+    try {
+      txa::assemble(m, *args, s, ws);
+    } catch (...) {
+      if (ws) (void)hipFreeAsync(ws, s);
+      throw;
+    }
+    if (ws) TXA_HIP(hipFreeAsync(ws, s));
   });
 }
 

@@ -635,10 +635,8 @@ void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws
   if (!a.f && !a.vals) return;
   size_t rbytes = 0, jbytes = 0;
   ws_parts(m, a.precision, a.f != nullptr, a.vals != nullptr, &rbytes, &jbytes);
-  // the caller's workspace, or a stream-ordered allocation released after the gather
   char *w = static_cast<char *>(ws);
-  const bool own = w == nullptr;
-  if (own) TXA_HIP(hipMallocAsync(reinterpret_cast<void **>(&w), rbytes + jbytes, s));
+  TXA_REQUIRE(w != nullptr, TXA_ERR_LOGIC, "elasticity assembly needs its workspace");
   ElasticParams p;
   p.nq = m.topo == TXA_HEX8 ? 8 : 27;
   p.nelem = m.nelem;
@@ -689,7 +687,6 @@ void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws
     else hipLaunchKernelGGL((k_elastic_gather<27, float>), dim3(grid), dim3(256), 0, s, g);
   }
   TXA_HIP(hipGetLastError());
-  if (own) TXA_HIP(hipFreeAsync(w, s));
 }
 
 }  // namespace txa

@@ -102,9 +102,32 @@ struct Context {
   std::map<int, Batch *> batches;
   std::map<int, Server *> servers;
   int next_id = 1;
+  // stream-ordered pool for assembly workspaces: freed blocks stay mapped for the
+  // next call (release threshold unbounded) until the context is destroyed
+  hipMemPool_t ws_pool = nullptr;
   ~Context() {
     for (auto &kv : servers) serve_free(kv.second);   // joins the worker threads first
     for (auto &kv : batches) batch_free(kv.second);
+    if (ws_pool) {
+      (void)hipSetDevice(device);
+      (void)hipDeviceSynchronize();
+      (void)hipMemPoolDestroy(ws_pool);
+    }
+  }
+  // device workspace of `bytes` for work on stream s; give it back with hipFreeAsync on s
+  void *ws_alloc(size_t bytes, hipStream_t s) {
+    if (!ws_pool) {
+      hipMemPoolProps props{};
+      props.allocType = hipMemAllocationTypePinned;
+      props.location.type = hipMemLocationTypeDevice;
+      props.location.id = device;
+      TXA_HIP(hipMemPoolCreate(&ws_pool, &props));
+      uint64_t keep = UINT64_MAX;
+      TXA_HIP(hipMemPoolSetAttribute(ws_pool, hipMemPoolAttrReleaseThreshold, &keep));
+    }
+    void *p = nullptr;
+    TXA_HIP(hipMallocFromPoolAsync(&p, bytes, ws_pool, s));
+    return p;
   }
   Server &server(int id) {
     auto it = servers.find(id);
@@ -131,9 +154,9 @@ void exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t 
 
 // assembly (txa_assemble.hip)
 void upload_refelem(Mesh &m, DevBuf<double> &tab, hipStream_t s);
-// ws: device workspace of assemble_workspace_bytes(m, a) bytes (256-byte aligned),
-// or null to allocate it stream-ordered on s for the call
-void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws = nullptr);
+// ws: device workspace of assemble_workspace_bytes(...) bytes, 256-byte aligned, for
+// the work on s (may be null when that size is 0)
+void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws);
 size_t assemble_workspace_bytes(const Mesh &m, int physics, int precision, bool want_f, bool want_vals);
 // element-centric elasticity (txa_elastic.hip)
 void node_perm_build(Mesh &m, hipStream_t s);
