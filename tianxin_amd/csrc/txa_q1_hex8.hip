@@ -37,6 +37,11 @@ constexpr double kG = 0.57735026918962576451;   // 1/sqrt(3)
 constexpr double kP = 0.5 * (1.0 + kG);         // P(bit == qbit)
 constexpr double kM = 0.5 * (1.0 - kG);         // P(bit != qbit)
 constexpr int NSTAGE = 44;                      // 36 packed K + 8 R per element
+// Stage slots: FP32 stages are double-buffered (element waves publish layer z in one
+// slot while the owner waves consume layer z-1 from the other, one barrier per
+// layer); an FP64 stage takes one slot (two barriers per layer) to stay within LDS.
+template <typename Real>
+constexpr int stage_slots() { return sizeof(Real) == 4 ? 2 : 1; }
 
 #ifndef TXA_Q1_FP32_RING
 #define TXA_Q1_FP32_RING 0    // 1: FP32 integration uses the one-role LDS-ring kernel (0: warp-specialised)
@@ -52,6 +57,9 @@ constexpr int NSTAGE = 44;                      // 36 packed K + 8 R per element
 #endif
 #ifndef TXA_Q1_NT_STORES
 #define TXA_Q1_NT_STORES 1    // non-temporal stores for the CRS values (written once, not re-read)
+#endif
+#ifndef TXA_Q1_ALIGNED_STORES
+#define TXA_Q1_ALIGNED_STORES 1   // FP32 forms: row stores in 256-byte aligned chunks (no partial lines inside a range)
 #endif
 
 struct Q1DiffParams {
@@ -468,6 +476,10 @@ __device__ __forceinline__ RowMeta load_meta(const Q1DiffParams &p, int32_t L) {
   return m;
 }
 
+// ALIGN: store the range in 256-byte aligned chunks.  It removes the partial lines inside
+// a range (WRITE_SIZE 4.32 -> 3.81 GB at 256^3, the bytes of f and the values), a small
+// gain in the FP32 kernels; the FP64 kernels ran slower with it, so they store unaligned.
+template <bool ALIGN>
 __device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &m, const double (&acc)[27], double r,
                                           double *rowbuf, int grp) {
   const int lane = threadIdx.x & 63, gl = lane & 31, half = lane >> 5;
@@ -514,11 +526,18 @@ __device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &
   wave_lds_sync();   // the group's lanes live in one wavefront
   if (contiguous) {
     const int total = static_cast<int>(end_last - base0);
+    // aligned: each store instruction covers one 256-byte aligned chunk of the range; lanes
+    // before the range's first entry idle in the first chunk, so only the two end chunks
+    // of the range are partial
+    const int lead = ALIGN ? static_cast<int>((reinterpret_cast<uintptr_t>(p.vals + base0) >> 3) & 31) : 0;
+    for (int j = gl - lead; j < total; j += 32) {
+      if (j < 0) continue;
 #if TXA_Q1_NT_STORES
-    for (int j = gl; j < total; j += 32) __builtin_nontemporal_store(buf[j], p.vals + base0 + j);
+      __builtin_nontemporal_store(buf[j], p.vals + base0 + j);
 #else
-    for (int j = gl; j < total; j += 32) p.vals[base0 + j] = buf[j];
+      p.vals[base0 + j] = buf[j];
 #endif
+    }
   } else if (act) {
     scatter_row(p, L, base, m.w, acc);
   }
@@ -567,11 +586,14 @@ __device__ __forceinline__ void load_face_lids(const Q1DiffParams &p, int64_t n0
 // Block = 4 element waves + 4 owner waves.  Per element layer ez the element
 // waves compute layer ez into registers while the owner waves sum layer ez-1
 // from the stage and store the finished rows; then the element waves publish
-// layer ez in the stage.  Loads (element waves) and stores (owner waves) sit in
-// different waves, so neither waits for the other's memory counter.
+// layer ez in the stage (in the other slot when the stage is double-buffered, so
+// the publication needs no wait for the owners).  Loads (element waves) and
+// stores (owner waves) sit in different waves, so neither waits for the other's
+// memory counter.
 template <int KIND, typename Real, bool TRANS>
 __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by, int bz, Real *stage,
                                            double *rowbuf) {
+  constexpr int NSLOT = stage_slots<Real>();
   const bool is_elem = threadIdx.x < NE;
   const int t = is_elem ? threadIdx.x : threadIdx.x - NE;
   const int tx = t % TX, ty = t / TX;
@@ -635,14 +657,15 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
           for (int i = 0; i < 8; ++i) o.R[i] = Real(0);
         }
       }
-      __syncthreads();   // the owners are done with the stage
+      if constexpr (NSLOT == 1) __syncthreads();   // the owners are done with the stage
       if (ez <= ez1) {
+        Real *st = stage + (NSLOT == 2 ? (ez & 1) * (NSTAGE * NE) : 0);
 #pragma unroll
-        for (int i = 0; i < 36; ++i) stage[i * NE + t] = o.K[i];
+        for (int i = 0; i < 36; ++i) st[i * NE + t] = o.K[i];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) stage[(36 + i) * NE + t] = o.R[i];
+        for (int i = 0; i < 8; ++i) st[(36 + i) * NE + t] = o.R[i];
       }
-      __syncthreads();   // layer ez is in the stage
+      __syncthreads();   // layer ez is in the stage (two slots: the owners are also done with layer ez-1)
     }
   } else {
     // ---------------- owner waves: vertex column (oi, oj) is corner (1,1) of element
@@ -667,16 +690,17 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
       if (ez > ez0) {
         // sum element layer e = ez-1 from the stage
         const int e = ez - 1;
+        const Real *st = stage + (NSLOT == 2 ? (e & 1) * (NSTAGE * NE) : 0);
         RowMeta next;
         if (owner) {
           next = load_meta(p, e + 1 >= k0 ? Lq : -1);
           Lq = e + 2 <= p.nz ? p.lid[ocol + sxy * (e + 2)] : -1;
-          take_corner<1, 1>(stage, t, acc, accn, rc, rn);
-          take_corner<0, 1>(stage, t + 1, acc, accn, rc, rn);
-          take_corner<1, 0>(stage, t + TX, acc, accn, rc, rn);
-          take_corner<0, 0>(stage, t + TX + 1, acc, accn, rc, rn);
+          take_corner<1, 1>(st, t, acc, accn, rc, rn);
+          take_corner<0, 1>(st, t + 1, acc, accn, rc, rn);
+          take_corner<1, 0>(st, t + TX, acc, accn, rc, rn);
+          take_corner<0, 0>(st, t + TX + 1, acc, accn, rc, rn);
         }
-        if (e >= k0) emit_rows(p, meta, acc, rc, rowbuf, t >> 5);
+        if (e >= k0) emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, meta, acc, rc, rowbuf, t >> 5);
         meta = next;
 #pragma unroll
         for (int s = 0; s < 18; ++s) { acc[s] = accn[s]; accn[s] = 0.0; }
@@ -685,11 +709,11 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
         rc = rn;
         rn = 0.0;
       }
-      __syncthreads();   // done with the stage
+      if constexpr (NSLOT == 1) __syncthreads();   // done with the stage
       __syncthreads();   // next layer is in the stage
     }
     // the top vertex layer of the mesh has no element layer above it
-    if (k1 == p.nz + 1) emit_rows(p, meta, acc, rc, rowbuf, t >> 5);
+    if (k1 == p.nz + 1) emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, meta, acc, rc, rowbuf, t >> 5);
   }
 }
 
@@ -829,7 +853,7 @@ __device__ __forceinline__ void ring_block(const Q1DiffParams &p, int bx, int by
     if (more)
 #pragma unroll
       for (int r = 0; r < 2; ++r) store_col(r, ez & 1, fv[r]);
-    if (ez >= k0) emit_rows(p, meta, acc, rc, lds, t >> 5);
+    if (ez >= k0) emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, meta, acc, rc, lds, t >> 5);
 #pragma unroll
     for (int s = 0; s < 18; ++s) { acc[s] = accn[s]; accn[s] = 0.0; }
 #pragma unroll
@@ -839,13 +863,13 @@ __device__ __forceinline__ void ring_block(const Q1DiffParams &p, int bx, int by
     Lcur = Lnext;
     __syncthreads();   // row buffers copied out, next face visible
   }
-  if (k1 == p.nz + 1) emit_rows(p, load_meta(p, Lcur), acc, rc, lds, t >> 5);
+  if (k1 == p.nz + 1) emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, load_meta(p, Lcur), acc, rc, lds, t >> 5);
 }
 
 
 template <int KIND, typename Real, bool TRANS>
 __global__ __launch_bounds__(NT) void k_q1_hex8_diffusion(Q1DiffParams p) {
-  __shared__ Real stage[NSTAGE * NE];
+  __shared__ Real stage[stage_slots<Real>() * NSTAGE * NE];
   __shared__ double rowbuf[8 * ROWBUF];
   spec_block<KIND, Real, TRANS>(p, blockIdx.x, blockIdx.y, blockIdx.z, stage, rowbuf);
 }
@@ -911,7 +935,7 @@ __global__ __launch_bounds__(NE) void k_q1_batch_ring(const Q1DiffParams *__rest
 template <int KIND, typename Real, bool TRANS>
 __global__ __launch_bounds__(NT) void k_q1_batch_spec(const Q1DiffParams *__restrict__ prm,
                                                       const int4 *__restrict__ items, int nitems, BatchQueue *q) {
-  __shared__ Real stage[NSTAGE * NE];
+  __shared__ Real stage[stage_slots<Real>() * NSTAGE * NE];
   __shared__ double rowbuf[8 * ROWBUF];
   __shared__ int slot;
   for (;;) {
