@@ -867,18 +867,48 @@ __device__ __forceinline__ void ring_block(const Q1DiffParams &p, int bx, int by
 }
 
 
+#ifndef TXA_Q1_XCD_ORDER
+#define TXA_Q1_XCD_ORDER 1   // FP32 forms: each XCD takes a contiguous run of tiles (neighbours share its L2)
+#endif
+
+// Tile of this workgroup.  Workgroups are dealt to the 8 XCDs round robin (linear id
+// mod 8); each XCD has its own L2.  XCD_ORDER hands XCD c the c-th contiguous run of
+// the tile order (x fastest, then y, then the z slab), which keeps tiles that share
+// halo vertices on one L2 at about the same time: FP32 1.957 -> 1.93 ms at 256^3.  The
+// FP64 kernels, which load each face without a prefetch, ran 11% slower with it.
+template <bool XCD_ORDER>
+__device__ __forceinline__ void tile_of_block(int &bx, int &by, int &bz) {
+  if (XCD_ORDER) {
+    constexpr int NXCD = 8;
+    const int gx = gridDim.x, gy = gridDim.y;
+    const int n = gx * gy * static_cast<int>(gridDim.z);
+    const int b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const int c = b % NXCD, i = b / NXCD, q = n / NXCD, r = n % NXCD;
+    const int t = c * q + min(c, r) + i;   // a bijection of [0, n): XCD c holds q + (c < r) blocks
+    bx = t % gx;
+    by = (t / gx) % gy;
+    bz = t / (gx * gy);
+  } else {
+    bx = blockIdx.x; by = blockIdx.y; bz = blockIdx.z;
+  }
+}
+
 template <int KIND, typename Real, bool TRANS>
 __global__ __launch_bounds__(NT) void k_q1_hex8_diffusion(Q1DiffParams p) {
   __shared__ Real stage[stage_slots<Real>() * NSTAGE * NE];
   __shared__ double rowbuf[8 * ROWBUF];
-  spec_block<KIND, Real, TRANS>(p, blockIdx.x, blockIdx.y, blockIdx.z, stage, rowbuf);
+  int bx, by, bz;
+  tile_of_block<TXA_Q1_XCD_ORDER && sizeof(Real) == 4>(bx, by, bz);
+  spec_block<KIND, Real, TRANS>(p, bx, by, bz, stage, rowbuf);
 }
 
 template <int KIND, typename Real, bool TRANS>
 __global__ __launch_bounds__(NE) void k_q1_hex8_ring(Q1DiffParams p) {
   __shared__ double lds[ring_union_doubles<Real>()];
   __shared__ double facebuf[2 * ring_nval<TRANS>() * FC];
-  ring_block<KIND, Real, TRANS>(p, blockIdx.x, blockIdx.y, blockIdx.z, lds, facebuf);
+  int bx, by, bz;
+  tile_of_block<TXA_Q1_XCD_ORDER && sizeof(Real) == 4>(bx, by, bz);
+  ring_block<KIND, Real, TRANS>(p, bx, by, bz, lds, facebuf);
 }
 
 
