@@ -168,6 +168,35 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir):
     print(out, TRAFFIC_FILE)
 
 
+def measured_hbm_peaks(gib=2, reps=10):
+    """Achievable HBM bandwidth on this GPU (GB/s): a device-to-device copy of `gib` GiB
+    (read + write bytes counted) and a fill of the same size (write bytes), each the
+    median of `reps` event-timed runs after a warm-up."""
+    import numpy as np
+    import torch
+    n = gib * (1 << 30) // 8
+    src = torch.ones(n, device="cuda", dtype=torch.float64)
+    dst = torch.empty_like(src)
+
+    def med(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1) / 1e3)
+        return float(np.median(ts))
+    t_copy = med(lambda: dst.copy_(src))
+    t_fill = med(lambda: dst.fill_(0.5))
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2 * 8 * n / t_copy / 1e9, 8 * n / t_fill / 1e9
+
+
 def log_uniform_sizes(seed, count, lo, hi):
     from tianxin_amd import synth
     import numpy as np
@@ -675,6 +704,7 @@ def main():
     ms = float(per32.mean())
     value = nel / (ms / 1e3)
     kt32 = float(kper32.mean())
+    copy_gbs, fill_gbs = measured_hbm_peaks()
     achieved = BYTES_PER_ELEM * nel / (kt32 / 1e3) / 1e9
     traffic = None
     if n == 256 and os.path.exists(TRAFFIC_FILE):
@@ -698,7 +728,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / (HBM_PEAK / 1e9), "traffic": traffic,
                      "kernel": FP32_KERNEL, "kernel_ms": kt32,
-                     "bytes_per_element": BYTES_PER_ELEM},
+                     "bytes_per_element": BYTES_PER_ELEM,
+                     # measured on this GPU: device-to-device copy (read + write bytes) and fill (write bytes)
+                     "measured_copy_GBs": copy_gbs, "measured_fill_GBs": fill_gbs,
+                     "traffic_GBs": traffic / (kt32 / 1e3) / 1e9 if traffic else None,
+                     "frac_of_measured_copy": achieved / copy_gbs},
     }
     if args.copy_calibration:
         del rp
