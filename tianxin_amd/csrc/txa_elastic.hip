@@ -92,6 +92,10 @@ struct Dims {
   static constexpr int S = 2 * R + 1;
   static constexpr int MP = (ND + 15) / 16 * 16;       // padded operand rows (MFMA)
   static constexpr int KP = 32;                        // padded quadrature dimension (MFMA)
+  // LDS row stride of the operand parts, in bf16: 40 (80 B) puts the 16 rows of an
+  // MFMA tile on distinct bank pairs (a 64 B stride put them on two banks: 8-way
+  // conflicts on every operand read) and keeps rows 16-byte aligned
+  static constexpr int KS = 40;
 };
 
 template <int NB, int NQ, int MODE>
@@ -116,12 +120,14 @@ struct Smem {
   // FP64 rows are not padded (81 doubles is already an odd bank stride), which keeps
   // the FP64 form under half of a CU's LDS: two workgroups per CU.
   static constexpr int TP = MODE == MODE_F64 ? D::ND : D::ND + 1;
+  // (the MFMA form uses neither T nor G of `v`: they shrink to one entry there)
+  static constexpr int VR = MODE == MODE_SPLIT ? 1 : D::ND;
   union alignas(16) {
     struct {
-      TReal T[D::ND][TP];
-      TReal G[NQ * 3 * NB];
+      TReal T[VR][TP];
+      TReal G[MODE == MODE_SPLIT ? 1 : NQ * 3 * NB];
     } v;
-    short ops[3 * D::MP * D::KP];
+    short ops[3 * D::MP * D::KS];
     float T32[D::ND][D::ND + 1];
   } w;
 };
@@ -223,10 +229,10 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
     }
     if constexpr (MODE == MODE_SPLIT) {
       // operand rows >= ND are zero (T covered them in the previous element)
-      constexpr int PADV = (D::MP - ND) * D::KP / 8;   // 16-byte vectors per part
+      constexpr int PADV = (D::MP - ND) * D::KS / 8;   // 16-byte vectors per part
       for (int i = t; i < 3 * PADV; i += BS) {
         const int s = i / PADV, o = i % PADV;
-        reinterpret_cast<uint4 *>(sm.w.ops + (s * D::MP + ND) * D::KP)[o] = make_uint4(0, 0, 0, 0);
+        reinterpret_cast<uint4 *>(sm.w.ops + (s * D::MP + ND) * D::KS)[o] = make_uint4(0, 0, 0, 0);
       }
     }
     __syncthreads();
@@ -246,15 +252,16 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
         sm.M[a][b] = static_cast<TReal>(sg * s);
       }
     __syncthreads();
-    // ---- physical gradients at the quadrature points (MFMA mode: also the zero columns q >= NQ)
+    // ---- physical gradients at the quadrature points (MFMA mode: also the zero columns q >= NQ);
+    // q runs fastest over the threads, so the operand writes of a wave are consecutive in LDS
     constexpr int QG = MODE == MODE_SPLIT ? D::KP : NQ;
     for (int i = t; i < QG * NB; i += BS) {
-      const int q = i / NB, a = i % NB;
+      const int q = i % QG, a = i / QG;
       if (MODE == MODE_SPLIT && q >= NQ) {
 #pragma unroll
         for (int k = 0; k < 3; ++k)
 #pragma unroll
-          for (int s = 0; s < 3; ++s) sm.w.ops[(s * D::MP + k * NB + a) * D::KP + q] = 0;
+          for (int s = 0; s < 3; ++s) sm.w.ops[(s * D::MP + k * NB + a) * D::KS + q] = 0;
         continue;
       }
       const int qx = q % Q1, qy = (q / Q1) % Q1, qz = q / (Q1 * Q1);
@@ -271,9 +278,9 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
           short s0, s1, s2;
           split3(static_cast<float>(sq * g), s0, s1, s2);
           const int m = k * NB + a;
-          sm.w.ops[(0 * D::MP + m) * D::KP + q] = s0;
-          sm.w.ops[(1 * D::MP + m) * D::KP + q] = s1;
-          sm.w.ops[(2 * D::MP + m) * D::KP + q] = s2;
+          sm.w.ops[(0 * D::MP + m) * D::KS + q] = s0;
+          sm.w.ops[(1 * D::MP + m) * D::KS + q] = s1;
+          sm.w.ops[(2 * D::MP + m) * D::KS + q] = s2;
         } else {
           sm.w.v.G[(q * 3 + k) * NB + a] = static_cast<TReal>(g);
         }
@@ -302,8 +309,8 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
           v4s a_[3], b_[3];
 #pragma unroll
           for (int s = 0; s < 3; ++s) {
-            a_[s] = *reinterpret_cast<const v4s *>(O + (s * D::MP + ma) * D::KP + k0);
-            b_[s] = *reinterpret_cast<const v4s *>(O + (s * D::MP + nb) * D::KP + k0);
+            a_[s] = *reinterpret_cast<const v4s *>(O + (s * D::MP + ma) * D::KS + k0);
+            b_[s] = *reinterpret_cast<const v4s *>(O + (s * D::MP + nb) * D::KS + k0);
           }
           // products of order 2^-16, 2^-8 and 1 (smallest first)
           acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[2], b_[0], acc[k], 0, 0, 0);
