@@ -85,6 +85,30 @@ __device__ __forceinline__ void split3(float x, short &s0, short &s1, short &s2)
   s2 = bf16_rne(r2, v2);
 }
 
+// One DPP lane move of an FP64 value (both halves); lanes of rows outside row_mask get 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROW_MASK, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Fixed-order sum over groups of GW (8 or 32) consecutive lanes, on the VALU (DPP):
+// quad swaps (lanes ^1, ^2), the half-row mirror (8 lanes) and the row mirror (16
+// lanes); for 32 lanes the first row's total is then broadcast into the second row.
+// The group sum ends in every lane of the group (GW = 8) or in its second row (GW = 32).
+template <int GW>
+__device__ __forceinline__ double group_sum(double x) {
+  x += dpp_f64<0xB1, 0xf>(x);    // quad_perm [1,0,3,2]
+  x += dpp_f64<0x4E, 0xf>(x);    // quad_perm [2,3,0,1]
+  x += dpp_f64<0x141, 0xf>(x);   // row_half_mirror
+  if constexpr (GW == 32) {
+    x += dpp_f64<0x140, 0xf>(x);   // row_mirror
+    x += dpp_f64<0x142, 0xa>(x);   // row_bcast15 into rows 1 and 3
+  }
+  return x;
+}
+
 template <int NB>
 struct Dims {
   static constexpr int ND = 3 * NB;                    // element DOFs
@@ -394,12 +418,11 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
             Tref(i * NB + a, j * NB + b) = static_cast<TReal>(jv);
           }
       }
-      // row residuals: fixed-order reduction over the group's b
+      // row residuals: fixed-order reduction over the group's b (lanes b >= NB add 0)
+      static_assert(GW == 8 || GW == 32, "group width");
 #pragma unroll
-      for (int off = GW / 2; off > 0; off >>= 1)
-#pragma unroll
-        for (int i = 0; i < 3; ++i) rr[i] += __shfl_xor(rr[i], off, 64);
-      if (a < NB && b == 0)
+      for (int i = 0; i < 3; ++i) rr[i] = group_sum<GW>(rr[i]);
+      if (a < NB && b == (GW == 32 ? 16 : 0))
 #pragma unroll
         for (int i = 0; i < 3; ++i) sm.r[3 * a + i] = rr[i];
     }
