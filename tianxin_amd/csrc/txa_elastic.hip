@@ -50,7 +50,8 @@ struct ElasticParams {
   const double *x, *xdot, *xddot;
   double alpha, beta, gamma2;
   int transient;
-  void *jbuf;        // element Jacobian blocks [E][ND][ND], row-major (FP32, FP64 in FP64 mode); null: none
+  void *jbuf;        // element Jacobian blocks [E][ND][ND] in the Gram layout: row i NB + a = DOF (a, i),
+                     // column j NB + b = DOF (b, j) (FP32, FP64 in FP64 mode); null: none
   double *rbuf;      // element residuals [E][ND]; null: none
 };
 
@@ -156,8 +157,10 @@ struct Smem {
   } w;
 };
 
+// The MFMA form is held to three waves per SIMD (at most 168 registers, AGPRs included):
+// its LDS allows four workgroups per CU, registers would otherwise allow two.
 template <int NB, int NQ, int BS, int MODE>
-__global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
+__global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 3 : 1) void k_elastic(ElasticParams p) {
   using D = Dims<NB>;
   using TReal = typename Smem<NB, NQ, MODE>::TReal;
   constexpr int ND = D::ND;
@@ -205,18 +208,41 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
   for (int i = t; i < NB * 3; i += BS) sm.off[i / 3][i % 3] = NB == 8 ? kHexV[i / 3][i % 3] : kQ2[i / 3][i % 3];
   __syncthreads();
 
-  for (int64_t e = blockIdx.x; e < p.nelem; e += gridDim.x) {
-    // ---- gather: vertices, state
+  // Gather data of the next element, fetched while the current one is computed: the
+  // connectivity and LIDs (thread t < 24: one vertex coordinate; t < ND: one DOF) once
+  // the Gram matrix is done, the coordinates and state they address before the block
+  // store (short register lifetimes: the pair phase is the register peak).
+  static_assert(BS >= ND && BS >= 24, "one gather entry per thread");
+  int64_t gn = 0;
+  int32_t gL = 0;
+  double gX = 0.0, gu = 0.0, gv = 0.0, gac = 0.0;
+  auto fetch_index = [&](int64_t e) {
+    if (e >= p.nelem) return;
+    if (t < 24) gn = p.conn[e * p.nconn + t / 3];
+    if (t < ND) gL = p.elem_lid[e * ND + t];
+  };
+  auto fetch_data = [&](int64_t e) {
+    if (e >= p.nelem) return;
     if (t < 24) {
-      const int vtx = t / 3, d = t % 3;
-      const int64_t n = p.conn[e * p.nconn + vtx];
-      sm.X[vtx][d] = d == 0 ? p.X[n] : (d == 1 ? p.Y[n] : p.Z[n]);
+      const int d = t % 3;
+      gX = d == 0 ? p.X[gn] : (d == 1 ? p.Y[gn] : p.Z[gn]);
     }
-    for (int i = t; i < ND; i += BS) {
-      const int32_t L = p.elem_lid[e * ND + i];
-      sm.u[i] = p.x[L];
-      sm.v[i] = tr ? p.xdot[L] : 0.0;
-      sm.a[i] = tra ? p.xddot[L] : 0.0;
+    if (t < ND) {
+      gu = p.x[gL];
+      gv = tr ? p.xdot[gL] : 0.0;
+      gac = tra ? p.xddot[gL] : 0.0;
+    }
+  };
+  fetch_index(blockIdx.x);
+  fetch_data(blockIdx.x);
+
+  for (int64_t e = blockIdx.x; e < p.nelem; e += gridDim.x) {
+    // ---- gather: vertices, state (fetched during the previous element)
+    if (t < 24) sm.X[t / 3][t % 3] = gX;
+    if (t < ND) {
+      sm.u[t] = gu;
+      sm.v[t] = gv;
+      sm.a[t] = gac;
     }
     __syncthreads();
     // ---- geometry (FP64)
@@ -376,6 +402,7 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
       }
     }
     __syncthreads();
+    fetch_index(e + gridDim.x);
     // ---- 3x3 blocks of node pairs: residual rows, Jacobian values written over the
     // pair's own T entries (the block of (a, b) depends only on those nine values)
     //   K_ij = lambda T[(i,a),(j,b)] + mu T[(j,a),(i,b)] + delta_ij mu sum_k T[(k,a),(k,b)]
@@ -427,13 +454,13 @@ __global__ __launch_bounds__(BS) void k_elastic(ElasticParams p) {
         for (int i = 0; i < 3; ++i) sm.r[3 * a + i] = rr[i];
     }
     __syncthreads();
-    // ---- element block to the workspace, row-major (DOF 3a + i), consecutive lanes on
-    // consecutive entries
+    fetch_data(e + gridDim.x);
+    // ---- element block to the workspace in the Gram layout (row i NB + a is DOF (a, i),
+    // column j NB + b is DOF (b, j)): consecutive lanes on consecutive entries
     if (p.jbuf)
       for (int idx = t; idx < ND * ND; idx += BS) {
-        const int P = idx / ND, Q = idx % ND;
-        const int a = P / 3, i = P % 3, b = Q / 3, j = Q % 3;
-        static_cast<TReal *>(p.jbuf)[e * (ND * ND) + idx] = Tref(i * NB + a, j * NB + b);
+        const int r = idx / ND, c = idx % ND;
+        static_cast<TReal *>(p.jbuf)[e * (ND * ND) + idx] = Tref(r, c);
       }
     __syncthreads();
     if (p.rbuf)
@@ -487,13 +514,17 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
   // the wave index is uniform: everything derived from the node stays in scalar registers
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   constexpr int NR = (3 * ND + 63) / 64;   // block entries per lane and element
-  // per lane and r: lattice offset index of the entry's column node, plus R in each axis
-  int lterm[NR];
+  // per lane and r: entry idx = lane + 64 r of the node's three block rows (row field
+  // idx / ND, Gram column idx % ND = field j, node bn), the lattice offset index of its
+  // column node plus R in each axis, and its offset in the element block
+  int lterm[NR], joff[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
-    const int bn = ((lane + 64 * r) % ND) / 3;
+    const int idx = lane + 64 * r, c = idx % ND;
+    const int bn = c % NB;
     const int8_t *o = NB == 8 ? kHexV[bn] : kQ2[bn];
     lterm[r] = (o[0] + R) + S * (o[1] + R) + S * S * (o[2] + R);
+    joff[r] = (idx / ND) * NB * ND + c;
   }
   for (int i = threadIdx.x; i < NB; i += 256) {
     const int8_t *o = NB == 8 ? kHexV[i] : kQ2[i];
@@ -543,7 +574,7 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
         pos[c][r] = 0xffffu;
         const int idx = lane + 64 * r;
         if (c < ne && idx < 3 * ND && J) {
-          val[c][r] = J[(ce[c] * ND + 3 * ca[c]) * ND + idx];   // rows 3a .. 3a+2 of the element block
+          val[c][r] = J[(ce[c] * ND + ca[c]) * ND + joff[r]];   // Gram rows a, NB + a, 2 NB + a
           pos[c][r] = np[lterm[r] - cs[c]];
         }
       }
@@ -561,7 +592,7 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
           const int idx = lane + 64 * r;
-          const int i = idx / ND, j = (idx % ND) % 3;
+          const int i = idx / ND, j = (idx % ND) / NB;
           const uint16_t ps = pos[c][r];
           if (idx < 3 * ND && ps != 0xffffu && ps + j < len) b[i * len + ps + j] += static_cast<double>(val[c][r]);
         }
