@@ -305,6 +305,18 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 3 : 1) void k_elastic(Elas
     // ---- physical gradients at the quadrature points (MFMA mode: also the zero columns q >= NQ);
     // q runs fastest over the threads, so the operand writes of a wave are consecutive in LDS
     constexpr int QG = MODE == MODE_SPLIT ? D::KP : NQ;
+    // when BS is a multiple of QG every thread keeps one q: its J^-1 and sqrt(w |det J|)
+    // are read from LDS once
+    constexpr bool QFIX = BS % QG == 0;
+    double Jq[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, sqq = 1.0;
+    if constexpr (QFIX) {
+      const int q = t % QG;
+      if (q < NQ) {
+#pragma unroll
+        for (int r = 0; r < 9; ++r) Jq[r] = sm.Ji[q][r];
+        if constexpr (MODE == MODE_SPLIT) sqq = sm.sw[q];
+      }
+    }
     for (int i = t; i < QG * NB; i += BS) {
       const int q = i % QG, a = i / QG;
       if (MODE == MODE_SPLIT && q >= NQ) {
@@ -319,10 +331,11 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 3 : 1) void k_elastic(Elas
       const double lx = sm.bL[ax][qx], ly = sm.bL[ay][qy], lz = sm.bL[az][qz];
       const double dn[3] = {sm.bdL[ax][qx] * ly * lz, lx * sm.bdL[ay][qy] * lz, lx * ly * sm.bdL[az][qz]};
       const double *Ji = sm.Ji[q];
-      const double sq = MODE == MODE_SPLIT ? sm.sw[q] : 1.0;
+      auto ji = [&](int r) { return QFIX ? Jq[r] : Ji[r]; };
+      const double sq = QFIX ? sqq : (MODE == MODE_SPLIT ? sm.sw[q] : 1.0);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const double g = Ji[0 * 3 + k] * dn[0] + Ji[1 * 3 + k] * dn[1] + Ji[2 * 3 + k] * dn[2];
+        const double g = ji(0 * 3 + k) * dn[0] + ji(1 * 3 + k) * dn[1] + ji(2 * 3 + k) * dn[2];
         if constexpr (MODE == MODE_SPLIT) {
           // operand [m = (k,a)][q] = sqrt(w |det J|) g in three bf16 parts: T = O^T O
           short s0, s1, s2;
