@@ -103,6 +103,23 @@ int txa_graph_build(txa_ctx ctx, int mesh_id, int64_t *nnz);
 int txa_graph_copy(txa_ctx ctx, int mesh_id, int64_t *rowptr, int32_t *col);   /* host outputs */
 int txa_graph_device(txa_ctx ctx, int mesh_id, const int64_t **rowptr, const int32_t **col);
 
+/* The assembled system in the MatrixMarket exchange format: A (CRS values vals
+ * on the mesh's graph) as "coordinate real general", f and x as "array real
+ * general".  vals, f and x are device arrays; a null array or path skips that
+ * file.  Indices are GID + 1, rows in LID order, values in the shortest
+ * decimal form that reads back to the same double.
+ * Replaces TpetraLinearObjContainer::writeMatrixMarket
+ * (disc-fe/src/lof/Panzer_TpetraLinearObjContainer.hpp:359-367). */
+int txa_write_matrix_market(txa_ctx ctx, int mesh_id, const double *vals, const double *f, const double *x,
+                            const char *a_path, const char *b_path, const char *x_path);
+
+/* Reads a MatrixMarket "array real general" vector of n entries into the device
+ * array dst (entry i to LID i).  TXA_ERR_INVALID for a missing or malformed
+ * file, or one of another length.
+ * Replaces TpetraLinearObjFactory::readVector
+ * (disc-fe/src/lof/Panzer_TpetraLinearObjFactory_impl.hpp:743-772). */
+int txa_read_vector_mm(txa_ctx ctx, const char *path, double *dst, int64_t n);
+
 /* One residual + Jacobian evaluation.  All arrays are device memory.
  * f (ndofs) and vals (nnz) are overwritten with the assembled sums, Dirichlet
  * rows applied last (f_i = x_i - g_i, identity matrix row).

@@ -109,6 +109,9 @@ def load_library(path: str = LIB_PATH):
                                  ctypes.POINTER(P), ctypes.POINTER(P)]
     L.txa_serve_release.argtypes = [P, ctypes.c_int, I64]
     L.txa_serve_stats.argtypes = [P, ctypes.c_int, P]
+    L.txa_write_matrix_market.argtypes = [P, ctypes.c_int, P, P, P, ctypes.c_char_p, ctypes.c_char_p,
+                                          ctypes.c_char_p]
+    L.txa_read_vector_mm.argtypes = [P, ctypes.c_char_p, P, I64]
     _lib = L
     return L
 
@@ -240,6 +243,16 @@ class BrickMesh:
         col = np.zeros(max(self.nnz, 1), np.int32) if with_cols else None
         self.eng.check(self.eng.lib.txa_graph_copy(self.eng.h, self.id, _np(rp), None if col is None else _np(col)))
         return rp, (col[: self.nnz] if with_cols else None)
+
+    def write_matrix_market(self, vals=None, f=None, x=None, a_path=None, b_path=None, x_path=None):
+        """A (vals on this graph), f and x to MatrixMarket files (TpetraLinearObjContainer::writeMatrixMarket)."""
+        enc = lambda s: None if s is None else os.fsencode(s)
+        self.eng.check(self.eng.lib.txa_write_matrix_market(self.eng.h, self.id, _dptr(vals), _dptr(f), _dptr(x),
+                                                            enc(a_path), enc(b_path), enc(x_path)))
+
+    def read_vector(self, path, dst):
+        """A MatrixMarket array file of ndofs entries into the device tensor dst (LinearObjFactory::readVector)."""
+        self.eng.check(self.eng.lib.txa_read_vector_mm(self.eng.h, os.fsencode(path), _dptr(dst), int(dst.numel())))
 
     def assemble(self, *, stream=None, **kw):
         """One residual + Jacobian evaluation into device tensors f [ndofs] and vals [nnz].

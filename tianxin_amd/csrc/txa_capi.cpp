@@ -171,6 +171,22 @@ int txa_graph_device(txa_ctx ctx, int mesh_id, const int64_t **rowptr, const int
   });
 }
 
+int txa_write_matrix_market(txa_ctx ctx, int mesh_id, const double *vals, const double *f, const double *x,
+                            const char *a_path, const char *b_path, const char *x_path) {
+  return guarded(ctx, [&] {
+    Mesh &m = ctx->mesh(mesh_id);
+    TXA_HIP(hipDeviceSynchronize());   // the arrays may still be written by queued work
+    txa::write_matrix_market(m, vals, f, x, a_path, b_path, x_path);
+  });
+}
+
+int txa_read_vector_mm(txa_ctx ctx, const char *path, double *dst, int64_t n) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(path && (dst || n == 0) && n >= 0, TXA_ERR_INVALID, "null argument or negative length");
+    txa::read_vector_mm(path, dst, n);
+  });
+}
+
 int txa_assemble(txa_ctx ctx, int mesh_id, const txa_assembly_args *args, void *stream) {
   return guarded(ctx, [&] {
     TXA_REQUIRE(args, TXA_ERR_INVALID, "null args");

@@ -152,6 +152,11 @@ void dofmap_build(Mesh &m, int nfields, hipStream_t s);
 void graph_build(Mesh &m, hipStream_t s);
 void exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s);
 
+// MatrixMarket export / import (txa_io.cpp)
+void write_matrix_market(Mesh &m, const double *vals, const double *f, const double *x, const char *a_path,
+                         const char *b_path, const char *x_path);
+void read_vector_mm(const char *path, double *dst, int64_t n);
+
 // assembly (txa_assemble.hip)
 void upload_refelem(Mesh &m, DevBuf<double> &tab, hipStream_t s);
 // ws: device workspace of assemble_workspace_bytes(...) bytes, 256-byte aligned, for
