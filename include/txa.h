@@ -37,6 +37,15 @@ extern "C" {
 #define TXA_ERR_NOMEM 4       /* device allocation failed */
 #define TXA_ERR_UNSUPPORTED 5 /* combination not implemented */
 
+/* Side sets of the brick: the shards hex side ordinals the cube mesh uses
+ * (adapters-stk/src/stk_interface/Panzer_STK_CubeHexMeshFactory.cpp:660-760). */
+#define TXA_SIDE_BOTTOM 0 /* y = 0 */
+#define TXA_SIDE_RIGHT 1  /* x = max */
+#define TXA_SIDE_TOP 2    /* y = max */
+#define TXA_SIDE_LEFT 3   /* x = 0 */
+#define TXA_SIDE_BACK 4   /* z = 0 */
+#define TXA_SIDE_FRONT 5  /* z = max */
+
 /* element topologies (shards: Hexahedron<8>, Tetrahedron<4> from a 6-tet split
  * of each hex, HGRAD_HEX_C2 on Hexahedron<8> geometry) */
 #define TXA_HEX8 1
@@ -146,6 +155,16 @@ typedef struct {
   const int32_t *bc_lids;      /* [n_bc] Dirichlet LIDs, or NULL */
   const double *bc_vals;       /* [n_bc] prescribed values, or NULL (= 0) */
   int64_t n_bc;
+  /* Neumann flux on sides of the brick (hex8 and hex27 meshes), added to f before
+   * the Dirichlet rows: f(b, i) += neumann_values[k nf + i] * (integral over side
+   * neumann_sides[k] of N_b dS), Gauss rule of the volume degree on each face.
+   * Sides (TXA_SIDE_*) are the shards hex side ordinals of the cube mesh's side
+   * sets.  Host arrays, read during the call.
+   * Replaces TianXin::Flux::evaluateFields
+   * (disc-fe/src/evaluators/TianXin_Neumann_impl.hpp:141-160) on a side workset. */
+  int n_neumann;
+  const int *neumann_sides;     /* [n_neumann] */
+  const double *neumann_values; /* [n_neumann][nf] flux per field */
 } txa_assembly_args;
 
 int txa_assemble(txa_ctx ctx, int mesh_id, const txa_assembly_args *args, void *stream);

@@ -209,3 +209,64 @@ def dirichlet(lids, g, x, rowptr, col, f, vals):
     lib().orc_dirichlet(lids.size, _p(lids), _p(g), _p(np.ascontiguousarray(x)), _p(rowptr), _p(col),
                         _p(f), _p(vals))
     return f, vals
+
+
+# Element-local node positions on the element's node lattice: shards vertex order (hex8) and the
+# Q2 order (hex27), the orders orc_mesh_build numbers the element's nodes in.
+HEX8_NODES = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+HEX27_NODES = np.array([[0, 0, 0], [2, 0, 0], [2, 2, 0], [0, 2, 0], [0, 0, 2], [2, 0, 2], [2, 2, 2], [0, 2, 2],
+                        [1, 0, 0], [2, 1, 0], [1, 2, 0], [0, 1, 0], [1, 0, 2], [2, 1, 2], [1, 2, 2], [0, 1, 2],
+                        [0, 0, 1], [2, 0, 1], [2, 2, 1], [0, 2, 1], [1, 0, 1], [2, 1, 1], [1, 2, 1], [0, 1, 1],
+                        [1, 1, 0], [1, 1, 2], [1, 1, 1]])
+# side ordinal -> (normal axis, at the upper end): the shards hex sides of the cube mesh's side sets
+# (adapters-stk/src/stk_interface/Panzer_STK_CubeHexMeshFactory.cpp:660-760)
+SIDES = {0: (1, 0), 1: (0, 1), 2: (1, 1), 3: (0, 0), 4: (2, 0), 5: (2, 1)}
+
+
+def _lagrange(step, a, t):
+    if step == 1:
+        return 0.5 * (1.0 - t) if a == 0 else 0.5 * (1.0 + t)
+    return (0.5 * t * (t - 1.0), 1.0 - t * t, 0.5 * t * (t + 1.0))[a]
+
+
+def neumann(topo, nx, ny, nz, X, ev, elem_lid, nf, side, values, f):
+    """Adds the Neumann flux of one side of the brick into f and returns f.
+
+    Restates TianXin::Flux::evaluateFields (disc-fe/src/evaluators/TianXin_Neumann_impl.hpp:141-160):
+    on the side workset, residual(cell, b) = val * sum_qp weighted_basis_scalar(cell, b, qp), scattered
+    into f by element LIDs (ScatterResidual adds).  weighted_basis_scalar on a side is N_b(qp) times the
+    face measure w_qp |x_xi x x_eta| (Intrepid2 computeFaceMeasure of the face map); the side rule is
+    the Gauss rule of the volume degree, 2x2 for hex8 and 3x3 for hex27.  Element by element, as the
+    side workset runs: for each element on the side, its face nodes (volume basis restricted to the face)
+    and its bilinear face from the four corner vertices."""
+    step = 1 if topo == HEX8 else 2
+    nodes = HEX8_NODES if topo == HEX8 else HEX27_NODES
+    d, at_max = SIDES[side]
+    u, v = [k for k in range(3) if k != d]
+    n = (nx, ny, nz)
+    if step == 1:
+        pts, wts = [-1 / np.sqrt(3), 1 / np.sqrt(3)], [1.0, 1.0]
+    else:
+        pts, wts = [-np.sqrt(0.6), 0.0, np.sqrt(0.6)], [5 / 9, 8 / 9, 5 / 9]
+    fixed = n[d] - 1 if at_max else 0
+    on_side = [k for k in range(len(nodes)) if nodes[k][d] == (step if at_max else 0)]
+    corners = [k for k in range(8) if HEX8_NODES[k][d] == at_max]
+    for ez in range(nz):
+        for ey in range(ny):
+            for ex in range(nx):
+                if (ex, ey, ez)[d] != fixed:
+                    continue
+                e = ex + nx * (ey + ny * ez)
+                C = {(HEX8_NODES[k][u], HEX8_NODES[k][v]): X[ev[e, k]] for k in corners}
+                for k in on_side:
+                    a, b = nodes[k][u], nodes[k][v]
+                    s = 0.0
+                    for xi, wx in zip(pts, wts):
+                        for eta, wy in zip(pts, wts):
+                            tx = 0.25 * ((1 - eta) * (C[1, 0] - C[0, 0]) + (1 + eta) * (C[1, 1] - C[0, 1]))
+                            ty = 0.25 * ((1 - xi) * (C[0, 1] - C[0, 0]) + (1 + xi) * (C[1, 1] - C[1, 0]))
+                            s += wx * wy * _lagrange(step, a, xi) * _lagrange(step, b, eta) * np.linalg.norm(
+                                np.cross(tx, ty))
+                    for i in range(nf):
+                        f[elem_lid[e, k * nf + i]] += values[i] * s
+    return f

@@ -79,3 +79,30 @@ def test_bench_reads_no_undefined_name():
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     for f in ("bench.py", "__graft_entry__.py"):
         assert _unbound_names(os.path.join(root, f)) == {}, f
+
+
+def test_ctypes_mirrors_match_the_header_layout(tmp_path):
+    # txa_assembly_args and txa_serve_request: every field offset and the size, C compiler vs ctypes
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    structs = {"txa_assembly_args": txa.AssemblyArgs, "txa_serve_request": txa.ServeRequest}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "txa.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("%zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("%zu\\n", offsetof({cname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(root, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = []
+    for py in structs.values():
+        want.append(ctypes.sizeof(py))
+        want += [getattr(py, fname).offset for fname, _ in py._fields_]
+    assert got == want

@@ -410,3 +410,68 @@ def test_dirichlet_rows_identity():
         e = np.zeros(owned.size); e[r] = 1.0
         assert np.array_equal(D[r], e)
         assert f[r] == x[r] - 0.25
+
+
+def _neumann(topo, n, X, nf, side, values):
+    conn, ev, _ = O.brick_mesh(topo, n, n, n)
+    _, owned, el = O.dofmap(conn, nf)
+    return O.neumann(topo, n, n, n, X, ev, el, nf, side, values, np.zeros(owned.size))
+
+
+def test_neumann_flat_faces_closed_forms():
+    # unit cube, h = 1/n: a hex8 face gives each corner h^2/4 (bilinear basis); the Q2 face gives its
+    # corners, edge midpoints and centre h^2 (1/36, 1/9, 4/9) (products of the 1D Simpson weights)
+    n = 3
+    h2 = 1.0 / n ** 2
+    X = synth.brick_coords(n, n, n)
+    for side in range(6):
+        f = _neumann(O.HEX8, n, X, 1, side, (2.0,))
+        assert abs(f.sum() - 2.0) <= 1e-14                       # val * area
+        got = sorted(set(np.round(f[f != 0] / (2.0 * h2), 12)))
+        assert got == [0.25, 0.5, 1.0]                           # side corners, side edges, interior
+        f = _neumann(O.HEX27, n, X, 1, side, (1.0,))
+        assert abs(f.sum() - 1.0) <= 1e-14
+        got = sorted(set(np.round(f[f != 0] / h2, 12)))
+        assert got == sorted(set(np.round([1 / 36, 2 / 36, 4 / 36, 1 / 9, 2 / 9, 4 / 9], 12)))
+
+
+def test_neumann_parallelogram_faces_and_fields():
+    # an affine brick has parallelogram faces: each corner of a face gets a quarter of its area
+    n = 2
+    A = np.array([[1.2, 0.3, -0.1], [0.1, 0.9, 0.2], [-0.2, 0.15, 1.1]])
+    X = synth.brick_coords(n, n, n) @ A.T + np.array([0.5, -1.0, 2.0])
+    for side, (d, _) in O.SIDES.items():
+        u, v = [k for k in range(3) if k != d]
+        area = np.linalg.norm(np.cross(A[:, u], A[:, v]))       # the side is the image of a unit square
+        f = _neumann(O.HEX8, n, X, 3, side, (1.0, -2.0, 0.5))
+        tot = [f[i::3].sum() for i in range(3)]
+        assert np.allclose(tot, [area, -2.0 * area, 0.5 * area], rtol=1e-13, atol=0)
+        face = area / n ** 2
+        vals = np.abs(f[0::3][f[0::3] != 0])
+        assert np.allclose(sorted(set(np.round(vals / face, 10))), [0.25, 0.5, 1.0])
+
+
+def test_neumann_jittered_side_total_is_the_side_area():
+    # partition of unity: the side total is the 2x2-Gauss area of the bilinear faces; against the
+    # faces' true areas (a 24x24 Gauss rule, an independent computation) for mildly warped faces
+    n = 4
+    X = synth.brick_coords(n, n, n, jitter=0.1, seed=7)
+    g, w = np.polynomial.legendre.leggauss(24)
+    for side, (d, at_max) in O.SIDES.items():
+        u, v = [k for k in range(3) if k != d]
+        tot = _neumann(O.HEX8, n, X, 1, side, (1.0,)).sum()
+        area = 0.0
+        for cv in range(n):
+            for cu in range(n):
+                c = np.zeros((2, 2, 3))
+                for du in (0, 1):
+                    for dv in (0, 1):
+                        idx = [0, 0, 0]
+                        idx[d], idx[u], idx[v] = (n if at_max else 0), cu + du, cv + dv
+                        c[du, dv] = X[idx[0] + (n + 1) * (idx[1] + (n + 1) * idx[2])]
+                for s_, ws in zip(g, w):
+                    for t_, wt in zip(g, w):
+                        xs = 0.25 * ((1 - t_) * (c[1, 0] - c[0, 0]) + (1 + t_) * (c[1, 1] - c[0, 1]))
+                        xt = 0.25 * ((1 - s_) * (c[0, 1] - c[0, 0]) + (1 + s_) * (c[1, 1] - c[1, 0]))
+                        area += ws * wt * np.linalg.norm(np.cross(xs, xt))
+        assert abs(tot - area) <= 1e-5 * area

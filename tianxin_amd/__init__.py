@@ -23,6 +23,8 @@ TXA_OK, TXA_ERR_INVALID, TXA_ERR_LOGIC, TXA_ERR_DEVICE, TXA_ERR_NOMEM, TXA_ERR_U
 HEX8, TET4, HEX27 = 1, 2, 3
 DIFFUSION, ELASTICITY = 0, 1
 FP64, FP32, SPLIT_BF16 = 0, 1, 2
+# side sets of the brick (shards hex side ordinals): y = 0, x = max, y = max, x = 0, z = 0, z = max
+SIDE_BOTTOM, SIDE_RIGHT, SIDE_TOP, SIDE_LEFT, SIDE_BACK, SIDE_FRONT = range(6)
 
 _lib = None
 
@@ -47,6 +49,7 @@ class AssemblyArgs(ctypes.Structure):
         ("transient", ctypes.c_int),
         ("f", P), ("vals", P),
         ("bc_lids", P), ("bc_vals", P), ("n_bc", I64),
+        ("n_neumann", ctypes.c_int), ("neumann_sides", P), ("neumann_values", P),
     ]
 
 
@@ -140,8 +143,9 @@ def _stream(stream):
 
 def make_args(*, physics, coef, x, f=None, vals=None, precision=FP64, xdot=None, xddot=None,
               alpha=0.0, beta=1.0, gamma2=0.0, transient=False, nu=0.3, rho=1.0, src=(0.0, 0.0, 0.0),
-              rayleigh=(0.0, 0.0), bc_lids=None, bc_vals=None):
-    """txa_assembly_args from device tensors (see include/txa.h)."""
+              rayleigh=(0.0, 0.0), bc_lids=None, bc_vals=None, neumann=None):
+    """txa_assembly_args from device tensors (see include/txa.h).  neumann: [(side, (flux per field, ...)), ...]
+    with sides SIDE_BOTTOM .. SIDE_FRONT."""
     a = AssemblyArgs()
     a.physics, a.precision = physics, precision
     a.coef = _dptr(coef)
@@ -155,6 +159,12 @@ def make_args(*, physics, coef, x, f=None, vals=None, precision=FP64, xdot=None,
     a.f, a.vals = _dptr(f), _dptr(vals)
     a.bc_lids, a.bc_vals = _dptr(bc_lids), _dptr(bc_vals)
     a.n_bc = 0 if bc_lids is None else int(bc_lids.numel())
+    if neumann:
+        sides = (ctypes.c_int * len(neumann))(*[int(s) for s, _ in neumann])
+        flat = [float(v) for _, vs in neumann for v in vs]
+        vals = (ctypes.c_double * len(flat))(*flat)
+        a.n_neumann, a.neumann_sides, a.neumann_values = len(neumann), ctypes.cast(sides, P), ctypes.cast(vals, P)
+        a._keep = (sides, vals)   # the host arrays live as long as the argument block
     return a
 
 
@@ -284,11 +294,14 @@ class Batch:
         if len(per_mesh) != len(self.meshes):
             raise ValueError("one argument set per mesh")
         arr = (AssemblyArgs * len(per_mesh))()
+        made = []
         for i, kw in enumerate(per_mesh):
             kw = dict(kw)
             kw.setdefault("precision", self.precision)
             kw.setdefault("transient", self.transient)
-            arr[i] = make_args(**kw)
+            made.append(make_args(**kw))
+            arr[i] = made[-1]
+        arr._keep = made   # host arrays the copied argument blocks point to
         return arr
 
     def assemble(self, per_mesh, use_graph=True, stream=None):

@@ -179,6 +179,76 @@ __global__ void k_dirichlet(int64_t n, int64_t nrows, const int32_t *__restrict_
     for (int64_t k = rowptr[r] + lane; k < rowptr[r + 1]; k += 64) vals[k] = col[k] == r ? 1.0 : 0.0;
 }
 
+// Neumann flux on one side of the brick: one thread per lattice point of the
+// side sums, in a fixed face order, its share of the faces around it,
+//   f(point, i) += val_i * sum_q w_q L_a(xi_q) L_b(eta_q) |x_xi x x_eta|(q),
+// with the bilinear face map of the four corner vertices, Gauss points of the
+// volume degree (2 for hex8, 3 for hex27) and the face basis of the element
+// restricted to the face (a, b: the point's position in the face, 0..STEP).
+struct NeumannParams {
+  int step;                       // lattice points per cell edge: 1 (hex8), 2 (hex27)
+  int nx, ny, nz, nf;
+  int axis, at_max;               // side normal axis, and whether the side is at its upper end
+  double val[3];
+  const double *X, *Y, *Z;        // vertex coordinates
+  const int32_t *point_lid;       // [points][nf]
+  double *f;
+};
+
+__device__ __forceinline__ double lagrange(int step, int a, double t) {
+  if (step == 1) return a == 0 ? 0.5 * (1.0 - t) : 0.5 * (1.0 + t);
+  return a == 0 ? 0.5 * t * (t - 1.0) : (a == 1 ? 1.0 - t * t : 0.5 * t * (t + 1.0));
+}
+
+__global__ void k_neumann(NeumannParams p) {
+  const int n[3] = {p.nx, p.ny, p.nz};
+  const int d = p.axis, u = d == 0 ? 1 : 0, v = d == 2 ? 1 : 2;   // in-plane axes, ascending
+  const int64_t pu = static_cast<int64_t>(p.step) * n[u] + 1, pv = static_cast<int64_t>(p.step) * n[v] + 1;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= pu * pv) return;
+  const int I = static_cast<int>(t % pu), J = static_cast<int>(t / pu);
+  const int nq = p.step == 1 ? 2 : 3;
+  const double g2 = 0.57735026918962576451, g3 = 0.77459666924148337704;   // 1/sqrt(3), sqrt(3/5)
+  double acc = 0.0;
+  const int cu_lo = I % p.step == 0 ? I / p.step - 1 : I / p.step, cv_lo = J % p.step == 0 ? J / p.step - 1 : J / p.step;
+  for (int cv = cv_lo; cv <= J / p.step; ++cv)
+    for (int cu = cu_lo; cu <= I / p.step; ++cu) {
+      if (cu < 0 || cu >= n[u] || cv < 0 || cv >= n[v]) continue;
+      const int a = I - p.step * cu, b = J - p.step * cv;
+      double C[2][2][3];   // corners [du][dv], coordinates
+      for (int dv = 0; dv < 2; ++dv)
+        for (int du = 0; du < 2; ++du) {
+          int c[3];
+          c[d] = p.at_max ? n[d] : 0;
+          c[u] = cu + du;
+          c[v] = cv + dv;
+          const int64_t vid = c[0] + static_cast<int64_t>(p.nx + 1) * (c[1] + static_cast<int64_t>(p.ny + 1) * c[2]);
+          C[du][dv][0] = p.X[vid]; C[du][dv][1] = p.Y[vid]; C[du][dv][2] = p.Z[vid];
+        }
+      for (int qv = 0; qv < nq; ++qv)
+        for (int qu = 0; qu < nq; ++qu) {
+          const double xi = nq == 2 ? (2 * qu - 1) * g2 : (qu - 1) * g3;
+          const double eta = nq == 2 ? (2 * qv - 1) * g2 : (qv - 1) * g3;
+          const double w = nq == 2 ? 1.0 : (qu == 1 ? 8.0 / 9.0 : 5.0 / 9.0) * (qv == 1 ? 8.0 / 9.0 : 5.0 / 9.0);
+          double tx[3], ty[3];
+          for (int k = 0; k < 3; ++k) {
+            tx[k] = 0.25 * ((1.0 - eta) * (C[1][0][k] - C[0][0][k]) + (1.0 + eta) * (C[1][1][k] - C[0][1][k]));
+            ty[k] = 0.25 * ((1.0 - xi) * (C[0][1][k] - C[0][0][k]) + (1.0 + xi) * (C[1][1][k] - C[1][0][k]));
+          }
+          const double nx_ = tx[1] * ty[2] - tx[2] * ty[1], ny_ = tx[2] * ty[0] - tx[0] * ty[2],
+                       nz_ = tx[0] * ty[1] - tx[1] * ty[0];
+          acc += w * lagrange(p.step, a, xi) * lagrange(p.step, b, eta) * sqrt(nx_ * nx_ + ny_ * ny_ + nz_ * nz_);
+        }
+    }
+  int64_t c[3];
+  c[d] = p.at_max ? static_cast<int64_t>(p.step) * n[d] : 0;
+  c[u] = I;
+  c[v] = J;
+  const int64_t px = static_cast<int64_t>(p.step) * p.nx + 1, py = static_cast<int64_t>(p.step) * p.ny + 1;
+  const int64_t pt = c[0] + px * (c[1] + py * c[2]);
+  for (int i = 0; i < p.nf; ++i) p.f[p.point_lid[pt * p.nf + i]] += p.val[i] * acc;
+}
+
 }  // namespace
 
 void upload_refelem(Mesh &m, DevBuf<double> &tab, hipStream_t s) {
@@ -218,7 +288,15 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
               "DOF map has " + std::to_string(m.nfields) + " fields, physics needs " + std::to_string(want_nf));
   TXA_REQUIRE(!(a.transient && a.xdot == nullptr), TXA_ERR_INVALID, "transient assembly needs xdot");
   TXA_REQUIRE(a.n_bc == 0 || a.bc_lids != nullptr, TXA_ERR_INVALID, "bc_lids missing");
-  const bool trans = a.transient != 0;
+  TXA_REQUIRE(a.n_neumann >= 0 && (a.n_neumann == 0 || (a.neumann_sides && a.neumann_values)), TXA_ERR_INVALID,
+              "neumann_sides / neumann_values missing");
+  if (a.n_neumann > 0) {
+    TXA_REQUIRE(m.topo == TXA_HEX8 || m.topo == TXA_HEX27, TXA_ERR_UNSUPPORTED,
+                "Neumann sides are implemented for hex8 and hex27 meshes");
+    for (int k = 0; k < a.n_neumann; ++k)
+      TXA_REQUIRE(a.neumann_sides[k] >= 0 && a.neumann_sides[k] <= 5, TXA_ERR_INVALID,
+                  "side " + std::to_string(a.neumann_sides[k]) + " is not a side of the brick (0..5)");
+  }
 
   if (uses_q1_path(m, a.physics, a.precision)) {
     launch_q1_hex8_diffusion(m, a, s);
@@ -244,6 +322,25 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
     if (a.vals) TXA_HIP(hipMemsetAsync(a.vals, 0, sizeof(double) * m.nnz, s));
     int64_t blocks = m.nelem < 65536 * 8 ? m.nelem : 65536 * 8;
     hipLaunchKernelGGL(k_generic, dim3(static_cast<unsigned>(blocks)), dim3(GB), 0, s, p);
+    TXA_HIP(hipGetLastError());
+  }
+  // Neumann sides, one launch each (stream order keeps the sums of shared edge points in a fixed order)
+  for (int k = 0; a.f && k < a.n_neumann; ++k) {
+    NeumannParams np;
+    np.step = m.topo == TXA_HEX27 ? 2 : 1;
+    np.nx = m.nx; np.ny = m.ny; np.nz = m.nz; np.nf = m.nfields;
+    const int side = a.neumann_sides[k];
+    static const int kAxis[6] = {1, 0, 1, 0, 2, 2}, kMax[6] = {0, 1, 1, 0, 0, 1};
+    np.axis = kAxis[side];
+    np.at_max = kMax[side];
+    for (int i = 0; i < 3; ++i) np.val[i] = i < m.nfields ? a.neumann_values[k * m.nfields + i] : 0.0;
+    np.X = m.xyz.p; np.Y = m.xyz.p + m.nverts; np.Z = m.xyz.p + 2 * m.nverts;
+    np.point_lid = m.point_lid.p;
+    np.f = a.f;
+    const int n3[3] = {m.nx, m.ny, m.nz};
+    const int u = np.axis == 0 ? 1 : 0, v = np.axis == 2 ? 1 : 2;
+    const int64_t npts = (static_cast<int64_t>(np.step) * n3[u] + 1) * (static_cast<int64_t>(np.step) * n3[v] + 1);
+    hipLaunchKernelGGL(k_neumann, dim3(static_cast<unsigned>((npts + 255) / 256)), dim3(256), 0, s, np);
     TXA_HIP(hipGetLastError());
   }
   if (a.n_bc > 0) {

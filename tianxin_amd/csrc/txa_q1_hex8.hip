@@ -1143,6 +1143,7 @@ void batch_fill(Batch &b, const txa_assembly_args *args) {
     const Mesh &m = *b.meshes[i];
     const txa_assembly_args &a = args[i];
     TXA_REQUIRE(a.physics == TXA_DIFFUSION, TXA_ERR_UNSUPPORTED, "batched assembly covers Q1 diffusion");
+    TXA_REQUIRE(a.n_neumann == 0, TXA_ERR_UNSUPPORTED, "Neumann sides are assembled by txa_assemble, not in batches");
     TXA_REQUIRE(a.precision == b.precision, TXA_ERR_INVALID, "every batch item must use the batch precision");
     TXA_REQUIRE((a.transient != 0) == b.transient, TXA_ERR_INVALID, "every batch item must match the batch transient flag");
     TXA_REQUIRE(a.x && a.coef, TXA_ERR_INVALID, "x and coef are required");
