@@ -84,6 +84,18 @@ const char *txa_version(void);
  * (adapters-stk/src/Panzer_STKConnManager.cpp:243-300). */
 int txa_mesh_brick(txa_ctx ctx, int topo, int nx, int ny, int nz, const double *vertex_coords,
                    int *mesh_id);
+/* A general mesh (e.g. read from Exodus, or a ConnManager's connectivity):
+ * conn host [nelem][npe] entity ids, npe = 8 (TXA_HEX8, shards vertex order),
+ * 4 (TXA_TET4), 27 (TXA_HEX27: 8 vertices, then 12 edges, 6 faces and the cell,
+ * as in the brick).  Vertex entries are vertex ids in [0, nverts); the others
+ * any ids below nentities (nentities = nverts for vertex-only topologies).
+ * vertex_coords host [nverts][3].  The DOF map and graph follow the same rules as
+ * for a brick; the element work runs on the generic kernel in FP64 (the lattice
+ * kernels, Neumann sides and batches need a brick).
+ * Replaces the connectivity STKConnManager / ConnManager hands to the DOF manager
+ * (adapters-stk/src/Panzer_STKConnManager.cpp:243-300). */
+int txa_mesh_upload(txa_ctx ctx, int topo, int64_t nelem, const int64_t *conn, int64_t nentities, int64_t nverts,
+                    const double *vertex_coords, int *mesh_id);
 int txa_mesh_destroy(txa_ctx ctx, int mesh_id);
 /* out[0]=elements out[1]=connectivity entries/element out[2]=vertices */
 int txa_mesh_info(txa_ctx ctx, int mesh_id, int64_t *out);

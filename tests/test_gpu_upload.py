@@ -1,0 +1,118 @@
+"""GPU: general meshes through txa_mesh_upload, against the oracle.
+
+A brick's own connectivity with its elements shuffled and its vertices
+renumbered, and the reference's Exodus fixtures (tests/golden), are uploaded as
+unstructured meshes.  The DOF map and graph must equal the oracle's on the same
+arrays bit for bit; the assembly (the generic kernel, FP64) must equal the
+oracle's within 1e-12 normwise, Dirichlet rows included.  Lattice-only
+features (reduced precision, Neumann sides, batches) must be refused.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import tianxin_amd as txa
+from test_gpu_parity import TOL64, dev, rel_err
+from tianxin_amd import exodus, synth
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+OTOPO = {txa.HEX8: O.HEX8, txa.TET4: O.TET4, txa.HEX27: O.HEX27}
+
+
+def renumbered_brick(topo, n, seed):
+    """Brick connectivity with elements shuffled and vertices relabelled."""
+    rng = np.random.default_rng(seed)
+    conn, ev, s = O.brick_mesh(OTOPO[topo], n, n + 1, n + 2)
+    X = synth.brick_coords(n, n + 1, n + 2, jitter=0.1, seed=seed)
+    nv = X.shape[0]
+    perm = rng.permutation(nv)                       # new id of each vertex
+    conn, ev = conn.copy(), perm[ev]
+    vmask = conn < nv
+    conn[vmask] = perm[conn[vmask]]
+    order = rng.permutation(conn.shape[0])
+    Xn = np.empty_like(X)
+    Xn[perm] = X
+    nent = int(conn.max()) + 1 if topo == txa.HEX27 else nv
+    return conn[order], ev[order], Xn, nent
+
+
+def check(engine, topo, conn, ev, X, nent, nf, physics):
+    m = engine.upload(topo, conn, X, nentities=nent)
+    try:
+        nd = m.build_dofmap(nf)
+        nnz = m.build_graph()
+        gids, owned, el = O.dofmap(conn, nf)
+        rp, col = O.graph(el, owned.size)
+        o, e = m.dofmap()
+        assert nd == owned.size and np.array_equal(o, owned) and np.array_equal(e, el)
+        r, c = m.graph()
+        assert nnz == col.size and np.array_equal(r, rp) and np.array_equal(c, col)
+        coef = synth.lognormal(41, conn.shape[0], 0.5)
+        x = synth.uniform(42, nd, -1, 1)
+        # Dirichlet rows: every field of the vertices with the smallest x
+        xmin = X[:, 0].min()
+        vert = np.nonzero(np.abs(X[:, 0] - xmin) < 1e-12)[0]
+        lid_of_vertex = np.full(X.shape[0], -1, np.int64)
+        nvtx = ev.shape[1]
+        lid_of_vertex[conn[:, :nvtx].ravel()] = el.reshape(el.shape[0], -1, nf)[:, :nvtx, 0].ravel()
+        bc = np.unique((lid_of_vertex[vert][:, None] + np.arange(nf)).ravel()).astype(np.int32)
+        A = O.Assembler(OTOPO[topo], physics, ev, X, el, coef, rp, col, nu=0.3, rho=1.0, src=(1.0, -0.5, 0.25))
+        fr, vr = A.evaluate(x, None, None, alpha=0.0, beta=1.0, gamma2=0.0, transient=False, mode=O.MODE_EXPLICIT)
+        O.dirichlet(bc, None, x, rp, col, fr, vr)
+        f = torch.empty(nd, device="cuda", dtype=torch.float64)
+        v = torch.empty(nnz, device="cuda", dtype=torch.float64)
+        m.assemble(physics=physics, coef=dev(coef), x=dev(x), f=f, vals=v, nu=0.3, rho=1.0, src=(1.0, -0.5, 0.25),
+                   bc_lids=dev(bc, torch.int32))
+        torch.cuda.synchronize()
+        assert rel_err(f.cpu().numpy(), fr) <= TOL64
+        assert rel_err(v.cpu().numpy(), vr) <= TOL64
+    finally:
+        m.destroy()
+
+
+@pytest.mark.parametrize("topo,nf", [(txa.HEX8, 1), (txa.HEX8, 3), (txa.TET4, 1), (txa.HEX27, 3)])
+def test_renumbered_brick(engine, topo, nf):
+    conn, ev, X, nent = renumbered_brick(topo, 3, seed=40 + topo)
+    check(engine, topo, conn, ev, X, nent, nf, txa.ELASTICITY if nf == 3 else txa.DIFFUSION)
+
+
+@pytest.mark.parametrize("name,block,topo", [("basic3d.gen", 0, txa.HEX8), ("twoblock_cube.gen", 0, txa.TET4)])
+def test_exodus_fixture(engine, name, block, topo):
+    X, blocks = exodus.read_exodus(os.path.join(GOLDEN, name))
+    conn = blocks[block][1]
+    used = np.unique(conn)                         # the block's own vertices, renumbered densely
+    new = np.full(X.shape[0], -1, np.int64)
+    new[used] = np.arange(used.size)
+    conn = new[conn]
+    X = X[used]
+    for nf, physics in ((1, txa.DIFFUSION), (3, txa.ELASTICITY)):
+        check(engine, topo, conn, conn, X, X.shape[0], nf, physics)
+
+
+def test_upload_rejections(engine):
+    conn, ev, X, nent = renumbered_brick(txa.HEX8, 2, seed=7)
+    bad = conn.copy()
+    bad[0, 3] = X.shape[0]                         # not a vertex
+    with pytest.raises(txa.TxaError) as e:
+        engine.upload(txa.HEX8, bad, X)
+    assert e.value.code == txa.TXA_ERR_INVALID
+    m = engine.upload(txa.HEX8, conn, X)
+    try:
+        nd = m.build_dofmap(1)
+        m.build_graph()
+        k = torch.ones(m.nelem, device="cuda", dtype=torch.float64)
+        x = torch.zeros(nd, device="cuda", dtype=torch.float64)
+        f = torch.empty(nd, device="cuda", dtype=torch.float64)
+        for kw in (dict(precision=txa.FP32), dict(neumann=[(txa.SIDE_LEFT, (1.0,))])):
+            with pytest.raises(txa.TxaError) as e:
+                m.assemble(physics=txa.DIFFUSION, coef=k, x=x, f=f, **kw)
+            assert e.value.code == txa.TXA_ERR_UNSUPPORTED
+        with pytest.raises(txa.TxaError) as e:
+            txa.Batch(engine, [m], precision=txa.FP64)
+        assert e.value.code == txa.TXA_ERR_UNSUPPORTED
+    finally:
+        m.destroy()

@@ -92,6 +92,7 @@ def load_library(path: str = LIB_PATH):
     L.txa_mesh_brick.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, P,
                                  ctypes.POINTER(ctypes.c_int)]
     L.txa_mesh_destroy.argtypes = [P, ctypes.c_int]
+    L.txa_mesh_upload.argtypes = [P, ctypes.c_int, I64, P, I64, I64, P, ctypes.POINTER(ctypes.c_int)]
     L.txa_mesh_info.argtypes = [P, ctypes.c_int, P]
     L.txa_dofmap_build.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(I64)]
     L.txa_dofmap_copy.argtypes = [P, ctypes.c_int, P, P]
@@ -201,18 +202,31 @@ class Engine:
     def brick(self, topo, nx, ny, nz, coords):
         return BrickMesh(self, topo, nx, ny, nz, coords)
 
+    def upload(self, topo, conn, coords, nentities=None):
+        """A general mesh (txa_mesh_upload): conn [elements][npe] entity ids (vertices first, 0-based),
+        coords [vertices][3].  nentities defaults to the number of vertices (vertex-only topologies)."""
+        return BrickMesh(self, topo, None, None, None, coords, conn=conn, nentities=nentities)
+
 
 class BrickMesh:
     """Structured brick mesh resident on the device, with its DOF map and CRS graph."""
 
-    def __init__(self, eng: Engine, topo, nx, ny, nz, coords):
+    def __init__(self, eng: Engine, topo, nx, ny, nz, coords, conn=None, nentities=None):
         self.eng = eng
         self.topo, self.nx, self.ny, self.nz = topo, nx, ny, nz
         c = np.ascontiguousarray(coords, np.float64)
-        if c.shape != ((nx + 1) * (ny + 1) * (nz + 1), 3):
-            raise ValueError("coords must be [(nx+1)(ny+1)(nz+1)][3]")
         mid = ctypes.c_int()
-        eng.check(eng.lib.txa_mesh_brick(eng.h, topo, nx, ny, nz, _np(c), ctypes.byref(mid)))
+        if conn is None:
+            if c.shape != ((nx + 1) * (ny + 1) * (nz + 1), 3):
+                raise ValueError("coords must be [(nx+1)(ny+1)(nz+1)][3]")
+            eng.check(eng.lib.txa_mesh_brick(eng.h, topo, nx, ny, nz, _np(c), ctypes.byref(mid)))
+        else:
+            k = np.ascontiguousarray(conn, np.int64)
+            if c.ndim != 2 or c.shape[1] != 3 or k.ndim != 2:
+                raise ValueError("conn must be [elements][npe] and coords [vertices][3]")
+            ne = c.shape[0] if nentities is None else int(nentities)
+            eng.check(eng.lib.txa_mesh_upload(eng.h, topo, k.shape[0], _np(k), ne, c.shape[0], _np(c),
+                                              ctypes.byref(mid)))
         self.id = mid.value
         info = np.zeros(3, np.int64)
         eng.check(eng.lib.txa_mesh_info(eng.h, self.id, _np(info)))

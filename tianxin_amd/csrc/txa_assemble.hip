@@ -265,7 +265,8 @@ void upload_refelem(Mesh &m, DevBuf<double> &tab, hipStream_t s) {
 }
 
 static bool uses_q1_path(const Mesh &m, int physics, int precision) {
-  return (m.topo == TXA_HEX8 || m.topo == TXA_TET4) && physics == TXA_DIFFUSION && precision != TXA_SPLIT_BF16;
+  return m.structured && (m.topo == TXA_HEX8 || m.topo == TXA_TET4) && physics == TXA_DIFFUSION &&
+         precision != TXA_SPLIT_BF16;
 }
 
 size_t assemble_workspace_bytes(const Mesh &m, int physics, int precision, bool want_f, bool want_vals) {
@@ -291,8 +292,8 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
   TXA_REQUIRE(a.n_neumann >= 0 && (a.n_neumann == 0 || (a.neumann_sides && a.neumann_values)), TXA_ERR_INVALID,
               "neumann_sides / neumann_values missing");
   if (a.n_neumann > 0) {
-    TXA_REQUIRE(m.topo == TXA_HEX8 || m.topo == TXA_HEX27, TXA_ERR_UNSUPPORTED,
-                "Neumann sides are implemented for hex8 and hex27 meshes");
+    TXA_REQUIRE(m.structured && (m.topo == TXA_HEX8 || m.topo == TXA_HEX27), TXA_ERR_UNSUPPORTED,
+                "Neumann sides are implemented for hex8 and hex27 bricks");
     for (int k = 0; k < a.n_neumann; ++k)
       TXA_REQUIRE(a.neumann_sides[k] >= 0 && a.neumann_sides[k] <= 5, TXA_ERR_INVALID,
                   "side " + std::to_string(a.neumann_sides[k]) + " is not a side of the brick (0..5)");
@@ -304,7 +305,8 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
     launch_elastic(m, a, s, ws);
   } else {
     TXA_REQUIRE(a.precision == TXA_FP64, TXA_ERR_UNSUPPORTED,
-                "reduced-precision integration is implemented for hex8 / tet4 diffusion and 3-field elasticity");
+                "reduced-precision integration is implemented for hex8 / tet4 diffusion and 3-field elasticity "
+                "on bricks");
     TXA_REQUIRE(m.ref.p != nullptr, TXA_ERR_LOGIC, "reference tables missing");
     RefElem r;
     refelem_build(r, m.topo);

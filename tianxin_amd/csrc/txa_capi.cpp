@@ -94,6 +94,20 @@ int txa_mesh_brick(txa_ctx ctx, int topo, int nx, int ny, int nz, const double *
   });
 }
 
+int txa_mesh_upload(txa_ctx ctx, int topo, int64_t nelem, const int64_t *conn, int64_t nentities, int64_t nverts,
+                    const double *vertex_coords, int *mesh_id) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(mesh_id && conn && vertex_coords, TXA_ERR_INVALID, "null argument");
+    auto m = std::make_unique<Mesh>();
+    m->topo = topo;
+    txa::mesh_upload(*m, nelem, conn, nentities, nverts, vertex_coords, nullptr);
+    txa::upload_refelem(*m, m->ref, nullptr);
+    int id = ctx->next_id++;
+    ctx->meshes[id] = std::move(m);
+    *mesh_id = id;
+  });
+}
+
 int txa_mesh_destroy(txa_ctx ctx, int mesh_id) {
   return guarded(ctx, [&] {
     auto it = ctx->meshes.find(mesh_id);

@@ -691,7 +691,7 @@ size_t elastic_workspace_bytes(const Mesh &m, int precision, bool want_f, bool w
 }
 
 void node_perm_build(Mesh &m, hipStream_t s) {
-  if (m.nfields != 3 || (m.topo != TXA_HEX8 && m.topo != TXA_HEX27)) return;
+  if (!m.structured || m.nfields != 3 || (m.topo != TXA_HEX8 && m.topo != TXA_HEX27)) return;
   const int R = m.topo == TXA_HEX8 ? 1 : 2;
   const int64_t px = m.topo == TXA_HEX8 ? m.nx + 1 : 2 * static_cast<int64_t>(m.nx) + 1;
   const int64_t py = m.topo == TXA_HEX8 ? m.ny + 1 : 2 * static_cast<int64_t>(m.ny) + 1;

@@ -63,6 +63,7 @@ struct DevBuf {
 // A structured brick mesh with its DOF map and CRS graph, resident on the device.
 struct Mesh {
   int topo = 0, nx = 0, ny = 0, nz = 0;
+  bool structured = true;  // a brick (txa_mesh_brick): lattice kernels; false: uploaded connectivity
   int64_t nelem = 0;     // elements (6 per cell for TET4)
   int nconn = 0;         // connectivity entities per element
   int nvert_elem = 0;    // geometry vertices per element
@@ -148,6 +149,8 @@ struct Context {
 
 // setup kernels (txa_setup.hip)
 void mesh_generate(Mesh &m, const double *host_coords, hipStream_t s);
+void mesh_upload(Mesh &m, int64_t nelem, const int64_t *host_conn, int64_t nentities, int64_t nverts,
+                 const double *host_coords, hipStream_t s);
 void dofmap_build(Mesh &m, int nfields, hipStream_t s);
 void graph_build(Mesh &m, hipStream_t s);
 void exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s);

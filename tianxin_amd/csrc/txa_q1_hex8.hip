@@ -1209,6 +1209,7 @@ Batch *batch_create(const std::vector<Mesh *> &meshes, const std::vector<int> &i
   b->transient = transient;
   for (Mesh *m : meshes) {
     TXA_REQUIRE(m->topo == TXA_HEX8 || m->topo == TXA_TET4, TXA_ERR_UNSUPPORTED, "batched assembly covers hex8 and tet4 meshes");
+    TXA_REQUIRE(m->structured, TXA_ERR_UNSUPPORTED, "batched assembly covers bricks (txa_mesh_brick)");
     TXA_REQUIRE(m->has_graph && m->nfields == 1, TXA_ERR_LOGIC, "batch meshes need a scalar DOF map and a graph");
   }
   b->prm.alloc(meshes.size());

@@ -353,6 +353,41 @@ void mesh_generate(Mesh &m, const double *host_coords, hipStream_t s) {
   TXA_HIP(hipStreamSynchronize(s));
 }
 
+// A general mesh: connectivity as entity ids (the first nvert_elem of an element are
+// its vertices, in [0, nverts); HEX27 adds its edges, faces and cell as further ids
+// below nentities), vertex coordinates [nverts][3].  Validated on the host.
+void mesh_upload(Mesh &m, int64_t nelem, const int64_t *host_conn, int64_t nentities, int64_t nverts,
+                 const double *host_coords, hipStream_t s) {
+  switch (m.topo) {
+    case TXA_HEX8: m.nconn = 8; m.nvert_elem = 8; break;
+    case TXA_TET4: m.nconn = 4; m.nvert_elem = 4; break;
+    case TXA_HEX27: m.nconn = 27; m.nvert_elem = 8; break;
+    default: throw Error(TXA_ERR_INVALID, "unknown topology " + std::to_string(m.topo));
+  }
+  TXA_REQUIRE(nelem >= 1 && nverts >= m.nvert_elem && nentities >= nverts, TXA_ERR_INVALID,
+              "need elements, vertices and nentities >= nverts");
+  for (int64_t e = 0; e < nelem; ++e)
+    for (int k = 0; k < m.nconn; ++k) {
+      const int64_t id = host_conn[e * m.nconn + k];
+      const int64_t hi = k < m.nvert_elem ? nverts : nentities;
+      TXA_REQUIRE(id >= 0 && id < hi, TXA_ERR_INVALID,
+                  "element " + std::to_string(e) + " entry " + std::to_string(k) + ": entity id " +
+                      std::to_string(id) + (k < m.nvert_elem ? " is not a vertex" : " is out of range"));
+    }
+  m.structured = false;
+  m.nelem = nelem;
+  m.nverts = nverts;
+  m.nentities = nentities;
+  std::vector<double> soa(static_cast<size_t>(3 * nverts));
+  for (int64_t i = 0; i < nverts; ++i)
+    for (int d = 0; d < 3; ++d) soa[d * nverts + i] = host_coords[3 * i + d];
+  m.xyz.alloc(3 * nverts);
+  TXA_HIP(hipMemcpyAsync(m.xyz.p, soa.data(), sizeof(double) * 3 * nverts, hipMemcpyHostToDevice, s));
+  m.conn.alloc(nelem * m.nconn);
+  TXA_HIP(hipMemcpyAsync(m.conn.p, host_conn, sizeof(int64_t) * nelem * m.nconn, hipMemcpyHostToDevice, s));
+  TXA_HIP(hipStreamSynchronize(s));
+}
+
 void dofmap_build(Mesh &m, int nfields, hipStream_t s) {
   TXA_REQUIRE(nfields >= 1 && nfields <= 3, TXA_ERR_INVALID, "nfields must be 1..3");
   m.nfields = nfields;
@@ -396,14 +431,17 @@ void dofmap_build(Mesh &m, int nfields, hipStream_t s) {
   hipLaunchKernelGGL(k_assign_lids, dim3(grid_for(npos)), dim3(kBlock), 0, s, gid.p, npos, first.p, pos.p, m.owned.p,
                      m.elem_lid.p);
   TXA_HIP(hipGetLastError());
-  // LID of every lattice point and field
-  m.npoints = m.topo == TXA_HEX27
-                  ? static_cast<int64_t>(2 * m.nx + 1) * (2 * m.ny + 1) * (2 * m.nz + 1)
-                  : m.nverts;
-  m.point_lid.alloc(m.npoints * nfields);
-  hipLaunchKernelGGL(k_point_lid, dim3(grid_for(ncon_tot)), dim3(kBlock), 0, s, m.topo, static_cast<int64_t>(m.nx),
-                     static_cast<int64_t>(m.ny), m.conn.p, m.nconn, m.nelem, nfields, m.elem_lid.p, m.point_lid.p);
-  TXA_HIP(hipGetLastError());
+  // LID of every lattice point and field (bricks only)
+  m.npoints = 0;
+  if (m.structured) {
+    m.npoints = m.topo == TXA_HEX27
+                    ? static_cast<int64_t>(2 * m.nx + 1) * (2 * m.ny + 1) * (2 * m.nz + 1)
+                    : m.nverts;
+    m.point_lid.alloc(m.npoints * nfields);
+    hipLaunchKernelGGL(k_point_lid, dim3(grid_for(ncon_tot)), dim3(kBlock), 0, s, m.topo, static_cast<int64_t>(m.nx),
+                       static_cast<int64_t>(m.ny), m.conn.p, m.nconn, m.nelem, nfields, m.elem_lid.p, m.point_lid.p);
+    TXA_HIP(hipGetLastError());
+  }
   TXA_HIP(hipStreamSynchronize(s));
   m.has_graph = false;
 }
@@ -413,8 +451,10 @@ void graph_build(Mesh &m, hipStream_t s) {
   int nd = m.ndof_elem;
   int64_t npos = m.nelem * nd;
   int64_t nrows = m.ndofs;
+  // a brick's rows have at most valence x nd candidates; an uploaded mesh's rows are
+  // checked against the capacity while they are sorted
   int valence = m.topo == TXA_TET4 ? 24 : 8;
-  TXA_REQUIRE(static_cast<int64_t>(valence) * nd <= kMaxCand, TXA_ERR_UNSUPPORTED,
+  TXA_REQUIRE(!m.structured || static_cast<int64_t>(valence) * nd <= kMaxCand, TXA_ERR_UNSUPPORTED,
               "row candidate count exceeds the per-row sort capacity");
   DevBuf<int64_t> cnt, cptr, cursor, ulen;
   cnt.alloc(nrows + 1);
@@ -444,7 +484,7 @@ void graph_build(Mesh &m, hipStream_t s) {
   TXA_HIP(hipGetLastError());
   int ovf = 0;
   TXA_HIP(hipMemcpy(&ovf, overflow.p, sizeof(int), hipMemcpyDeviceToHost));
-  TXA_REQUIRE(ovf == 0, TXA_ERR_LOGIC, "graph row exceeded the sort capacity");
+  TXA_REQUIRE(ovf == 0, TXA_ERR_UNSUPPORTED, "a graph row has more than 1024 candidate columns (element valence)");
   m.rowptr.alloc(nrows + 1);
   exclusive_scan_i64(ulen.p, m.rowptr.p, nrows + 1, s);
   TXA_HIP(hipMemcpy(&m.nnz, m.rowptr.p + nrows, sizeof(int64_t), hipMemcpyDeviceToHost));
@@ -453,7 +493,7 @@ void graph_build(Mesh &m, hipStream_t s) {
                      m.rowptr.p, m.col.p);
   TXA_HIP(hipGetLastError());
   // stencil slot map for the vertex-lattice kernels
-  if (m.topo == TXA_HEX8 || m.topo == TXA_TET4) {
+  if (m.structured && (m.topo == TXA_HEX8 || m.topo == TXA_TET4)) {
     m.perm_width = (27 * m.nfields + 15) / 16 * 16;
     m.perm.alloc(static_cast<size_t>(nrows) * m.perm_width);
     int64_t nt = m.nverts * m.nfields;
