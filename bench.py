@@ -30,7 +30,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n=128, reps=8, n1=48, reps1=3):
+def cpu_baseline(n=128, reps=10, n1=48, reps1=10):
     """Oracle (reference algorithm: cached worksets, forward-AD Jacobian, row-search
     scatter) on host cores, on an n^3 sample of the same workload with all threads
     (at most 16), and on an n1^3 sample with one thread."""
@@ -60,7 +60,14 @@ def cpu_baseline(n=128, reps=8, n1=48, reps1=3):
 
     v = run(n, cores, reps)
     v1 = run(n1, 1, reps1)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((l.split(":", 1)[1].strip() for l in fh if l.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": v, "unit": "elements/s", "cores": cores, "kind": "port",
+            "host": {"cpu_model": model, "logical_cpus": os.cpu_count(), "threads_used": cores},
             "sample": f"Q1 Poisson {n}^3 hex8 ({n ** 3} elements), FP64, forward-AD Jacobian, workset 1000, "
                       f"median of {reps} after 2 warm-ups, {cores} OpenMP threads",
             "single_core": {"value": v1, "unit": "elements/s", "cores": 1,
