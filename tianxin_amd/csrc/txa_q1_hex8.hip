@@ -417,26 +417,38 @@ __device__ __forceinline__ void tet6_cell(const double (&X)[3][8], const double 
   for (int i = 0; i < 36; ++i) o.K[i] = TRANS ? Real(beta) * K[i] + Real(alpha) * M[i] : Real(beta) * K[i];
 }
 
+// Coefficients of one cell: k of the hex8 element, or of the six tets (element 6*cell + t).
+template <int KIND>
+struct CellCoef {
+  double k[KIND == TXA_TET4 ? 6 : 1];
+};
+
+template <int KIND>
+__device__ __forceinline__ void load_coef(const Q1DiffParams &p, int64_t cell, CellCoef<KIND> &c) {
+  constexpr int n = KIND == TXA_TET4 ? 6 : 1;
+#pragma unroll
+  for (int t = 0; t < n; ++t) c.k[t] = p.coef[n * cell + t];
+}
+
 // Cell routine of the element kind: one hex8 element, or the six tets of the cell.
 template <int KIND, typename Real, bool TRANS>
-__device__ __forceinline__ void cell_element(const Q1DiffParams &p, int64_t cell, const double (&X)[3][8],
+__device__ __forceinline__ void cell_element(const Q1DiffParams &p, const CellCoef<KIND> &c, const double (&X)[3][8],
                                              const double (&u)[8], const double (&ud)[8], Hex8Out<Real, TRANS> &o) {
-  if (KIND == TXA_TET4) {
-    double k[6];
-#pragma unroll
-    for (int t = 0; t < 6; ++t) k[t] = p.coef[6 * cell + t];
-    tet6_cell<Real, TRANS>(X, u, ud, k, p.rhoc, p.src, p.alpha, p.beta, o);
-  } else {
-    hex8_element<Real, TRANS>(X, u, ud, p.coef[cell], p.rhoc, p.src, p.alpha, p.beta, o);
-  }
+  if constexpr (KIND == TXA_TET4) tet6_cell<Real, TRANS>(X, u, ud, c.k, p.rhoc, p.src, p.alpha, p.beta, o);
+  else hex8_element<Real, TRANS>(X, u, ud, c.k[0], p.rhoc, p.src, p.alpha, p.beta, o);
 }
+
+// Global-memory view of an output pointer.  Stores through a generic pointer become
+// flat stores, which the LDS counter also tracks, so an LDS wait would wait for them.
+using gdouble = __attribute__((address_space(1))) double;
+__device__ __forceinline__ gdouble *global_ptr(double *q) { return (gdouble *)q; }
 
 __device__ __forceinline__ void scatter_row(const Q1DiffParams &p, int32_t L, int64_t base, const uint32_t (&w)[8],
                                             const double (&acc)[27]) {
 #pragma unroll
   for (int s = 0; s < 27; ++s) {
     const uint32_t o = (w[s >> 2] >> (8 * (s & 3))) & 0xffu;
-    if (o != 0xffu) p.vals[base + o] = acc[s];
+    if (o != 0xffu) global_ptr(p.vals)[base + o] = acc[s];
   }
 }
 
@@ -486,7 +498,7 @@ __device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &
   const bool act = m.L >= 0;
   const int32_t L = m.L;
   const int64_t base = m.base, end = m.end;
-  if (act && p.f) p.f[L] = r;
+  if (act && p.f) global_ptr(p.f)[L] = r;
   if (!p.vals) return;   // uniform over the block
   // owners of a group are a prefix of its lanes; contiguous when successive LIDs differ by one
   const int32_t Lnext = __shfl_down(L, 1, 32);
@@ -498,7 +510,7 @@ __device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &
   const bool contiguous = gok == 0xffffffffu && nact > 0;
   const int64_t base0 = __shfl(base, 0, 32);
   const int64_t end_last = __shfl(end, nact > 0 ? nact - 1 : 0, 32);
-  double *buf = rowbuf + grp * ROWBUF;
+  double *buf = rowbuf + grp * ROWBUF;   // dereferenced only by groups with a row: grp < TY-1
 #if TXA_Q1_IDENT_FAST
   // interior rows: the 27 neighbours sit in the row in lattice order (slot s at s)
   const bool ident = m.w[0] == 0x03020100u && m.w[1] == 0x07060504u && m.w[2] == 0x0b0a0908u &&
@@ -524,23 +536,81 @@ __device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &
     }
   }
   wave_lds_sync();   // the group's lanes live in one wavefront
+  // aligned: each store instruction covers one 256-byte aligned chunk of the range; lanes
+  // before the range's first entry idle in the first chunk, so only the two end chunks
+  // of the range are partial
+  const int total = static_cast<int>(end_last - base0);
+  const int lead = ALIGN ? static_cast<int>((reinterpret_cast<uintptr_t>(p.vals + base0) >> 3) & 31) : 0;
+  const int nch = contiguous ? (total + lead + 31) >> 5 : 0;    // 32-entry chunks of the group's range
+  const int nchw = max(nch, __shfl_xor(nch, 32));               // the wave's two groups: uniform count
   if (contiguous) {
-    const int total = static_cast<int>(end_last - base0);
-    // aligned: each store instruction covers one 256-byte aligned chunk of the range; lanes
-    // before the range's first entry idle in the first chunk, so only the two end chunks
-    // of the range are partial
-    const int lead = ALIGN ? static_cast<int>((reinterpret_cast<uintptr_t>(p.vals + base0) >> 3) & 31) : 0;
-    for (int j = gl - lead; j < total; j += 32) {
-      if (j < 0) continue;
+    // chunk c: lane gl stores range entry j = 32c + gl - lead when 0 <= j < total.  Four LDS
+    // reads are issued before their four stores (one LDS round trip per four chunks); the
+    // reads of idle lanes may fall outside the group's buffer and are not used.
+    const int j0 = gl - lead;
+    const double *src = buf + j0;
+    gdouble *dst = global_ptr(p.vals + base0 + j0);
+    for (int c = 0; c < nchw; c += 4) {
+      double v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = src[32 * (c + i)];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (static_cast<unsigned>(j0 + 32 * (c + i)) < static_cast<unsigned>(total)) {
 #if TXA_Q1_NT_STORES
-      __builtin_nontemporal_store(buf[j], p.vals + base0 + j);
+          __builtin_nontemporal_store(v[i], dst + 32 * (c + i));
 #else
-      p.vals[base0 + j] = buf[j];
+          dst[32 * (c + i)] = v[i];
 #endif
+        }
     }
   } else if (act) {
     scatter_row(p, L, base, m.w, acc);
   }
+}
+
+// Row metadata loads for the element waves: no branch, so the loaded registers
+// reach their first use unchanged (lanes without a row read row 0 and publish
+// L = -1; their values are never used).
+__device__ __forceinline__ RowMeta load_meta_flat(const Q1DiffParams &p, int32_t L) {
+  RowMeta m;
+  m.L = L;
+  const int32_t r = L >= 0 ? L : 0;
+  m.base = p.rowptr[r];
+  m.end = p.rowptr[r + 1];
+  const uint4 *pp = reinterpret_cast<const uint4 *>(p.perm + static_cast<int64_t>(r) * 32);
+  const uint4 w0 = pp[0], w1 = pp[1];
+  m.w[0] = w0.x; m.w[1] = w0.y; m.w[2] = w0.z; m.w[3] = w0.w;
+  m.w[4] = w1.x; m.w[5] = w1.y; m.w[6] = w1.z; m.w[7] = w1.w;
+  return m;
+}
+
+// Row metadata handed from the element waves to the owner waves through LDS, one
+// slot per stage slot, structure of arrays over the owner lanes (ty < TY-1):
+// [0] LID (-1: no row), [1] [2] low and high words of the row's first CRS
+// position, [3..9] the slot map words 0-6 with the row length in the top byte of
+// word 6 (byte 27 of the map, which carries no slot).
+constexpr int NOWN = TX * (TY - 1);
+constexpr int MWORDS = 10;
+constexpr int MSLOT = MWORDS * NOWN;
+
+__device__ __forceinline__ void put_meta(uint32_t *ms, int t, const RowMeta &m) {
+  ms[t] = static_cast<uint32_t>(m.L);
+  ms[NOWN + t] = static_cast<uint32_t>(static_cast<uint64_t>(m.base));
+  ms[2 * NOWN + t] = static_cast<uint32_t>(static_cast<uint64_t>(m.base) >> 32);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) ms[(3 + i) * NOWN + t] = m.w[i];
+  ms[9 * NOWN + t] = (m.w[6] & 0x00ffffffu) | (static_cast<uint32_t>(m.end - m.base) << 24);
+}
+
+__device__ __forceinline__ RowMeta get_meta(const uint32_t *ms, int t) {
+  RowMeta m;
+  m.L = static_cast<int32_t>(ms[t]);
+  m.base = static_cast<int64_t>(ms[NOWN + t] | static_cast<uint64_t>(ms[2 * NOWN + t]) << 32);
+#pragma unroll
+  for (int i = 0; i < 7; ++i) m.w[i] = ms[(3 + i) * NOWN + t];
+  m.end = m.base + (m.w[6] >> 24);
+  return m;
 }
 
 // Add rows a_lo = CX + 2CY (bottom) and a_lo + 4 (top) of the element at
@@ -589,10 +659,14 @@ __device__ __forceinline__ void load_face_lids(const Q1DiffParams &p, int64_t n0
 // layer ez in the stage (in the other slot when the stage is double-buffered, so
 // the publication needs no wait for the owners).  Loads (element waves) and
 // stores (owner waves) sit in different waves, so neither waits for the other's
-// memory counter.
+// memory counter.  The owner waves issue no global loads at all: one counter
+// tracks both loads and stores in a wave, so a wait for a load issued after a
+// layer's row stores also waits for those stores to complete, which held each
+// owner wave to one layer of stores in flight.  The element waves therefore load
+// the owners' LIDs, row pointers and slot maps and publish them with the layer.
 template <int KIND, typename Real, bool TRANS>
 __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by, int bz, Real *stage,
-                                           double *rowbuf) {
+                                           double *rowbuf, uint32_t *meta) {
   constexpr int NSLOT = stage_slots<Real>();
   const bool is_elem = threadIdx.x < NE;
   const int t = is_elem ? threadIdx.x : threadIdx.x - NE;
@@ -612,31 +686,56 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
     const int ex = x0 + tx, ey = y0 + ty;
     const bool evalid = ex >= 0 && ex < p.nx && ey >= 0 && ey < p.ny;
     const int64_t col0 = evalid ? ex + sx * ey : 0;   // vertex (ex, ey, 0)
+    const int64_t cell0 = ex + static_cast<int64_t>(p.nx) * ey;   // element (ex, ey, 0)
+    const int64_t sz = static_cast<int64_t>(p.nx) * p.ny;
     Face<TRANS> fb, ft, fn;        // bottom, top and prefetched next-top faces
     int32_t Ln[4] = {0, 0, 0, 0};  // LIDs of the face after the prefetched one
+    CellCoef<KIND> kc, kn;         // coefficients of this layer's element and (prefetched) the next one's
     if (evalid) {
       int32_t L[4];
       load_face_lids(p, col0 + sxy * ez0, sx, L);
       load_face(p, col0 + sxy * ez0, sx, L, fb);
       load_face_lids(p, col0 + sxy * (ez0 + 1), sx, L);
       load_face(p, col0 + sxy * (ez0 + 1), sx, L, ft);
-      if (kPrefetch && ez0 + 2 <= p.nz) load_face_lids(p, col0 + sxy * (ez0 + 2), sx, Ln);
+      if (kPrefetch) {
+        load_face_lids(p, col0 + sxy * min(ez0 + 2, p.nz), sx, Ln);
+        load_coef<KIND>(p, cell0 + sz * ez0, kc);
+      }
     }
+    // the vertex column of owner lane t: (ex+1, ey+1), rows of layers [k0, k1)
+    const int oi = ex + 1, oj = ey + 1;
+    const bool owns = tx < TX - 1 && ty < TY - 1 && oi <= p.nx && oj <= p.ny;
+    const int64_t ocol = owns ? oi + sx * oj : 0;
+    auto own_lid = [&](int z) -> int32_t { return owns && z >= k0 && z < k1 ? p.lid[ocol + sxy * z] : -1; };
+    int32_t Lo = own_lid(ez0);
     for (int ez = ez0; ez <= ez1 + 1; ++ez) {
+      // metadata of the owner's row in vertex layer ez (published with element layer ez),
+      // then the LID of the next layer; both ahead of this layer's face loads
+      RowMeta om = load_meta_flat(p, Lo);
+      const int32_t Lo_next = own_lid(ez + 1);
       Hex8Out<Real, TRANS> o;
+      Real done = Real(0);   // set from the layer's last result
       if (ez <= ez1) {
         if (evalid) {
           if (kPrefetch) {
-            // prefetch the face above the top one (layer ez+2) and the LIDs beyond it
-            if (ez + 2 <= p.nz) {
-              load_face(p, col0 + sxy * (ez + 2), sx, Ln, fn);
-              if (ez + 3 <= p.nz) load_face_lids(p, col0 + sxy * (ez + 3), sx, Ln);
+            // prefetch the face above the top one (layer ez+2), the LIDs beyond it and the next
+            // element's coefficients, with the layers clamped to the mesh (what lies past it is
+            // loaded again and never used).  No branch: a conditional load would be copied,
+            // and so waited for, where the branch joins.
+            load_face(p, col0 + sxy * min(ez + 2, p.nz), sx, Ln, fn);
+            load_face_lids(p, col0 + sxy * min(ez + 3, p.nz), sx, Ln);
+            load_coef<KIND>(p, cell0 + sz * min(ez + 1, p.nz - 1), kn);
+            // keep these loads ahead of the element work: the scheduler would otherwise sink
+            // them toward their use at the end of the layer and expose their latency there
+            __builtin_amdgcn_sched_barrier(0);
+          } else {
+            load_coef<KIND>(p, cell0 + sz * ez, kc);
+            if (ez > ez0) {
+              // no register room for a prefetch (FP64): load the top face now
+              int32_t L[4];
+              load_face_lids(p, col0 + sxy * (ez + 1), sx, L);
+              load_face(p, col0 + sxy * (ez + 1), sx, L, ft);
             }
-          } else if (ez > ez0) {
-            // no register room for a prefetch (FP64): load the top face now
-            int32_t L[4];
-            load_face_lids(p, col0 + sxy * (ez + 1), sx, L);
-            load_face(p, col0 + sxy * (ez + 1), sx, L, ft);
           }
           double X[3][8], u[8], ud[8];
 #pragma unroll
@@ -646,25 +745,33 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
             u[v] = fb.u[v]; u[v + 4] = ft.u[v];
             ud[v] = fb.ud[v]; ud[v + 4] = ft.ud[v];
           }
-          cell_element<KIND, Real, TRANS>(p, ex + static_cast<int64_t>(p.nx) * (ey + static_cast<int64_t>(p.ny) * ez),
-                                          X, u, ud, o);
+          cell_element<KIND, Real, TRANS>(p, kc, X, u, ud, o);
           fb = ft;
-          if (kPrefetch) ft = fn;
+          if (kPrefetch) { ft = fn; kc = kn; }
         } else {
 #pragma unroll
           for (int i = 0; i < 36; ++i) o.K[i] = Real(0);
 #pragma unroll
           for (int i = 0; i < 8; ++i) o.R[i] = Real(0);
         }
+        done = o.R[7];
       }
+      // the metadata registers are first read here, after the element work (an input the
+      // work produces orders it), so the wave waits for those loads only now; earlier
+      // uses, or a register of the loads reused early, would wait for them before the work
+      asm volatile("" : "+v"(om.base), "+v"(om.end), "+v"(om.w[0]), "+v"(om.w[1]), "+v"(om.w[2]), "+v"(om.w[3]),
+                   "+v"(om.w[4]), "+v"(om.w[5]), "+v"(om.w[6]) : "v"(done));
       if constexpr (NSLOT == 1) __syncthreads();   // the owners are done with the stage
+      const int sl = NSLOT == 2 ? (ez & 1) : 0;
       if (ez <= ez1) {
-        Real *st = stage + (NSLOT == 2 ? (ez & 1) * (NSTAGE * NE) : 0);
+        Real *st = stage + sl * (NSTAGE * NE);
 #pragma unroll
         for (int i = 0; i < 36; ++i) st[i * NE + t] = o.K[i];
 #pragma unroll
         for (int i = 0; i < 8; ++i) st[(36 + i) * NE + t] = o.R[i];
       }
+      if (t < NOWN) put_meta(meta + sl * MSLOT, t, om);
+      Lo = Lo_next;
       __syncthreads();   // layer ez is in the stage (two slots: the owners are also done with layer ez-1)
     }
   } else {
@@ -672,36 +779,27 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
     // column (tx, ty), corner (0,1) of (tx+1, ty), (1,0) of (tx, ty+1), (0,0) of (tx+1, ty+1)
     const int oi = x0 + 1 + tx, oj = y0 + 1 + ty;
     const bool owner = tx < TX - 1 && ty < TY - 1 && oi <= p.nx && oj <= p.ny;
-    const int64_t ocol = owner ? oi + sx * oj : 0;
     double acc[27], accn[18];
     double rc = 0.0, rn = 0.0;
 #pragma unroll
     for (int s = 0; s < 27; ++s) acc[s] = 0.0;
 #pragma unroll
     for (int s = 0; s < 18; ++s) accn[s] = 0.0;
-    RowMeta meta;        // rows of vertex layer e, consumed at step e
-    int32_t Lq = -1;     // LID of the owned vertex in layer e+1
-    if (owner) {
-      const int32_t L0 = p.lid[ocol + sxy * ez0];
-      meta = load_meta(p, ez0 >= k0 ? L0 : -1);
-      Lq = p.lid[ocol + sxy * (ez0 + 1)];
-    }
     for (int ez = ez0; ez <= ez1 + 1; ++ez) {
       if (ez > ez0) {
         // sum element layer e = ez-1 from the stage
         const int e = ez - 1;
-        const Real *st = stage + (NSLOT == 2 ? (e & 1) * (NSTAGE * NE) : 0);
-        RowMeta next;
+        const int sl = NSLOT == 2 ? (e & 1) : 0;
+        const Real *st = stage + sl * (NSTAGE * NE);
+        RowMeta m;
         if (owner) {
-          next = load_meta(p, e + 1 >= k0 ? Lq : -1);
-          Lq = e + 2 <= p.nz ? p.lid[ocol + sxy * (e + 2)] : -1;
+          m = get_meta(meta + sl * MSLOT, t);
           take_corner<1, 1>(st, t, acc, accn, rc, rn);
           take_corner<0, 1>(st, t + 1, acc, accn, rc, rn);
           take_corner<1, 0>(st, t + TX, acc, accn, rc, rn);
           take_corner<0, 0>(st, t + TX + 1, acc, accn, rc, rn);
         }
-        if (e >= k0) emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, meta, acc, rc, rowbuf, t >> 5);
-        meta = next;
+        if (e >= k0) emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, m, acc, rc, rowbuf, t >> 5);
 #pragma unroll
         for (int s = 0; s < 18; ++s) { acc[s] = accn[s]; accn[s] = 0.0; }
 #pragma unroll
@@ -713,7 +811,11 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
       __syncthreads();   // next layer is in the stage
     }
     // the top vertex layer of the mesh has no element layer above it
-    if (k1 == p.nz + 1) emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, meta, acc, rc, rowbuf, t >> 5);
+    if (k1 == p.nz + 1) {
+      RowMeta m;
+      if (owner) m = get_meta(meta + (NSLOT == 2 ? ((ez1 + 1) & 1) : 0) * MSLOT, t);
+      emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, m, acc, rc, rowbuf, t >> 5);
+    }
   }
 }
 
@@ -830,8 +932,9 @@ __device__ __forceinline__ void ring_block(const Q1DiffParams &p, int bx, int by
         u[v] = face(slot, 3, c);
         ud[v] = TRANS ? face(slot, NVAL - 1, c) : 0.0;
       }
-      cell_element<KIND, Real, TRANS>(p, ex + static_cast<int64_t>(p.nx) * (ey + static_cast<int64_t>(p.ny) * ez), X,
-                                      u, ud, o);
+      CellCoef<KIND> kc;
+      load_coef<KIND>(p, ex + static_cast<int64_t>(p.nx) * (ey + static_cast<int64_t>(p.ny) * ez), kc);
+      cell_element<KIND, Real, TRANS>(p, kc, X, u, ud, o);
     } else {
 #pragma unroll
       for (int i = 0; i < 36; ++i) o.K[i] = Real(0);
@@ -896,10 +999,11 @@ __device__ __forceinline__ void tile_of_block(int &bx, int &by, int &bz) {
 template <int KIND, typename Real, bool TRANS>
 __global__ __launch_bounds__(NT) void k_q1_hex8_diffusion(Q1DiffParams p) {
   __shared__ Real stage[stage_slots<Real>() * NSTAGE * NE];
-  __shared__ double rowbuf[8 * ROWBUF];
+  __shared__ double rowbuf[(TY - 1) * ROWBUF];
+  __shared__ uint32_t meta[stage_slots<Real>() * MSLOT];
   int bx, by, bz;
   tile_of_block<TXA_Q1_XCD_ORDER && sizeof(Real) == 4>(bx, by, bz);
-  spec_block<KIND, Real, TRANS>(p, bx, by, bz, stage, rowbuf);
+  spec_block<KIND, Real, TRANS>(p, bx, by, bz, stage, rowbuf, meta);
 }
 
 template <int KIND, typename Real, bool TRANS>
@@ -966,7 +1070,8 @@ template <int KIND, typename Real, bool TRANS>
 __global__ __launch_bounds__(NT) void k_q1_batch_spec(const Q1DiffParams *__restrict__ prm,
                                                       const int4 *__restrict__ items, int nitems, BatchQueue *q) {
   __shared__ Real stage[stage_slots<Real>() * NSTAGE * NE];
-  __shared__ double rowbuf[8 * ROWBUF];
+  __shared__ double rowbuf[(TY - 1) * ROWBUF];
+  __shared__ uint32_t meta[stage_slots<Real>() * MSLOT];
   __shared__ int slot;
   for (;;) {
     const int it = fetch_item(q, &slot);
@@ -974,7 +1079,7 @@ __global__ __launch_bounds__(NT) void k_q1_batch_spec(const Q1DiffParams *__rest
     const int4 w = items[it];
     const Q1DiffParams &p = prm[w.x];
     if (w.w * p.zc > p.nz) continue;   // slab past the mesh: nothing to do (uniform over the block)
-    spec_block<KIND, Real, TRANS>(p, w.y, w.z, w.w, stage, rowbuf);
+    spec_block<KIND, Real, TRANS>(p, w.y, w.z, w.w, stage, rowbuf, meta);
     __syncthreads();
   }
   leave_queue(q);
