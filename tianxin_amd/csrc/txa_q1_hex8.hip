@@ -328,6 +328,11 @@ __device__ __forceinline__ void hex8_element(const double (&X)[3][8], const doub
     }
     o.R[a] = Real(double(r) - F[a]);
   }
+  if (!TRANS && beta == 1.0) {   // the steady Jacobian dR/du (uniform branch): K as it is
+#pragma unroll
+    for (int i = 0; i < 36; ++i) o.K[i] = K[i];
+    return;
+  }
 #pragma unroll
   for (int a = 0; a < 8; ++a)
 #pragma unroll
@@ -613,8 +618,18 @@ __device__ __forceinline__ RowMeta get_meta(const uint32_t *ms, int t) {
   return m;
 }
 
-// Add rows a_lo = CX + 2CY (bottom) and a_lo + 4 (top) of the element at
-// corner (CX, CY) of the owner's vertex column, read from the stage.
+// Whether corner (CX, CY) is the first of the order (1,1), (0,1), (1,0), (0,0) to reach
+// stencil column (x, y): corner (CX, CY) reaches x in {1-CX, 2-CX}, y in {1-CY, 2-CY}.
+__device__ __forceinline__ constexpr bool first_touch(int CX, int CY, int x, int y) {
+  return (CX == 1 && CY == 1) || (CX == 0 && CY == 1 && x == 2) || (CX == 1 && CY == 0 && y == 2) ||
+         (CX == 0 && CY == 0 && x == 2 && y == 2);
+}
+
+// Add rows a_lo = CX + 2CY (bottom) and a_lo + 4 (top) of the element at corner
+// (CX, CY) of the owner's vertex column, read from the stage.  Called for the corners
+// in the order (1,1), (0,1), (1,0), (0,0): the first contribution to a slot of the
+// next layer's row (accn, rn) or to the upper stencil layer of this row (acc[18..26])
+// is stored rather than added, so those slots need no clearing between layers.
 template <int CX, int CY, typename Real>
 __device__ __forceinline__ void take_corner(const Real *s, int col, double (&acc)[27], double (&accn)[18], double &rc,
                                             double &rn) {
@@ -622,12 +637,17 @@ __device__ __forceinline__ void take_corner(const Real *s, int col, double (&acc
 #pragma unroll
   for (int b = 0; b < 8; ++b) {
     const int bx = b & 1, by = (b >> 1) & 1, bz = (b >> 2) & 1;
-    const int st = (bx - CX + 1) + 3 * (by - CY + 1);
-    acc[st + 9 * (bz + 1)] += double(s[sym8(alo, b) * NE + col]);
-    accn[st + 9 * bz] += double(s[sym8(ahi, b) * NE + col]);
+    const int x = bx - CX + 1, y = by - CY + 1, st = x + 3 * y;
+    const bool first = first_touch(CX, CY, x, y);
+    const double lo = double(s[sym8(alo, b) * NE + col]), hi = double(s[sym8(ahi, b) * NE + col]);
+    if (bz == 1 && first) acc[st + 18] = lo;
+    else acc[st + 9 * (bz + 1)] += lo;
+    if (first) accn[st + 9 * bz] = hi;
+    else accn[st + 9 * bz] += hi;
   }
   rc += double(s[(36 + alo) * NE + col]);
-  rn += double(s[(36 + ahi) * NE + col]);
+  if (CX == 1 && CY == 1) rn = double(s[(36 + ahi) * NE + col]);
+  else rn += double(s[(36 + ahi) * NE + col]);
 }
 
 // Vertex data of one face (4 vertices) of an element column.
@@ -801,11 +821,8 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
         }
         if (e >= k0) emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, m, acc, rc, rowbuf, t >> 5);
 #pragma unroll
-        for (int s = 0; s < 18; ++s) { acc[s] = accn[s]; accn[s] = 0.0; }
-#pragma unroll
-        for (int s = 18; s < 27; ++s) acc[s] = 0.0;
+        for (int s = 0; s < 18; ++s) acc[s] = accn[s];
         rc = rn;
-        rn = 0.0;
       }
       if constexpr (NSLOT == 1) __syncthreads();   // done with the stage
       __syncthreads();   // next layer is in the stage
@@ -958,11 +975,8 @@ __device__ __forceinline__ void ring_block(const Q1DiffParams &p, int bx, int by
       for (int r = 0; r < 2; ++r) store_col(r, ez & 1, fv[r]);
     if (ez >= k0) emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, meta, acc, rc, lds, t >> 5);
 #pragma unroll
-    for (int s = 0; s < 18; ++s) { acc[s] = accn[s]; accn[s] = 0.0; }
-#pragma unroll
-    for (int s = 18; s < 27; ++s) acc[s] = 0.0;
+    for (int s = 0; s < 18; ++s) acc[s] = accn[s];
     rc = rn;
-    rn = 0.0;
     Lcur = Lnext;
     __syncthreads();   // row buffers copied out, next face visible
   }
