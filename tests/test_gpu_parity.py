@@ -159,6 +159,18 @@ def test_hex8_diffusion_fp64_many_blocks_and_slabs(engine):
     c.m.destroy()
 
 
+@pytest.mark.parametrize("nx,ny,nz", [(14, 14, 2), (15, 16, 3), (29, 31, 2), (30, 6, 2), (31, 8, 3)])
+@pytest.mark.parametrize("precision,tol", [(txa.FP64, TOL64), (txa.FP32, TOL32)])
+def test_hex8_diffusion_tile_edges(engine, nx, ny, nz, precision, tol):
+    # vertex counts at and next to multiples of the owned patch of a workgroup
+    # (15 x 15 or 31 x 7 vertex columns), Dirichlet rows included, against the oracle
+    c = Case(engine, txa.HEX8, nx, ny, nz, 1, jitter=0.1, seed=nx + ny)
+    f, v, fr, vr = run_both(engine, c, txa.DIFFUSION, precision=precision, bc=True)
+    assert rel_err(f, fr) <= tol
+    assert rel_err(v, vr) <= tol
+    c.m.destroy()
+
+
 def test_panzer_shim_driver():
     # C++ evaluator layer (include/txa_panzer.hpp) driven like AssemblyEngine, checked against the oracle
     import os

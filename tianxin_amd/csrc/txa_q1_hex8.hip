@@ -31,7 +31,11 @@
 namespace txa {
 namespace {
 
-constexpr int TX = 32, TY = 8, NE = TX * TY;   // element tile: one element per thread of waves 0-3
+#ifndef TXA_Q1_TX
+#define TXA_Q1_TX 32   // element tile width: 32 x 8, or 16 x 16 (fewer recomputed elements, 1.5% slower at 256^3)
+#endif
+constexpr int TX = TXA_Q1_TX, TY = 256 / TX, NE = TX * TY;
+static_assert(TX == 16 || TX == 32, "tile width: a group of owner lanes (one x line) is 16 or 32 lanes");
 constexpr int NT = 2 * NE;                     // block: 4 element waves + 4 owner waves
 constexpr double kG = 0.57735026918962576451;   // 1/sqrt(3)
 constexpr double kP = 0.5 * (1.0 + kG);         // P(bit == qbit)
@@ -457,8 +461,9 @@ __device__ __forceinline__ void scatter_row(const Q1DiffParams &p, int32_t L, in
   }
 }
 
-constexpr int ROWBUF = 27 * 32;   // doubles per 32-lane group: 31 rows of at most 27 entries (837),
-constexpr int ROWBUF_SPARE = 27 * 31;   // then a spare slot for the entries a row does not have
+// Owner groups: the TX lanes of one x line of owners (TX-1 of them own rows).
+constexpr int ROWBUF = 27 * TX;          // doubles per group: TX-1 rows of at most 27 entries,
+constexpr int ROWBUF_SPARE = 27 * (TX - 1);   // then a spare slot for the entries a row does not have
 
 // Orders LDS writes and reads between the lanes of one wavefront.
 __device__ __forceinline__ void wave_lds_sync() {
@@ -493,28 +498,31 @@ __device__ __forceinline__ RowMeta load_meta(const Q1DiffParams &p, int32_t L) {
   return m;
 }
 
-// ALIGN: store the range in 256-byte aligned chunks.  It removes the partial lines inside
-// a range (WRITE_SIZE 4.32 -> 3.81 GB at 256^3, the bytes of f and the values), a small
-// gain in the FP32 kernels; the FP64 kernels ran slower with it, so they store unaligned.
+// ALIGN: store the range in aligned chunks of TX entries.  It removes the partial lines
+// inside a range (WRITE_SIZE 4.32 -> 3.81 GB at 256^3, the bytes of f and the values), a
+// small gain in the FP32 kernels; the FP64 kernels ran slower with it, so they store
+// unaligned.
 template <bool ALIGN>
 __device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &m, const double (&acc)[27], double r,
                                           double *rowbuf, int grp) {
-  const int lane = threadIdx.x & 63, gl = lane & 31, half = lane >> 5;
+  constexpr int GW = TX;   // lanes per group
+  constexpr uint64_t GMASK = (1ull << GW) - 1;
+  const int lane = threadIdx.x & 63, gl = lane & (GW - 1), gi = lane / GW;
   const bool act = m.L >= 0;
   const int32_t L = m.L;
   const int64_t base = m.base, end = m.end;
   if (act && p.f) global_ptr(p.f)[L] = r;
   if (!p.vals) return;   // uniform over the block
   // owners of a group are a prefix of its lanes; contiguous when successive LIDs differ by one
-  const int32_t Lnext = __shfl_down(L, 1, 32);
-  const bool next_act = __shfl_down(static_cast<int>(act), 1, 32) != 0 && gl < 31;
+  const int32_t Lnext = __shfl_down(L, 1, GW);
+  const bool next_act = __shfl_down(static_cast<int>(act), 1, GW) != 0 && gl < GW - 1;
   const bool ok = !act || !next_act || Lnext == L + 1;
   const uint64_t okm = __ballot(ok), actm = __ballot(act);
-  const uint32_t gok = static_cast<uint32_t>(okm >> (32 * half)), gact = static_cast<uint32_t>(actm >> (32 * half));
-  const int nact = __popc(gact);
-  const bool contiguous = gok == 0xffffffffu && nact > 0;
-  const int64_t base0 = __shfl(base, 0, 32);
-  const int64_t end_last = __shfl(end, nact > 0 ? nact - 1 : 0, 32);
+  const uint64_t gok = (okm >> (GW * gi)) & GMASK, gact = (actm >> (GW * gi)) & GMASK;
+  const int nact = __popcll(gact);
+  const bool contiguous = gok == GMASK && nact > 0;
+  const int64_t base0 = __shfl(base, 0, GW);
+  const int64_t end_last = __shfl(end, nact > 0 ? nact - 1 : 0, GW);
   double *buf = rowbuf + grp * ROWBUF;   // dereferenced only by groups with a row: grp < TY-1
 #if TXA_Q1_IDENT_FAST
   // interior rows: the 27 neighbours sit in the row in lattice order (slot s at s)
@@ -541,31 +549,32 @@ __device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &
     }
   }
   wave_lds_sync();   // the group's lanes live in one wavefront
-  // aligned: each store instruction covers one 256-byte aligned chunk of the range; lanes
-  // before the range's first entry idle in the first chunk, so only the two end chunks
-  // of the range are partial
+  // aligned: each group's part of a store instruction covers one aligned chunk of GW
+  // entries of the range; lanes before the range's first entry idle in the first chunk,
+  // so only the two end chunks of the range are partial
   const int total = static_cast<int>(end_last - base0);
-  const int lead = ALIGN ? static_cast<int>((reinterpret_cast<uintptr_t>(p.vals + base0) >> 3) & 31) : 0;
-  const int nch = contiguous ? (total + lead + 31) >> 5 : 0;    // 32-entry chunks of the group's range
-  const int nchw = max(nch, __shfl_xor(nch, 32));               // the wave's two groups: uniform count
+  const int lead = ALIGN ? static_cast<int>((reinterpret_cast<uintptr_t>(p.vals + base0) >> 3) & (GW - 1)) : 0;
+  int nchw = contiguous ? (total + lead + GW - 1) / GW : 0;   // chunks of the group's range
+#pragma unroll
+  for (int o = GW; o < 64; o <<= 1) nchw = max(nchw, __shfl_xor(nchw, o));   // uniform over the wave
   if (contiguous) {
-    // chunk c: lane gl stores range entry j = 32c + gl - lead when 0 <= j < total.  Four LDS
-    // reads are issued before their four stores (one LDS round trip per four chunks); the
-    // reads of idle lanes may fall outside the group's buffer and are not used.
+    // chunk c: lane gl stores range entry j = GW c + gl - lead when 0 <= j < total.  Four
+    // LDS reads are issued before their four stores (one LDS round trip per four chunks);
+    // the reads of idle lanes may fall outside the group's buffer and are not used.
     const int j0 = gl - lead;
     const double *src = buf + j0;
     gdouble *dst = global_ptr(p.vals + base0 + j0);
     for (int c = 0; c < nchw; c += 4) {
       double v[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) v[i] = src[32 * (c + i)];
+      for (int i = 0; i < 4; ++i) v[i] = src[GW * (c + i)];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
-        if (static_cast<unsigned>(j0 + 32 * (c + i)) < static_cast<unsigned>(total)) {
+        if (static_cast<unsigned>(j0 + GW * (c + i)) < static_cast<unsigned>(total)) {
 #if TXA_Q1_NT_STORES
-          __builtin_nontemporal_store(v[i], dst + 32 * (c + i));
+          __builtin_nontemporal_store(v[i], dst + GW * (c + i));
 #else
-          dst[32 * (c + i)] = v[i];
+          dst[GW * (c + i)] = v[i];
 #endif
         }
     }
@@ -819,7 +828,7 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
           take_corner<1, 0>(st, t + TX, acc, accn, rc, rn);
           take_corner<0, 0>(st, t + TX + 1, acc, accn, rc, rn);
         }
-        if (e >= k0) emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, m, acc, rc, rowbuf, t >> 5);
+        if (e >= k0) emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, m, acc, rc, rowbuf, t / TX);
 #pragma unroll
         for (int s = 0; s < 18; ++s) acc[s] = accn[s];
         rc = rn;
@@ -831,14 +840,14 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
     if (k1 == p.nz + 1) {
       RowMeta m;
       if (owner) m = get_meta(meta + (NSLOT == 2 ? ((ez1 + 1) & 1) : 0) * MSLOT, t);
-      emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, m, acc, rc, rowbuf, t >> 5);
+      emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, m, acc, rc, rowbuf, t / TX);
     }
   }
 }
 
 
 // One-role variant: every thread computes an element and owns a vertex column.
-// The vertex data of the tile's faces (33 x 9 columns) is staged in an LDS ring
+// The vertex data of the tile's faces ((TX+1) x (TY+1) columns) is staged in an LDS ring
 // of two z faces; the face two layers up is loaded into registers at the start
 // of a layer and stored into the ring after the element work, so every global
 // load of a layer is issued before that layer's row stores and no wait on a
@@ -848,9 +857,9 @@ template <bool TRANS>
 constexpr int ring_nval() { return TRANS ? 5 : 4; }  // x, y, z, u (, udot)
 template <typename Real>
 constexpr int ring_union_doubles() {
-  return (NSTAGE * NE * static_cast<int>(sizeof(Real)) + 7) / 8 > 8 * ROWBUF
+  return (NSTAGE * NE * static_cast<int>(sizeof(Real)) + 7) / 8 > TY * ROWBUF
              ? (NSTAGE * NE * static_cast<int>(sizeof(Real)) + 7) / 8
-             : 8 * ROWBUF;
+             : TY * ROWBUF;
 }
 
 template <int KIND, typename Real, bool TRANS>
@@ -973,26 +982,27 @@ __device__ __forceinline__ void ring_block(const Q1DiffParams &p, int bx, int by
     if (more)
 #pragma unroll
       for (int r = 0; r < 2; ++r) store_col(r, ez & 1, fv[r]);
-    if (ez >= k0) emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, meta, acc, rc, lds, t >> 5);
+    if (ez >= k0) emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, meta, acc, rc, lds, t / TX);
 #pragma unroll
     for (int s = 0; s < 18; ++s) acc[s] = accn[s];
     rc = rn;
     Lcur = Lnext;
     __syncthreads();   // row buffers copied out, next face visible
   }
-  if (k1 == p.nz + 1) emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, load_meta(p, Lcur), acc, rc, lds, t >> 5);
+  if (k1 == p.nz + 1) emit_rows<TXA_Q1_ALIGNED_STORES && sizeof(Real) == 4>(p, load_meta(p, Lcur), acc, rc, lds, t / TX);
 }
 
 
 #ifndef TXA_Q1_XCD_ORDER
-#define TXA_Q1_XCD_ORDER 1   // FP32 forms: each XCD takes a contiguous run of tiles (neighbours share its L2)
+#define TXA_Q1_XCD_ORDER 0   // 1: FP32 forms hand each XCD a contiguous run of tiles (neighbours share its L2)
 #endif
 
 // Tile of this workgroup.  Workgroups are dealt to the 8 XCDs round robin (linear id
 // mod 8); each XCD has its own L2.  XCD_ORDER hands XCD c the c-th contiguous run of
 // the tile order (x fastest, then y, then the z slab), which keeps tiles that share
-// halo vertices on one L2 at about the same time: FP32 1.957 -> 1.93 ms at 256^3.  The
-// FP64 kernels, which load each face without a prefetch, ran 11% slower with it.
+// halo vertices on one L2 at about the same time.  It gained 1.4% (FP32, 256^3) while
+// the owner waves loaded their own row metadata; with the store-only owner waves the
+// plain order is 2-3% faster, and the FP64 kernels were always slower with it.
 template <bool XCD_ORDER>
 __device__ __forceinline__ void tile_of_block(int &bx, int &by, int &bz) {
   if (XCD_ORDER) {
@@ -1033,7 +1043,7 @@ __global__ __launch_bounds__(NE) void k_q1_hex8_ring(Q1DiffParams p) {
 // ============================================================================
 // Batched assembly: many meshes in one persistent launch
 // ============================================================================
-// Work item = one tile (workgroup's worth: a 31x7 patch of vertex columns and a
+// Work item = one tile (workgroup's worth: a (TX-1) x (TY-1) patch of vertex columns and a
 // z slab) of one mesh.  Workgroups pull items from a counter until the list is
 // exhausted; the last workgroup to leave resets the counter, so the launch can
 // be replayed (as a graph node) without a separate clear.
