@@ -161,18 +161,13 @@ __device__ __forceinline__ void hex8_element(const double (&X)[3][8], const doub
     if (q == 0) sg = det < 0.0 ? -1.0 : 1.0;   // computeCellMeasure: sign of det at the first point
     wm[q] = det * sg;                          // Gauss weights are 1
     const double cw = 0.25 * k * sg * recip(det);   // k wm / det^2, times the 1/4 of (s/2)(s/2)
-    Real cr[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) cr[i][j] = Real(c[i][j]);
-    const Real cwr = Real(cw);
-    A[q][0] = cwr * (cr[0][0] * cr[0][0] + cr[1][0] * cr[1][0] + cr[2][0] * cr[2][0]);
-    A[q][1] = cwr * (cr[0][1] * cr[0][1] + cr[1][1] * cr[1][1] + cr[2][1] * cr[2][1]);
-    A[q][2] = cwr * (cr[0][2] * cr[0][2] + cr[1][2] * cr[1][2] + cr[2][2] * cr[2][2]);
-    A[q][3] = cwr * (cr[0][0] * cr[0][1] + cr[1][0] * cr[1][1] + cr[2][0] * cr[2][1]);
-    A[q][4] = cwr * (cr[0][1] * cr[0][2] + cr[1][1] * cr[1][2] + cr[2][1] * cr[2][2]);
-    A[q][5] = cwr * (cr[0][0] * cr[0][2] + cr[1][0] * cr[1][2] + cr[2][0] * cr[2][2]);
+    // the metric in FP64, rounded once to Real (six conversions per point instead of ten)
+    A[q][0] = Real(cw * (c[0][0] * c[0][0] + c[1][0] * c[1][0] + c[2][0] * c[2][0]));
+    A[q][1] = Real(cw * (c[0][1] * c[0][1] + c[1][1] * c[1][1] + c[2][1] * c[2][1]));
+    A[q][2] = Real(cw * (c[0][2] * c[0][2] + c[1][2] * c[1][2] + c[2][2] * c[2][2]));
+    A[q][3] = Real(cw * (c[0][0] * c[0][1] + c[1][0] * c[1][1] + c[2][0] * c[2][1]));
+    A[q][4] = Real(cw * (c[0][1] * c[0][2] + c[1][1] * c[1][2] + c[2][1] * c[2][2]));
+    A[q][5] = Real(cw * (c[0][0] * c[0][2] + c[1][0] * c[1][2] + c[2][0] * c[2][2]));
   }
   // ---- diagonal tables S_dd[c1][c2]: sum over the two other axes after summing axis d
   Real S[3][3][3];
