@@ -50,10 +50,16 @@ struct ElasticParams {
   const double *x, *xdot, *xddot;
   double alpha, beta, gamma2;
   int transient;
-  void *jbuf;        // element Jacobian blocks [E][ND][ND] in the Gram layout: row i NB + a = DOF (a, i),
-                     // column j NB + b = DOF (b, j) (FP32, FP64 in FP64 mode); null: none
+  void *jbuf;        // element Jacobian blocks [E][NB][block_stride(NB)], node-major: node a holds the
+                     // rows of DOFs (a, 0), (a, 1), (a, 2), each over the Gram columns j NB + b = DOF
+                     // (b, j), then zero padding (FP32, FP64 in FP64 mode); null: none
   double *rbuf;      // element residuals [E][ND]; null: none
 };
+
+// Entries per node in the workspace block: the node's three rows (9 NB entries), padded for
+// hex27 to 256 so every node's rows start on a 1 KiB boundary (FP32) and the gather reads
+// whole cache lines; hex8's 72 entries stay unpadded.
+__host__ __device__ constexpr int block_stride(int NB) { return NB == 27 ? 256 : 9 * NB; }
 
 // lattice offsets of the element's nodes: shards vertex order (hex8), Q2 order (hex27)
 __device__ const int8_t kHexV[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
@@ -468,13 +474,15 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 3 : 1) void k_elastic(Elas
     }
     __syncthreads();
     fetch_data(e + gridDim.x);
-    // ---- element block to the workspace in the Gram layout (row i NB + a is DOF (a, i),
-    // column j NB + b is DOF (b, j)): consecutive lanes on consecutive entries
-    if (p.jbuf)
-      for (int idx = t; idx < ND * ND; idx += BS) {
-        const int r = idx / ND, c = idx % ND;
-        static_cast<TReal *>(p.jbuf)[e * (ND * ND) + idx] = Tref(r, c);
+    // ---- element block to the workspace, node-major (node a: rows i NB + a, i = 0..2, of
+    // the Gram layout, then padding): consecutive lanes on consecutive entries
+    if (p.jbuf) {
+      constexpr int NP = block_stride(NB);
+      for (int idx = t; idx < NB * NP; idx += BS) {
+        const int a = idx / NP, q = idx % NP;
+        static_cast<TReal *>(p.jbuf)[e * (NB * NP) + idx] = q < 3 * ND ? Tref((q / ND) * NB + a, q % ND) : TReal(0);
       }
+    }
     __syncthreads();
     if (p.rbuf)
       for (int i = t; i < ND; i += BS) {
@@ -527,17 +535,17 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
   // the wave index is uniform: everything derived from the node stays in scalar registers
   const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   constexpr int NR = (3 * ND + 63) / 64;   // block entries per lane and element
+  constexpr int NP = block_stride(NB);
   // per lane and r: entry idx = lane + 64 r of the node's three block rows (row field
-  // idx / ND, Gram column idx % ND = field j, node bn), the lattice offset index of its
-  // column node plus R in each axis, and its offset in the element block
-  int lterm[NR], joff[NR];
+  // idx / ND, Gram column idx % ND = field j, node bn), and the lattice offset index of
+  // its column node plus R in each axis; the entry sits at idx in the node's part of the block
+  int lterm[NR];
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
     const int idx = lane + 64 * r, c = idx % ND;
     const int bn = c % NB;
     const int8_t *o = NB == 8 ? kHexV[bn] : kQ2[bn];
     lterm[r] = (o[0] + R) + S * (o[1] + R) + S * S * (o[2] + R);
-    joff[r] = (idx / ND) * NB * ND + c;
   }
   for (int i = threadIdx.x; i < NB; i += 256) {
     const int8_t *o = NB == 8 ? kHexV[i] : kQ2[i];
@@ -587,7 +595,7 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
         pos[c][r] = 0xffffu;
         const int idx = lane + 64 * r;
         if (c < ne && idx < 3 * ND && J) {
-          val[c][r] = J[(ce[c] * ND + ca[c]) * ND + joff[r]];   // Gram rows a, NB + a, 2 NB + a
+          val[c][r] = J[(ce[c] * NB + ca[c]) * NP + idx];   // node ca's rows: one contiguous run
           pos[c][r] = np[lterm[r] - cs[c]];
         }
       }
@@ -673,7 +681,10 @@ size_t ws_round(size_t b) { return (b + kWsAlign - 1) / kWsAlign * kWsAlign; }
 void ws_parts(const Mesh &m, int precision, bool want_f, bool want_vals, size_t *rbytes, size_t *jbytes) {
   const size_t nd = static_cast<size_t>(m.ndof_elem), ne = static_cast<size_t>(m.nelem);
   *rbytes = want_f ? ws_round(ne * nd * sizeof(double)) : 0;
-  *jbytes = want_vals ? ws_round(ne * nd * nd * (precision == TXA_FP64 ? sizeof(double) : sizeof(float))) : 0;
+  const size_t nb = nd / 3;
+  *jbytes = want_vals ? ws_round(ne * nb * static_cast<size_t>(block_stride(static_cast<int>(nb))) *
+                                 (precision == TXA_FP64 ? sizeof(double) : sizeof(float)))
+                      : 0;
 }
 
 }  // namespace
