@@ -554,12 +554,25 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
   __syncthreads();
   double *b = buf[w];
   const JT *J = static_cast<const JT *>(g.jbuf);
-  for (int64_t pt = static_cast<int64_t>(blockIdx.x) * 4 + w; pt < g.npoints; pt += static_cast<int64_t>(gridDim.x) * 4) {
+  // The node's LID and row pointers are wave-uniform: they are read through the scalar
+  // cache (constant address space; neither array changes during the kernel), whose
+  // counter is not the one the row stores use, and one node ahead.  Read as vector loads,
+  // each node began by waiting for them and with them for every store of the node before.
+  using cint = const __attribute__((address_space(4))) int32_t;
+  using clong = const __attribute__((address_space(4))) int64_t;
+  const cint *plid = (const cint *)g.point_lid;
+  const clong *prow = (const clong *)g.rowptr;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * 4;
+  int64_t pt = static_cast<int64_t>(blockIdx.x) * 4 + w;
+  int32_t Lc = plid[min(pt, g.npoints - 1) * 3];
+  int64_t bc = prow[Lc], ec = prow[Lc + 1];
+  for (; pt < g.npoints; pt += stride) {
     const int64_t I = pt % g.px, Jy = (pt / g.px) % g.py, K = pt / (g.px * g.py);
-    const int32_t L0 = g.point_lid[pt * 3];
-    const int64_t base = g.rowptr[L0];
+    const int32_t L0 = Lc;
+    const int64_t base = bc;
     // row length (at most S^3 neighbours x 3 fields; clamped so the LDS rows never overflow)
-    const int len = min(static_cast<int>(g.rowptr[L0 + 1] - base), MAXLEN);
+    const int len = min(static_cast<int>(ec - bc), MAXLEN);
+    const int32_t Ln = plid[min(pt + stride, g.npoints - 1) * 3];   // the next node's LID
     const int64_t node = L0 / 3;
     const uint16_t *np = g.nperm + node * g.nperm_w;
     for (int k = lane; k < 3 * len; k += 64) b[k] = 0.0;
@@ -601,6 +614,7 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
       }
       if (c < ne && lane < 3 && g.rbuf) rr[c] = g.rbuf[ce[c] * ND + 3 * ca[c] + lane];
     }
+    const int64_t bn = prow[Ln], en = prow[Ln + 1];   // the next node's row pointers
     // the values stay opaque until here, so no conversion pulls a wait in among the loads
 #pragma unroll
     for (int c = 0; c < 8; ++c)
@@ -625,6 +639,9 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
       for (int k = lane; k < 3 * len; k += 64) g.vals[base + k] = b[k];
     if (g.f && lane < 3) g.f[L0 + lane] = racc;
     gather_lds_sync();   // the buffer is reused by the next node
+    Lc = Ln;
+    bc = bn;
+    ec = en;
   }
 }
 
