@@ -214,43 +214,50 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 3 : 1) void k_elastic(Elas
   for (int i = t; i < NB * 3; i += BS) sm.off[i / 3][i % 3] = NB == 8 ? kHexV[i / 3][i % 3] : kQ2[i / 3][i % 3];
   __syncthreads();
 
-  // Gather data of the next element, fetched while the current one is computed: the
-  // connectivity and LIDs (thread t < 24: one vertex coordinate; t < ND: one DOF) once
-  // the Gram matrix is done, the coordinates and state they address before the block
-  // store (short register lifetimes: the pair phase is the register peak).
+  // Gather data of the element after this one, fetched while this one is computed: the
+  // coordinates, state and coefficient (thread t < 24: one vertex coordinate; t < ND:
+  // one DOF) once the Gram matrix is done, from the connectivity and LIDs fetched one
+  // element earlier; they are put in LDS after the pair phase, before the block store.
+  // No global load is then pending when the block store goes out, so the next element
+  // starts without waiting for those stores (one counter tracks both).  The loads have
+  // no branch (indices clamped to a valid element and lane; the extra values are not
+  // used): a conditional load would be copied, and so waited for, where the branch joins.
   static_assert(BS >= ND && BS >= 24, "one gather entry per thread");
   int64_t gn = 0;
   int32_t gL = 0;
-  double gX = 0.0, gu = 0.0, gv = 0.0, gac = 0.0;
+  double gX = 0.0, gu = 0.0, gv = 0.0, gac = 0.0, gE = 0.0;
   auto fetch_index = [&](int64_t e) {
-    if (e >= p.nelem) return;
-    if (t < 24) gn = p.conn[e * p.nconn + t / 3];
-    if (t < ND) gL = p.elem_lid[e * ND + t];
+    const int64_t ec = e < p.nelem ? e : p.nelem - 1;
+    gn = p.conn[ec * p.nconn + min(t, 23) / 3];
+    gL = p.elem_lid[ec * ND + min(t, ND - 1)];
   };
   auto fetch_data = [&](int64_t e) {
-    if (e >= p.nelem) return;
-    if (t < 24) {
-      const int d = t % 3;
-      gX = d == 0 ? p.X[gn] : (d == 1 ? p.Y[gn] : p.Z[gn]);
-    }
-    if (t < ND) {
-      gu = p.x[gL];
-      gv = tr ? p.xdot[gL] : 0.0;
-      gac = tra ? p.xddot[gL] : 0.0;
-    }
+    const int64_t ec = e < p.nelem ? e : p.nelem - 1;
+    const int d = t % 3;
+    const double *P = d == 0 ? p.X : (d == 1 ? p.Y : p.Z);
+    gX = P[gn];
+    gu = p.x[gL];
+    gv = tr ? p.xdot[gL] : 0.0;
+    gac = tra ? p.xddot[gL] : 0.0;
+    gE = p.coef[ec];
   };
-  fetch_index(blockIdx.x);
-  fetch_data(blockIdx.x);
-
-  for (int64_t e = blockIdx.x; e < p.nelem; e += gridDim.x) {
-    // ---- gather: vertices, state (fetched during the previous element)
+  auto put_data = [&]() {
     if (t < 24) sm.X[t / 3][t % 3] = gX;
     if (t < ND) {
       sm.u[t] = gu;
       sm.v[t] = gv;
       sm.a[t] = gac;
     }
-    __syncthreads();
+    asm volatile("" : "+v"(gE));   // the coefficient is taken here too, not after the block store
+  };
+  fetch_index(blockIdx.x);
+  fetch_data(blockIdx.x);
+  put_data();
+  double E = gE;
+  fetch_index(blockIdx.x + gridDim.x);
+
+  for (int64_t e = blockIdx.x; e < p.nelem; e += gridDim.x) {
+    __syncthreads();   // this element's vertices and state are in LDS
     // ---- geometry (FP64)
     if (t < NQ) {
       const int qx = t % Q1, qy = (t / Q1) % Q1, qz = t / (Q1 * Q1);
@@ -421,11 +428,11 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 3 : 1) void k_elastic(Elas
       }
     }
     __syncthreads();
-    fetch_index(e + gridDim.x);
+    fetch_data(e + gridDim.x);
+    fetch_index(e + 2 * static_cast<int64_t>(gridDim.x));
     // ---- 3x3 blocks of node pairs: residual rows, Jacobian values written over the
     // pair's own T entries (the block of (a, b) depends only on those nine values)
     //   K_ij = lambda T[(i,a),(j,b)] + mu T[(j,a),(i,b)] + delta_ij mu sum_k T[(k,a),(k,b)]
-    const double E = p.coef[e];
     const double lam = E * p.nu / ((1.0 + p.nu) * (1.0 - 2.0 * p.nu));
     const double mu = E / (2.0 * (1.0 + p.nu));
     auto Tref = [&](int r, int c) -> TReal & {
@@ -473,7 +480,7 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 3 : 1) void k_elastic(Elas
         for (int i = 0; i < 3; ++i) sm.r[3 * a + i] = rr[i];
     }
     __syncthreads();
-    fetch_data(e + gridDim.x);
+    put_data();   // the next element's vertices and state (LDS X, u, v, a are free now)
     // ---- element block to the workspace, node-major (node a: rows i NB + a, i = 0..2, of
     // the Gram layout, then padding): consecutive lanes on consecutive entries
     if (p.jbuf) {
@@ -494,11 +501,14 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 3 : 1) void k_elastic(Elas
             const int qx = q % Q1, qy = (q / Q1) % Q1, qz = q / (Q1 * Q1);
             F += sm.wm[q] * sm.bL[ax][qx] * sm.bL[ay][qy] * sm.bL[az][qz];
           }
-          F *= sg * p.src[i % 3];
+          // a select, not an indexed read: src[i % 3] would be a vector load from the
+          // parameters, and waiting for it would wait for the block stores
+          const int c = i % 3;
+          F *= sg * (c == 0 ? p.src[0] : (c == 1 ? p.src[1] : p.src[2]));
         }
         p.rbuf[e * ND + i] = sm.r[i] - F;
       }
-    __syncthreads();
+    E = gE;
   }
 }
 
