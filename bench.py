@@ -482,15 +482,21 @@ def bench_serve(args):
     for i, (m, E, bc) in enumerate(pool):
         outs[i] = (torch.empty(m.ndofs, device="cuda", dtype=torch.float64),
                    torch.empty(m.nnz, device="cuda", dtype=torch.float64))
-    torch.cuda.synchronize()
-    tp = time.perf_counter()
-    for r in range(args.requests):
+    def plain(r):
         m, E, bc = pool[pick[r]]
         u, v, a = states[r]
         f, vals = outs[pick[r]]
         m.assemble(physics=txa.ELASTICITY, precision=txa.FP64, coef=E, x=u, xdot=v, xddot=a, f=f, vals=vals,
                    alpha=0.5 / (0.25 * 1e-3), beta=1.0, gamma2=1.0 / (0.25 * 1e-6), transient=True, nu=0.3, rho=1.0,
                    rayleigh=(0.01, 0.001), bc_lids=bc)
+    # warm-up: one call per resident mesh, so the context's workspace pool has grown to the
+    # largest request before the timed pass (the server's arena is allocated up front too)
+    for i in sorted(set(int(k) for k in pick[:args.requests])):
+        plain(int(np.nonzero(pick == i)[0][0]))
+    torch.cuda.synchronize()
+    tp = time.perf_counter()
+    for r in range(args.requests):
+        plain(r)
     torch.cuda.synchronize()
     wall_plain = time.perf_counter() - tp
     line = {"metric": "serving: p50/p99 latency and elements/s (transient elasticity residual+Jacobian)",
