@@ -577,8 +577,12 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
   int64_t pt = static_cast<int64_t>(blockIdx.x) * 4 + w;
   int32_t Lc = plid[min(pt, g.npoints - 1) * 3];
   int64_t bc = prow[Lc], ec = prow[Lc + 1];
+  // lattice coordinates of the node, advanced by the stride's own coordinates with
+  // carries: the divisions happen once, not per node
+  const int64_t pxy = g.px * g.py;
+  int64_t I = pt % g.px, Jy = (pt / g.px) % g.py, K = pt / pxy;
+  const int64_t sI = stride % g.px, sJ = (stride / g.px) % g.py, sK = stride / pxy;
   for (; pt < g.npoints; pt += stride) {
-    const int64_t I = pt % g.px, Jy = (pt / g.px) % g.py, K = pt / (g.px * g.py);
     const int32_t L0 = Lc;
     const int64_t base = bc;
     // row length (at most S^3 neighbours x 3 fields; clamped so the LDS rows never overflow)
@@ -653,6 +657,11 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
     Lc = Ln;
     bc = bn;
     ec = en;
+    I += sI;
+    Jy += sJ;
+    K += sK;
+    if (I >= g.px) { I -= g.px; ++Jy; }
+    if (Jy >= g.py) { Jy -= g.py; ++K; }
   }
 }
 
