@@ -635,6 +635,9 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
     for (int c = 0; c < 8; ++c)
 #pragma unroll
       for (int r = 0; r < NR; ++r) asm volatile("" : "+v"(val[c][r]));
+    // LDS atomic adds, issued back to back: the LDS executes one wave's operations in
+    // order and no two lanes of one instruction share an entry, so every entry is summed
+    // in cell order, as one read-add-write round per cell did, without a round trip per cell
     double racc = 0.0;
 #pragma unroll
     for (int c = 0; c < 8; ++c) {
@@ -644,12 +647,12 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
           const int idx = lane + 64 * r;
           const int i = idx / ND, j = (idx % ND) / NB;
           const uint16_t ps = pos[c][r];
-          if (idx < 3 * ND && ps != 0xffffu && ps + j < len) b[i * len + ps + j] += static_cast<double>(val[c][r]);
+          if (idx < 3 * ND && ps != 0xffffu && ps + j < len) atomicAdd(&b[i * len + ps + j], static_cast<double>(val[c][r]));
         }
         racc += rr[c];
       }
-      gather_lds_sync();   // the next element may add to the same entries
     }
+    gather_lds_sync();   // all additions done before the rows are read
     if (g.vals)
       for (int k = lane; k < 3 * len; k += 64) g.vals[base + k] = b[k];
     if (g.f && lane < 3) g.f[L0 + lane] = racc;
