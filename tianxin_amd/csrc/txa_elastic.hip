@@ -26,7 +26,8 @@
 // Jacobian block (FP32, FP64 in FP64 mode) and residual (FP64) to a workspace,
 // coalesced; a node-centric gather then sums, for every node, the blocks of the
 // elements around it into the node's three CRS rows in FP64, in a fixed element
-// order, and writes each row once.  No atomics: results are bitwise reproducible.
+// order, and writes each row once.  No global-memory atomics (the gather's LDS adds
+// run in one wave's program order): results are bitwise reproducible.
 // The row position of every (node, neighbour) pair comes from a per-node table
 // built with the graph.
 #include <type_traits>
