@@ -51,7 +51,7 @@ constexpr int stage_slots() { return sizeof(Real) == 4 ? 2 : 1; }
 #define TXA_Q1_FP32_RING 0    // 1: FP32 integration uses the one-role LDS-ring kernel (0: warp-specialised)
 #endif
 #ifndef TXA_Q1_TARGET_BLOCKS
-#define TXA_Q1_TARGET_BLOCKS 2048   // z slabs are sized to give about this many workgroups
+#define TXA_Q1_TARGET_BLOCKS 2560   // z slabs are sized to give about this many workgroups
 #endif
 #ifndef TXA_Q1_IDENT_FAST
 #define TXA_Q1_IDENT_FAST 1   // rows whose slot map is the identity skip the per-entry decode
