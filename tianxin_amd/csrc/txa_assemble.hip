@@ -28,7 +28,9 @@ namespace {
 // ============================================================================
 // Generic path: one workgroup per element, atomic scatter
 // ============================================================================
-constexpr int GB = 256;
+// Two shapes: 256 threads with tables for up to 27 basis functions and points
+// (hex27), and one wave with tables for up to 8 (hex8, tet4) whose LDS is small
+// enough for many elements in flight per CU.
 
 struct GenericParams {
   int topo, physics, nb, nq, nv, nf, ndof, nconn;
@@ -56,10 +58,12 @@ __device__ __forceinline__ int64_t find_slot(const int64_t *rowptr, const int32_
   return lo;
 }
 
+template <int GB, int MNB, int MNQ>
 __global__ __launch_bounds__(GB) void k_generic(GenericParams p) {
-  __shared__ double sX[8][3], sDet[kMaxNQ], sJi[kMaxNQ][9], sWm[kMaxNQ];
-  __shared__ double sG[kMaxNB * kMaxNQ * 3];
-  __shared__ double sU[81], sV[81], sA[81], sR[81];
+  constexpr int MND = 3 * MNB;   // element DOFs, at most
+  __shared__ double sX[8][3], sDet[MNQ], sJi[MNQ][9], sWm[MNQ];
+  __shared__ double sG[MNB * MNQ * 3];
+  __shared__ double sU[MND], sV[MND], sA[MND], sR[MND];
   const double *tN = p.tab;
   const double *tdN = tN + p.nb * p.nq;
   const double *tgdN = tdN + p.nb * p.nq * 3;
@@ -69,25 +73,26 @@ __global__ __launch_bounds__(GB) void k_generic(GenericParams p) {
   const bool tr = p.transient && p.xdot;
   for (int64_t e = blockIdx.x; e < p.nelem; e += gridDim.x) {
     const int32_t *lids = p.elem_lid + e * p.ndof;
+    static_assert(GB >= 24, "one vertex coordinate per thread");
     if (t < p.nv * 3) {
       const int v = t / 3, d = t % 3;
       const int64_t n = p.conn[e * p.nconn + v];
       sX[v][d] = d == 0 ? p.X[n] : (d == 1 ? p.Y[n] : p.Z[n]);
     }
-    if (t < p.ndof) {
-      const int32_t L = lids[t];
-      sU[t] = p.x[L];
-      sV[t] = tr ? p.xdot[L] : 0.0;
-      sA[t] = (tr && p.xddot) ? p.xddot[L] : 0.0;
-      sR[t] = 0.0;
+    for (int i = t; i < p.ndof; i += GB) {
+      const int32_t L = lids[i];
+      sU[i] = p.x[L];
+      sV[i] = tr ? p.xdot[L] : 0.0;
+      sA[i] = (tr && p.xddot) ? p.xddot[L] : 0.0;
+      sR[i] = 0.0;
     }
     __syncthreads();
-    if (t < p.nq) {
+    for (int q = t; q < p.nq; q += GB) {
       double J[3][3];
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) {
           double s = 0.0;
-          for (int v = 0; v < p.nv; ++v) s += sX[v][i] * tgdN[(v * p.nq + t) * 3 + j];
+          for (int v = 0; v < p.nv; ++v) s += sX[v][i] * tgdN[(v * p.nq + q) * 3 + j];
           J[i][j] = s;
         }
       const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
@@ -95,7 +100,7 @@ __global__ __launch_bounds__(GB) void k_generic(GenericParams p) {
       const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
       const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
       const double id = 1.0 / det;
-      double *Ji = sJi[t];
+      double *Ji = sJi[q];
       Ji[0] = c00 * id;
       Ji[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
       Ji[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
@@ -105,10 +110,10 @@ __global__ __launch_bounds__(GB) void k_generic(GenericParams p) {
       Ji[6] = c02 * id;
       Ji[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
       Ji[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
-      sDet[t] = det;
+      sDet[q] = det;
     }
     __syncthreads();
-    if (t < p.nq) sWm[t] = tqw[t] * sDet[t] * (sDet[0] < 0.0 ? -1.0 : 1.0);
+    for (int q = t; q < p.nq; q += GB) sWm[q] = tqw[q] * sDet[q] * (sDet[0] < 0.0 ? -1.0 : 1.0);
     for (int idx = t; idx < p.nb * p.nq; idx += GB) {
       const int b = idx / p.nq, q = idx % p.nq;
       const double *Ji = sJi[q];
@@ -152,11 +157,11 @@ __global__ __launch_bounds__(GB) void k_generic(GenericParams p) {
       }
     }
     __syncthreads();
-    if (t < p.ndof && p.f) {
-      const int a = t / p.nf, i = t % p.nf;
+    for (int k = t; p.f && k < p.ndof; k += GB) {
+      const int a = k / p.nf, i = k % p.nf;
       double F = 0.0;
       for (int q = 0; q < p.nq; ++q) F += sWm[q] * tN[a * p.nq + q] * p.src[i];
-      atomicAdd(p.f + lids[t], sR[t] - F);
+      atomicAdd(p.f + lids[k], sR[k] - F);
     }
     __syncthreads();
   }
@@ -323,7 +328,10 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
     if (a.f) TXA_HIP(hipMemsetAsync(a.f, 0, sizeof(double) * m.ndofs, s));
     if (a.vals) TXA_HIP(hipMemsetAsync(a.vals, 0, sizeof(double) * m.nnz, s));
     int64_t blocks = m.nelem < 65536 * 8 ? m.nelem : 65536 * 8;
-    hipLaunchKernelGGL(k_generic, dim3(static_cast<unsigned>(blocks)), dim3(GB), 0, s, p);
+    if (r.nb <= 8 && r.nq <= 8)
+      hipLaunchKernelGGL((k_generic<64, 8, 8>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, p);
+    else
+      hipLaunchKernelGGL((k_generic<256, kMaxNB, kMaxNQ>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, p);
     TXA_HIP(hipGetLastError());
   }
   // Neumann sides, one launch each (stream order keeps the sums of shared edge points in a fixed order)
