@@ -64,6 +64,15 @@ __global__ __launch_bounds__(GB) void k_generic(GenericParams p) {
   __shared__ double sX[8][3], sDet[MNQ], sJi[MNQ][9], sWm[MNQ];
   __shared__ double sG[MNB * MNQ * 3];
   __shared__ double sU[MND], sV[MND], sA[MND], sR[MND];
+  __shared__ int32_t sL[MND];   // the element's LIDs
+  // One-wave shape: the column lists of the element's rows are staged in LDS (up to RCAP
+  // entries; longer ones fall back to the search in global memory), so the row search of
+  // every entry runs in LDS instead of as a chain of dependent global loads.
+  constexpr bool kStage = GB == 64;
+  constexpr int RCAP = kStage ? 512 : 1;
+  __shared__ int32_t sCol[RCAP];
+  __shared__ int64_t sRB[kStage ? MND : 1];       // CRS start of each element row
+  __shared__ int32_t sRO[kStage ? MND + 1 : 1];   // start of each element row in sCol
   const double *tN = p.tab;
   const double *tdN = tN + p.nb * p.nq;
   const double *tgdN = tdN + p.nb * p.nq * 3;
@@ -81,10 +90,51 @@ __global__ __launch_bounds__(GB) void k_generic(GenericParams p) {
     }
     for (int i = t; i < p.ndof; i += GB) {
       const int32_t L = lids[i];
+      sL[i] = L;
       sU[i] = p.x[L];
       sV[i] = tr ? p.xdot[L] : 0.0;
       sA[i] = (tr && p.xddot) ? p.xddot[L] : 0.0;
       sR[i] = 0.0;
+    }
+    bool staged = false;   // uniform over the block
+    if constexpr (kStage) {
+      int64_t rb = 0;
+      int rl = 0;
+      if (t < p.ndof && p.vals) {
+        const int32_t L = lids[t];
+        rb = p.rowptr[L];
+        rl = static_cast<int>(p.rowptr[L + 1] - rb);
+      }
+      int inc = rl;   // inclusive prefix sum of the row lengths over the wave
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o);
+        if (t >= o) inc += v;
+      }
+      const int total = __shfl(inc, p.ndof - 1);
+      staged = p.vals && total <= RCAP;
+      if (t < p.ndof) { sRB[t] = rb; sRO[t + 1] = inc; }
+      if (t == 0) sRO[0] = 0;
+      __syncthreads();
+      if (staged)
+        for (int k0 = t; k0 < total; k0 += 4 * GB) {
+          int32_t v[4];
+          int kk[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {   // four independent loads before their LDS writes
+            const int k = k0 + u * GB;
+            kk[u] = k;
+            v[u] = 0;
+            if (k < total) {
+              int I = 0;   // the element row of entry k: the last row starting at or before k
+              for (int r = 1; r < p.ndof; ++r) I = sRO[r] <= k ? r : I;
+              v[u] = p.col[sRB[I] + (k - sRO[I])];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (kk[u] < total) sCol[kk[u]] = v[u];
+        }
     }
     __syncthreads();
     for (int q = t; q < p.nq; q += GB) {
@@ -152,8 +202,19 @@ __global__ __launch_bounds__(GB) void k_generic(GenericParams p) {
       }
       atomicAdd(&sR[I], rcontrib);
       if (p.vals) {
-        const int64_t slot = find_slot(p.rowptr, p.col, lids[I], lids[Jd]);
-        if (slot < p.rowptr[lids[I] + 1] && p.col[slot] == lids[Jd]) atomicAdd(p.vals + slot, jv);
+        if (staged) {
+          const int32_t key = sL[Jd];
+          int lo = sRO[I], hi = sRO[I + 1];
+          const int end = hi;
+          while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (sCol[mid] < key) lo = mid + 1; else hi = mid;
+          }
+          if (lo < end && sCol[lo] == key) atomicAdd(p.vals + sRB[I] + (lo - sRO[I]), jv);
+        } else {
+          const int64_t slot = find_slot(p.rowptr, p.col, sL[I], sL[Jd]);
+          if (slot < p.rowptr[sL[I] + 1] && p.col[slot] == sL[Jd]) atomicAdd(p.vals + slot, jv);
+        }
       }
     }
     __syncthreads();
@@ -161,7 +222,7 @@ __global__ __launch_bounds__(GB) void k_generic(GenericParams p) {
       const int a = k / p.nf, i = k % p.nf;
       double F = 0.0;
       for (int q = 0; q < p.nq; ++q) F += sWm[q] * tN[a * p.nq + q] * p.src[i];
-      atomicAdd(p.f + lids[k], sR[k] - F);
+      atomicAdd(p.f + sL[k], sR[k] - F);
     }
     __syncthreads();
   }
