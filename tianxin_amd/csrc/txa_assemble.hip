@@ -228,6 +228,118 @@ __global__ __launch_bounds__(GB) void k_generic(GenericParams p) {
   }
 }
 
+// Generic diffusion on hex8 / tet4 meshes (one field, steady): one element per lane.
+// The reference tables sit in LDS; each lane integrates its element (the same
+// formulas as k_generic, FP64), adds its residual rows and Jacobian entries with
+// atomics, and finds each entry's CRS position by a binary search whose steps depend
+// only on the row length, so the NB searches of one row run in lockstep (NB
+// independent loads per step instead of one dependent chain per entry).
+template <int NB, int NQ>
+__global__ __launch_bounds__(256, 2) void k_generic_lane(GenericParams p) {
+  __shared__ double tN[NB * NQ], tdN[NB * NQ * 3], tgdN[NB * NQ * 3], tqw[NQ];
+  for (int i = threadIdx.x; i < NB * NQ; i += 256) tN[i] = p.tab[i];
+  for (int i = threadIdx.x; i < NB * NQ * 3; i += 256) {
+    tdN[i] = p.tab[NB * NQ + i];
+    tgdN[i] = p.tab[NB * NQ * 4 + i];   // geometry vertices are the basis nodes (nv = nb)
+  }
+  for (int i = threadIdx.x; i < NQ; i += 256) tqw[i] = p.tab[NB * NQ * 7 + i];
+  __syncthreads();
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= p.nelem) return;
+  double X[NB][3], u[NB];
+  int32_t L[NB];
+#pragma unroll
+  for (int v = 0; v < NB; ++v) {
+    const int64_t n = p.conn[e * p.nconn + v];
+    X[v][0] = p.X[n]; X[v][1] = p.Y[n]; X[v][2] = p.Z[n];
+    L[v] = p.elem_lid[e * NB + v];
+  }
+#pragma unroll
+  for (int v = 0; v < NB; ++v) u[v] = p.x[L[v]];
+  const double cf = p.coef[e];
+  double K[NB * (NB + 1) / 2], F[NB];
+#pragma unroll
+  for (int i = 0; i < NB * (NB + 1) / 2; ++i) K[i] = 0.0;
+#pragma unroll
+  for (int a = 0; a < NB; ++a) F[a] = 0.0;
+  double sg = 1.0;
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    double J[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double s = 0.0;
+#pragma unroll
+        for (int v = 0; v < NB; ++v) s += X[v][i] * tgdN[(v * NQ + q) * 3 + j];
+        J[i][j] = s;
+      }
+    const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    const double det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+    const double id = 1.0 / det;
+    double Ji[9];
+    Ji[0] = c00 * id;
+    Ji[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * id;
+    Ji[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * id;
+    Ji[3] = c01 * id;
+    Ji[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * id;
+    Ji[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * id;
+    Ji[6] = c02 * id;
+    Ji[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * id;
+    Ji[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * id;
+    if (q == 0) sg = det < 0.0 ? -1.0 : 1.0;
+    const double w = tqw[q] * det * sg;
+    double G[NB][3];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const double *dn = &tdN[(b * NQ + q) * 3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) G[b][i] = Ji[0 * 3 + i] * dn[0] + Ji[1 * 3 + i] * dn[1] + Ji[2 * 3 + i] * dn[2];
+    }
+    int k = 0;
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      F[a] += w * tN[a * NQ + q] * p.src[0];
+#pragma unroll
+      for (int b = a; b < NB; ++b, ++k) K[k] += w * cf * (G[a][0] * G[b][0] + G[a][1] * G[b][1] + G[a][2] * G[b][2]);
+    }
+  }
+  auto kab = [&](int a, int b) -> double {
+    const int lo = a < b ? a : b, hi = a < b ? b : a;
+    return K[lo * NB - (lo * (lo - 1)) / 2 + (hi - lo)];
+  };
+  if (p.f)
+#pragma unroll
+    for (int a = 0; a < NB; ++a) {
+      double r = 0.0;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) r += kab(a, b) * u[b];
+      atomicAdd(p.f + L[a], r - F[a]);
+    }
+  if (!p.vals) return;
+#pragma unroll
+  for (int a = 0; a < NB; ++a) {
+    const int64_t r0 = p.rowptr[L[a]];
+    const int64_t n = p.rowptr[L[a] + 1] - r0;
+    int64_t step = 1;
+    while (step * 2 <= n) step *= 2;
+    int64_t pos[NB];   // lower bound of L[b] in the row: columns before it, in [0, n]
+#pragma unroll
+    for (int b = 0; b < NB; ++b) pos[b] = 0;
+    for (; step > 0; step >>= 1) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b)
+        if (pos[b] + step <= n && p.col[r0 + pos[b] + step - 1] < L[b]) pos[b] += step;
+    }
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+      if (pos[b] < n && p.col[r0 + pos[b]] == L[b]) atomicAdd(p.vals + r0 + pos[b], p.beta * kab(a, b));
+  }
+}
+
 // ============================================================================
 // Dirichlet rows: f_i = x_i - g_i; matrix row -> identity row
 // (disc-fe/src/lof/Panzer_TpetraLinearObjContainer.hpp:228-237, 300-318)
@@ -389,7 +501,15 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
     if (a.f) TXA_HIP(hipMemsetAsync(a.f, 0, sizeof(double) * m.ndofs, s));
     if (a.vals) TXA_HIP(hipMemsetAsync(a.vals, 0, sizeof(double) * m.nnz, s));
     int64_t blocks = m.nelem < 65536 * 8 ? m.nelem : 65536 * 8;
-    if (r.nb <= 8 && r.nq <= 8)
+    const bool lane = a.physics == TXA_DIFFUSION && !a.transient && m.nfields == 1 &&
+                      ((m.topo == TXA_HEX8 && r.nb == 8 && r.nv == 8 && r.nq == 8) ||
+                       (m.topo == TXA_TET4 && r.nb == 4 && r.nv == 4 && r.nq == 4));
+    const unsigned lblocks = static_cast<unsigned>((m.nelem + 255) / 256);
+    if (lane && m.topo == TXA_HEX8)
+      hipLaunchKernelGGL((k_generic_lane<8, 8>), dim3(lblocks), dim3(256), 0, s, p);
+    else if (lane)
+      hipLaunchKernelGGL((k_generic_lane<4, 4>), dim3(lblocks), dim3(256), 0, s, p);
+    else if (r.nb <= 8 && r.nq <= 8)
       hipLaunchKernelGGL((k_generic<64, 8, 8>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, p);
     else
       hipLaunchKernelGGL((k_generic<256, kMaxNB, kMaxNQ>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, p);
