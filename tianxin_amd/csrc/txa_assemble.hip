@@ -13,9 +13,11 @@
 // disc-fe/src/Panzer_BasisValues2_impl.hpp:1379-1460) and the Sacado-FAD
 // Jacobian replaced by the explicit tangent beta*K + alpha*C + gamma2*M.
 //
-// Dispatch: hex8 diffusion goes to the row-owner kernel (txa_q1_hex8.hip);
-// every other topology / physics combination runs k_generic here: one
-// workgroup per element, FP64 atomic scatter into a zeroed CRS.
+// Dispatch: diffusion on hex8 / tet4 bricks goes to the row-owner kernel
+// (txa_q1_hex8.hip), three-field elasticity on bricks to txa_elastic.hip; steady
+// one-field diffusion on uploaded hex8 / tet4 meshes runs k_generic_lane here (one
+// element per lane) and every other combination k_generic (one workgroup per
+// element), both with FP64 atomic scatter into a zeroed CRS.
 #include "txa_internal.hpp"
 #include "txa_refelem.hpp"
 
