@@ -80,6 +80,14 @@ def test_renumbered_brick(engine, topo, nf):
     check(engine, topo, conn, ev, X, nent, nf, txa.ELASTICITY if nf == 3 else txa.DIFFUSION)
 
 
+@pytest.mark.parametrize("topo", [txa.HEX8, txa.TET4])
+def test_renumbered_brick_many_blocks(engine, topo):
+    # steady one-field diffusion on an uploaded mesh runs one element per lane: several
+    # workgroups of 256 elements and a partial last one
+    conn, ev, X, nent = renumbered_brick(topo, 10, seed=60 + topo)
+    check(engine, topo, conn, ev, X, nent, 1, txa.DIFFUSION)
+
+
 @pytest.mark.parametrize("name,block,topo", [("basic3d.gen", 0, txa.HEX8), ("twoblock_cube.gen", 0, txa.TET4)])
 def test_exodus_fixture(engine, name, block, topo):
     X, blocks = exodus.read_exodus(os.path.join(GOLDEN, name))
