@@ -16,8 +16,9 @@
 // Dispatch: diffusion on hex8 / tet4 bricks goes to the row-owner kernel
 // (txa_q1_hex8.hip), three-field elasticity on bricks to txa_elastic.hip; steady
 // one-field diffusion on uploaded hex8 / tet4 meshes runs k_generic_lane here (one
-// element per lane) and every other combination k_generic (one workgroup per
-// element), both with FP64 atomic scatter into a zeroed CRS.
+// element per lane; its Jacobian through the row gather k_lane_gather, atomic-free)
+// and every other combination k_generic (one workgroup per element, FP64 atomic
+// scatter into a zeroed CRS).
 #include "txa_internal.hpp"
 #include "txa_refelem.hpp"
 
@@ -49,6 +50,7 @@ struct GenericParams {
   double *f, *vals;
   const int64_t *rowptr;
   const int32_t *col;
+  double *ke;                      // k_generic_lane: element matrices [nelem][nb][nb] for the row gather, or null
 };
 
 __device__ __forceinline__ int64_t find_slot(const int64_t *rowptr, const int32_t *col, int32_t row, int32_t key) {
@@ -232,10 +234,11 @@ __global__ __launch_bounds__(GB) void k_generic(GenericParams p) {
 
 // Generic diffusion on hex8 / tet4 meshes (one field, steady): one element per lane.
 // The reference tables sit in LDS; each lane integrates its element (the same
-// formulas as k_generic, FP64), adds its residual rows and Jacobian entries with
-// atomics, and finds each entry's CRS position by a binary search whose steps depend
-// only on the row length, so the NB searches of one row run in lockstep (NB
-// independent loads per step instead of one dependent chain per entry).
+// formulas as k_generic, FP64) and adds its residual rows with atomics.  The element
+// matrix goes to the workspace for k_lane_gather when one is given (ke); otherwise its
+// entries are added with atomics, each entry's CRS position found by a binary search
+// whose steps depend only on the row length, so the NB searches of one row run in
+// lockstep (NB independent loads per step instead of one dependent chain per entry).
 template <int NB, int NQ>
 __global__ __launch_bounds__(256, 2) void k_generic_lane(GenericParams p) {
   __shared__ double tN[NB * NQ], tdN[NB * NQ * 3], tgdN[NB * NQ * 3], tqw[NQ];
@@ -322,6 +325,14 @@ __global__ __launch_bounds__(256, 2) void k_generic_lane(GenericParams p) {
       atomicAdd(p.f + L[a], r - F[a]);
     }
   if (!p.vals) return;
+  if (p.ke) {
+    double *ke = p.ke + e * (NB * NB);
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+      for (int b = 0; b < NB; ++b) ke[a * NB + b] = kab(a, b);
+    return;
+  }
 #pragma unroll
   for (int a = 0; a < NB; ++a) {
     const int64_t r0 = p.rowptr[L[a]];
@@ -339,6 +350,63 @@ __global__ __launch_bounds__(256, 2) void k_generic_lane(GenericParams p) {
 #pragma unroll
     for (int b = 0; b < NB; ++b)
       if (pos[b] < n && p.col[r0 + pos[b]] == L[b]) atomicAdd(p.vals + r0 + pos[b], p.beta * kab(a, b));
+  }
+}
+
+// Row gather of the uploaded-mesh Jacobian (after k_generic_lane wrote the element
+// matrices): 32 lanes per CRS row, lane j on columns j, j+32, ... of the row.  The row's
+// items are the entries (L_e[b], K_e(a, b)) of its element rows p = e*NB + a, in
+// ascending p and b.  The 32 lanes load 64 items at a time (two each, all loads in
+// flight together), then every lane walks the 64 through shuffles, in item order, and
+// adds those whose column is its own.  Every entry of the row is stored once, with no
+// atomics, so the values do not depend on the order in which elements ran.
+template <int NB>
+__global__ __launch_bounds__(256) void k_lane_gather(int64_t nrows, const int64_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ col,
+                                                     const int64_t *__restrict__ occ_ptr,
+                                                     const int64_t *__restrict__ occ,
+                                                     const int32_t *__restrict__ elem_lid,
+                                                     const double *__restrict__ ke, double beta,
+                                                     double *__restrict__ vals) {
+  const int j = threadIdx.x % 32;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * (256 / 32);
+  // rows are uniform across each half-wave, so all 32 lanes of a half run the loops below together
+  for (int64_t row = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) / 32; row < nrows; row += stride) {
+    const int64_t r0 = rowptr[row], n = rowptr[row + 1] - r0;
+    const int64_t o0 = occ_ptr[row];
+    const int64_t nitem = (occ_ptr[row + 1] - o0) * NB;
+    for (int64_t c0 = 0; c0 < n; c0 += 32) {
+      const int64_t jj = c0 + j;
+      const int32_t c = jj < n ? col[r0 + jj] : -1;
+      double acc = 0.0;
+      for (int64_t i0 = 0; i0 < nitem; i0 += 64) {
+        int32_t Li[2];
+        double Ki[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int64_t it = i0 + h * 32 + j;
+          Li[h] = -2;   // matches no column
+          Ki[h] = 0.0;
+          if (it < nitem) {
+            const int64_t pe = occ[o0 + it / NB];   // e*NB + a
+            const int b = static_cast<int>(it % NB);
+            Li[h] = elem_lid[(pe / NB) * NB + b];
+            Ki[h] = ke[pe * NB + b];
+          }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int64_t left = nitem - (i0 + h * 32);
+          const int cnt = left < 32 ? static_cast<int>(left) : 32;
+          for (int k = 0; k < cnt; ++k) {
+            const int32_t Lk = __shfl(Li[h], k, 32);
+            const double Kk = __shfl(Ki[h], k, 32);
+            if (Lk == c) acc += Kk;
+          }
+        }
+      }
+      if (jj < n) vals[r0 + jj] = beta * acc;
+    }
   }
 }
 
@@ -449,7 +517,16 @@ static bool uses_q1_path(const Mesh &m, int physics, int precision) {
          precision != TXA_SPLIT_BF16;
 }
 
+// Uploaded one-field hex8 / tet4 diffusion: the element matrices for the row gather
+// (the mesh's element-row lists exist; transient calls run k_generic and leave it unused).
+static bool lane_gather_mesh(const Mesh &m, int physics, int precision) {
+  return !m.structured && m.nfields == 1 && physics == TXA_DIFFUSION && precision == TXA_FP64 &&
+         m.occ.p != nullptr && ((m.topo == TXA_HEX8 && m.ndof_elem == 8) || (m.topo == TXA_TET4 && m.ndof_elem == 4));
+}
+
 size_t assemble_workspace_bytes(const Mesh &m, int physics, int precision, bool want_f, bool want_vals) {
+  if (lane_gather_mesh(m, physics, precision))
+    return want_vals ? sizeof(double) * static_cast<size_t>(m.nelem) * m.ndof_elem * m.ndof_elem : 0;
   txa_assembly_args a{};
   a.physics = physics;
   a.precision = precision;
@@ -500,12 +577,15 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
     p.x = a.x; p.xdot = a.xdot; p.xddot = a.xddot;
     p.alpha = a.alpha; p.beta = a.beta; p.gamma2 = a.gamma2; p.transient = a.transient;
     p.f = a.f; p.vals = a.vals; p.rowptr = m.rowptr.p; p.col = m.col.p;
-    if (a.f) TXA_HIP(hipMemsetAsync(a.f, 0, sizeof(double) * m.ndofs, s));
-    if (a.vals) TXA_HIP(hipMemsetAsync(a.vals, 0, sizeof(double) * m.nnz, s));
     int64_t blocks = m.nelem < 65536 * 8 ? m.nelem : 65536 * 8;
     const bool lane = a.physics == TXA_DIFFUSION && !a.transient && m.nfields == 1 &&
                       ((m.topo == TXA_HEX8 && r.nb == 8 && r.nv == 8 && r.nq == 8) ||
                        (m.topo == TXA_TET4 && r.nb == 4 && r.nv == 4 && r.nq == 4));
+    // the Jacobian of the lane kernel by the row gather: element matrices in ws, rows stored whole
+    const bool gather = lane && a.vals != nullptr && ws != nullptr && lane_gather_mesh(m, a.physics, a.precision);
+    p.ke = gather ? static_cast<double *>(ws) : nullptr;
+    if (a.f) TXA_HIP(hipMemsetAsync(a.f, 0, sizeof(double) * m.ndofs, s));
+    if (a.vals && !gather) TXA_HIP(hipMemsetAsync(a.vals, 0, sizeof(double) * m.nnz, s));
     const unsigned lblocks = static_cast<unsigned>((m.nelem + 255) / 256);
     if (lane && m.topo == TXA_HEX8)
       hipLaunchKernelGGL((k_generic_lane<8, 8>), dim3(lblocks), dim3(256), 0, s, p);
@@ -516,6 +596,16 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
     else
       hipLaunchKernelGGL((k_generic<256, kMaxNB, kMaxNQ>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, p);
     TXA_HIP(hipGetLastError());
+    if (gather) {
+      const int64_t gblocks = std::min<int64_t>((m.ndofs + 7) / 8, int64_t(1) << 20);   // 8 rows per block
+      if (m.topo == TXA_HEX8)
+        hipLaunchKernelGGL((k_lane_gather<8>), dim3(static_cast<unsigned>(gblocks)), dim3(256), 0, s, m.ndofs,
+                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, p.ke, a.beta, a.vals);
+      else
+        hipLaunchKernelGGL((k_lane_gather<4>), dim3(static_cast<unsigned>(gblocks)), dim3(256), 0, s, m.ndofs,
+                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, p.ke, a.beta, a.vals);
+      TXA_HIP(hipGetLastError());
+    }
   }
   // Neumann sides, one launch each (stream order keeps the sums of shared edge points in a fixed order)
   for (int k = 0; a.f && k < a.n_neumann; ++k) {

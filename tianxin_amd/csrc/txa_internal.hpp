@@ -88,6 +88,9 @@ struct Mesh {
   DevBuf<uint8_t> perm;       // [ndofs][perm_width] position of each stencil neighbour in the row, 255 = absent
   int nperm_w = 0;            // entries per node of nperm
   DevBuf<uint16_t> nperm;     // 3-field HEX8 / HEX27: [nodes][nperm_w] position of (neighbour, field 0) in the node's rows
+  // uploaded one-field HEX8 / TET4: the element rows p = e*ndof_elem + a of each LID, ascending
+  // (occ[occ_ptr[r] .. occ_ptr[r+1]) for row r), for the row gather of the assembly
+  DevBuf<int64_t> occ_ptr, occ;
   DevBuf<double> ref;         // reference-element tables of the topology
 };
 

@@ -272,6 +272,29 @@ __global__ void k_row_copy(int64_t nrows, const int64_t *__restrict__ cptr, cons
   for (int64_t i = threadIdx.x; i < n; i += blockDim.x) col[rowptr[row] + i] = cand[cptr[row] + i];
 }
 
+// Element rows of each LID (uploaded one-field meshes): p = e*ndof + a lands in the list
+// of row elem_lid[p]; the lists are then sorted so that the row gather sums in a fixed order.
+__global__ void k_occ_fill(const int32_t *__restrict__ elem_lid, int64_t n, int64_t *__restrict__ cursor,
+                           int64_t *__restrict__ occ) {
+  int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int64_t slot = static_cast<int64_t>(
+      atomicAdd(reinterpret_cast<unsigned long long *>(cursor + elem_lid[p]), 1ull));
+  occ[slot] = p;
+}
+
+__global__ void k_occ_sort(int64_t nrows, const int64_t *__restrict__ optr, int64_t *__restrict__ occ) {
+  int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (row >= nrows) return;
+  const int64_t lo = optr[row], hi = optr[row + 1];
+  for (int64_t i = lo + 1; i < hi; ++i) {   // insertion sort: a row has a few element rows
+    const int64_t v = occ[i];
+    int64_t j = i;
+    for (; j > lo && occ[j - 1] > v; --j) occ[j] = occ[j - 1];
+    occ[j] = v;
+  }
+}
+
 // Per-row stencil slot map for nodal fields on the vertex lattice: slot
 // s*nf + g holds the position, within the row, of field g at the neighbour
 // point offset s = (dx+1) + 3(dy+1) + 9(dz+1), or 255 when that column is
@@ -503,6 +526,27 @@ void graph_build(Mesh &m, hipStream_t s) {
     TXA_HIP(hipGetLastError());
   }
   node_perm_build(m, s);   // neighbour positions for the 3-field elasticity kernels
+  // element rows of each LID for the row gather of uploaded one-field hex8 / tet4 meshes
+  m.occ_ptr.release();
+  m.occ.release();
+  if (!m.structured && m.nfields == 1 && ((m.topo == TXA_HEX8 && nd == 8) || (m.topo == TXA_TET4 && nd == 4))) {
+    DevBuf<int64_t> ocnt, ocur;
+    ocnt.alloc(nrows + 1);
+    TXA_HIP(hipMemsetAsync(ocnt.p, 0, ocnt.bytes(), s));
+    hipLaunchKernelGGL(k_row_count, dim3(grid_for(npos)), dim3(kBlock), 0, s, m.elem_lid.p, npos, 1, ocnt.p);
+    TXA_HIP(hipGetLastError());
+    m.occ_ptr.alloc(nrows + 1);
+    exclusive_scan_i64(ocnt.p, m.occ_ptr.p, nrows + 1, s);
+    ocnt.release();
+    ocur.alloc(nrows + 1);
+    TXA_HIP(hipMemcpyAsync(ocur.p, m.occ_ptr.p, sizeof(int64_t) * (nrows + 1), hipMemcpyDeviceToDevice, s));
+    m.occ.alloc(npos);
+    hipLaunchKernelGGL(k_occ_fill, dim3(grid_for(npos)), dim3(kBlock), 0, s, m.elem_lid.p, npos, ocur.p, m.occ.p);
+    TXA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_occ_sort, dim3(grid_for(nrows)), dim3(kBlock), 0, s, nrows, m.occ_ptr.p, m.occ.p);
+    TXA_HIP(hipGetLastError());
+    TXA_HIP(hipStreamSynchronize(s));   // ocur is freed at scope end
+  }
   TXA_HIP(hipStreamSynchronize(s));
   m.has_graph = true;
 }
