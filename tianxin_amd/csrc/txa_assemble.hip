@@ -357,9 +357,12 @@ __global__ __launch_bounds__(256, 2) void k_generic_lane(GenericParams p) {
 // matrices): 32 lanes per CRS row, lane j on columns j, j+32, ... of the row.  The row's
 // items are the entries (L_e[b], K_e(a, b)) of its element rows p = e*NB + a, in
 // ascending p and b.  The 32 lanes load 64 items at a time (two each, all loads in
-// flight together), then every lane walks the 64 through shuffles, in item order, and
-// adds those whose column is its own.  Every entry of the row is stored once, with no
-// atomics, so the values do not depend on the order in which elements ran.
+// flight together) into their half-wave's LDS region; every lane then walks the 64 in
+// item order (broadcast reads) and adds those whose column is its own.  Every entry of
+// the row is stored once, with no atomics, so the values do not depend on the order in
+// which elements ran.  A half-wave's region is written and read by that half-wave
+// only, and one wave's LDS operations complete in program order, so no barrier is
+// needed (the two halves of a wave may run different trip counts).
 template <int NB>
 __global__ __launch_bounds__(256) void k_lane_gather(int64_t nrows, const int64_t *__restrict__ rowptr,
                                                      const int32_t *__restrict__ col,
@@ -368,9 +371,10 @@ __global__ __launch_bounds__(256) void k_lane_gather(int64_t nrows, const int64_
                                                      const int32_t *__restrict__ elem_lid,
                                                      const double *__restrict__ ke, double beta,
                                                      double *__restrict__ vals) {
-  const int j = threadIdx.x % 32;
+  __shared__ __align__(16) double sK[8][64];
+  __shared__ __align__(16) int32_t sL[8][64];
+  const int j = threadIdx.x % 32, hw = threadIdx.x / 32;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * (256 / 32);
-  // rows are uniform across each half-wave, so all 32 lanes of a half run the loops below together
   for (int64_t row = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) / 32; row < nrows; row += stride) {
     const int64_t r0 = rowptr[row], n = rowptr[row + 1] - r0;
     const int64_t o0 = occ_ptr[row];
@@ -394,15 +398,31 @@ __global__ __launch_bounds__(256) void k_lane_gather(int64_t nrows, const int64_
             Ki[h] = ke[pe * NB + b];
           }
         }
+        // the previous round's reads of this region are issued before these writes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-          const int64_t left = nitem - (i0 + h * 32);
-          const int cnt = left < 32 ? static_cast<int>(left) : 32;
-          for (int k = 0; k < cnt; ++k) {
-            const int32_t Lk = __shfl(Li[h], k, 32);
-            const double Kk = __shfl(Ki[h], k, 32);
-            if (Lk == c) acc += Kk;
+          sL[hw][h * 32 + j] = Li[h];
+          sK[hw][h * 32 + j] = Ki[h];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // past the round's last item the slots hold column -2, which matches no lane; the
+        // reads are unconditional (16-byte, eight items a step) and the sum takes a select
+        const int cnt = nitem - i0 < 64 ? static_cast<int>(nitem - i0) : 64;
+        const int cnt8 = (cnt + 7) & ~7;
+        for (int k = 0; k < cnt8; k += 8) {
+          const int4 la = *reinterpret_cast<const int4 *>(&sL[hw][k]);
+          const int4 lb = *reinterpret_cast<const int4 *>(&sL[hw][k + 4]);
+          const int32_t lv[8] = {la.x, la.y, la.z, la.w, lb.x, lb.y, lb.z, lb.w};
+          double kv[8];
+#pragma unroll
+          for (int u = 0; u < 8; u += 2) {
+            const double2 k2 = *reinterpret_cast<const double2 *>(&sK[hw][k + u]);
+            kv[u] = k2.x;
+            kv[u + 1] = k2.y;
           }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc = lv[u] == c ? acc + kv[u] : acc;
         }
       }
       if (jj < n) vals[r0 + jj] = beta * acc;
