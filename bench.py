@@ -524,6 +524,96 @@ Q2_GATHER = {"split_bf16": "k_elastic_gather<27, float>", "fp32": "k_elastic_gat
              "fp64": "k_elastic_gather<27, double>"}
 
 
+def bench_upload(args):
+    """Uploaded-mesh path (txa_mesh_upload, the connectivity a ConnManager hands over):
+    an n^3 hex8 brick (default 128) uploaded as unstructured connectivity, FP64, steady
+    Poisson (k lognormal sigma 0.5 per element, f = 1, u ~ U[-1, 1], u = 0 on x = 0).
+    Times the cells in brick order and in a seeded random order.  Checks: in brick order
+    the upload numbers LIDs as the lattice mesh does, so f and J must equal the lattice
+    FP64 path's to rounding; Poisson rows off the Dirichlet set sum to zero."""
+    import numpy as np
+    import torch
+
+    import tianxin_amd as txa
+    from tianxin_amd import synth
+
+    n = args.n if args.n != 256 else 128
+    eng = txa.Engine(0)
+    X = synth.brick_coords(n, n, n, jitter=0.1, seed=3)
+    k = torch.as_tensor(synth.lognormal(5, n ** 3, 0.5), device="cuda")
+    out, checks = {}, {}
+    ref = None
+    for name, seed in (("brick_order", None), ("shuffled", 11)):
+        t0 = time.time()
+        conn = synth.brick_hex8_conn(n, n, n, shuffle_seed=seed)
+        m = eng.upload(txa.HEX8, conn, X)
+        nd = m.build_dofmap(1)
+        nnz = m.build_graph()
+        owned, _ = m.dofmap()
+        lid_of_gid = np.empty(nd, np.int64)
+        lid_of_gid[owned] = np.arange(nd)
+        vx = (np.arange((n + 1) ** 2) * (n + 1)).astype(np.int64)     # vertices with i = 0 (GID = vertex id)
+        bc = torch.as_tensor(np.sort(lid_of_gid[vx]).astype(np.int32), device="cuda")
+        x = torch.as_tensor(synth.uniform(6, nd, -1, 1), device="cuda")
+        if seed is not None:
+            kk = torch.as_tensor(synth.lognormal(5, n ** 3, 0.5)[np.argsort(synth.splitmix64(seed, n ** 3), kind="stable")],
+                                 device="cuda")
+        else:
+            kk = k
+        log(f"{name}: setup {time.time() - t0:.1f}s, {m.nelem} elements, {nd} dofs, {nnz} nnz")
+        f = torch.empty(nd, device="cuda", dtype=torch.float64)
+        v = torch.empty(nnz, device="cuda", dtype=torch.float64)
+
+        def step():
+            m.assemble(physics=txa.DIFFUSION, precision=txa.FP64, coef=kk, x=x, f=f, vals=v, src=(1.0,), bc_lids=bc)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+        ev[0].record()
+        for i in range(args.steps):
+            step()
+            ev[i + 1].record()
+        torch.cuda.synchronize()
+        per = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)])
+        ms = float(per.mean())
+        out[name] = {"ms_per_step": ms, "elements_per_s": m.nelem / (ms / 1e3),
+                     "p50": float(np.percentile(per, 50)), "p99": float(np.percentile(per, 99))}
+        log(f"{name}: {ms:.3f} ms/step")
+        # Poisson rows off the Dirichlet set sum to zero
+        rp, _ = m.graph(with_cols=False)
+        rows = torch.repeat_interleave(torch.arange(nd, device="cuda"), torch.as_tensor(np.diff(rp), device="cuda"))
+        rs = torch.zeros(nd, device="cuda", dtype=torch.float64).index_add_(0, rows, v)
+        keep = torch.ones(nd, dtype=torch.bool, device="cuda")
+        keep[bc.long()] = False
+        checks[name] = {"row_sum_rel_max": float(rs[keep].abs().max() / v.abs().max())}
+        if seed is None:
+            ref = (f.clone(), v.clone(), x, bc)
+        m.destroy()
+        del f, v
+    # the lattice FP64 path on the same brick, same LIDs and graph
+    ml = eng.brick(txa.HEX8, n, n, n, X)
+    ml.build_dofmap(1)
+    ml.build_graph()
+    fu, vu, x, bc = ref
+    fl = torch.empty_like(fu)
+    vl = torch.empty_like(vu)
+    ml.assemble(physics=txa.DIFFUSION, precision=txa.FP64, coef=k, x=x, f=fl, vals=vl, src=(1.0,), bc_lids=bc)
+    torch.cuda.synchronize()
+    checks["brick_order_vs_lattice_fp64"] = {"R_normwise": float((fu - fl).abs().max() / fl.abs().max()),
+                                            "J_normwise": float((vu - vl).abs().max() / vl.abs().max())}
+    ml.destroy()
+    best = out["brick_order"]
+    line = {"metric": "elements/s (residual+Jacobian), uploaded mesh", "value": best["elements_per_s"],
+            "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": best["ms_per_step"], "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"Q1 Poisson hex8 {n}^3 brick uploaded as unstructured connectivity",
+                       "elements": n ** 3, "quadrature": "2x2x2 Gauss", "k": "lognormal sigma=0.5",
+                       "dirichlet": "u=0 on x=0"},
+            "orders": out, "checks": checks}
+    print(json.dumps(line), flush=True)
+
+
 def summarize_q2(stats_dir, pmc_dir):
     """Writes profiles/q2_hex27_elastic_64_kernel_stats.md from a kernel-stats pass and
     one --pmc pass (SQ_VALU_MFMA_BUSY_CYCLES, GRBM_GUI_ACTIVE, SQ_INSTS_VALU,
@@ -606,8 +696,9 @@ def main():
     ap.add_argument("--summarize-profiles", nargs=3, metavar=("STATS_DIR", "FETCH_DIR", "WRITE_DIR"),
                     help="CPU only: turn rocprofv3 csv output (kernel stats, FETCH_SIZE pass, WRITE_SIZE pass) "
                          "into the summaries under profiles/")
-    ap.add_argument("--workload", choices=("single", "batch", "serve", "q2"), default="single",
-                    help="single: the 256^3 Q1 line (default); batch / serve: the other two forms")
+    ap.add_argument("--workload", choices=("single", "batch", "serve", "q2", "upload"), default="single",
+                    help="single: the 256^3 Q1 line (default); batch / serve / q2: the other forms; "
+                         "upload: a 128^3 hex8 brick through txa_mesh_upload")
     ap.add_argument("--meshes", type=int, default=512)
     ap.add_argument("--pool", type=int, default=8)
     ap.add_argument("--requests", type=int, default=48)
@@ -634,6 +725,8 @@ def main():
         return bench_serve(args)
     if args.workload == "q2":
         return bench_q2(args)
+    if args.workload == "upload":
+        return bench_upload(args)
 
     import numpy as np
     import torch

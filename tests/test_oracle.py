@@ -475,3 +475,14 @@ def test_neumann_jittered_side_total_is_the_side_area():
                         xt = 0.25 * ((1 - s_) * (c[0, 1] - c[0, 0]) + (1 + s_) * (c[1, 1] - c[1, 0]))
                         area += ws * wt * np.linalg.norm(np.cross(xs, xt))
         assert abs(tot - area) <= 1e-5 * area
+
+
+def test_synth_brick_hex8_conn_is_the_brick_mesh():
+    # the upload benchmark's connectivity is the oracle's brick mesh (same vertex and
+    # cell order), and the shuffled form holds the same cells in another order
+    for dims in ((3, 4, 5), (2, 2, 2), (1, 3, 2)):
+        conn, _, _ = O.brick_mesh(O.HEX8, *dims)
+        assert np.array_equal(synth.brick_hex8_conn(*dims), conn)
+        sh = synth.brick_hex8_conn(*dims, shuffle_seed=9)
+        assert not np.array_equal(sh, conn) or conn.shape[0] < 3
+        assert np.array_equal(np.unique(sh, axis=0), np.unique(conn, axis=0))

@@ -68,6 +68,23 @@ def brick_coords(nx: int, ny: int, nz: int, *, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0,
     return X.reshape(-1, 3)
 
 
+def brick_hex8_conn(nx: int, ny: int, nz: int, shuffle_seed=None) -> np.ndarray:
+    """Hex8 connectivity [nx*ny*nz][8] of the brick whose vertices brick_coords numbers.
+
+    Cells in x-fastest order, vertices of a cell in the order (0,0,0) (1,0,0) (1,1,0)
+    (0,1,0) (0,0,1) (1,0,1) (1,1,1) (0,1,1) (CubeHexMeshFactory's element order).  With
+    ``shuffle_seed`` the cells come in a seeded random order.
+    """
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    c = (i + (nx + 1) * (j + (ny + 1) * k)).transpose(2, 1, 0).reshape(-1).astype(np.int64)
+    sx, sxy = 1, (nx + 1) * (ny + 1)
+    off = np.array([0, sx, sx + nx + 1, nx + 1, sxy, sxy + sx, sxy + sx + nx + 1, sxy + nx + 1], np.int64)
+    conn = c[:, None] + off[None, :]
+    if shuffle_seed is not None:
+        conn = conn[np.argsort(splitmix64(shuffle_seed, conn.shape[0]), kind="stable")]
+    return conn
+
+
 def lognormal(seed: int, n: int, sigma: float) -> np.ndarray:
     """exp(g), g ~ N(0, sigma^2) iid."""
     return np.exp(normal(seed, n, sigma))
