@@ -50,7 +50,7 @@ struct GenericParams {
   double *f, *vals;
   const int64_t *rowptr;
   const int32_t *col;
-  double *ke;                      // k_generic_lane: element matrices [nelem][nb][nb] for the row gather, or null
+  double *ke;                      // k_generic_lane: element matrices, packed upper triangles [nelem][nb(nb+1)/2], or null
 };
 
 __device__ __forceinline__ int64_t find_slot(const int64_t *rowptr, const int32_t *col, int32_t row, int32_t key) {
@@ -326,11 +326,10 @@ __global__ __launch_bounds__(256, 2) void k_generic_lane(GenericParams p) {
     }
   if (!p.vals) return;
   if (p.ke) {
-    double *ke = p.ke + e * (NB * NB);
+    // the packed upper triangle as accumulated: entry (lo, hi) at lo*NB - lo(lo-1)/2 + (hi - lo)
+    double *ke = p.ke + e * (NB * (NB + 1) / 2);
 #pragma unroll
-    for (int a = 0; a < NB; ++a)
-#pragma unroll
-      for (int b = 0; b < NB; ++b) ke[a * NB + b] = kab(a, b);
+    for (int i = 0; i < NB * (NB + 1) / 2; ++i) ke[i] = K[i];
     return;
   }
 #pragma unroll
@@ -354,8 +353,8 @@ __global__ __launch_bounds__(256, 2) void k_generic_lane(GenericParams p) {
 }
 
 // Row gather of the uploaded-mesh Jacobian (after k_generic_lane wrote the element
-// matrices): 32 lanes per CRS row, lane j on columns j, j+32, ... of the row.  The row's
-// items are the entries (L_e[b], K_e(a, b)) of its element rows p = e*NB + a, in
+// matrices, packed upper triangles): 32 lanes per CRS row, lane j on columns j, j+32, ...
+// of the row.  The row's items are the entries (L_e[b], K_e(a, b)) of its element rows p = e*NB + a, in
 // ascending p and b.  The 32 lanes load 64 items at a time (two each, all loads in
 // flight together) into their half-wave's LDS region; every lane then walks the 64 in
 // item order (broadcast reads) and adds those whose column is its own.  Every entry of
@@ -393,9 +392,11 @@ __global__ __launch_bounds__(256) void k_lane_gather(int64_t nrows, const int64_
           Ki[h] = 0.0;
           if (it < nitem) {
             const int64_t pe = occ[o0 + it / NB];   // e*NB + a
-            const int b = static_cast<int>(it % NB);
-            Li[h] = elem_lid[(pe / NB) * NB + b];
-            Ki[h] = ke[pe * NB + b];
+            const int64_t e = pe / NB;
+            const int a = static_cast<int>(pe - e * NB), b = static_cast<int>(it % NB);
+            const int lo = a < b ? a : b, hi = a < b ? b : a;   // K_e is symmetric, stored packed
+            Li[h] = elem_lid[e * NB + b];
+            Ki[h] = ke[e * (NB * (NB + 1) / 2) + lo * NB - (lo * (lo - 1)) / 2 + (hi - lo)];
           }
         }
         // the previous round's reads of this region are issued before these writes
@@ -546,7 +547,7 @@ static bool lane_gather_mesh(const Mesh &m, int physics, int precision) {
 
 size_t assemble_workspace_bytes(const Mesh &m, int physics, int precision, bool want_f, bool want_vals) {
   if (lane_gather_mesh(m, physics, precision))
-    return want_vals ? sizeof(double) * static_cast<size_t>(m.nelem) * m.ndof_elem * m.ndof_elem : 0;
+    return want_vals ? sizeof(double) * static_cast<size_t>(m.nelem) * (m.ndof_elem * (m.ndof_elem + 1) / 2) : 0;
   txa_assembly_args a{};
   a.physics = physics;
   a.precision = precision;
