@@ -131,7 +131,7 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir):
     COPY_BYTES written)."""
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
     stats = _rows(stats_dir, "kernel_stats.csv")
-    cmd_stats = ("rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_stats -- "
+    cmd_stats = ("rocprofv3 --kernel-trace --stats --output-format csv -d <out>/prof_stats -- "
                  "python3 bench.py --steps 5 --warmup 1 --no-cpu")
     lines = ["# Kernel statistics: Q1 Poisson hex8 256^3 (bench.py workload)", "",
              "Command (from the repository root, on one MI355X):", "", "    " + cmd_stats, "",
@@ -154,7 +154,7 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir):
           "copy_fetch_kib": cf, "copy_write_kib": cw, "fetch_factor": ff, "write_factor": fw,
           "fetch_bytes": fetch_b, "write_bytes": write_b, "bytes_per_launch": fetch_b + write_b,
           "algorithmic_bytes": BYTES_PER_ELEM * nel,
-          "commands": [f"rocprofv3 --pmc {c} --output-format csv -d gpurun_out/pmc_{c.lower()} -- "
+          "commands": [f"rocprofv3 --pmc {c} --output-format csv -d <out>/pmc_{c.lower()} -- "
                        "python3 bench.py --steps 2 --warmup 1 --no-cpu --copy-calibration"
                        for c in ("FETCH_SIZE", "WRITE_SIZE")]}
     json.dump(tr, open(TRAFFIC_FILE, "w"), indent=1)
@@ -622,10 +622,10 @@ def summarize_q2(stats_dir, pmc_dir):
     nel = 64 ** 3
     lines = ["# Kernel statistics: Q2 hex27 elasticity 64^3 (bench.py --workload q2)", "",
              "Commands (from the repository root, on one MI355X):", "",
-             "    rocprofv3 --kernel-trace --stats --output-format csv -d gpurun_out/prof_q2 -- "
+             "    rocprofv3 --kernel-trace --stats --output-format csv -d <out>/prof_q2 -- "
              "python3 bench.py --workload q2 --steps 3 --warmup 1",
              "    rocprofv3 --pmc SQ_VALU_MFMA_BUSY_CYCLES GRBM_GUI_ACTIVE SQ_INSTS_VALU SQ_INSTS_VALU_MFMA_MOPS_BF16 "
-             "SQ_INSTS_LDS SQ_WAVES --output-format csv -d gpurun_out/pmc_q2 -- "
+             "SQ_INSTS_LDS SQ_WAVES --output-format csv -d <out>/pmc_q2 -- "
              "python3 bench.py --workload q2 --steps 1 --warmup 0", "",
              "Each form is two kernels: the element kernel (geometry, Gram matrix, element block and",
              "residual to a workspace) and the node-centric gather (sums the blocks around each node into",
