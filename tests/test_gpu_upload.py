@@ -116,6 +116,33 @@ def test_uploaded_jacobian_row_gather(engine, topo):
         m.destroy()
 
 
+@pytest.mark.parametrize("topo", [txa.HEX8, txa.TET4])
+def test_uploaded_transient_diffusion(engine, topo):
+    # R = K u + M udot - F and J = beta K + alpha M (M = rho int N N^T) against the oracle;
+    # tet4 runs one element per lane with the row gather, hex8 one workgroup per element
+    conn, ev, X, nent = renumbered_brick(topo, 6, seed=90 + topo)
+    m = engine.upload(topo, conn, X, nentities=nent)
+    try:
+        nd = m.build_dofmap(1)
+        nnz = m.build_graph()
+        gids, owned, el = O.dofmap(conn, 1)
+        rp, col = O.graph(el, owned.size)
+        coef = synth.lognormal(91, conn.shape[0], 0.5)
+        x, xd = synth.uniform(92, nd, -1, 1), synth.uniform(93, nd, -1, 1)
+        kw = dict(alpha=2.5, beta=0.75, gamma2=0.0)
+        A = O.Assembler(OTOPO[topo], txa.DIFFUSION, ev, X, el, coef, rp, col, nu=0.3, rho=1.7, src=(1.0, 0.0, 0.0))
+        fr, vr = A.evaluate(x, xd, None, transient=True, mode=O.MODE_EXPLICIT, **kw)
+        f = torch.empty(nd, device="cuda", dtype=torch.float64)
+        v = torch.empty(nnz, device="cuda", dtype=torch.float64)
+        m.assemble(physics=txa.DIFFUSION, coef=dev(coef), x=dev(x), xdot=dev(xd), f=f, vals=v, rho=1.7, src=(1.0,),
+                   transient=True, **kw)
+        torch.cuda.synchronize()
+        assert rel_err(f.cpu().numpy(), fr) <= TOL64
+        assert rel_err(v.cpu().numpy(), vr) <= TOL64
+    finally:
+        m.destroy()
+
+
 @pytest.mark.parametrize("name,block,topo", [("basic3d.gen", 0, txa.HEX8), ("twoblock_cube.gen", 0, txa.TET4)])
 def test_exodus_fixture(engine, name, block, topo):
     X, blocks = exodus.read_exodus(os.path.join(GOLDEN, name))

@@ -14,8 +14,8 @@
 // Jacobian replaced by the explicit tangent beta*K + alpha*C + gamma2*M.
 //
 // Dispatch: diffusion on hex8 / tet4 bricks goes to the row-owner kernel
-// (txa_q1_hex8.hip), three-field elasticity on bricks to txa_elastic.hip; steady
-// one-field diffusion on uploaded hex8 / tet4 meshes runs k_generic_lane here (one
+// (txa_q1_hex8.hip), three-field elasticity on bricks to txa_elastic.hip; one-field
+// diffusion on uploaded hex8 (steady) / tet4 (steady or transient) meshes runs k_generic_lane here (one
 // element per lane; its Jacobian through the row gather k_lane_gather, atomic-free)
 // and every other combination k_generic (one workgroup per element, FP64 atomic
 // scatter into a zeroed CRS).
@@ -232,14 +232,18 @@ __global__ __launch_bounds__(GB) void k_generic(GenericParams p) {
   }
 }
 
-// Generic diffusion on hex8 / tet4 meshes (one field, steady): one element per lane.
+// Generic diffusion on hex8 / tet4 meshes (one field): one element per lane.
 // The reference tables sit in LDS; each lane integrates its element (the same
-// formulas as k_generic, FP64) and adds its residual rows with atomics.  The element
+// formulas as k_generic, FP64) and adds its residual rows with atomics.  Steady (TR
+// false): R = K u - F from the element matrix, J = beta K.  Transient (TR true): the
+// matrix accumulated is J_e = beta K + alpha M (M = rho int N N^T) and the residual
+// K u + M udot - F is summed at the quadrature points (w (k grad N_a . grad u +
+// rho N_a udot - N_a f)), so no second matrix is held.  The element
 // matrix goes to the workspace for k_lane_gather when one is given (ke); otherwise its
 // entries are added with atomics, each entry's CRS position found by a binary search
 // whose steps depend only on the row length, so the NB searches of one row run in
 // lockstep (NB independent loads per step instead of one dependent chain per entry).
-template <int NB, int NQ>
+template <int NB, int NQ, bool TR>
 __global__ __launch_bounds__(256, 2) void k_generic_lane(GenericParams p) {
   __shared__ double tN[NB * NQ], tdN[NB * NQ * 3], tgdN[NB * NQ * 3], tqw[NQ];
   for (int i = threadIdx.x; i < NB * NQ; i += 256) tN[i] = p.tab[i];
@@ -251,7 +255,7 @@ __global__ __launch_bounds__(256, 2) void k_generic_lane(GenericParams p) {
   __syncthreads();
   const int64_t e = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
   if (e >= p.nelem) return;
-  double X[NB][3], u[NB];
+  double X[NB][3], u[NB], ud[TR ? NB : 1];
   int32_t L[NB];
 #pragma unroll
   for (int v = 0; v < NB; ++v) {
@@ -261,6 +265,10 @@ __global__ __launch_bounds__(256, 2) void k_generic_lane(GenericParams p) {
   }
 #pragma unroll
   for (int v = 0; v < NB; ++v) u[v] = p.x[L[v]];
+  if constexpr (TR) {
+#pragma unroll
+    for (int v = 0; v < NB; ++v) ud[v] = p.xdot[L[v]];
+  }
   const double cf = p.coef[e];
   double K[NB * (NB + 1) / 2], F[NB];
 #pragma unroll
@@ -305,25 +313,51 @@ __global__ __launch_bounds__(256, 2) void k_generic_lane(GenericParams p) {
       for (int i = 0; i < 3; ++i) G[b][i] = Ji[0 * 3 + i] * dn[0] + Ji[1 * 3 + i] * dn[1] + Ji[2 * 3 + i] * dn[2];
     }
     int k = 0;
+    if constexpr (TR) {
+      double gu[3] = {0.0, 0.0, 0.0}, vq = 0.0;   // grad u and udot at the point
 #pragma unroll
-    for (int a = 0; a < NB; ++a) {
-      F[a] += w * tN[a * NQ + q] * p.src[0];
+      for (int b = 0; b < NB; ++b) {
 #pragma unroll
-      for (int b = a; b < NB; ++b, ++k) K[k] += w * cf * (G[a][0] * G[b][0] + G[a][1] * G[b][1] + G[a][2] * G[b][2]);
+        for (int i = 0; i < 3; ++i) gu[i] += G[b][i] * u[b];
+        vq += tN[b * NQ + q] * ud[b];
+      }
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        const double na = tN[a * NQ + q];
+        F[a] += w * (cf * (G[a][0] * gu[0] + G[a][1] * gu[1] + G[a][2] * gu[2]) + p.rho * na * vq - na * p.src[0]);
+#pragma unroll
+        for (int b = a; b < NB; ++b, ++k)
+          K[k] += w * (p.beta * cf * (G[a][0] * G[b][0] + G[a][1] * G[b][1] + G[a][2] * G[b][2]) +
+                       p.alpha * p.rho * na * tN[b * NQ + q]);
+      }
+    } else {
+#pragma unroll
+      for (int a = 0; a < NB; ++a) {
+        F[a] += w * tN[a * NQ + q] * p.src[0];
+#pragma unroll
+        for (int b = a; b < NB; ++b, ++k) K[k] += w * cf * (G[a][0] * G[b][0] + G[a][1] * G[b][1] + G[a][2] * G[b][2]);
+      }
     }
   }
   auto kab = [&](int a, int b) -> double {
     const int lo = a < b ? a : b, hi = a < b ? b : a;
     return K[lo * NB - (lo * (lo - 1)) / 2 + (hi - lo)];
   };
-  if (p.f)
+  if (p.f) {
+    if constexpr (TR) {
 #pragma unroll
-    for (int a = 0; a < NB; ++a) {
-      double r = 0.0;
+      for (int a = 0; a < NB; ++a) atomicAdd(p.f + L[a], F[a]);
+    } else {
 #pragma unroll
-      for (int b = 0; b < NB; ++b) r += kab(a, b) * u[b];
-      atomicAdd(p.f + L[a], r - F[a]);
+      for (int a = 0; a < NB; ++a) {
+        double r = 0.0;
+#pragma unroll
+        for (int b = 0; b < NB; ++b) r += kab(a, b) * u[b];
+        atomicAdd(p.f + L[a], r - F[a]);
+      }
     }
+  }
+  const double jb = TR ? 1.0 : p.beta;   // the transient matrix already holds beta and alpha
   if (!p.vals) return;
   if (p.ke) {
     // the packed upper triangle as accumulated: entry (lo, hi) at lo*NB - lo(lo-1)/2 + (hi - lo)
@@ -348,7 +382,7 @@ __global__ __launch_bounds__(256, 2) void k_generic_lane(GenericParams p) {
     }
 #pragma unroll
     for (int b = 0; b < NB; ++b)
-      if (pos[b] < n && p.col[r0 + pos[b]] == L[b]) atomicAdd(p.vals + r0 + pos[b], p.beta * kab(a, b));
+      if (pos[b] < n && p.col[r0 + pos[b]] == L[b]) atomicAdd(p.vals + r0 + pos[b], jb * kab(a, b));
   }
 }
 
@@ -599,7 +633,8 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
     p.alpha = a.alpha; p.beta = a.beta; p.gamma2 = a.gamma2; p.transient = a.transient;
     p.f = a.f; p.vals = a.vals; p.rowptr = m.rowptr.p; p.col = m.col.p;
     int64_t blocks = m.nelem < 65536 * 8 ? m.nelem : 65536 * 8;
-    const bool lane = a.physics == TXA_DIFFUSION && !a.transient && m.nfields == 1 &&
+    // transient hex8 stays on k_generic: its lane form holds too many registers (it spills)
+    const bool lane = a.physics == TXA_DIFFUSION && m.nfields == 1 && (!a.transient || m.topo == TXA_TET4) &&
                       ((m.topo == TXA_HEX8 && r.nb == 8 && r.nv == 8 && r.nq == 8) ||
                        (m.topo == TXA_TET4 && r.nb == 4 && r.nv == 4 && r.nq == 4));
     // the Jacobian of the lane kernel by the row gather: element matrices in ws, rows stored whole
@@ -609,22 +644,25 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
     if (a.vals && !gather) TXA_HIP(hipMemsetAsync(a.vals, 0, sizeof(double) * m.nnz, s));
     const unsigned lblocks = static_cast<unsigned>((m.nelem + 255) / 256);
     if (lane && m.topo == TXA_HEX8)
-      hipLaunchKernelGGL((k_generic_lane<8, 8>), dim3(lblocks), dim3(256), 0, s, p);
+      hipLaunchKernelGGL((k_generic_lane<8, 8, false>), dim3(lblocks), dim3(256), 0, s, p);
+    else if (lane && a.transient)
+      hipLaunchKernelGGL((k_generic_lane<4, 4, true>), dim3(lblocks), dim3(256), 0, s, p);
     else if (lane)
-      hipLaunchKernelGGL((k_generic_lane<4, 4>), dim3(lblocks), dim3(256), 0, s, p);
+      hipLaunchKernelGGL((k_generic_lane<4, 4, false>), dim3(lblocks), dim3(256), 0, s, p);
     else if (r.nb <= 8 && r.nq <= 8)
       hipLaunchKernelGGL((k_generic<64, 8, 8>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, p);
     else
       hipLaunchKernelGGL((k_generic<256, kMaxNB, kMaxNQ>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, p);
     TXA_HIP(hipGetLastError());
     if (gather) {
+      const double gbeta = a.transient ? 1.0 : a.beta;   // the transient element matrices hold beta already
       const int64_t gblocks = std::min<int64_t>((m.ndofs + 7) / 8, int64_t(1) << 20);   // 8 rows per block
       if (m.topo == TXA_HEX8)
         hipLaunchKernelGGL((k_lane_gather<8>), dim3(static_cast<unsigned>(gblocks)), dim3(256), 0, s, m.ndofs,
-                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, p.ke, a.beta, a.vals);
+                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, p.ke, gbeta, a.vals);
       else
         hipLaunchKernelGGL((k_lane_gather<4>), dim3(static_cast<unsigned>(gblocks)), dim3(256), 0, s, m.ndofs,
-                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, p.ke, a.beta, a.vals);
+                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, p.ke, gbeta, a.vals);
       TXA_HIP(hipGetLastError());
     }
   }
