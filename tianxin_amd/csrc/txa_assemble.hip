@@ -15,7 +15,7 @@
 //
 // Dispatch: diffusion on hex8 / tet4 bricks goes to the row-owner kernel
 // (txa_q1_hex8.hip), three-field elasticity on bricks to txa_elastic.hip; one-field
-// diffusion on uploaded hex8 (steady) / tet4 (steady or transient) meshes runs k_generic_lane here (one
+// diffusion (steady or transient) on uploaded hex8 / tet4 meshes runs k_generic_lane here (one
 // element per lane; its Jacobian through the row gather k_lane_gather, atomic-free)
 // and every other combination k_generic (one workgroup per element, FP64 atomic
 // scatter into a zeroed CRS).
@@ -236,9 +236,9 @@ __global__ __launch_bounds__(GB) void k_generic(GenericParams p) {
 // The reference tables sit in LDS; each lane integrates its element (the same
 // formulas as k_generic, FP64) and adds its residual rows with atomics.  Steady (TR
 // false): R = K u - F from the element matrix, J = beta K.  Transient (TR true): the
-// matrix accumulated is J_e = beta K + alpha M (M = rho int N N^T) and the residual
-// K u + M udot - F is summed at the quadrature points (w (k grad N_a . grad u +
-// rho N_a udot - N_a f)), so no second matrix is held.  The element
+// quadrature loop is the same and keeps the point weights; afterwards each mass entry
+// M_ab = rho sum_q w_q N_a N_b is formed once, added to R (M udot) and to the matrix,
+// which becomes J_e = beta K + alpha M, so no second matrix is held.  The element
 // matrix goes to the workspace for k_lane_gather when one is given (ke); otherwise its
 // entries are added with atomics, each entry's CRS position found by a binary search
 // whose steps depend only on the row length, so the NB searches of one row run in
@@ -276,6 +276,7 @@ __global__ __launch_bounds__(256, 2) void k_generic_lane(GenericParams p) {
 #pragma unroll
   for (int a = 0; a < NB; ++a) F[a] = 0.0;
   double sg = 1.0;
+  double wq[TR ? NQ : 1];   // the point weights w detJ, for the mass entries
 #pragma unroll
   for (int q = 0; q < NQ; ++q) {
     double J[3][3];
@@ -312,51 +313,46 @@ __global__ __launch_bounds__(256, 2) void k_generic_lane(GenericParams p) {
 #pragma unroll
       for (int i = 0; i < 3; ++i) G[b][i] = Ji[0 * 3 + i] * dn[0] + Ji[1 * 3 + i] * dn[1] + Ji[2 * 3 + i] * dn[2];
     }
+    if constexpr (TR) wq[q] = w;
     int k = 0;
-    if constexpr (TR) {
-      double gu[3] = {0.0, 0.0, 0.0}, vq = 0.0;   // grad u and udot at the point
 #pragma unroll
-      for (int b = 0; b < NB; ++b) {
+    for (int a = 0; a < NB; ++a) {
+      F[a] += w * tN[a * NQ + q] * p.src[0];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) gu[i] += G[b][i] * u[b];
-        vq += tN[b * NQ + q] * ud[b];
-      }
-#pragma unroll
-      for (int a = 0; a < NB; ++a) {
-        const double na = tN[a * NQ + q];
-        F[a] += w * (cf * (G[a][0] * gu[0] + G[a][1] * gu[1] + G[a][2] * gu[2]) + p.rho * na * vq - na * p.src[0]);
-#pragma unroll
-        for (int b = a; b < NB; ++b, ++k)
-          K[k] += w * (p.beta * cf * (G[a][0] * G[b][0] + G[a][1] * G[b][1] + G[a][2] * G[b][2]) +
-                       p.alpha * p.rho * na * tN[b * NQ + q]);
-      }
-    } else {
-#pragma unroll
-      for (int a = 0; a < NB; ++a) {
-        F[a] += w * tN[a * NQ + q] * p.src[0];
-#pragma unroll
-        for (int b = a; b < NB; ++b, ++k) K[k] += w * cf * (G[a][0] * G[b][0] + G[a][1] * G[b][1] + G[a][2] * G[b][2]);
-      }
+      for (int b = a; b < NB; ++b, ++k) K[k] += w * cf * (G[a][0] * G[b][0] + G[a][1] * G[b][1] + G[a][2] * G[b][2]);
     }
   }
   auto kab = [&](int a, int b) -> double {
     const int lo = a < b ? a : b, hi = a < b ? b : a;
     return K[lo * NB - (lo * (lo - 1)) / 2 + (hi - lo)];
   };
-  if (p.f) {
-    if constexpr (TR) {
+  // residual rows K u - F (then + M udot below, for the transient form)
+  if (TR || p.f)
 #pragma unroll
-      for (int a = 0; a < NB; ++a) atomicAdd(p.f + L[a], F[a]);
-    } else {
+    for (int a = 0; a < NB; ++a) {
+      double r = 0.0;
 #pragma unroll
-      for (int a = 0; a < NB; ++a) {
-        double r = 0.0;
-#pragma unroll
-        for (int b = 0; b < NB; ++b) r += kab(a, b) * u[b];
-        atomicAdd(p.f + L[a], r - F[a]);
-      }
+      for (int b = 0; b < NB; ++b) r += kab(a, b) * u[b];
+      F[a] = r - F[a];
     }
+  if constexpr (TR) {
+    int k = 0;
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+      for (int b = a; b < NB; ++b, ++k) {
+        double mab = 0.0;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) mab += wq[q] * tN[a * NQ + q] * tN[b * NQ + q];
+        mab *= p.rho;
+        F[a] += mab * ud[b];
+        if (b != a) F[b] += mab * ud[a];
+        K[k] = p.beta * K[k] + p.alpha * mab;
+      }
   }
+  if (p.f)
+#pragma unroll
+    for (int a = 0; a < NB; ++a) atomicAdd(p.f + L[a], F[a]);
   const double jb = TR ? 1.0 : p.beta;   // the transient matrix already holds beta and alpha
   if (!p.vals) return;
   if (p.ke) {
@@ -633,8 +629,7 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
     p.alpha = a.alpha; p.beta = a.beta; p.gamma2 = a.gamma2; p.transient = a.transient;
     p.f = a.f; p.vals = a.vals; p.rowptr = m.rowptr.p; p.col = m.col.p;
     int64_t blocks = m.nelem < 65536 * 8 ? m.nelem : 65536 * 8;
-    // transient hex8 stays on k_generic: its lane form holds too many registers (it spills)
-    const bool lane = a.physics == TXA_DIFFUSION && m.nfields == 1 && (!a.transient || m.topo == TXA_TET4) &&
+    const bool lane = a.physics == TXA_DIFFUSION && m.nfields == 1 &&
                       ((m.topo == TXA_HEX8 && r.nb == 8 && r.nv == 8 && r.nq == 8) ||
                        (m.topo == TXA_TET4 && r.nb == 4 && r.nv == 4 && r.nq == 4));
     // the Jacobian of the lane kernel by the row gather: element matrices in ws, rows stored whole
@@ -643,7 +638,9 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
     if (a.f) TXA_HIP(hipMemsetAsync(a.f, 0, sizeof(double) * m.ndofs, s));
     if (a.vals && !gather) TXA_HIP(hipMemsetAsync(a.vals, 0, sizeof(double) * m.nnz, s));
     const unsigned lblocks = static_cast<unsigned>((m.nelem + 255) / 256);
-    if (lane && m.topo == TXA_HEX8)
+    if (lane && m.topo == TXA_HEX8 && a.transient)
+      hipLaunchKernelGGL((k_generic_lane<8, 8, true>), dim3(lblocks), dim3(256), 0, s, p);
+    else if (lane && m.topo == TXA_HEX8)
       hipLaunchKernelGGL((k_generic_lane<8, 8, false>), dim3(lblocks), dim3(256), 0, s, p);
     else if (lane && a.transient)
       hipLaunchKernelGGL((k_generic_lane<4, 4, true>), dim3(lblocks), dim3(256), 0, s, p);
