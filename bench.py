@@ -75,8 +75,8 @@ def cpu_baseline(n=128, reps=10, n1=48, reps1=10):
 
 
 COPY_BYTES = 1 << 30
-FP32_KERNEL = "k_q1_hex8_diffusion<1, float, false>"   # HEX8, FP32 integration, steady
-FP64_KERNEL = "k_q1_hex8_diffusion<1, double, false>"  # HEX8, FP64
+FP32_KERNEL = "k_q1_hex8_diffusion<1, float, false, false>"   # HEX8, FP32 integration, steady
+FP64_KERNEL = "k_q1_hex8_diffusion<1, double, false, true>"   # HEX8, FP64, steady, beta = 1
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "q1_hex8_diffusion_fp32_256_traffic.json")
 
 
@@ -794,6 +794,8 @@ def main():
         return float(((a - b).abs()[mask] / b.abs()[mask]).max())
 
     err = {"R_normwise": rel(f32, f64), "J_normwise": rel(v32, v64),
+           # entrywise over |ref| > 1e-12 max|ref| (the stated measure), and over 1e-3 max|ref|
+           "R_entrywise_over_1e-12_max": ent(f32, f64, 1e-12), "J_entrywise_over_1e-12_max": ent(v32, v64, 1e-12),
            "R_entrywise_over_1e-3_max": ent(f32, f64), "J_entrywise_over_1e-3_max": ent(v32, v64)}
     # size-independent property: each interior Jacobian row sums to zero (k grad.grad form)
     rp = m.graph(with_cols=False)[0]
@@ -834,6 +836,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / (HBM_PEAK / 1e9), "traffic": traffic,
                      "kernel": FP32_KERNEL, "kernel_ms": kt32,
+                     # the same bytes over the whole step (volume kernel + Dirichlet rows): value x 408 B
+                     "frac_whole_step": value * BYTES_PER_ELEM / HBM_PEAK,
                      "bytes_per_element": BYTES_PER_ELEM,
                      # measured on this GPU: device-to-device copy (read + write bytes) and fill (write bytes)
                      "measured_copy_GBs": copy_gbs, "measured_fill_GBs": fill_gbs,

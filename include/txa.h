@@ -69,6 +69,9 @@ typedef struct txa_context *txa_ctx;
 int txa_create(int device, txa_ctx *out);
 int txa_destroy(txa_ctx ctx);
 const char *txa_last_error(txa_ctx ctx);
+/* Waits for the context's device work.  Returns TXA_ERR_INVALID when an assembly or
+ * batch since the previous txa_sync was given a Dirichlet LID outside its DOF map
+ * (that entry touched no row; the reference throws for it), and clears the report. */
 int txa_sync(txa_ctx ctx);
 /* Library version string (static). */
 const char *txa_version(void);
@@ -234,6 +237,10 @@ typedef struct {
   const double *bc_vals;       /* device [n_bc] or NULL (= 0) */
   int64_t n_bc;
   int want_jacobian;           /* 0: residual only */
+  void *stream;                /* hipStream_t that produced u/v/a/coef/bc (NULL = the default
+                                  stream): the request runs after the work queued on it before
+                                  txa_serve_submit, so a time loop may write the state on its own
+                                  stream and submit without synchronising */
 } txa_serve_request;
 
 /* hbm_budget: bytes of the output arena (f and CRS values of admitted requests
@@ -244,8 +251,9 @@ int txa_serve_destroy(txa_ctx ctx, int server_id);
 /* Non-blocking: queues the request (TXA_ERR_NOMEM if it can never fit the budget). */
 int txa_serve_submit(txa_ctx ctx, int server_id, const txa_serve_request *req, int64_t *request_id);
 /* status: 0 queued, 1 running, 2 done (f / vals point at the result, device memory),
- * 3 failed (returns TXA_ERR_DEVICE with the message).  timeout_s > 0 waits up to
- * that long for the request to finish. */
+ * 3 failed (returns TXA_ERR_INVALID for a Dirichlet LID outside the DOF map,
+ * TXA_ERR_DEVICE otherwise, with the message).  timeout_s > 0 waits up to that long
+ * for the request to finish. */
 int txa_serve_poll(txa_ctx ctx, int server_id, int64_t request_id, double timeout_s, int *status, const double **f,
                    const double **vals);
 /* Returns the request's memory to the arena (done or failed requests only). */

@@ -66,20 +66,32 @@ def test_unsupported_precision_on_the_generic_path(engine):
     m.destroy()
 
 
-def test_dirichlet_lids_outside_the_map_are_ignored(engine):
+def test_dirichlet_lids_outside_the_map_are_reported(engine):
+    # a LID outside the map touches no row; the next txa_sync reports it (TXA_ERR_INVALID,
+    # std::invalid_argument in the reference) and clears the report
+    engine.sync()
     m = brick(engine, txa.HEX8, 4, 1)
     x = dev(synth.uniform(1, m.ndofs, -1, 1))
     k = dev(synth.lognormal(2, m.nelem, 0.5))
     f0 = torch.empty(m.ndofs, device="cuda", dtype=torch.float64)
     v0 = torch.empty(m.nnz, device="cuda", dtype=torch.float64)
     m.assemble(physics=txa.DIFFUSION, coef=k, x=x, f=f0, vals=v0, src=(1.0,))
+    engine.sync()
     f1, v1 = torch.empty_like(f0), torch.empty_like(v0)
     bad = dev(np.array([-1, m.ndofs, 1 << 30], np.int32), torch.int32)
     m.assemble(physics=txa.DIFFUSION, coef=k, x=x, f=f1, vals=v1, src=(1.0,), bc_lids=bad)
-    torch.cuda.synchronize()
+    assert code_of(engine.sync) == txa.TXA_ERR_INVALID
+    assert b"outside the DOF map" in engine.lib.txa_last_error(engine.h)
+    engine.sync()                                  # reported once
     assert torch.equal(f0, f1) and torch.equal(v0, v1)
     empty = dev(np.zeros(0, np.int32), torch.int32)
     m.assemble(physics=txa.DIFFUSION, coef=k, x=x, f=f1, vals=v1, src=(1.0,), bc_lids=empty)
-    torch.cuda.synchronize()
+    engine.sync()
     assert torch.equal(f0, f1) and torch.equal(v0, v1)
+    # the same report from a batch
+    b = txa.Batch(engine, [m], precision=txa.FP64)
+    b.assemble([dict(physics=txa.DIFFUSION, coef=k, x=x, f=f1, vals=v1, src=(1.0,), bc_lids=bad)],
+               use_graph=False)
+    assert code_of(engine.sync) == txa.TXA_ERR_INVALID
+    b.destroy()
     m.destroy()

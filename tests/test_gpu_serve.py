@@ -60,11 +60,12 @@ class Model:
         u, v, a = st
         return dev(u), dev(v), None if a is None else dev(a)
 
-    def submit(self, srv, st, d=None):
+    def submit(self, srv, st, d=None, precision=txa.FP64, bc=None, stream=None):
         d = self.upload(st) if d is None else d
         rid = srv.submit(self.m, physics=self.physics, coef=self.d_coef, u=d[0], v=d[1], a=d[2], dt=DT,
                          newmark=(NB, NG), nu=NU, rho=RHO, src=(1.0, 0.0, -0.5),
-                         rayleigh=RAY if self.physics == txa.ELASTICITY else (0.0, 0.0), bc_lids=self.d_bc)
+                         rayleigh=RAY if self.physics == txa.ELASTICITY else (0.0, 0.0),
+                         bc_lids=self.d_bc if bc is None else bc, precision=precision, stream=stream)
         return rid, d   # keep the state tensors alive until the request is done
 
     def oracle(self, st):
@@ -136,6 +137,84 @@ def test_serve_alone_and_in_the_mix(engine, models):
         srv.destroy()
 
 
+def test_serve_fp32_alone_and_in_the_mix(engine, models):
+    """The served precision form (FP32 element integration): each request within 1e-6 of the
+    FP64 oracle, and the same request gives the same bits alone and in a mix of eight."""
+    srv = txa.Server(engine, [m.m for m in models], hbm_budget=1 << 30, slots=4)
+    try:
+        probe = models[4]
+        st0 = probe.state(900)
+        rid, keep = probe.submit(srv, st0, precision=txa.FP32)
+        f_alone, v_alone = srv.result(rid, probe.m)
+        srv.release(rid)
+        order = [1, 4, 3, 2, 0, 4, 3, 1]
+        states = [st0 if i == 5 else models[k].state(910 + 10 * i) for i, k in enumerate(order)]
+        dstates = [models[k].upload(st) for k, st in zip(order, states)]
+        torch.cuda.synchronize()
+        jobs = [(k, states[i], *models[k].submit(srv, states[i], dstates[i], precision=txa.FP32))
+                for i, k in enumerate(order)]
+        for i, (k, st, rid, keep) in enumerate(jobs):
+            f, v = srv.result(rid, models[k].m)
+            srv.release(rid)
+            fr, vr = models[k].oracle(st)
+            e = (rel_err(f, fr), rel_err(v, vr))
+            print(f"fp32 request {i} model {k}: oracle R {e[0]:.2e} J {e[1]:.2e}", flush=True)
+            assert e[0] <= 1e-6 and e[1] <= 1e-6
+            if i == 5:
+                assert np.array_equal(f, f_alone) and np.array_equal(v, v_alone)
+        assert srv.stats()["failed"] == 0
+    finally:
+        srv.destroy()
+
+
+def test_serve_waits_for_the_producer_stream(engine, models):
+    """State written on the caller's stream and submitted at once (no synchronisation): the
+    request runs after that work, so its result is the oracle's for the new state."""
+    mdl = models[3]
+    srv = txa.Server(engine, [mdl.m], hbm_budget=1 << 30, slots=2)
+    side = torch.cuda.Stream()
+    try:
+        st_old, st_new = mdl.state(1200), mdl.state(1300)
+        d = mdl.upload(st_old)
+        src = mdl.upload(st_new)
+        busy = torch.rand(2048, 2048, device="cuda", dtype=torch.float32)
+        torch.cuda.synchronize()
+        with torch.cuda.stream(side):
+            for _ in range(20):          # keeps the producer stream busy before the state is written
+                busy = busy @ busy
+                busy = busy / busy.norm()
+            for dst, s_ in zip(d, src):
+                if dst is not None:
+                    dst.copy_(s_)
+            rid, keep = mdl.submit(srv, st_new, d, stream=side.cuda_stream)
+        f, v = srv.result(rid, mdl.m)
+        srv.release(rid)
+        fr, vr = mdl.oracle(st_new)
+        assert rel_err(f, fr) <= 1e-12 and rel_err(v, vr) <= 1e-12
+    finally:
+        srv.destroy()
+
+
+def test_serve_dirichlet_outside_the_map_fails(engine, models):
+    mdl = models[2]
+    srv = txa.Server(engine, [mdl.m], hbm_budget=1 << 28, slots=2)
+    try:
+        bad = dev(np.array([0, mdl.ndofs + 5], np.int32), torch.int32)
+        rid, keep = mdl.submit(srv, mdl.state(5), bc=bad)
+        with pytest.raises(txa.TxaError) as e:
+            srv.poll(rid, timeout=60.0)
+        assert e.value.code == txa.TXA_ERR_INVALID and "outside" in str(e.value)
+        srv.release(rid)
+        rid, keep = mdl.submit(srv, mdl.state(6))          # the slot is fine afterwards
+        f, v = srv.result(rid, mdl.m)
+        srv.release(rid)
+        fr, vr = mdl.oracle(mdl.state(6))
+        assert rel_err(f, fr) <= 1e-12
+        assert srv.stats()["bytes_in_use"] == 0
+    finally:
+        srv.destroy()
+
+
 def test_serve_admission_by_footprint(engine, models):
     big = models[3]
     # room for one such request at a time: the workspace goes back when the kernels end,
@@ -181,6 +260,10 @@ def test_serve_rejections(engine, models):
         with pytest.raises(txa.TxaError) as e:
             models[0].m.destroy()                            # resident while the server lives
         assert e.value.code == txa.TXA_ERR_LOGIC
+        for rebuild in (lambda: models[0].m.build_dofmap(3), lambda: models[0].m.build_graph()):
+            with pytest.raises(txa.TxaError) as e:           # its DOF map and graph stay as served
+                rebuild()
+            assert e.value.code == txa.TXA_ERR_LOGIC
         with pytest.raises(txa.TxaError):
             srv.poll(12345)
     finally:

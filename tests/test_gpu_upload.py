@@ -119,7 +119,7 @@ def test_uploaded_jacobian_row_gather(engine, topo):
 @pytest.mark.parametrize("topo", [txa.HEX8, txa.TET4])
 def test_uploaded_transient_diffusion(engine, topo):
     # R = K u + M udot - F and J = beta K + alpha M (M = rho int N N^T) against the oracle;
-    # tet4 runs one element per lane with the row gather, hex8 one workgroup per element
+    # both topologies run one element per lane with the row gather
     conn, ev, X, nent = renumbered_brick(topo, 6, seed=90 + topo)
     m = engine.upload(topo, conn, X, nentities=nent)
     try:

@@ -61,6 +61,7 @@ class ServeRequest(ctypes.Structure):
         ("dt", ctypes.c_double), ("newmark_beta", ctypes.c_double), ("newmark_gamma", ctypes.c_double),
         ("u", P), ("v", P), ("a", P),
         ("bc_lids", P), ("bc_vals", P), ("n_bc", I64), ("want_jacobian", ctypes.c_int),
+        ("stream", P),
     ]
 
 
@@ -133,6 +134,42 @@ def _dptr(t):
     if not t.is_contiguous():
         raise ValueError("expected a contiguous tensor")
     return ctypes.c_void_p(t.data_ptr())
+
+
+def check_tensors(specs, device=None):
+    """Raises ValueError unless every (name, tensor, dtype, n, required) of specs has the dtype
+    ("float64" / "int32") and n entries (n None: any), then that every one lives on CUDA device
+    `device`: the kernels read or write n entries of that width through the raw pointers, so
+    anything else would read past a buffer, reinterpret the data or fault.  Absent optional
+    tensors (None) pass."""
+    for name, t, dtype, n, required in specs:
+        if t is None:
+            if required:
+                raise ValueError(f"{name} is required")
+            continue
+        if str(t.dtype) != "torch." + dtype:
+            raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+        if n is not None and t.numel() != n:
+            raise ValueError(f"{name} must have {n} entries, got {t.numel()}")
+    for name, t, _, _, _ in specs:
+        if t is None:
+            continue
+        if not t.is_cuda:
+            raise ValueError(f"{name} must be a device tensor")
+        if device is not None and t.device.index != device:
+            raise ValueError(f"{name} is on cuda:{t.device.index}, the engine on cuda:{device}")
+
+
+def check_assembly_tensors(mesh, device, *, coef, x, f=None, vals=None, xdot=None, xddot=None, bc_lids=None,
+                           bc_vals=None, **_):
+    """The device arrays of one assembly against the mesh's sizes (elements, DOFs, nonzeros)."""
+    if bc_vals is not None and bc_lids is None:
+        raise ValueError("bc_vals without bc_lids")
+    check_tensors([("coef", coef, "float64", mesh.nelem, True), ("x", x, "float64", mesh.ndofs, True),
+                   ("xdot", xdot, "float64", mesh.ndofs, False), ("xddot", xddot, "float64", mesh.ndofs, False),
+                   ("f", f, "float64", mesh.ndofs, False), ("vals", vals, "float64", mesh.nnz, False),
+                   ("bc_lids", bc_lids, "int32", None, False),
+                   ("bc_vals", bc_vals, "float64", None if bc_lids is None else bc_lids.numel(), False)], device)
 
 
 def _stream(stream):
@@ -282,6 +319,7 @@ class BrickMesh:
         """One residual + Jacobian evaluation into device tensors f [ndofs] and vals [nnz].
 
         Keywords as make_args."""
+        check_assembly_tensors(self, self.eng.device, **kw)
         a = make_args(**kw)
         self.eng.check(self.eng.lib.txa_assemble(self.eng.h, self.id, ctypes.byref(a), _stream(stream)))
 
@@ -310,6 +348,7 @@ class Batch:
         arr = (AssemblyArgs * len(per_mesh))()
         made = []
         for i, kw in enumerate(per_mesh):
+            check_assembly_tensors(self.meshes[i], self.eng.device, **kw)
             kw = dict(kw)
             kw.setdefault("precision", self.precision)
             kw.setdefault("transient", self.transient)
@@ -357,8 +396,14 @@ class Server:
 
     def submit(self, mesh, *, physics, coef, u, v, a=None, dt=1e-3, newmark=(0.25, 0.5), nu=0.3, rho=1.0,
                src=(0.0, 0.0, 0.0), rayleigh=(0.0, 0.0), bc_lids=None, bc_vals=None, want_jacobian=True,
-               precision=FP64):
-        """Queues one request; the device arrays must stay alive until it is done.  Returns its id."""
+               precision=FP64, stream=None):
+        """Queues one request; the device arrays must stay alive until it is done.  Returns its id.
+        The request runs after the work queued on `stream` (default: torch's current stream) at
+        submission, so state written there needs no synchronisation first."""
+        check_assembly_tensors(mesh, self.eng.device, coef=coef, x=u, xdot=v, xddot=a, bc_lids=bc_lids,
+                               bc_vals=bc_vals)
+        if physics == ELASTICITY and a is None:
+            raise ValueError("elasticity requests need the acceleration state a")
         r = ServeRequest()
         r.mesh_id, r.physics, r.precision = mesh.id, physics, precision
         r.coef = _dptr(coef)
@@ -371,6 +416,7 @@ class Server:
         r.bc_lids, r.bc_vals = _dptr(bc_lids), _dptr(bc_vals)
         r.n_bc = 0 if bc_lids is None else int(bc_lids.numel())
         r.want_jacobian = int(want_jacobian)
+        r.stream = _stream(stream)
         rid = I64()
         self.eng.check(self.eng.lib.txa_serve_submit(self.eng.h, self.id, ctypes.byref(r), ctypes.byref(rid)))
         return int(rid.value)

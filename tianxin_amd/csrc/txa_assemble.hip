@@ -467,12 +467,16 @@ __global__ __launch_bounds__(256) void k_lane_gather(int64_t nrows, const int64_
 // ============================================================================
 __global__ void k_dirichlet(int64_t n, int64_t nrows, const int32_t *__restrict__ lids, const double *__restrict__ g,
                             const double *__restrict__ x, const int64_t *__restrict__ rowptr,
-                            const int32_t *__restrict__ col, double *__restrict__ f, double *__restrict__ vals) {
+                            const int32_t *__restrict__ col, double *__restrict__ f, double *__restrict__ vals,
+                            int *err) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64;
   const int lane = threadIdx.x % 64;
   if (i >= n) return;
   const int32_t r = lids[i];
-  if (r < 0 || r >= nrows) return;  // LIDs outside the map are ignored
+  if (r < 0 || r >= nrows) {   // outside the map: no row is touched, the context reports it
+    if (lane == 0 && err) *err = 1;
+    return;
+  }
   if (lane == 0 && f) f[r] = x[r] - (g ? g[i] : 0.0);
   if (vals)
     for (int64_t k = rowptr[r] + lane; k < rowptr[r + 1]; k += 64) vals[k] = col[k] == r ? 1.0 : 0.0;
@@ -568,8 +572,8 @@ static bool uses_q1_path(const Mesh &m, int physics, int precision) {
          precision != TXA_SPLIT_BF16;
 }
 
-// Uploaded one-field hex8 / tet4 diffusion: the element matrices for the row gather
-// (the mesh's element-row lists exist; transient calls run k_generic and leave it unused).
+// Uploaded one-field hex8 / tet4 diffusion, steady or transient: the element matrices go to
+// the workspace for the row gather (the mesh's element-row lists exist).
 static bool lane_gather_mesh(const Mesh &m, int physics, int precision) {
   return !m.structured && m.nfields == 1 && physics == TXA_DIFFUSION && precision == TXA_FP64 &&
          m.occ.p != nullptr && ((m.topo == TXA_HEX8 && m.ndof_elem == 8) || (m.topo == TXA_TET4 && m.ndof_elem == 4));
@@ -585,7 +589,7 @@ size_t assemble_workspace_bytes(const Mesh &m, int physics, int precision, bool 
   return elastic_workspace_bytes(m, precision, want_f, want_vals);
 }
 
-void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
+void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws, int *err) {
   TXA_REQUIRE(m.has_graph, TXA_ERR_LOGIC, "assemble before graph_build");
   TXA_REQUIRE(a.x != nullptr, TXA_ERR_INVALID, "x is required");
   TXA_REQUIRE(a.coef != nullptr, TXA_ERR_INVALID, "coef is required");
@@ -685,7 +689,7 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
   if (a.n_bc > 0) {
     const int64_t per = 4;  // rows per 256-thread block
     hipLaunchKernelGGL(k_dirichlet, dim3(static_cast<unsigned>((a.n_bc + per - 1) / per)), dim3(256), 0, s, a.n_bc,
-                       m.ndofs, a.bc_lids, a.bc_vals, a.x, m.rowptr.p, m.col.p, a.f, a.vals);
+                       m.ndofs, a.bc_lids, a.bc_vals, a.x, m.rowptr.p, m.col.p, a.f, a.vals, err);
     TXA_HIP(hipGetLastError());
   }
 }

@@ -35,6 +35,19 @@ int guarded(txa_ctx ctx, F &&f) {
   }
 }
 
+// Rebuilding a mesh's DOF map or graph reallocates arrays that batches and servers
+// launch on (and can change its DOF count): refused while one holds the mesh.
+void require_unshared(txa_ctx ctx, int mesh_id, const char *what) {
+  for (auto &kv : ctx->batches)
+    for (int id : txa::batch_mesh_ids(*kv.second))
+      TXA_REQUIRE(id != mesh_id, TXA_ERR_LOGIC,
+                  std::string(what) + ": mesh is part of batch " + std::to_string(kv.first));
+  for (auto &kv : ctx->servers)
+    for (int id : txa::serve_mesh_ids(*kv.second))
+      TXA_REQUIRE(id != mesh_id, TXA_ERR_LOGIC,
+                  std::string(what) + ": mesh is served by server " + std::to_string(kv.first));
+}
+
 }  // namespace
 
 extern "C" {
@@ -60,6 +73,14 @@ int txa_create(int device, txa_ctx *out) {
   txa_context *c = new (std::nothrow) txa_context();
   if (!c) return TXA_ERR_NOMEM;
   c->device = device;
+  try {
+    c->err.alloc(1);
+    TXA_HIP(hipMemset(c->err.p, 0, sizeof(int)));
+  } catch (const Error &e) {
+    g_create_error = e.what();
+    delete c;
+    return e.code;
+  }
   *out = c;
   return TXA_OK;
 }
@@ -75,7 +96,17 @@ int txa_destroy(txa_ctx ctx) {
 const char *txa_last_error(txa_ctx ctx) { return ctx ? ctx->last_error.c_str() : g_create_error.c_str(); }
 
 int txa_sync(txa_ctx ctx) {
-  return guarded(ctx, [&] { TXA_HIP(hipDeviceSynchronize()); });
+  return guarded(ctx, [&] {
+    TXA_HIP(hipDeviceSynchronize());
+    int flag = 0;
+    TXA_HIP(hipMemcpy(&flag, ctx->err.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) {
+      TXA_HIP(hipMemset(ctx->err.p, 0, sizeof(int)));
+      throw Error(TXA_ERR_INVALID,
+                  "a Dirichlet LID of an assembly since the last txa_sync lies outside the DOF map "
+                  "(its row was not applied)");
+    }
+  });
 }
 
 int txa_mesh_brick(txa_ctx ctx, int topo, int nx, int ny, int nz, const double *vertex_coords, int *mesh_id) {
@@ -112,12 +143,7 @@ int txa_mesh_destroy(txa_ctx ctx, int mesh_id) {
   return guarded(ctx, [&] {
     auto it = ctx->meshes.find(mesh_id);
     TXA_REQUIRE(it != ctx->meshes.end(), TXA_ERR_INVALID, "unknown mesh id");
-    for (auto &kv : ctx->batches)
-      for (int id : txa::batch_mesh_ids(*kv.second))
-        TXA_REQUIRE(id != mesh_id, TXA_ERR_LOGIC, "mesh is part of batch " + std::to_string(kv.first));
-    for (auto &kv : ctx->servers)
-      for (int id : txa::serve_mesh_ids(*kv.second))
-        TXA_REQUIRE(id != mesh_id, TXA_ERR_LOGIC, "mesh is served by server " + std::to_string(kv.first));
+    require_unshared(ctx, mesh_id, "txa_mesh_destroy");
     TXA_HIP(hipDeviceSynchronize());
     ctx->meshes.erase(it);
   });
@@ -134,6 +160,7 @@ int txa_mesh_info(txa_ctx ctx, int mesh_id, int64_t *out) {
 int txa_dofmap_build(txa_ctx ctx, int mesh_id, int nfields, int64_t *ndofs) {
   return guarded(ctx, [&] {
     Mesh &m = ctx->mesh(mesh_id);
+    require_unshared(ctx, mesh_id, "txa_dofmap_build");
     txa::dofmap_build(m, nfields, nullptr);
     if (ndofs) *ndofs = m.ndofs;
   });
@@ -162,6 +189,7 @@ int txa_dofmap_point_lids(txa_ctx ctx, int mesh_id, int32_t *point_lids, int64_t
 int txa_graph_build(txa_ctx ctx, int mesh_id, int64_t *nnz) {
   return guarded(ctx, [&] {
     Mesh &m = ctx->mesh(mesh_id);
+    require_unshared(ctx, mesh_id, "txa_graph_build");
     txa::graph_build(m, nullptr);
     if (nnz) *nnz = m.nnz;
   });
@@ -210,7 +238,7 @@ int txa_assemble(txa_ctx ctx, int mesh_id, const txa_assembly_args *args, void *
                                                      args->vals != nullptr);
     void *ws = wsb ? ctx->ws_alloc(wsb, s) : nullptr;
     try {
-      txa::assemble(m, *args, s, ws);
+      txa::assemble(m, *args, s, ws, ctx->err.p);
     } catch (...) {
       if (ws) (void)hipFreeAsync(ws, s);
       throw;
@@ -244,7 +272,7 @@ int txa_batch_destroy(txa_ctx ctx, int batch_id) {
 
 int txa_batch_assemble(txa_ctx ctx, int batch_id, const txa_assembly_args *args, int use_graph, void *stream) {
   return guarded(ctx, [&] {
-    txa::batch_assemble(ctx->batch(batch_id), args, use_graph != 0, static_cast<hipStream_t>(stream));
+    txa::batch_assemble(ctx->batch(batch_id), args, use_graph != 0, static_cast<hipStream_t>(stream), ctx->err.p);
   });
 }
 
@@ -253,7 +281,8 @@ int txa_batch_tune(txa_ctx ctx, int batch_id, const txa_assembly_args *args, int
   return guarded(ctx, [&] {
     TXA_REQUIRE(args, TXA_ERR_INVALID, "null args");
     TXA_REQUIRE(reps >= 1 && reps <= 1000, TXA_ERR_INVALID, "reps must be in [1, 1000]");
-    const double ms = txa::batch_tune(ctx->batch(batch_id), args, static_cast<hipStream_t>(stream), reps);
+    const double ms = txa::batch_tune(ctx->batch(batch_id), args, static_cast<hipStream_t>(stream), reps,
+                                      ctx->err.p);
     if (best_ms) *best_ms = ms;
   });
 }

@@ -109,6 +109,9 @@ struct Context {
   // stream-ordered pool for assembly workspaces: freed blocks stay mapped for the
   // next call (release threshold unbounded) until the context is destroyed
   hipMemPool_t ws_pool = nullptr;
+  // device flag the assembly kernels set when a Dirichlet LID lies outside the DOF map;
+  // txa_sync reports and clears it (the reference throws at that point)
+  DevBuf<int> err;
   ~Context() {
     for (auto &kv : servers) serve_free(kv.second);   // joins the worker threads first
     for (auto &kv : batches) batch_free(kv.second);
@@ -167,7 +170,8 @@ void read_vector_mm(const char *path, double *dst, int64_t n);
 void upload_refelem(Mesh &m, DevBuf<double> &tab, hipStream_t s);
 // ws: device workspace of assemble_workspace_bytes(...) bytes, 256-byte aligned, for
 // the work on s (may be null when that size is 0)
-void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws);
+// err: device (or host-mapped) int set to 1 when a Dirichlet LID lies outside the map (may be null)
+void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws, int *err = nullptr);
 size_t assemble_workspace_bytes(const Mesh &m, int physics, int precision, bool want_f, bool want_vals);
 // element-centric elasticity (txa_elastic.hip)
 void node_perm_build(Mesh &m, hipStream_t s);
@@ -177,8 +181,8 @@ void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws
 // Q1 diffusion fast path and batches (txa_q1_hex8.hip)
 void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s);
 Batch *batch_create(const std::vector<Mesh *> &meshes, const std::vector<int> &ids, int precision, bool transient);
-void batch_assemble(Batch &b, const txa_assembly_args *args, bool use_graph, hipStream_t s);
-double batch_tune(Batch &b, const txa_assembly_args *args, hipStream_t s, int reps);
+void batch_assemble(Batch &b, const txa_assembly_args *args, bool use_graph, hipStream_t s, int *err);
+double batch_tune(Batch &b, const txa_assembly_args *args, hipStream_t s, int reps, int *err);
 void batch_info(const Batch &b, int64_t *out);
 const std::vector<int> &batch_mesh_ids(const Batch &b);
 // serving loop (txa_serve.cpp)

@@ -22,7 +22,15 @@
 //
 // Geometry (trilinear-map coefficients, J, cofactors, det, 1/det) is FP64 in
 // every precision; Real is the precision of the metric tensor and the
-// contraction; row sums over elements are FP64.
+// contraction (the Jacobian); row sums over elements are FP64.
+//
+// Residual.  The FP64 form forms R = K u - F from its FP64 element matrix.  The
+// FP32 form computes R in FP64 without the matrix, as the reference's
+// DOFGradient -> GradBasisDotVector chain does: the reference gradient of u at
+// each point from the Walsh coefficients of u (u(xi) = sum_s u^_s prod_{d in s} xi_d),
+// the flux G_q grad u with the FP64 metric, and its transposed contraction back
+// to the vertices in the same basis; the FP32 element matrix never touches R, so
+// R keeps FP64 accuracy entry by entry (cancelling rows included).
 #include "txa_internal.hpp"
 
 #include <algorithm>
@@ -31,33 +39,24 @@
 namespace txa {
 namespace {
 
-#ifndef TXA_Q1_TX
-#define TXA_Q1_TX 32   // element tile width: 32 x 8, or 16 x 16 (fewer recomputed elements, 1.5% slower at 256^3)
-#endif
-constexpr int TX = TXA_Q1_TX, TY = 256 / TX, NE = TX * TY;
-static_assert(TX == 16 || TX == 32, "tile width: a group of owner lanes (one x line) is 16 or 32 lanes");
+// element tile 32 x 8 (a 16 x 16 tile recomputes fewer halo elements but ran 1.5% slower at 256^3)
+constexpr int TX = 32, TY = 256 / TX, NE = TX * TY;
 constexpr int NT = 2 * NE;                     // block: 4 element waves + 4 owner waves
 constexpr double kG = 0.57735026918962576451;   // 1/sqrt(3)
 constexpr double kP = 0.5 * (1.0 + kG);         // P(bit == qbit)
 constexpr double kM = 0.5 * (1.0 - kG);         // P(bit != qbit)
-constexpr int NSTAGE = 44;                      // 36 packed K + 8 R per element
+constexpr int NSTAGE = 44;   // per element: 36 packed K, then 8 FP64 R (FP64) or 4 FP64 x-pair sums of R (FP32)
 // Stage slots: FP32 stages are double-buffered (element waves publish layer z in one
 // slot while the owner waves consume layer z-1 from the other, one barrier per
 // layer); an FP64 stage takes one slot (two barriers per layer) to stay within LDS.
 template <typename Real>
 constexpr int stage_slots() { return sizeof(Real) == 4 ? 2 : 1; }
 
-#ifndef TXA_Q1_FP32_RING
-#define TXA_Q1_FP32_RING 0    // 1: FP32 integration uses the one-role LDS-ring kernel (0: warp-specialised)
-#endif
 #ifndef TXA_Q1_TARGET_BLOCKS
 #define TXA_Q1_TARGET_BLOCKS 2560   // z slabs are sized to give about this many workgroups
 #endif
 #ifndef TXA_Q1_IDENT_FAST
 #define TXA_Q1_IDENT_FAST 1   // rows whose slot map is the identity skip the per-entry decode
-#endif
-#ifndef TXA_Q1_UNROLL2
-#define TXA_Q1_UNROLL2 1      // layer loop unrolled by two: the accumulator hand-over becomes renaming
 #endif
 #ifndef TXA_Q1_NT_STORES
 #define TXA_Q1_NT_STORES 1    // non-temporal stores for the CRS values (written once, not re-read)
@@ -102,15 +101,74 @@ __device__ __forceinline__ double recip(double x) {
 
 template <typename Real, bool TRANS>
 struct Hex8Out {
-  Real K[36];   // beta K + alpha M, packed symmetric, binary vertex order
-  Real R[8];    // K u + M udot - F
+  Real K[36];     // beta K + alpha M, packed symmetric, binary vertex order
+  double R[8];    // K u + M udot - F, FP64 in every precision
 };
 
-// One element.  X: vertex coordinates in binary order v = bx + 2by + 4bz.
-template <typename Real, bool TRANS>
+// Whether the residual is formed without the element matrix (FP32 form: FP64 flux).
+template <typename Real>
+constexpr bool flux_residual() { return sizeof(Real) == 4; }
+
+// In-place Walsh butterfly over the three vertex bits: v[s] <- sum_a prod_{d in s} sgn(a,d) v[a].
+__device__ __forceinline__ void walsh8(double (&v)[8]) {
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (!bit(i, b)) {
+        const double lo = v[i], hi = v[i | (1 << b)];
+        v[i] = lo + hi;
+        v[i | (1 << b)] = hi - lo;
+      }
+}
+
+// Its transpose: v[a] <- sum_s prod_{d in s} sgn(a,d) v[s].
+__device__ __forceinline__ void walsh8_t(double (&v)[8]) {
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (!bit(i, b)) {
+        const double lo = v[i], hi = v[i | (1 << b)];
+        v[i] = lo - hi;
+        v[i | (1 << b)] = lo + hi;
+      }
+}
+
+// Trilinear interpolation to the 2x2x2 Gauss points and its transpose (the matrix
+// P(a_d, q_d) is symmetric): v[q] <- sum_a prod_d P(a_d, q_d) v[a].
+__device__ __forceinline__ void interp8(double (&v)[8]) {
+#pragma unroll
+  for (int b = 0; b < 3; ++b)
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (!bit(i, b)) {
+        const double lo = v[i], hi = v[i | (1 << b)];
+        v[i] = kP * lo + kM * hi;
+        v[i | (1 << b)] = kM * lo + kP * hi;
+      }
+}
+
+// One element.  X: vertex coordinates in binary order v = bx + 2by + 4bz.  mid() runs
+// once the quadrature loop is done (the caller issues its prefetch loads there, where
+// the geometry's registers are free).
+template <typename Real, bool TRANS, bool UNIT, class Mid>
 __device__ __forceinline__ void hex8_element(const double (&X)[3][8], const double (&u)[8], const double (&ud)[8],
                                              double k, double rhoc, double src, double alpha, double beta,
-                                             Hex8Out<Real, TRANS> &o) {
+                                             Hex8Out<Real, TRANS> &o, Mid &&mid) {
+  constexpr bool kFlux = flux_residual<Real>();
+  // FP64 flux residual: Walsh coefficients of u (times 1/8; the xi-linear terms of the
+  // gradient prescaled by the point coordinate 1/sqrt3, the bilinear one by 1/3) and the
+  // accumulators rh[s] = sum_q sum_j d(phi_s)/d(xi_j)(q) z_j(q) of the flux z = A_q grad u
+  double uh[8], rh[8];
+  if constexpr (kFlux) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) { uh[v] = u[v]; rh[v] = 0.0; }
+    walsh8(uh);
+#pragma unroll
+    for (int v = 1; v < 8; ++v) uh[v] *= 0.125;
+    uh[3] *= kG; uh[5] *= kG; uh[6] *= kG; uh[7] *= kG * kG;
+  }
   // Walsh transform of the vertex coordinates: h[s] = sum_v prod_{d in s} sgn(v,d) X_v,
   // the coefficients (times 8) of 1, xi, eta, xi eta, zeta, xi zeta, eta zeta, xi eta zeta
   double h[3][8];
@@ -159,16 +217,48 @@ __device__ __forceinline__ void hex8_element(const double (&X)[3][8], const doub
     c[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
     const double det = J[0][0] * c[0][0] + J[0][1] * c[0][1] + J[0][2] * c[0][2];
     if (q == 0) sg = det < 0.0 ? -1.0 : 1.0;   // computeCellMeasure: sign of det at the first point
-    wm[q] = det * sg;                          // Gauss weights are 1
+    if (!kFlux || TRANS) wm[q] = det * sg;     // Gauss weights are 1
     const double cw = 0.25 * k * sg * recip(det);   // k wm / det^2, times the 1/4 of (s/2)(s/2)
     // the metric in FP64, rounded once to Real (six conversions per point instead of ten)
-    A[q][0] = Real(cw * (c[0][0] * c[0][0] + c[1][0] * c[1][0] + c[2][0] * c[2][0]));
-    A[q][1] = Real(cw * (c[0][1] * c[0][1] + c[1][1] * c[1][1] + c[2][1] * c[2][1]));
-    A[q][2] = Real(cw * (c[0][2] * c[0][2] + c[1][2] * c[1][2] + c[2][2] * c[2][2]));
-    A[q][3] = Real(cw * (c[0][0] * c[0][1] + c[1][0] * c[1][1] + c[2][0] * c[2][1]));
-    A[q][4] = Real(cw * (c[0][1] * c[0][2] + c[1][1] * c[1][2] + c[2][1] * c[2][2]));
-    A[q][5] = Real(cw * (c[0][0] * c[0][2] + c[1][0] * c[1][2] + c[2][0] * c[2][2]));
+    const double mxx = cw * (c[0][0] * c[0][0] + c[1][0] * c[1][0] + c[2][0] * c[2][0]);
+    const double myy = cw * (c[0][1] * c[0][1] + c[1][1] * c[1][1] + c[2][1] * c[2][1]);
+    const double mzz = cw * (c[0][2] * c[0][2] + c[1][2] * c[1][2] + c[2][2] * c[2][2]);
+    const double mxy = cw * (c[0][0] * c[0][1] + c[1][0] * c[1][1] + c[2][0] * c[2][1]);
+    const double myz = cw * (c[0][1] * c[0][2] + c[1][1] * c[1][2] + c[2][1] * c[2][2]);
+    const double mxz = cw * (c[0][0] * c[0][2] + c[1][0] * c[1][2] + c[2][0] * c[2][2]);
+    A[q][0] = Real(mxx); A[q][1] = Real(myy); A[q][2] = Real(mzz);
+    A[q][3] = Real(mxy); A[q][4] = Real(myz); A[q][5] = Real(mxz);
+    if constexpr (kFlux) {
+      // reference gradient of u at the point, the flux z = A_q grad u (A_q = G_q / 4), and its
+      // contraction with the gradients of the Walsh basis functions
+      const double s0 = bit(q, 0) ? 1.0 : -1.0, s1 = bit(q, 1) ? 1.0 : -1.0, s2 = bit(q, 2) ? 1.0 : -1.0;
+      const double g0 = uh[1] + s1 * uh[3] + s2 * uh[5] + (s1 * s2) * uh[7];
+      const double g1 = uh[2] + s0 * uh[3] + s2 * uh[6] + (s0 * s2) * uh[7];
+      const double g2 = uh[4] + s0 * uh[5] + s1 * uh[6] + (s0 * s1) * uh[7];
+      const double z0 = mxx * g0 + mxy * g1 + mxz * g2;
+      const double z1 = mxy * g0 + myy * g1 + myz * g2;
+      const double z2 = mxz * g0 + myz * g1 + mzz * g2;
+      rh[1] += z0;
+      rh[2] += z1;
+      rh[4] += z2;
+      rh[3] += s1 * z0 + s0 * z1;
+      rh[5] += s2 * z0 + s0 * z2;
+      rh[6] += s2 * z1 + s1 * z2;
+      rh[7] += (s1 * s2) * z0 + (s0 * s2) * z1 + (s0 * s1) * z2;
+      // source: -F_a = -src sum_q wm_q N_a(q) enters as -(src/4) wm_q phi_s(q), with
+      // phi_s(q) = 3^(-|s|/2) prod sgn, one power of 1/sqrt3 fewer than the later scaling supplies
+      const double ws = -0.25 * src * (det * sg), wg = ws * kG;
+      rh[0] += ws;
+      rh[1] += s0 * wg;
+      rh[2] += s1 * wg;
+      rh[4] += s2 * wg;
+      rh[3] += (s0 * s1) * wg;
+      rh[5] += (s0 * s2) * wg;
+      rh[6] += (s1 * s2) * wg;
+      rh[7] += (s0 * s1 * s2) * wg;
+    }
   }
+  mid();
   // ---- diagonal tables S_dd[c1][c2]: sum over the two other axes after summing axis d
   Real S[3][3][3];
 #pragma unroll
@@ -274,7 +364,7 @@ __device__ __forceinline__ void hex8_element(const double (&X)[3][8], const doub
     }
   // ---- source load F_a = src sum_q wm_q N_a(q) and mass classes (FP64), axis by axis
   double F[8];
-  {
+  if constexpr (!kFlux) {
     double t1[2][2][2];   // [ax][qy][qz]
 #pragma unroll
     for (int ax = 0; ax < 2; ++ax)
@@ -314,20 +404,39 @@ __device__ __forceinline__ void hex8_element(const double (&X)[3][8], const doub
       }
   }
   // ---- residual and tangent
+  if constexpr (kFlux) {
+    // R_a = 4 (1/8) sum_s w_s(a) rh_s (A = G/4, N_a = 1/8 sum_s w_s(a) phi_s), source included;
+    // then M udot
+    rh[3] *= kG; rh[5] *= kG; rh[6] *= kG; rh[7] *= kG * kG;
+    walsh8_t(rh);
+    double mr[8];
+    if constexpr (TRANS) {
+      // M udot = rho_c sum_q wm_q N_a(q) udot(q): udot at the points, weighted, back to the vertices
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    Real r = Real(0);
+      for (int v = 0; v < 8; ++v) mr[v] = ud[v];
+      interp8(mr);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      r += K[sym8(a, b)] * Real(u[b]);
-      if (TRANS) {
-        const int mc = cls(bit(a, 0), bit(b, 0)) + 3 * (cls(bit(a, 1), bit(b, 1)) + 3 * cls(bit(a, 2), bit(b, 2)));
-        r += M[mc] * Real(ud[b]);
-      }
+      for (int q = 0; q < 8; ++q) mr[q] *= rhoc * wm[q];
+      interp8(mr);
     }
-    o.R[a] = Real(double(r) - F[a]);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) o.R[a] = TRANS ? 0.5 * rh[a] + mr[a] : 0.5 * rh[a];
+  } else {
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      Real r = Real(0);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        r += K[sym8(a, b)] * Real(u[b]);
+        if (TRANS) {
+          const int mc = cls(bit(a, 0), bit(b, 0)) + 3 * (cls(bit(a, 1), bit(b, 1)) + 3 * cls(bit(a, 2), bit(b, 2)));
+          r += M[mc] * Real(ud[b]);
+        }
+      }
+      o.R[a] = double(r) - F[a];
+    }
   }
-  if (!TRANS && beta == 1.0) {   // the steady Jacobian dR/du (uniform branch): K as it is
+  if constexpr (!TRANS && UNIT) {   // the steady Jacobian dR/du with beta = 1: K as it is
 #pragma unroll
     for (int i = 0; i < 36; ++i) o.K[i] = K[i];
     return;
@@ -359,16 +468,18 @@ __device__ __forceinline__ constexpr int tet_v(int t, int j) {
 // (j = 1..3), grad N_0 = -(sum); measure w |det| with w = 1/6; the degree-2
 // rule integrates the source (|T|/4 per vertex) and mass (|T|/20 (1 + delta)) exactly.
 // k: the six tet coefficients, in element order 6*cell + t.
-template <typename Real, bool TRANS>
+template <typename Real, bool TRANS, bool UNIT, class Mid>
 __device__ __forceinline__ void tet6_cell(const double (&X)[3][8], const double (&u)[8], const double (&ud)[8],
                                           const double (&k)[6], double rhoc, double src, double alpha, double beta,
-                                          Hex8Out<Real, TRANS> &o) {
+                                          Hex8Out<Real, TRANS> &o, Mid &&mid) {
+  constexpr bool kFlux = flux_residual<Real>();
+  mid();
   Real K[36], M[TRANS ? 36 : 1];
-  double F[8];
+  double F[8], Rf[8];   // source load; FP64 residual K u + M udot of the FP32 form
 #pragma unroll
   for (int i = 0; i < 36; ++i) { K[i] = Real(0); if (TRANS) M[i] = Real(0); }
 #pragma unroll
-  for (int v = 0; v < 8; ++v) F[v] = 0.0;
+  for (int v = 0; v < 8; ++v) { F[v] = 0.0; Rf[v] = 0.0; }
 #pragma unroll
   for (int t = 0; t < 6; ++t) {
     double J[3][3];
@@ -389,7 +500,31 @@ __device__ __forceinline__ void tet6_cell(const double (&X)[3][8], const double 
     const double det = J[0][0] * c[0][0] + J[0][1] * c[0][1] + J[0][2] * c[0][2];
     const double sg = det < 0.0 ? -1.0 : 1.0;
     const double vol = det * sg / 6.0;                 // sum of the weighted measure
-    const Real cw = Real(k[t] * sg * recip(det) / 6.0);
+    const double cw64 = k[t] * sg * recip(det) / 6.0;
+    const Real cw = Real(cw64);
+    if constexpr (kFlux) {
+      // FP64: grad u (times det) = cof (u_j - u_0), R_j = cw g_j . (cof du), R_0 = -(R_1 + R_2 + R_3);
+      // mass rho_c |T| / 20 (udot_a + sum_b udot_b)
+      const double du0 = u[tet_v(t, 1)] - u[tet_v(t, 0)], du1 = u[tet_v(t, 2)] - u[tet_v(t, 0)],
+                   du2 = u[tet_v(t, 3)] - u[tet_v(t, 0)];
+      double tt[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) tt[i] = cw64 * (c[i][0] * du0 + c[i][1] * du1 + c[i][2] * du2);
+      double rs = 0.0;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double r = c[0][j] * tt[0] + c[1][j] * tt[1] + c[2][j] * tt[2];
+        Rf[tet_v(t, j + 1)] += r;
+        rs += r;
+      }
+      Rf[tet_v(t, 0)] -= rs;
+      if constexpr (TRANS) {
+        const double sud = ud[tet_v(t, 0)] + ud[tet_v(t, 1)] + ud[tet_v(t, 2)] + ud[tet_v(t, 3)];
+        const double mw = rhoc * vol * 0.05;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) Rf[tet_v(t, a)] += mw * (sud + ud[tet_v(t, a)]);
+      }
+    }
     Real g[4][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -407,18 +542,24 @@ __device__ __forceinline__ void tet6_cell(const double (&X)[3][8], const double 
       }
     }
   }
+  if constexpr (kFlux) {
 #pragma unroll
-  for (int a = 0; a < 8; ++a) {
-    Real r = Real(0);
+    for (int a = 0; a < 8; ++a) o.R[a] = Rf[a] - F[a];
+  } else {
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
-      r += K[sym8(a, b)] * Real(u[b]);
-      if (TRANS) r += M[sym8(a, b)] * Real(ud[b]);
+    for (int a = 0; a < 8; ++a) {
+      Real r = Real(0);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        r += K[sym8(a, b)] * Real(u[b]);
+        if (TRANS) r += M[sym8(a, b)] * Real(ud[b]);
+      }
+      o.R[a] = double(r) - F[a];
     }
-    o.R[a] = Real(double(r) - F[a]);
   }
 #pragma unroll
-  for (int i = 0; i < 36; ++i) o.K[i] = TRANS ? Real(beta) * K[i] + Real(alpha) * M[i] : Real(beta) * K[i];
+  for (int i = 0; i < 36; ++i)
+    o.K[i] = TRANS ? Real(beta) * K[i] + Real(alpha) * M[i] : (UNIT ? K[i] : Real(beta) * K[i]);
 }
 
 // Coefficients of one cell: k of the hex8 element, or of the six tets (element 6*cell + t).
@@ -435,11 +576,12 @@ __device__ __forceinline__ void load_coef(const Q1DiffParams &p, int64_t cell, C
 }
 
 // Cell routine of the element kind: one hex8 element, or the six tets of the cell.
-template <int KIND, typename Real, bool TRANS>
+template <int KIND, typename Real, bool TRANS, bool UNIT, class Mid>
 __device__ __forceinline__ void cell_element(const Q1DiffParams &p, const CellCoef<KIND> &c, const double (&X)[3][8],
-                                             const double (&u)[8], const double (&ud)[8], Hex8Out<Real, TRANS> &o) {
-  if constexpr (KIND == TXA_TET4) tet6_cell<Real, TRANS>(X, u, ud, c.k, p.rhoc, p.src, p.alpha, p.beta, o);
-  else hex8_element<Real, TRANS>(X, u, ud, c.k[0], p.rhoc, p.src, p.alpha, p.beta, o);
+                                             const double (&u)[8], const double (&ud)[8], Hex8Out<Real, TRANS> &o,
+                                             Mid &&mid) {
+  if constexpr (KIND == TXA_TET4) tet6_cell<Real, TRANS, UNIT>(X, u, ud, c.k, p.rhoc, p.src, p.alpha, p.beta, o, mid);
+  else hex8_element<Real, TRANS, UNIT>(X, u, ud, c.k[0], p.rhoc, p.src, p.alpha, p.beta, o, mid);
 }
 
 // Global-memory view of an output pointer.  Stores through a generic pointer become
@@ -648,9 +790,24 @@ __device__ __forceinline__ void take_corner(const Real *s, int col, double (&acc
     if (first) accn[st + 9 * bz] = hi;
     else accn[st + 9 * bz] += hi;
   }
-  rc += double(s[(36 + alo) * NE + col]);
-  if (CX == 1 && CY == 1) rn = double(s[(36 + ahi) * NE + col]);
-  else rn += double(s[(36 + ahi) * NE + col]);
+  if constexpr (!flux_residual<Real>()) {   // residual rows: FP64 values, 8 per element
+    const double *rs = reinterpret_cast<const double *>(s + 36 * NE);
+    rc += rs[alo * NE + col];
+    if (CX == 1 && CY == 1) rn = rs[ahi * NE + col];
+    else rn += rs[ahi * NE + col];
+  }
+}
+
+// Residual of the FP32 form: the stage holds, per element column c, the x-pair sums
+// P_c[by + 2 bz] = R_c[1 + 2by + 4bz] + R_{c+1}[2by + 4bz] (FP64) of the element and its
+// +x neighbour, so the owner at corner (1,1) of column c adds two values per vertex layer.
+template <typename Real>
+__device__ __forceinline__ void take_pairs(const Real *s, int col, double &rc, double &rn) {
+  if constexpr (flux_residual<Real>()) {
+    const double *ps = reinterpret_cast<const double *>(s + 36 * NE);
+    rc += ps[1 * NE + col] + ps[0 * NE + col + TX];
+    rn = ps[3 * NE + col] + ps[2 * NE + col + TX];
+  }
 }
 
 // Vertex data of one face (4 vertices) of an element column.
@@ -687,7 +844,8 @@ __device__ __forceinline__ void load_face_lids(const Q1DiffParams &p, int64_t n0
 // layer's row stores also waits for those stores to complete, which held each
 // owner wave to one layer of stores in flight.  The element waves therefore load
 // the owners' LIDs, row pointers and slot maps and publish them with the layer.
-template <int KIND, typename Real, bool TRANS>
+// UNIT: the steady tangent with beta = 1 (the element matrix is stored unscaled).
+template <int KIND, typename Real, bool TRANS, bool UNIT>
 __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by, int bz, Real *stage,
                                            double *rowbuf, uint32_t *meta) {
   constexpr int NSLOT = stage_slots<Real>();
@@ -737,21 +895,26 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
       RowMeta om = load_meta_flat(p, Lo);
       const int32_t Lo_next = own_lid(ez + 1);
       Hex8Out<Real, TRANS> o;
-      Real done = Real(0);   // set from the layer's last result
+      double done = 0.0;   // set from the layer's last result
       if (ez <= ez1) {
         if (evalid) {
-          if (kPrefetch) {
-            // prefetch the face above the top one (layer ez+2), the LIDs beyond it and the next
-            // element's coefficients, with the layers clamped to the mesh (what lies past it is
-            // loaded again and never used).  No branch: a conditional load would be copied,
-            // and so waited for, where the branch joins.
-            load_face(p, col0 + sxy * min(ez + 2, p.nz), sx, Ln, fn);
-            load_face_lids(p, col0 + sxy * min(ez + 3, p.nz), sx, Ln);
-            load_coef<KIND>(p, cell0 + sz * min(ez + 1, p.nz - 1), kn);
-            // keep these loads ahead of the element work: the scheduler would otherwise sink
-            // them toward their use at the end of the layer and expose their latency there
-            __builtin_amdgcn_sched_barrier(0);
-          } else {
+          // prefetch (FP32) of the face above the top one (layer ez+2), the LIDs beyond it and the
+          // next element's coefficients, with the layers clamped to the mesh (what lies past it is
+          // loaded again and never used).  No branch: a conditional load would be copied, and so
+          // waited for, where the branch joins.  Issued after the element's quadrature loop, whose
+          // geometry holds the most registers, and ahead of the rest of the element work.
+          auto prefetch = [&]() {
+            if constexpr (kPrefetch) {
+              __builtin_amdgcn_sched_barrier(0);   // not hoisted into the quadrature loop either
+              load_face(p, col0 + sxy * min(ez + 2, p.nz), sx, Ln, fn);
+              load_face_lids(p, col0 + sxy * min(ez + 3, p.nz), sx, Ln);
+              load_coef<KIND>(p, cell0 + sz * min(ez + 1, p.nz - 1), kn);
+              // keep these loads ahead of the rest of the element work: the scheduler would
+              // otherwise sink them toward their use at the end of the layer
+              __builtin_amdgcn_sched_barrier(0);
+            }
+          };
+          if (!kPrefetch) {
             load_coef<KIND>(p, cell0 + sz * ez, kc);
             if (ez > ez0) {
               // no register room for a prefetch (FP64): load the top face now
@@ -768,16 +931,25 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
             u[v] = fb.u[v]; u[v + 4] = ft.u[v];
             ud[v] = fb.ud[v]; ud[v + 4] = ft.ud[v];
           }
-          cell_element<KIND, Real, TRANS>(p, kc, X, u, ud, o);
+          cell_element<KIND, Real, TRANS, UNIT>(p, kc, X, u, ud, o, prefetch);
           fb = ft;
           if (kPrefetch) { ft = fn; kc = kn; }
         } else {
 #pragma unroll
           for (int i = 0; i < 36; ++i) o.K[i] = Real(0);
 #pragma unroll
-          for (int i = 0; i < 8; ++i) o.R[i] = Real(0);
+          for (int i = 0; i < 8; ++i) o.R[i] = 0.0;
         }
-        done = o.R[7];
+        if constexpr (flux_residual<Real>()) {
+          // x-pair sums of the residual rows with the +x neighbour (the next lane of the
+          // element row; the last column's pairs are not read)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const int v1 = 1 + 2 * (j & 1) + 4 * (j >> 1);
+            o.R[j] = o.R[v1] + __shfl_down(o.R[v1 - 1], 1, TX);
+          }
+        }
+        done = o.R[3];
       }
       // the metadata registers are first read here, after the element work (an input the
       // work produces orders it), so the wave waits for those loads only now; earlier
@@ -791,7 +963,8 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
 #pragma unroll
         for (int i = 0; i < 36; ++i) st[i * NE + t] = o.K[i];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) st[(36 + i) * NE + t] = o.R[i];
+        for (int i = 0; i < (flux_residual<Real>() ? 4 : 8); ++i)
+          reinterpret_cast<double *>(st + 36 * NE)[i * NE + t] = o.R[i];
       }
       if (t < NOWN) put_meta(meta + sl * MSLOT, t, om);
       Lo = Lo_next;
@@ -821,6 +994,7 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
           take_corner<0, 1>(st, t + 1, acc, accn, rc, rn);
           take_corner<1, 0>(st, t + TX, acc, accn, rc, rn);
           take_corner<0, 0>(st, t + TX + 1, acc, accn, rc, rn);
+          take_pairs(st, t, rc, rn);
         }
         if (e >= k0) emit_rows<TXA_Q1_ALIGNED_STORES>(p, m, acc, rc, rowbuf, t / TX);
 #pragma unroll
@@ -840,199 +1014,13 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
 }
 
 
-// One-role variant: every thread computes an element and owns a vertex column.
-// The vertex data of the tile's faces ((TX+1) x (TY+1) columns) is staged in an LDS ring
-// of two z faces; the face two layers up is loaded into registers at the start
-// of a layer and stored into the ring after the element work, so every global
-// load of a layer is issued before that layer's row stores and no wait on a
-// load also waits for fresh stores.
-constexpr int FW = TX + 1, FC = FW * (TY + 1);     // vertex columns of the tile
-template <bool TRANS>
-constexpr int ring_nval() { return TRANS ? 5 : 4; }  // x, y, z, u (, udot)
-template <typename Real>
-constexpr int ring_union_doubles() {
-  return (NSTAGE * NE * static_cast<int>(sizeof(Real)) + 7) / 8 > TY * ROWBUF
-             ? (NSTAGE * NE * static_cast<int>(sizeof(Real)) + 7) / 8
-             : TY * ROWBUF;
-}
-
-template <int KIND, typename Real, bool TRANS>
-__device__ __forceinline__ void ring_block(const Q1DiffParams &p, int bx, int by, int bz, double *lds,
-                                           double *facebuf) {
-  constexpr int NVAL = ring_nval<TRANS>();
-  // element stage, then the row buffers, in `lds`; faces [2][NVAL][FC] in `facebuf`
-  Real *stage = reinterpret_cast<Real *>(lds);
-  auto face = [&](int slot, int i, int c) -> double & { return facebuf[(slot * NVAL + i) * FC + c]; };
-  const int t = threadIdx.x, tx = t % TX, ty = t / TX;
-  const int x0 = -1 + bx * (TX - 1);
-  const int y0 = -1 + by * (TY - 1);
-  const int64_t sx = p.nx + 1, sxy = static_cast<int64_t>(p.nx + 1) * (p.ny + 1);
-  const int k0 = bz * p.zc;
-  const int k1 = min(k0 + p.zc, p.nz + 1);
-  const int ez0 = max(k0 - 1, 0), ez1 = min(k1 - 1, p.nz - 1);
-
-  const int ex = x0 + tx, ey = y0 + ty;
-  const bool evalid = ex >= 0 && ex < p.nx && ey >= 0 && ey < p.ny;
-  const int oi = ex + 1, oj = ey + 1;
-  const bool owner = tx < TX - 1 && ty < TY - 1 && oi <= p.nx && oj <= p.ny;
-  const int64_t ocol = owner ? oi + sx * oj : 0;
-
-  // face columns loaded by this thread: c = t and c = t + NE
-  int64_t vcol[2];
-  bool vval[2];
-#pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const int c = t + r * NE;
-    const int vx = x0 + c % FW, vy = y0 + c / FW;
-    vval[r] = c < FC && vx >= 0 && vx <= p.nx && vy >= 0 && vy <= p.ny;
-    vcol[r] = vval[r] ? vx + sx * vy : 0;
-  }
-  auto load_col = [&](int r, int z, int32_t L, double (&v)[NVAL]) {
-    const int64_t n = vcol[r] + sxy * z;
-    v[0] = p.X[n]; v[1] = p.Y[n]; v[2] = p.Z[n];
-    v[3] = p.x[L];
-    if (TRANS) v[NVAL - 1] = p.xdot[L];
-  };
-  auto store_col = [&](int r, int slot, const double (&v)[NVAL]) {
-    const int c = t + r * NE;
-    if (c < FC)
-#pragma unroll
-      for (int i = 0; i < NVAL; ++i) face(slot, i, c) = vval[r] ? v[i] : 0.0;
-  };
-  int32_t lq[2] = {0, 0};   // LIDs of the face two layers up
-#pragma unroll
-  for (int z = 0; z < 2; ++z)
-#pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      double v[NVAL] = {};
-      if (vval[r]) load_col(r, ez0 + z, p.lid[vcol[r] + sxy * (ez0 + z)], v);
-      store_col(r, (ez0 + z) & 1, v);
-    }
-#pragma unroll
-  for (int r = 0; r < 2; ++r)
-    if (vval[r] && ez0 + 2 <= p.nz) lq[r] = p.lid[vcol[r] + sxy * (ez0 + 2)];
-
-  double acc[27], accn[18];
-  double rc = 0.0, rn = 0.0;
-#pragma unroll
-  for (int s = 0; s < 27; ++s) acc[s] = 0.0;
-#pragma unroll
-  for (int s = 0; s < 18; ++s) accn[s] = 0.0;
-  int32_t Lcur = owner ? p.lid[ocol + sxy * ez0] : -1;
-  __syncthreads();
-
-#if TXA_Q1_UNROLL2
-#pragma unroll 2
-#endif
-  for (int ez = ez0; ez <= ez1; ++ez) {
-    // loads of this layer, all before its stores: the face two layers up, the LIDs
-    // beyond it, the owner's row pointers and slot map, the owner's next LID
-    const bool more = ez + 2 <= p.nz;
-    double fv[2][NVAL];
-    if (more) {
-#pragma unroll
-      for (int r = 0; r < 2; ++r)
-        if (vval[r]) {
-          load_col(r, ez + 2, lq[r], fv[r]);
-          if (ez + 3 <= p.nz) lq[r] = p.lid[vcol[r] + sxy * (ez + 3)];
-        }
-    }
-    const RowMeta meta = load_meta(p, ez >= k0 ? Lcur : -1);
-    const int32_t Lnext = owner ? p.lid[ocol + sxy * (ez + 1)] : -1;
-
-    Hex8Out<Real, TRANS> o;
-    if (evalid) {
-      double X[3][8], u[8], ud[8];
-#pragma unroll
-      for (int v = 0; v < 8; ++v) {
-        const int c = (tx + (v & 1)) + FW * (ty + ((v >> 1) & 1));
-        const int slot = (ez + (v >> 2)) & 1;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) X[d][v] = face(slot, d, c);
-        u[v] = face(slot, 3, c);
-        ud[v] = TRANS ? face(slot, NVAL - 1, c) : 0.0;
-      }
-      CellCoef<KIND> kc;
-      load_coef<KIND>(p, ex + static_cast<int64_t>(p.nx) * (ey + static_cast<int64_t>(p.ny) * ez), kc);
-      cell_element<KIND, Real, TRANS>(p, kc, X, u, ud, o);
-    } else {
-#pragma unroll
-      for (int i = 0; i < 36; ++i) o.K[i] = Real(0);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) o.R[i] = Real(0);
-    }
-#pragma unroll
-    for (int i = 0; i < 36; ++i) stage[i * NE + t] = o.K[i];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) stage[(36 + i) * NE + t] = o.R[i];
-    __syncthreads();
-    if (owner) {
-      take_corner<1, 1>(stage, t, acc, accn, rc, rn);
-      take_corner<0, 1>(stage, t + 1, acc, accn, rc, rn);
-      take_corner<1, 0>(stage, t + TX, acc, accn, rc, rn);
-      take_corner<0, 0>(stage, t + TX + 1, acc, accn, rc, rn);
-    }
-    __syncthreads();   // the stage is free: row buffers and the next face may be written
-    if (more)
-#pragma unroll
-      for (int r = 0; r < 2; ++r) store_col(r, ez & 1, fv[r]);
-    if (ez >= k0) emit_rows<TXA_Q1_ALIGNED_STORES>(p, meta, acc, rc, lds, t / TX);
-#pragma unroll
-    for (int s = 0; s < 18; ++s) acc[s] = accn[s];
-    rc = rn;
-    Lcur = Lnext;
-    __syncthreads();   // row buffers copied out, next face visible
-  }
-  if (k1 == p.nz + 1) emit_rows<TXA_Q1_ALIGNED_STORES>(p, load_meta(p, Lcur), acc, rc, lds, t / TX);
-}
-
-
-#ifndef TXA_Q1_XCD_ORDER
-#define TXA_Q1_XCD_ORDER 0   // 1: FP32 forms hand each XCD a contiguous run of tiles (neighbours share its L2)
-#endif
-
-// Tile of this workgroup.  Workgroups are dealt to the 8 XCDs round robin (linear id
-// mod 8); each XCD has its own L2.  XCD_ORDER hands XCD c the c-th contiguous run of
-// the tile order (x fastest, then y, then the z slab), which keeps tiles that share
-// halo vertices on one L2 at about the same time.  It gained 1.4% (FP32, 256^3) while
-// the owner waves loaded their own row metadata; with the store-only owner waves the
-// plain order is 2-3% faster, and the FP64 kernels were always slower with it.
-template <bool XCD_ORDER>
-__device__ __forceinline__ void tile_of_block(int &bx, int &by, int &bz) {
-  if (XCD_ORDER) {
-    constexpr int NXCD = 8;
-    const int gx = gridDim.x, gy = gridDim.y;
-    const int n = gx * gy * static_cast<int>(gridDim.z);
-    const int b = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
-    const int c = b % NXCD, i = b / NXCD, q = n / NXCD, r = n % NXCD;
-    const int t = c * q + min(c, r) + i;   // a bijection of [0, n): XCD c holds q + (c < r) blocks
-    bx = t % gx;
-    by = (t / gx) % gy;
-    bz = t / (gx * gy);
-  } else {
-    bx = blockIdx.x; by = blockIdx.y; bz = blockIdx.z;
-  }
-}
-
-template <int KIND, typename Real, bool TRANS>
+template <int KIND, typename Real, bool TRANS, bool UNIT>
 __global__ __launch_bounds__(NT) void k_q1_hex8_diffusion(Q1DiffParams p) {
-  __shared__ Real stage[stage_slots<Real>() * NSTAGE * NE];
+  __shared__ __align__(16) Real stage[stage_slots<Real>() * NSTAGE * NE];
   __shared__ double rowbuf[(TY - 1) * ROWBUF];
   __shared__ uint32_t meta[stage_slots<Real>() * MSLOT];
-  int bx, by, bz;
-  tile_of_block<TXA_Q1_XCD_ORDER && sizeof(Real) == 4>(bx, by, bz);
-  spec_block<KIND, Real, TRANS>(p, bx, by, bz, stage, rowbuf, meta);
+  spec_block<KIND, Real, TRANS, UNIT>(p, blockIdx.x, blockIdx.y, blockIdx.z, stage, rowbuf, meta);
 }
-
-template <int KIND, typename Real, bool TRANS>
-__global__ __launch_bounds__(NE) void k_q1_hex8_ring(Q1DiffParams p) {
-  __shared__ double lds[ring_union_doubles<Real>()];
-  __shared__ double facebuf[2 * ring_nval<TRANS>() * FC];
-  int bx, by, bz;
-  tile_of_block<TXA_Q1_XCD_ORDER && sizeof(Real) == 4>(bx, by, bz);
-  ring_block<KIND, Real, TRANS>(p, bx, by, bz, lds, facebuf);
-}
-
 
 // ============================================================================
 // Batched assembly: many meshes in one persistent launch
@@ -1067,27 +1055,9 @@ __device__ __forceinline__ void leave_queue(BatchQueue *q) {
 // One element kind per launch: a kernel holding both bodies needs the register
 // budget of the larger one plus spills.
 template <int KIND, typename Real, bool TRANS>
-__global__ __launch_bounds__(NE) void k_q1_batch_ring(const Q1DiffParams *__restrict__ prm,
-                                                      const int4 *__restrict__ items, int nitems, BatchQueue *q) {
-  __shared__ double lds[ring_union_doubles<Real>()];
-  __shared__ double facebuf[2 * ring_nval<TRANS>() * FC];
-  __shared__ int slot;
-  for (;;) {
-    const int it = fetch_item(q, &slot);
-    if (it >= nitems) break;
-    const int4 w = items[it];
-    const Q1DiffParams &p = prm[w.x];
-    if (w.w * p.zc > p.nz) continue;   // slab past the mesh: nothing to do (uniform over the block)
-    ring_block<KIND, Real, TRANS>(p, w.y, w.z, w.w, lds, facebuf);
-    __syncthreads();
-  }
-  leave_queue(q);
-}
-
-template <int KIND, typename Real, bool TRANS>
 __global__ __launch_bounds__(NT) void k_q1_batch_spec(const Q1DiffParams *__restrict__ prm,
                                                       const int4 *__restrict__ items, int nitems, BatchQueue *q) {
-  __shared__ Real stage[stage_slots<Real>() * NSTAGE * NE];
+  __shared__ __align__(16) Real stage[stage_slots<Real>() * NSTAGE * NE];
   __shared__ double rowbuf[(TY - 1) * ROWBUF];
   __shared__ uint32_t meta[stage_slots<Real>() * MSLOT];
   __shared__ int slot;
@@ -1097,7 +1067,7 @@ __global__ __launch_bounds__(NT) void k_q1_batch_spec(const Q1DiffParams *__rest
     const int4 w = items[it];
     const Q1DiffParams &p = prm[w.x];
     if (w.w * p.zc > p.nz) continue;   // slab past the mesh: nothing to do (uniform over the block)
-    spec_block<KIND, Real, TRANS>(p, w.y, w.z, w.w, stage, rowbuf, meta);
+    spec_block<KIND, Real, TRANS, false>(p, w.y, w.z, w.w, stage, rowbuf, meta);
     __syncthreads();
   }
   leave_queue(q);
@@ -1115,34 +1085,36 @@ struct BcItem {
 };
 
 // Dirichlet rows of every mesh of a batch: blockIdx.x = mesh, blockIdx.y strides its rows.
-__global__ void k_dirichlet_batch(const BcItem *__restrict__ items) {
+// A LID outside a mesh's map touches no row and sets *err.
+__global__ void k_dirichlet_batch(const BcItem *__restrict__ items, int *err) {
   const BcItem b = items[blockIdx.x];
   const int lane = threadIdx.x % 64;
   for (int64_t i = static_cast<int64_t>(blockIdx.y) * (blockDim.x / 64) + threadIdx.x / 64; i < b.n;
        i += static_cast<int64_t>(gridDim.y) * (blockDim.x / 64)) {
     const int32_t r = b.lids[i];
-    if (r < 0 || r >= b.nrows) continue;
+    if (r < 0 || r >= b.nrows) {
+      if (lane == 0 && err) *err = 1;
+      continue;
+    }
     if (lane == 0 && b.f) b.f[r] = b.x[r] - (b.g ? b.g[i] : 0.0);
     if (b.vals)
       for (int64_t k = b.rowptr[r] + lane; k < b.rowptr[r + 1]; k += 64) b.vals[k] = b.col[k] == r ? 1.0 : 0.0;
   }
 }
 
-// FP32 integration: one-role kernel with the LDS face ring (2 workgroups per CU);
-// FP64: element and owner waves split (the FP64 element state leaves no room for the ring)
+// Element and owner waves split in both precisions (the stage is double-buffered in FP32).
 template <int KIND>
 void launch_kind(const Q1DiffParams &p, dim3 grid, int precision, bool trans, hipStream_t s) {
+  const bool unit = p.beta == 1.0;
   if (precision == TXA_FP64) {
-    if (trans) hipLaunchKernelGGL((k_q1_hex8_diffusion<KIND, double, true>), grid, dim3(NT), 0, s, p);
-    else hipLaunchKernelGGL((k_q1_hex8_diffusion<KIND, double, false>), grid, dim3(NT), 0, s, p);
+    if (trans) hipLaunchKernelGGL((k_q1_hex8_diffusion<KIND, double, true, false>), grid, dim3(NT), 0, s, p);
+    else if (unit) hipLaunchKernelGGL((k_q1_hex8_diffusion<KIND, double, false, true>), grid, dim3(NT), 0, s, p);
+    else hipLaunchKernelGGL((k_q1_hex8_diffusion<KIND, double, false, false>), grid, dim3(NT), 0, s, p);
   } else {
-#if TXA_Q1_FP32_RING
-    if (trans) hipLaunchKernelGGL((k_q1_hex8_ring<KIND, float, true>), grid, dim3(NE), 0, s, p);
-    else hipLaunchKernelGGL((k_q1_hex8_ring<KIND, float, false>), grid, dim3(NE), 0, s, p);
-#else
-    if (trans) hipLaunchKernelGGL((k_q1_hex8_diffusion<KIND, float, true>), grid, dim3(NT), 0, s, p);
-    else hipLaunchKernelGGL((k_q1_hex8_diffusion<KIND, float, false>), grid, dim3(NT), 0, s, p);
-#endif
+    // FP32 scales by beta in every case: without the 36 multiplies the compiler's schedule
+    // of the unit-beta instance spills (47 registers) where the scaled one fits in 240
+    if (trans) hipLaunchKernelGGL((k_q1_hex8_diffusion<KIND, float, true, false>), grid, dim3(NT), 0, s, p);
+    else hipLaunchKernelGGL((k_q1_hex8_diffusion<KIND, float, false, false>), grid, dim3(NT), 0, s, p);
   }
 }
 
@@ -1285,18 +1257,13 @@ void launch_batch(const Batch &b, dim3 g, const int4 *items, int n, BatchQueue *
     if (b.transient) hipLaunchKernelGGL((k_q1_batch_spec<KIND, double, true>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
     else hipLaunchKernelGGL((k_q1_batch_spec<KIND, double, false>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
   } else {
-#if TXA_Q1_FP32_RING
-    if (b.transient) hipLaunchKernelGGL((k_q1_batch_ring<KIND, float, true>), g, dim3(NE), 0, s, b.prm.p, items, n, q);
-    else hipLaunchKernelGGL((k_q1_batch_ring<KIND, float, false>), g, dim3(NE), 0, s, b.prm.p, items, n, q);
-#else
     if (b.transient) hipLaunchKernelGGL((k_q1_batch_spec<KIND, float, true>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
     else hipLaunchKernelGGL((k_q1_batch_spec<KIND, float, false>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
-#endif
   }
   TXA_HIP(hipGetLastError());
 }
 
-void batch_enqueue(Batch &b, hipStream_t s) {
+void batch_enqueue(Batch &b, hipStream_t s, int *err) {
   const size_t n = b.meshes.size();
   for (size_t i = 0; i < n; ++i) {
     const Q1DiffParams &p = b.prm_h[i];
@@ -1315,7 +1282,7 @@ void batch_enqueue(Batch &b, hipStream_t s) {
     if (k == 0) launch_batch<TXA_HEX8>(b, g, it[k], nk[k], q, s);
     else launch_batch<TXA_TET4>(b, g, it[k], nk[k], q, s);
   }
-  hipLaunchKernelGGL(k_dirichlet_batch, dim3(static_cast<unsigned>(n), 8), dim3(256), 0, s, b.bc.p);
+  hipLaunchKernelGGL(k_dirichlet_batch, dim3(static_cast<unsigned>(n), 8), dim3(256), 0, s, b.bc.p, err);
   TXA_HIP(hipGetLastError());
 }
 
@@ -1355,7 +1322,7 @@ Batch *batch_create(const std::vector<Mesh *> &meshes, const std::vector<int> &i
 
 void batch_free(Batch *b) { delete b; }
 
-void batch_assemble(Batch &b, const txa_assembly_args *args, bool use_graph, hipStream_t s) {
+void batch_assemble(Batch &b, const txa_assembly_args *args, bool use_graph, hipStream_t s, int *err) {
   TXA_REQUIRE(args, TXA_ERR_INVALID, "null args");
   // the previous launch must be done reading the pinned parameter blocks
   if (b.launched) TXA_HIP(hipEventSynchronize(b.done));
@@ -1363,13 +1330,13 @@ void batch_assemble(Batch &b, const txa_assembly_args *args, bool use_graph, hip
   if (use_graph) {
     if (!b.exec) {
       TXA_HIP(hipStreamBeginCapture(b.cap, hipStreamCaptureModeThreadLocal));
-      batch_enqueue(b, b.cap);
+      batch_enqueue(b, b.cap, err);
       TXA_HIP(hipStreamEndCapture(b.cap, &b.graph));
       TXA_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
     }
     TXA_HIP(hipGraphLaunch(b.exec, s));
   } else {
-    batch_enqueue(b, s);
+    batch_enqueue(b, s, err);
   }
   TXA_HIP(hipEventRecord(b.done, s));
   b.launched = true;
@@ -1377,7 +1344,7 @@ void batch_assemble(Batch &b, const txa_assembly_args *args, bool use_graph, hip
 
 // Autotuner: times the batch over slab heights (the analogue of Panzer's workset
 // size sweep) and persistent grid sizes, keeps the fastest.  out: best ms.
-double batch_tune(Batch &b, const txa_assembly_args *args, hipStream_t s, int reps) {
+double batch_tune(Batch &b, const txa_assembly_args *args, hipStream_t s, int reps, int *err) {
   int dev = 0, ncu = 0;
   TXA_HIP(hipGetDevice(&dev));
   TXA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1395,9 +1362,9 @@ double batch_tune(Batch &b, const txa_assembly_args *args, hipStream_t s, int re
     batch_build_items(b, sl);
     for (int g : grids) {
       b.grid = g;
-      batch_enqueue(b, s);   // warm-up
+      batch_enqueue(b, s, err);   // warm-up
       TXA_HIP(hipEventRecord(e0, s));
-      for (int r = 0; r < reps; ++r) batch_enqueue(b, s);
+      for (int r = 0; r < reps; ++r) batch_enqueue(b, s, err);
       TXA_HIP(hipEventRecord(e1, s));
       TXA_HIP(hipEventSynchronize(e1));
       float ms = 0.f;

@@ -92,9 +92,11 @@ struct Request {
   bool has_mem = false, has_ws = false;
   int slot = -1;
   int status = QUEUED;
+  int error_code = TXA_ERR_DEVICE;
   std::string error;
   Clock::time_point t_submit, t_start, t_done;
-  hipEvent_t ev = nullptr;
+  hipEvent_t ev = nullptr;      // completion (after the Dirichlet report is copied back)
+  hipEvent_t ready = nullptr;   // the caller's producer stream at submission
 };
 
 }  // namespace
@@ -106,6 +108,8 @@ struct Server {
   Arena arena;
   std::vector<hipStream_t> streams;
   std::vector<int64_t> slot_req;   // request running on each slot, -1 = free
+  DevBuf<int> bc_flag;             // [slots] Dirichlet-LID-outside-the-map flag of the slot's request
+  int *bc_flag_h = nullptr;        // pinned copy, written back on the slot stream before completion
   std::mutex mu;
   std::condition_variable cv;      // wakes the worker (new work, stop)
   std::condition_variable done_cv; // wakes waiters (a request finished)
@@ -127,9 +131,13 @@ struct Server {
     cv.notify_all();
     if (worker.joinable()) worker.join();
     (void)hipSetDevice(device);
-    for (auto &kv : reqs)
+    for (hipStream_t s : streams) (void)hipStreamSynchronize(s);
+    for (auto &kv : reqs) {
       if (kv.second->ev) (void)hipEventDestroy(kv.second->ev);
+      if (kv.second->ready) (void)hipEventDestroy(kv.second->ready);
+    }
     for (hipStream_t s : streams) (void)hipStreamDestroy(s);
+    if (bc_flag_h) (void)hipHostFree(bc_flag_h);
   }
 
   double *fptr(const Request &q) { return reinterpret_cast<double *>(arena.mem.p + q.off_f); }
@@ -181,7 +189,11 @@ struct Server {
     q.slot = s;
     q.t_start = Clock::now();
     try {
-      assemble(*q.mesh, a, streams[s], q.has_ws ? arena.mem.p + q.off_w : nullptr);
+      // after the caller's producer stream; the slot's Dirichlet flag cleared, reported back
+      TXA_HIP(hipStreamWaitEvent(streams[s], q.ready, 0));
+      TXA_HIP(hipMemsetAsync(bc_flag.p + s, 0, sizeof(int), streams[s]));
+      assemble(*q.mesh, a, streams[s], q.has_ws ? arena.mem.p + q.off_w : nullptr, bc_flag.p + s);
+      TXA_HIP(hipMemcpyAsync(bc_flag_h + s, bc_flag.p + s, sizeof(int), hipMemcpyDeviceToHost, streams[s]));
       TXA_HIP(hipEventRecord(q.ev, streams[s]));
       q.status = RUNNING;
       slot_req[s] = q.id;
@@ -209,7 +221,13 @@ struct Server {
         if (e == hipErrorNotReady) continue;
         q.t_done = Clock::now();
         release_ws(q);
-        if (e == hipSuccess) {
+        if (e == hipSuccess && bc_flag_h[s]) {
+          q.status = FAILED;
+          q.error_code = TXA_ERR_INVALID;
+          q.error = "a Dirichlet LID of the request lies outside the DOF map (its row was not applied)";
+          ++n_failed;
+          release_mem(q);
+        } else if (e == hipSuccess) {
           q.status = DONE;
           lat_ms.push_back(std::chrono::duration<double, std::milli>(q.t_done - q.t_submit).count());
           elements_done += q.mesh->nelem;
@@ -281,6 +299,10 @@ Server *serve_create(int device, const std::vector<Mesh *> &meshes, const std::v
   s->arena.init(static_cast<size_t>(budget));
   s->streams.resize(slots);
   for (auto &st : s->streams) TXA_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  s->bc_flag.alloc(slots);
+  TXA_HIP(hipMemset(s->bc_flag.p, 0, sizeof(int) * slots));
+  TXA_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->bc_flag_h), sizeof(int) * slots, hipHostMallocDefault));
+  for (int i = 0; i < slots; ++i) s->bc_flag_h[i] = 0;
   s->slot_req.assign(slots, -1);
   Server *raw = s.get();
   s->worker = std::thread([raw] { raw->run(); });
@@ -311,6 +333,8 @@ int64_t serve_submit(Server &s, const txa_serve_request &r) {
   TXA_REQUIRE(round_up(q->bytes_f) + round_up(q->bytes_v) + round_up(q->bytes_w) <= s.arena.cap, TXA_ERR_NOMEM,
               "request footprint exceeds the server's HBM budget");
   TXA_HIP(hipEventCreateWithFlags(&q->ev, hipEventDisableTiming));
+  TXA_HIP(hipEventCreateWithFlags(&q->ready, hipEventDisableTiming));
+  TXA_HIP(hipEventRecord(q->ready, static_cast<hipStream_t>(r.stream)));
   std::lock_guard<std::mutex> lk(s.mu);
   TXA_REQUIRE(!s.stop, TXA_ERR_LOGIC, "server is stopping");
   q->id = s.next_id++;
@@ -343,7 +367,7 @@ void serve_poll(Server &s, int64_t id, double timeout_s, int *status, const doub
                        [&] { return q.status == DONE || q.status == FAILED; });
   }
   if (status) *status = q.status;
-  if (q.status == FAILED) throw Error(TXA_ERR_DEVICE, "request " + std::to_string(id) + " failed: " + q.error);
+  if (q.status == FAILED) throw Error(q.error_code, "request " + std::to_string(id) + " failed: " + q.error);
   if (f) *f = q.status == DONE ? s.fptr(q) : nullptr;
   if (vals) *vals = q.status == DONE ? s.vptr(q) : nullptr;
 }
@@ -354,6 +378,7 @@ void serve_release(Server &s, int64_t id) {
   TXA_REQUIRE(q.status == DONE || q.status == FAILED, TXA_ERR_LOGIC, "request still queued or running");
   s.release_mem(q);
   if (q.ev) (void)hipEventDestroy(q.ev);
+  if (q.ready) (void)hipEventDestroy(q.ready);
   s.reqs.erase(id);
   s.cv.notify_all();
 }
