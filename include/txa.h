@@ -184,6 +184,17 @@ typedef struct {
 
 int txa_assemble(txa_ctx ctx, int mesh_id, const txa_assembly_args *args, void *stream);
 
+/* txa_assemble without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The assembly workspace (the
+ * element blocks of the two-phase elasticity kernels and the element matrices of
+ * the uploaded-mesh row gather) comes from one static device array of 128 MiB per
+ * device instead of the context's pool; an assembly that needs more is refused with
+ * TXA_ERR_NOMEM before anything is launched.  Results equal txa_assemble's.  Every
+ * call on one device shares that array: calls whose work overlaps on one GPU (other
+ * streams, other contexts, other threads) race and must be ordered by the caller. */
+int txa_assemble_noalloc(txa_ctx ctx, int mesh_id, const txa_assembly_args *args, void *stream);
+
 /* Batched assembly of many resident meshes (HEX8 and TET4, Q1 diffusion) in one
  * persistent launch: every mesh contributes its tiles to one work list that the
  * launch drains, then one Dirichlet launch covers all meshes.  With use_graph

@@ -102,6 +102,7 @@ def load_library(path: str = LIB_PATH):
     L.txa_graph_copy.argtypes = [P, ctypes.c_int, P, P]
     L.txa_graph_device.argtypes = [P, ctypes.c_int, ctypes.POINTER(P), ctypes.POINTER(P)]
     L.txa_assemble.argtypes = [P, ctypes.c_int, ctypes.POINTER(AssemblyArgs), P]
+    L.txa_assemble_noalloc.argtypes = [P, ctypes.c_int, ctypes.POINTER(AssemblyArgs), P]
     L.txa_batch_create.argtypes = [P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.txa_batch_destroy.argtypes = [P, ctypes.c_int]
     L.txa_batch_assemble.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P]
@@ -322,6 +323,12 @@ class BrickMesh:
         check_assembly_tensors(self, self.eng.device, **kw)
         a = make_args(**kw)
         self.eng.check(self.eng.lib.txa_assemble(self.eng.h, self.id, ctypes.byref(a), _stream(stream)))
+
+    def assemble_noalloc(self, *, stream=None, **kw):
+        """As assemble, through txa_assemble_noalloc (workspace in the library's static array)."""
+        check_assembly_tensors(self, self.eng.device, **kw)
+        a = make_args(**kw)
+        self.eng.check(self.eng.lib.txa_assemble_noalloc(self.eng.h, self.id, ctypes.byref(a), _stream(stream)))
 
     def destroy(self):
         if self.id is not None:

@@ -461,6 +461,20 @@ __global__ __launch_bounds__(256) void k_lane_gather(int64_t nrows, const int64_
   }
 }
 
+// Zero fill of an output the atomic kernels add into (a kernel rather than hipMemsetAsync,
+// so the assembly can be recorded in a HIP graph).
+__global__ void k_zero(double *__restrict__ p, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    p[i] = 0.0;
+}
+
+void zero_async(double *p, int64_t n, hipStream_t s) {
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_zero, dim3(static_cast<unsigned>(std::max<int64_t>(blocks, 1))), dim3(256), 0, s, p, n);
+  TXA_HIP(hipGetLastError());
+}
+
 // ============================================================================
 // Dirichlet rows: f_i = x_i - g_i; matrix row -> identity row
 // (disc-fe/src/lof/Panzer_TpetraLinearObjContainer.hpp:228-237, 300-318)
@@ -639,8 +653,8 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws, int 
     // the Jacobian of the lane kernel by the row gather: element matrices in ws, rows stored whole
     const bool gather = lane && a.vals != nullptr && ws != nullptr && lane_gather_mesh(m, a.physics, a.precision);
     p.ke = gather ? static_cast<double *>(ws) : nullptr;
-    if (a.f) TXA_HIP(hipMemsetAsync(a.f, 0, sizeof(double) * m.ndofs, s));
-    if (a.vals && !gather) TXA_HIP(hipMemsetAsync(a.vals, 0, sizeof(double) * m.nnz, s));
+    if (a.f) zero_async(a.f, m.ndofs, s);
+    if (a.vals && !gather) zero_async(a.vals, m.nnz, s);
     const unsigned lblocks = static_cast<unsigned>((m.nelem + 255) / 256);
     if (lane && m.topo == TXA_HEX8 && a.transient)
       hipLaunchKernelGGL((k_generic_lane<8, 8, true>), dim3(lblocks), dim3(256), 0, s, p);

@@ -1,6 +1,7 @@
 // C ABI of the assembly engine: status codes + per-context last-error string,
 // no exception crosses the boundary.
 #include <cstring>
+#include <mutex>
 #include <new>
 
 #include "txa_internal.hpp"
@@ -14,6 +15,26 @@ struct txa_context : Context {};
 namespace {
 
 thread_local std::string g_create_error;
+
+// Workspace of txa_assemble_noalloc: one static device array, so that entry point allocates
+// nothing.  Each device the library runs on holds its own copy until the process ends.
+constexpr size_t kScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) char scratch[kScratchBytes];
+
+// The device address of this device's copy, looked up once per device with the device
+// current (host code must not use the array's name as an address).
+void *scratch_address(int device) {
+  static std::mutex mu;
+  static std::map<int, void *> addr;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = addr.find(device);
+  if (it != addr.end()) return it->second;
+  void *p = nullptr;
+  TXA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  TXA_REQUIRE(p != nullptr, TXA_ERR_DEVICE, "no device address for the static workspace");
+  addr[device] = p;
+  return p;
+}
 
 template <class F>
 int guarded(txa_ctx ctx, F &&f) {
@@ -76,6 +97,7 @@ int txa_create(int device, txa_ctx *out) {
   try {
     c->err.alloc(1);
     TXA_HIP(hipMemset(c->err.p, 0, sizeof(int)));
+    c->scratch = scratch_address(device);
   } catch (const Error &e) {
     g_create_error = e.what();
     delete c;
@@ -244,6 +266,21 @@ int txa_assemble(txa_ctx ctx, int mesh_id, const txa_assembly_args *args, void *
       throw;
     }
     if (ws) TXA_HIP(hipFreeAsync(ws, s));
+  });
+}
+
+int txa_assemble_noalloc(txa_ctx ctx, int mesh_id, const txa_assembly_args *args, void *stream) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(args, TXA_ERR_INVALID, "null args");
+    Mesh &m = ctx->mesh(mesh_id);
+    const size_t wsb = txa::assemble_workspace_bytes(m, args->physics, args->precision, args->f != nullptr,
+                                                     args->vals != nullptr);
+    TXA_REQUIRE(wsb <= kScratchBytes, TXA_ERR_NOMEM,
+                "txa_assemble_noalloc: this assembly needs " + std::to_string(wsb) +
+                    " bytes of workspace; the static workspace holds " + std::to_string(kScratchBytes) +
+                    " (use txa_assemble)");
+    TXA_REQUIRE(ctx->scratch != nullptr, TXA_ERR_LOGIC, "static workspace not set up");
+    txa::assemble(m, *args, static_cast<hipStream_t>(stream), wsb ? ctx->scratch : nullptr, ctx->err.p);
   });
 }
 

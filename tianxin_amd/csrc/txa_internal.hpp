@@ -112,6 +112,8 @@ struct Context {
   // device flag the assembly kernels set when a Dirichlet LID lies outside the DOF map;
   // txa_sync reports and clears it (the reference throws at that point)
   DevBuf<int> err;
+  // this device's copy of the static workspace of txa_assemble_noalloc (its device address)
+  void *scratch = nullptr;
   ~Context() {
     for (auto &kv : servers) serve_free(kv.second);   // joins the worker threads first
     for (auto &kv : batches) batch_free(kv.second);
