@@ -51,6 +51,7 @@ extern "C" {
 #define TXA_HEX8 1
 #define TXA_TET4 2
 #define TXA_HEX27 3
+#define TXA_WEDGE6 4   /* shards Wedge<6>: triangle 0,1,2, then 3,4,5 above it (HGRAD_WEDGE_C1) */
 
 /* physics */
 #define TXA_DIFFUSION 0       /* k grad(u).grad(v) + rho_c udot v - f v   (scalar) */
@@ -99,8 +100,32 @@ int txa_mesh_brick(txa_ctx ctx, int topo, int nx, int ny, int nz, const double *
  * (adapters-stk/src/Panzer_STKConnManager.cpp:243-300). */
 int txa_mesh_upload(txa_ctx ctx, int topo, int64_t nelem, const int64_t *conn, int64_t nentities, int64_t nverts,
                     const double *vertex_coords, int *mesh_id);
+/* One element block of a mesh: elements of one topology (TXA_HEX8, TXA_TET4, TXA_WEDGE6 or
+ * TXA_HEX27) with their connectivity, host [nelem][npe] as for txa_mesh_upload. */
+typedef struct {
+  int topo;
+  int64_t nelem;
+  const int64_t *conn;
+} txa_element_block;
+
+/* A mesh of several element blocks, of one or of different topologies, sharing the
+ * vertices (and entity ids) where they meet: one DOF map and one CRS graph over all of
+ * them.  Elements are numbered block after block in the order given (block b's element
+ * e is element offset_b + e: coefficient arrays run over that numbering), and the first
+ * touch that orders the LIDs walks the blocks in that order, as DOFManager walks its
+ * blockOrder_ (dof-mgr/src/Panzer_DOFManager.cpp:604-628).  txa_dofmap_copy's element
+ * LIDs are the blocks' [nelem][npe * nfields] arrays one after another.  Assembly runs
+ * each block with its own basis and cubature and sums into the same rows, as
+ * AssemblyEngine evaluates each block's volume field manager into one container
+ * (disc-fe/src/Panzer_AssemblyEngine_impl.hpp:134-180; element blocks of different
+ * topology: adapters-stk/src/Panzer_STK_WorksetFactory.cpp:279-292). */
+int txa_mesh_upload_blocks(txa_ctx ctx, int nblocks, const txa_element_block *blocks, int64_t nentities,
+                           int64_t nverts, const double *vertex_coords, int *mesh_id);
+/* out (host, 1 + 3 * nblocks entries): out[0] = blocks, then per block its topology,
+ * element count and first element.  cap: entries available in out. */
+int txa_mesh_blocks(txa_ctx ctx, int mesh_id, int64_t *out, int64_t cap);
 int txa_mesh_destroy(txa_ctx ctx, int mesh_id);
-/* out[0]=elements out[1]=connectivity entries/element out[2]=vertices */
+/* out[0]=elements out[1]=connectivity entries/element (0 when the blocks differ) out[2]=vertices */
 int txa_mesh_info(txa_ctx ctx, int mesh_id, int64_t *out);
 
 /* DOF map for `nfields` fields on every connectivity entity (1 = scalar,

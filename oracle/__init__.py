@@ -18,7 +18,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboracle.so")
 
-QUAD4, HEX8, TET4, HEX27 = 0, 1, 2, 3
+QUAD4, HEX8, TET4, HEX27, WEDGE6 = 0, 1, 2, 3, 4
 DIFFUSION, ELASTICITY = 0, 1
 MODE_FAD, MODE_EXPLICIT = 0, 1
 
@@ -133,6 +133,69 @@ def graph(elem_lid, nrows):
     if lib().orc_graph_cols(E, nd, _p(el), nrows, _p(rowptr), _p(col)):
         raise RuntimeError("oracle graph failed")
     return rowptr, col[:nnz]
+
+
+def dofmap_blocks(conns, nfields):
+    """DOF map over several element blocks: (owned GIDs, [elem_lid of each block]).
+
+    Restates DOFManager::buildGlobalUnknowns on one rank for a mesh of several blocks
+    (dof-mgr/src/Panzer_DOFManager.cpp:449-715): GIDs by the GUN rule over every entity id
+    any block uses (nf * rank of the id among the used ids + field, element-local order
+    entity-major then field, as orc_dofmap), owned_ by first touch walking blockOrder_
+    (the blocks in the order given), each block's elements in order and each element's
+    GIDs in order (:604-628), LID = position in owned_ (Panzer_GlobalIndexer.hpp:605-640)."""
+    conns = [np.ascontiguousarray(c, np.int64) for c in conns]
+    ids = np.unique(np.concatenate([c.ravel() for c in conns]))
+    egids = []
+    for c in conns:
+        rank = np.searchsorted(ids, c)
+        g = nfields * rank[:, :, None] + np.arange(nfields)[None, None, :]
+        egids.append(g.reshape(c.shape[0], c.shape[1] * nfields))
+    flat = np.concatenate([g.ravel() for g in egids])
+    uniq, first = np.unique(flat, return_index=True)
+    order = np.argsort(first, kind="stable")
+    owned = uniq[order]
+    lid_of_uniq = np.empty(uniq.size, np.int64)
+    lid_of_uniq[order] = np.arange(uniq.size)
+    els = [lid_of_uniq[np.searchsorted(uniq, g)].astype(np.int32) for g in egids]
+    return owned, els
+
+
+def graph_blocks(elem_lids, nrows):
+    """CRS graph over several blocks' element LIDs: row i holds the union of the closures of
+    every element (of any block) containing LID i, sorted (TpetraLinearObjFactory::buildGraph,
+    disc-fe/src/lof/Panzer_TpetraLinearObjFactory_impl.hpp:559-650, one insertGlobalIndices per
+    element row over all element blocks, then fillComplete's sorted, merged rows)."""
+    keys = []
+    for el in elem_lids:
+        el = np.asarray(el, np.int64)
+        r = np.repeat(el, el.shape[1], axis=1).ravel()
+        c = np.tile(el, (1, el.shape[1])).ravel()
+        keys.append(r * nrows + c)
+    k = np.unique(np.concatenate(keys))
+    rows, cols = k // nrows, (k % nrows).astype(np.int32)
+    rowptr = np.zeros(nrows + 1, np.int64)
+    np.cumsum(np.bincount(rows, minlength=nrows), out=rowptr[1:])
+    return rowptr, cols
+
+
+class BlockAssembler:
+    """Volume assembly of a mesh of several element blocks: one cached-workset Assembler per
+    block, every block's scatter adding into the same f and CRS values, as AssemblyEngine
+    evaluates the volume field manager of each block in turn into one container
+    (disc-fe/src/Panzer_AssemblyEngine_impl.hpp:134-180).  blocks: [(topo, elem_vert, elem_lid,
+    coef)], coefficients per element of the block."""
+
+    def __init__(self, physics, blocks, coords, rowptr, col, **kw):
+        self.nrows, self.nnz = rowptr.size - 1, int(rowptr[-1])
+        self.parts = [Assembler(t, physics, ev, coords, el, cf, rowptr, col, **kw) for t, ev, el, cf in blocks]
+
+    def evaluate(self, x, xdot=None, xddot=None, **kw):
+        f = np.zeros(self.nrows)
+        vals = np.zeros(max(self.nnz, 1))
+        for a in self.parts:
+            a.evaluate(x, xdot, xddot, f=f, vals=vals, **kw)
+        return f, vals[: self.nnz]
 
 
 def refelem(topo):

@@ -424,6 +424,41 @@ static void refelem_init(refelem *r, int topo)
     }
     return;
   }
+  if (topo == ORC_WEDGE6) {
+    /* HGRAD_WEDGE_C1 on shards Wedge<6> (Panzer_IntrepidBasisFactory.hpp:159-184 selects the
+     * Intrepid2 HGRAD basis of the cell topology): nodes (0,0,-1) (1,0,-1) (0,1,-1) and the same
+     * triangle at zeta = +1; N_i = lam_i (1 - zeta)/2, N_{i+3} = lam_i (1 + zeta)/2 with
+     * lam = (1 - xi - eta, xi, eta).  Cubature of degree 2 as a tensor rule (the reference's
+     * IntegrationRule asks Intrepid2 for degree 2, Panzer_IntegrationRule.cpp:130-166): the
+     * 3-point degree-2 triangle rule (1/6,1/6) (2/3,1/6) (1/6,2/3), weights 1/6, times 2-point
+     * Gauss in zeta, triangle point fastest. */
+    static const double tq[3][2] = {{1.0 / 6.0, 1.0 / 6.0}, {2.0 / 3.0, 1.0 / 6.0}, {1.0 / 6.0, 2.0 / 3.0}};
+    const double zg[2] = {-1.0 / sqrt(3.0), 1.0 / sqrt(3.0)};
+    r->dim = 3; r->nb = 6; r->nv = 6; r->nq = 6;
+    for (int k = 0; k < 2; ++k)
+      for (int t = 0; t < 3; ++t) {
+        const int q = 3 * k + t;
+        r->qp[q][0] = tq[t][0]; r->qp[q][1] = tq[t][1]; r->qp[q][2] = zg[k];
+        r->qw[q] = 1.0 / 6.0;
+      }
+    for (int q = 0; q < 6; ++q) {
+      const double x = r->qp[q][0], y = r->qp[q][1], z = r->qp[q][2];
+      const double lam[3] = {1.0 - x - y, x, y};
+      const double dlam[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
+      for (int l = 0; l < 2; ++l) {          /* l = 0: bottom (1 - zeta)/2, l = 1: top (1 + zeta)/2 */
+        const double h = l == 0 ? 0.5 * (1.0 - z) : 0.5 * (1.0 + z), dh = l == 0 ? -0.5 : 0.5;
+        for (int i = 0; i < 3; ++i) {
+          const int v = 3 * l + i;
+          r->N[v][q] = lam[i] * h;
+          r->dN[v][q][0] = dlam[i][0] * h;
+          r->dN[v][q][1] = dlam[i][1] * h;
+          r->dN[v][q][2] = lam[i] * dh;
+          for (int d = 0; d < 3; ++d) r->gdN[v][q][d] = r->dN[v][q][d];
+        }
+      }
+    }
+    return;
+  }
   /* TET4: linear basis, degree-2 four-point rule */
   r->dim = 3; r->nb = 4; r->nv = 4; r->nq = 4;
   double a = 0.5854101966249685, b = 0.1381966011250105;

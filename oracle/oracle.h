@@ -5,7 +5,7 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
-enum { ORC_QUAD4 = 0, ORC_HEX8 = 1, ORC_TET4 = 2, ORC_HEX27 = 3 };
+enum { ORC_QUAD4 = 0, ORC_HEX8 = 1, ORC_TET4 = 2, ORC_HEX27 = 3, ORC_WEDGE6 = 4 };
 enum { ORC_DIFFUSION = 0, ORC_ELASTICITY = 1 };
 enum { ORC_MODE_FAD = 0, ORC_MODE_EXPLICIT = 1 };
 

@@ -20,7 +20,8 @@ LIB_PATH = os.path.join(HERE, "libtxa.so")
 HEADER = os.path.join(ROOT, "include", "txa.h")
 
 TXA_OK, TXA_ERR_INVALID, TXA_ERR_LOGIC, TXA_ERR_DEVICE, TXA_ERR_NOMEM, TXA_ERR_UNSUPPORTED = range(6)
-HEX8, TET4, HEX27 = 1, 2, 3
+HEX8, TET4, HEX27, WEDGE6 = 1, 2, 3, 4
+NODES_PER_ELEMENT = {HEX8: 8, TET4: 4, HEX27: 27, WEDGE6: 6}
 DIFFUSION, ELASTICITY = 0, 1
 FP64, FP32, SPLIT_BF16 = 0, 1, 2
 # side sets of the brick (shards hex side ordinals): y = 0, x = max, y = max, x = 0, z = 0, z = max
@@ -51,6 +52,10 @@ class AssemblyArgs(ctypes.Structure):
         ("bc_lids", P), ("bc_vals", P), ("n_bc", I64),
         ("n_neumann", ctypes.c_int), ("neumann_sides", P), ("neumann_values", P),
     ]
+
+
+class ElementBlock(ctypes.Structure):
+    _fields_ = [("topo", ctypes.c_int), ("nelem", I64), ("conn", P)]
 
 
 class ServeRequest(ctypes.Structure):
@@ -95,6 +100,8 @@ def load_library(path: str = LIB_PATH):
     L.txa_mesh_destroy.argtypes = [P, ctypes.c_int]
     L.txa_mesh_upload.argtypes = [P, ctypes.c_int, I64, P, I64, I64, P, ctypes.POINTER(ctypes.c_int)]
     L.txa_mesh_info.argtypes = [P, ctypes.c_int, P]
+    L.txa_mesh_upload_blocks.argtypes = [P, ctypes.c_int, P, I64, I64, P, ctypes.POINTER(ctypes.c_int)]
+    L.txa_mesh_blocks.argtypes = [P, ctypes.c_int, P, I64]
     L.txa_dofmap_build.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(I64)]
     L.txa_dofmap_copy.argtypes = [P, ctypes.c_int, P, P]
     L.txa_dofmap_point_lids.argtypes = [P, ctypes.c_int, P, ctypes.POINTER(I64)]
@@ -245,16 +252,31 @@ class Engine:
         coords [vertices][3].  nentities defaults to the number of vertices (vertex-only topologies)."""
         return BrickMesh(self, topo, None, None, None, coords, conn=conn, nentities=nentities)
 
+    def upload_blocks(self, blocks, coords, nentities=None):
+        """A mesh of several element blocks (txa_mesh_upload_blocks): blocks [(topo, conn [elements][npe])],
+        one DOF map and graph over all of them; elements numbered block after block."""
+        return BrickMesh(self, 0, None, None, None, coords, blocks=blocks, nentities=nentities)
+
 
 class BrickMesh:
     """Structured brick mesh resident on the device, with its DOF map and CRS graph."""
 
-    def __init__(self, eng: Engine, topo, nx, ny, nz, coords, conn=None, nentities=None):
+    def __init__(self, eng: Engine, topo, nx, ny, nz, coords, conn=None, nentities=None, blocks=None):
         self.eng = eng
         self.topo, self.nx, self.ny, self.nz = topo, nx, ny, nz
         c = np.ascontiguousarray(coords, np.float64)
         mid = ctypes.c_int()
-        if conn is None:
+        if blocks is not None:
+            conns = [np.ascontiguousarray(k, np.int64) for _, k in blocks]
+            arr = (ElementBlock * len(blocks))()
+            for i, ((t, _), k) in enumerate(zip(blocks, conns)):
+                if k.ndim != 2 or k.shape[1] != NODES_PER_ELEMENT.get(int(t), -1):
+                    raise ValueError(f"block {i}: conn must be [elements][{NODES_PER_ELEMENT.get(int(t), '?')}]")
+                arr[i].topo, arr[i].nelem, arr[i].conn = int(t), k.shape[0], _np(k)
+            ne = c.shape[0] if nentities is None else int(nentities)
+            eng.check(eng.lib.txa_mesh_upload_blocks(eng.h, len(blocks), ctypes.cast(arr, P), ne, c.shape[0], _np(c),
+                                                     ctypes.byref(mid)))
+        elif conn is None:
             if c.shape != ((nx + 1) * (ny + 1) * (nz + 1), 3):
                 raise ValueError("coords must be [(nx+1)(ny+1)(nz+1)][3]")
             eng.check(eng.lib.txa_mesh_brick(eng.h, topo, nx, ny, nz, _np(c), ctypes.byref(mid)))
@@ -272,6 +294,10 @@ class BrickMesh:
         self.ndofs = None
         self.nnz = None
         self.nfields = None
+        bi = np.zeros(1 + 3 * 64, np.int64)
+        eng.check(eng.lib.txa_mesh_blocks(eng.h, self.id, _np(bi), bi.size))
+        # (topology, elements, first element) of each element block
+        self.blocks = [tuple(int(v) for v in bi[1 + 3 * k: 4 + 3 * k]) for k in range(int(bi[0]))]
 
     def build_dofmap(self, nfields):
         n = I64()
@@ -281,10 +307,19 @@ class BrickMesh:
         return self.ndofs
 
     def dofmap(self):
+        """(owned GIDs, element LIDs [elements][npe * nfields]); for a mesh whose blocks differ in
+        topology the element LIDs are a list with one such array per block."""
         owned = np.zeros(self.ndofs, np.int64)
-        el = np.zeros((self.nelem, self.nconn * self.nfields), np.int32)
-        self.eng.check(self.eng.lib.txa_dofmap_copy(self.eng.h, self.id, _np(owned), _np(el)))
-        return owned, el
+        npes = [NODES_PER_ELEMENT[t] for t, _, _ in self.blocks]
+        flat = np.zeros(sum(n * ne for n, (_, ne, _) in zip(npes, self.blocks)) * self.nfields, np.int32)
+        self.eng.check(self.eng.lib.txa_dofmap_copy(self.eng.h, self.id, _np(owned), _np(flat)))
+        if self.nconn:
+            return owned, flat.reshape(self.nelem, self.nconn * self.nfields)
+        out, off = [], 0
+        for n, (_, ne, _) in zip(npes, self.blocks):
+            out.append(flat[off:off + ne * n * self.nfields].reshape(ne, n * self.nfields))
+            off += ne * n * self.nfields
+        return owned, out
 
     def point_lids(self):
         """LID of each lattice point and field, [points][nfields] (int32)."""

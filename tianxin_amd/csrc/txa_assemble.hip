@@ -461,6 +461,90 @@ __global__ __launch_bounds__(256) void k_lane_gather(int64_t nrows, const int64_
   }
 }
 
+// The row gather over a mesh of several element blocks (hex8 / tet4 / wedge6): as k_lane_gather
+// with every element row padded to 8 items (a row of a block with NB < 8 basis functions has
+// NB real items; the others carry column -2, which matches no lane).  An element row is a
+// position p of elem_lid; its block is the last one whose positions start at or before p.
+constexpr int kMaxGatherBlocks = 8;
+struct GatherBlocks {
+  int n;
+  int64_t pos_begin[kMaxGatherBlocks];   // first elem_lid position of the block
+  int64_t ke_begin[kMaxGatherBlocks];    // first packed element-matrix entry of the block
+  int nb[kMaxGatherBlocks];
+};
+
+__global__ __launch_bounds__(256) void k_lane_gather_blocks(int64_t nrows, const int64_t *__restrict__ rowptr,
+                                                            const int32_t *__restrict__ col,
+                                                            const int64_t *__restrict__ occ_ptr,
+                                                            const int64_t *__restrict__ occ,
+                                                            const int32_t *__restrict__ elem_lid,
+                                                            const double *__restrict__ ke, GatherBlocks gb,
+                                                            double beta, double *__restrict__ vals) {
+  constexpr int NBP = 8;
+  __shared__ __align__(16) double sK[8][64];
+  __shared__ __align__(16) int32_t sL[8][64];
+  const int j = threadIdx.x % 32, hw = threadIdx.x / 32;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * (256 / 32);
+  for (int64_t row = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) / 32; row < nrows; row += stride) {
+    const int64_t r0 = rowptr[row], n = rowptr[row + 1] - r0;
+    const int64_t o0 = occ_ptr[row];
+    const int64_t nitem = (occ_ptr[row + 1] - o0) * NBP;
+    for (int64_t c0 = 0; c0 < n; c0 += 32) {
+      const int64_t jj = c0 + j;
+      const int32_t c = jj < n ? col[r0 + jj] : -1;
+      double acc = 0.0;
+      for (int64_t i0 = 0; i0 < nitem; i0 += 64) {
+        int32_t Li[2];
+        double Ki[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int64_t it = i0 + h * 32 + j;
+          Li[h] = -2;
+          Ki[h] = 0.0;
+          if (it < nitem) {
+            const int64_t pe = occ[o0 + it / NBP];
+            int k = 0;
+            for (int q = 1; q < gb.n; ++q) k = gb.pos_begin[q] <= pe ? q : k;
+            const int nb = gb.nb[k];
+            const int64_t loc = pe - gb.pos_begin[k], e = loc / nb;
+            const int a = static_cast<int>(loc - e * nb), b = static_cast<int>(it % NBP);
+            if (b < nb) {
+              const int lo = a < b ? a : b, hi = a < b ? b : a;
+              Li[h] = elem_lid[gb.pos_begin[k] + e * nb + b];
+              Ki[h] = ke[gb.ke_begin[k] + e * (nb * (nb + 1) / 2) + lo * nb - (lo * (lo - 1)) / 2 + (hi - lo)];
+            }
+          }
+        }
+        // the previous round's reads of this region are issued before these writes
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          sL[hw][h * 32 + j] = Li[h];
+          sK[hw][h * 32 + j] = Ki[h];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int cnt = nitem - i0 < 64 ? static_cast<int>(nitem - i0) : 64;
+        const int cnt8 = (cnt + 7) & ~7;
+        for (int k = 0; k < cnt8; k += 8) {
+          const int4 la = *reinterpret_cast<const int4 *>(&sL[hw][k]);
+          const int4 lb = *reinterpret_cast<const int4 *>(&sL[hw][k + 4]);
+          const int32_t lv[8] = {la.x, la.y, la.z, la.w, lb.x, lb.y, lb.z, lb.w};
+          double kv[8];
+#pragma unroll
+          for (int u = 0; u < 8; u += 2) {
+            const double2 k2 = *reinterpret_cast<const double2 *>(&sK[hw][k + u]);
+            kv[u] = k2.x;
+            kv[u + 1] = k2.y;
+          }
+#pragma unroll
+          for (int u = 0; u < 8; ++u) acc = lv[u] == c ? acc + kv[u] : acc;
+        }
+      }
+      if (jj < n) vals[r0 + jj] = beta * acc;
+    }
+  }
+}
+
 // Zero fill of an output the atomic kernels add into (a kernel rather than hipMemsetAsync,
 // so the assembly can be recorded in a HIP graph).
 __global__ void k_zero(double *__restrict__ p, int64_t n) {
@@ -568,9 +652,9 @@ __global__ void k_neumann(NeumannParams p) {
 
 }  // namespace
 
-void upload_refelem(Mesh &m, DevBuf<double> &tab, hipStream_t s) {
+void upload_refelem(int topo, DevBuf<double> &tab, hipStream_t s) {
   RefElem r;
-  refelem_build(r, m.topo);
+  refelem_build(r, topo);
   std::vector<double> h;
   for (int b = 0; b < r.nb; ++b) for (int q = 0; q < r.nq; ++q) h.push_back(r.N[b][q]);
   for (int b = 0; b < r.nb; ++b) for (int q = 0; q < r.nq; ++q) for (int d = 0; d < 3; ++d) h.push_back(r.dN[b][q][d]);
@@ -581,21 +665,45 @@ void upload_refelem(Mesh &m, DevBuf<double> &tab, hipStream_t s) {
   TXA_HIP(hipStreamSynchronize(s));
 }
 
+void upload_block_refs(Mesh &m, hipStream_t s) {
+  if (m.topo != 0) upload_refelem(m.topo, m.ref, s);
+  for (const MeshBlock &b : m.blocks)
+    if (!m.block_ref[b.topo].p) upload_refelem(b.topo, m.block_ref[b.topo], s);
+}
+
 static bool uses_q1_path(const Mesh &m, int physics, int precision) {
   return m.structured && (m.topo == TXA_HEX8 || m.topo == TXA_TET4) && physics == TXA_DIFFUSION &&
          precision != TXA_SPLIT_BF16;
 }
 
-// Uploaded one-field hex8 / tet4 diffusion, steady or transient: the element matrices go to
-// the workspace for the row gather (the mesh's element-row lists exist).
+// Basis functions of a block's topology that runs one element per lane (0: none).
+static int lane_nb(int topo) { return topo == TXA_HEX8 ? 8 : topo == TXA_TET4 ? 4 : topo == TXA_WEDGE6 ? 6 : 0; }
+
+// Uploaded one-field hex8 / tet4 / wedge6 diffusion, steady or transient: the element matrices
+// go to the workspace for the row gather (the mesh's element-row lists exist; a mesh of more
+// blocks than the gather's table holds adds its matrices with atomics instead).
 static bool lane_gather_mesh(const Mesh &m, int physics, int precision) {
-  return !m.structured && m.nfields == 1 && physics == TXA_DIFFUSION && precision == TXA_FP64 &&
-         m.occ.p != nullptr && ((m.topo == TXA_HEX8 && m.ndof_elem == 8) || (m.topo == TXA_TET4 && m.ndof_elem == 4));
+  if (m.structured || m.nfields != 1 || physics != TXA_DIFFUSION || precision != TXA_FP64 || !m.occ.p) return false;
+  if (m.blocks.size() > static_cast<size_t>(kMaxGatherBlocks)) return false;
+  for (const MeshBlock &b : m.blocks)
+    if (!lane_nb(b.topo)) return false;
+  return true;
+}
+
+// Offset (in doubles, 256-byte aligned per block) of each block's packed element matrices.
+static size_t gather_ke_doubles(const Mesh &m, std::vector<int64_t> *begin) {
+  size_t off = 0;
+  for (const MeshBlock &b : m.blocks) {
+    if (begin) begin->push_back(static_cast<int64_t>(off));
+    const int nb = lane_nb(b.topo);
+    off += (static_cast<size_t>(b.nelem) * (nb * (nb + 1) / 2) + 31) / 32 * 32;
+  }
+  return off;
 }
 
 size_t assemble_workspace_bytes(const Mesh &m, int physics, int precision, bool want_f, bool want_vals) {
-  if (lane_gather_mesh(m, physics, precision))
-    return want_vals ? sizeof(double) * static_cast<size_t>(m.nelem) * (m.ndof_elem * (m.ndof_elem + 1) / 2) : 0;
+  if (lane_gather_mesh(m, physics, precision)) return want_vals ? sizeof(double) * gather_ke_doubles(m, nullptr) : 0;
+
   txa_assembly_args a{};
   a.physics = physics;
   a.precision = precision;
@@ -633,51 +741,66 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws, int 
     TXA_REQUIRE(a.precision == TXA_FP64, TXA_ERR_UNSUPPORTED,
                 "reduced-precision integration is implemented for hex8 / tet4 diffusion and 3-field elasticity "
                 "on bricks");
-    TXA_REQUIRE(m.ref.p != nullptr, TXA_ERR_LOGIC, "reference tables missing");
-    RefElem r;
-    refelem_build(r, m.topo);
-    GenericParams p;
-    p.topo = m.topo; p.physics = a.physics; p.nb = r.nb; p.nq = r.nq; p.nv = r.nv; p.nf = m.nfields;
-    p.ndof = m.ndof_elem; p.nconn = m.nconn; p.nelem = m.nelem; p.conn = m.conn.p;
-    p.X = m.xyz.p; p.Y = m.xyz.p + m.nverts; p.Z = m.xyz.p + 2 * m.nverts;
-    p.elem_lid = m.elem_lid.p; p.tab = m.ref.p; p.coef = a.coef;
-    p.nu = a.nu; p.rho = a.rho; p.src[0] = a.src[0]; p.src[1] = a.src[1]; p.src[2] = a.src[2];
-    p.ra = a.rayleigh_a; p.rb = a.rayleigh_b;
-    p.x = a.x; p.xdot = a.xdot; p.xddot = a.xddot;
-    p.alpha = a.alpha; p.beta = a.beta; p.gamma2 = a.gamma2; p.transient = a.transient;
-    p.f = a.f; p.vals = a.vals; p.rowptr = m.rowptr.p; p.col = m.col.p;
-    int64_t blocks = m.nelem < 65536 * 8 ? m.nelem : 65536 * 8;
-    const bool lane = a.physics == TXA_DIFFUSION && m.nfields == 1 &&
-                      ((m.topo == TXA_HEX8 && r.nb == 8 && r.nv == 8 && r.nq == 8) ||
-                       (m.topo == TXA_TET4 && r.nb == 4 && r.nv == 4 && r.nq == 4));
-    // the Jacobian of the lane kernel by the row gather: element matrices in ws, rows stored whole
-    const bool gather = lane && a.vals != nullptr && ws != nullptr && lane_gather_mesh(m, a.physics, a.precision);
-    p.ke = gather ? static_cast<double *>(ws) : nullptr;
+    // the Jacobian of the lane kernels by the row gather: element matrices in ws, rows stored whole
+    const bool gather = a.vals != nullptr && ws != nullptr && lane_gather_mesh(m, a.physics, a.precision);
+    std::vector<int64_t> ke_begin;
+    if (gather) gather_ke_doubles(m, &ke_begin);
     if (a.f) zero_async(a.f, m.ndofs, s);
     if (a.vals && !gather) zero_async(a.vals, m.nnz, s);
-    const unsigned lblocks = static_cast<unsigned>((m.nelem + 255) / 256);
-    if (lane && m.topo == TXA_HEX8 && a.transient)
-      hipLaunchKernelGGL((k_generic_lane<8, 8, true>), dim3(lblocks), dim3(256), 0, s, p);
-    else if (lane && m.topo == TXA_HEX8)
-      hipLaunchKernelGGL((k_generic_lane<8, 8, false>), dim3(lblocks), dim3(256), 0, s, p);
-    else if (lane && a.transient)
-      hipLaunchKernelGGL((k_generic_lane<4, 4, true>), dim3(lblocks), dim3(256), 0, s, p);
-    else if (lane)
-      hipLaunchKernelGGL((k_generic_lane<4, 4, false>), dim3(lblocks), dim3(256), 0, s, p);
-    else if (r.nb <= 8 && r.nq <= 8)
-      hipLaunchKernelGGL((k_generic<64, 8, 8>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, p);
-    else
-      hipLaunchKernelGGL((k_generic<256, kMaxNB, kMaxNQ>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, p);
-    TXA_HIP(hipGetLastError());
+    // each element block with its own basis and cubature, all adding into the same rows
+    for (size_t bi = 0; bi < m.blocks.size(); ++bi) {
+      const MeshBlock &b = m.blocks[bi];
+      TXA_REQUIRE(m.block_ref[b.topo].p != nullptr, TXA_ERR_LOGIC, "reference tables missing");
+      RefElem r;
+      refelem_build(r, b.topo);
+      GenericParams p;
+      p.topo = b.topo; p.physics = a.physics; p.nb = r.nb; p.nq = r.nq; p.nv = r.nv; p.nf = m.nfields;
+      p.ndof = b.nconn * m.nfields; p.nconn = b.nconn; p.nelem = b.nelem; p.conn = m.conn.p + b.conn_begin;
+      p.X = m.xyz.p; p.Y = m.xyz.p + m.nverts; p.Z = m.xyz.p + 2 * m.nverts;
+      p.elem_lid = m.elem_lid.p + m.pos_begin(b); p.tab = m.block_ref[b.topo].p; p.coef = a.coef + b.elem_begin;
+      p.nu = a.nu; p.rho = a.rho; p.src[0] = a.src[0]; p.src[1] = a.src[1]; p.src[2] = a.src[2];
+      p.ra = a.rayleigh_a; p.rb = a.rayleigh_b;
+      p.x = a.x; p.xdot = a.xdot; p.xddot = a.xddot;
+      p.alpha = a.alpha; p.beta = a.beta; p.gamma2 = a.gamma2; p.transient = a.transient;
+      p.f = a.f; p.vals = a.vals; p.rowptr = m.rowptr.p; p.col = m.col.p;
+      const int64_t blocks = b.nelem < 65536 * 8 ? b.nelem : 65536 * 8;
+      const int nbl = a.physics == TXA_DIFFUSION && m.nfields == 1 ? lane_nb(b.topo) : 0;
+      p.ke = gather ? static_cast<double *>(ws) + ke_begin[bi] : nullptr;
+      const unsigned lblocks = static_cast<unsigned>((b.nelem + 255) / 256);
+      const bool tr = a.transient != 0;
+      if (nbl == 8 && tr) hipLaunchKernelGGL((k_generic_lane<8, 8, true>), dim3(lblocks), dim3(256), 0, s, p);
+      else if (nbl == 8) hipLaunchKernelGGL((k_generic_lane<8, 8, false>), dim3(lblocks), dim3(256), 0, s, p);
+      else if (nbl == 6 && tr) hipLaunchKernelGGL((k_generic_lane<6, 6, true>), dim3(lblocks), dim3(256), 0, s, p);
+      else if (nbl == 6) hipLaunchKernelGGL((k_generic_lane<6, 6, false>), dim3(lblocks), dim3(256), 0, s, p);
+      else if (nbl == 4 && tr) hipLaunchKernelGGL((k_generic_lane<4, 4, true>), dim3(lblocks), dim3(256), 0, s, p);
+      else if (nbl == 4) hipLaunchKernelGGL((k_generic_lane<4, 4, false>), dim3(lblocks), dim3(256), 0, s, p);
+      else if (r.nb <= 8 && r.nq <= 8)
+        hipLaunchKernelGGL((k_generic<64, 8, 8>), dim3(static_cast<unsigned>(blocks)), dim3(64), 0, s, p);
+      else
+        hipLaunchKernelGGL((k_generic<256, kMaxNB, kMaxNQ>), dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, p);
+      TXA_HIP(hipGetLastError());
+    }
     if (gather) {
       const double gbeta = a.transient ? 1.0 : a.beta;   // the transient element matrices hold beta already
       const int64_t gblocks = std::min<int64_t>((m.ndofs + 7) / 8, int64_t(1) << 20);   // 8 rows per block
-      if (m.topo == TXA_HEX8)
+      const double *ke = static_cast<const double *>(ws);
+      if (m.blocks.size() == 1 && m.topo == TXA_HEX8)
         hipLaunchKernelGGL((k_lane_gather<8>), dim3(static_cast<unsigned>(gblocks)), dim3(256), 0, s, m.ndofs,
-                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, p.ke, gbeta, a.vals);
-      else
+                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, ke, gbeta, a.vals);
+      else if (m.blocks.size() == 1 && m.topo == TXA_TET4)
         hipLaunchKernelGGL((k_lane_gather<4>), dim3(static_cast<unsigned>(gblocks)), dim3(256), 0, s, m.ndofs,
-                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, p.ke, gbeta, a.vals);
+                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, ke, gbeta, a.vals);
+      else {
+        GatherBlocks gb{};
+        gb.n = static_cast<int>(m.blocks.size());
+        for (int k = 0; k < gb.n; ++k) {
+          gb.pos_begin[k] = m.pos_begin(m.blocks[k]);
+          gb.ke_begin[k] = ke_begin[k];
+          gb.nb[k] = lane_nb(m.blocks[k].topo);
+        }
+        hipLaunchKernelGGL(k_lane_gather_blocks, dim3(static_cast<unsigned>(gblocks)), dim3(256), 0, s, m.ndofs,
+                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, ke, gb, gbeta, a.vals);
+      }
       TXA_HIP(hipGetLastError());
     }
   }

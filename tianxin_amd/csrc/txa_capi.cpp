@@ -140,7 +140,7 @@ int txa_mesh_brick(txa_ctx ctx, int topo, int nx, int ny, int nz, const double *
     auto m = std::make_unique<Mesh>();
     m->topo = topo; m->nx = nx; m->ny = ny; m->nz = nz;
     txa::mesh_generate(*m, vertex_coords, nullptr);
-    txa::upload_refelem(*m, m->ref, nullptr);
+    txa::upload_block_refs(*m, nullptr);
     int id = ctx->next_id++;
     ctx->meshes[id] = std::move(m);
     *mesh_id = id;
@@ -154,10 +154,39 @@ int txa_mesh_upload(txa_ctx ctx, int topo, int64_t nelem, const int64_t *conn, i
     auto m = std::make_unique<Mesh>();
     m->topo = topo;
     txa::mesh_upload(*m, nelem, conn, nentities, nverts, vertex_coords, nullptr);
-    txa::upload_refelem(*m, m->ref, nullptr);
+    txa::upload_block_refs(*m, nullptr);
     int id = ctx->next_id++;
     ctx->meshes[id] = std::move(m);
     *mesh_id = id;
+  });
+}
+
+int txa_mesh_upload_blocks(txa_ctx ctx, int nblocks, const txa_element_block *blocks, int64_t nentities,
+                           int64_t nverts, const double *vertex_coords, int *mesh_id) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(mesh_id && blocks && vertex_coords, TXA_ERR_INVALID, "null argument");
+    auto m = std::make_unique<Mesh>();
+    txa::mesh_upload_blocks(*m, nblocks, blocks, nentities, nverts, vertex_coords, nullptr);
+    txa::upload_block_refs(*m, nullptr);
+    int id = ctx->next_id++;
+    ctx->meshes[id] = std::move(m);
+    *mesh_id = id;
+  });
+}
+
+int txa_mesh_blocks(txa_ctx ctx, int mesh_id, int64_t *out, int64_t cap) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(out, TXA_ERR_INVALID, "null argument");
+    Mesh &m = ctx->mesh(mesh_id);
+    const int64_t need = 1 + 3 * static_cast<int64_t>(m.blocks.size());
+    TXA_REQUIRE(cap >= need, TXA_ERR_INVALID, "out holds " + std::to_string(cap) + " entries, " +
+                                                  std::to_string(need) + " needed");
+    out[0] = static_cast<int64_t>(m.blocks.size());
+    for (size_t k = 0; k < m.blocks.size(); ++k) {
+      out[1 + 3 * k] = m.blocks[k].topo;
+      out[2 + 3 * k] = m.blocks[k].nelem;
+      out[3 + 3 * k] = m.blocks[k].elem_begin;
+    }
   });
 }
 
@@ -193,8 +222,7 @@ int txa_dofmap_copy(txa_ctx ctx, int mesh_id, int64_t *owned_gids, int32_t *elem
     Mesh &m = ctx->mesh(mesh_id);
     TXA_REQUIRE(m.ndofs > 0, TXA_ERR_LOGIC, "DOF map not built");
     if (owned_gids) TXA_HIP(hipMemcpy(owned_gids, m.owned.p, m.owned.bytes(), hipMemcpyDeviceToHost));
-    if (elem_lids)
-      TXA_HIP(hipMemcpy(elem_lids, m.elem_lid.p, sizeof(int32_t) * m.nelem * m.ndof_elem, hipMemcpyDeviceToHost));
+    if (elem_lids) TXA_HIP(hipMemcpy(elem_lids, m.elem_lid.p, sizeof(int32_t) * m.npos(), hipMemcpyDeviceToHost));
   });
 }
 

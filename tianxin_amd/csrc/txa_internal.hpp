@@ -60,12 +60,22 @@ struct DevBuf {
   size_t bytes() const { return n * sizeof(T); }
 };
 
-// A structured brick mesh with its DOF map and CRS graph, resident on the device.
+// One element block of a mesh: elements of one topology, numbered globally from
+// elem_begin, connectivity at conn[conn_begin .. conn_begin + nelem * nconn).
+struct MeshBlock {
+  int topo = 0, nconn = 0, nvert_elem = 0;
+  int64_t nelem = 0, elem_begin = 0, conn_begin = 0;
+};
+
+// A mesh with its DOF map and CRS graph, resident on the device: a structured brick,
+// or uploaded connectivity of one or several element blocks.
 struct Mesh {
-  int topo = 0, nx = 0, ny = 0, nz = 0;
+  int topo = 0, nx = 0, ny = 0, nz = 0;   // topo: that of every block (0 when the blocks differ)
   bool structured = true;  // a brick (txa_mesh_brick): lattice kernels; false: uploaded connectivity
-  int64_t nelem = 0;     // elements (6 per cell for TET4)
-  int nconn = 0;         // connectivity entities per element
+  std::vector<MeshBlock> blocks;   // one for bricks and single-topology uploads
+  int64_t nelem = 0;     // elements of all blocks (6 per cell for a TET4 brick)
+  int64_t nconn_tot = 0; // connectivity entries of all blocks
+  int nconn = 0;         // connectivity entities per element (0 when the blocks differ)
   int nvert_elem = 0;    // geometry vertices per element
   int64_t nverts = 0;    // vertex nodes
   int64_t nentities = 0; // entity-id bound
@@ -73,10 +83,10 @@ struct Mesh {
   DevBuf<int64_t> conn;  // [nelem][nconn] entity ids
   // DOF map
   int nfields = 0;
-  int ndof_elem = 0;
+  int ndof_elem = 0;          // element DOFs (0 when the blocks differ)
   int64_t ndofs = 0;
   DevBuf<int64_t> owned;      // [ndofs] GID of each LID
-  DevBuf<int32_t> elem_lid;   // [nelem][ndof_elem]
+  DevBuf<int32_t> elem_lid;   // per block [nelem][nconn * nfields], blocks one after another
   int64_t npoints = 0;        // lattice points carrying DOFs (vertices; (2n+1)^3 for HEX27)
   DevBuf<int32_t> point_lid;  // [npoints][nfields] LID of each lattice point and field
   // graph
@@ -88,10 +98,15 @@ struct Mesh {
   DevBuf<uint8_t> perm;       // [ndofs][perm_width] position of each stencil neighbour in the row, 255 = absent
   int nperm_w = 0;            // entries per node of nperm
   DevBuf<uint16_t> nperm;     // 3-field HEX8 / HEX27: [nodes][nperm_w] position of (neighbour, field 0) in the node's rows
-  // uploaded one-field HEX8 / TET4: the element rows p = e*ndof_elem + a of each LID, ascending
+  // uploaded one-field HEX8 / TET4 / WEDGE6 blocks: the element rows of each LID, as positions p
+  // in elem_lid (element e of a block, row a: block's position base + e*ndof + a), ascending
   // (occ[occ_ptr[r] .. occ_ptr[r+1]) for row r), for the row gather of the assembly
   DevBuf<int64_t> occ_ptr, occ;
-  DevBuf<double> ref;         // reference-element tables of the topology
+  DevBuf<double> ref;         // reference-element tables of the topology (single-topology meshes)
+  DevBuf<double> block_ref[5];   // reference-element tables by topology (TXA_HEX8 .. TXA_WEDGE6)
+  bool mixed() const { return blocks.size() > 1 && topo == 0; }
+  int64_t npos() const { return nconn_tot * nfields; }   // element DOF positions of all blocks
+  int64_t pos_begin(const MeshBlock &b) const { return b.conn_begin * nfields; }
 };
 
 struct Batch;
@@ -159,6 +174,8 @@ struct Context {
 void mesh_generate(Mesh &m, const double *host_coords, hipStream_t s);
 void mesh_upload(Mesh &m, int64_t nelem, const int64_t *host_conn, int64_t nentities, int64_t nverts,
                  const double *host_coords, hipStream_t s);
+void mesh_upload_blocks(Mesh &m, int nblocks, const txa_element_block *blocks, int64_t nentities, int64_t nverts,
+                        const double *host_coords, hipStream_t s);
 void dofmap_build(Mesh &m, int nfields, hipStream_t s);
 void graph_build(Mesh &m, hipStream_t s);
 void exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s);
@@ -169,7 +186,8 @@ void write_matrix_market(Mesh &m, const double *vals, const double *f, const dou
 void read_vector_mm(const char *path, double *dst, int64_t n);
 
 // assembly (txa_assemble.hip)
-void upload_refelem(Mesh &m, DevBuf<double> &tab, hipStream_t s);
+void upload_refelem(int topo, DevBuf<double> &tab, hipStream_t s);
+void upload_block_refs(Mesh &m, hipStream_t s);   // tables of the mesh's topology and of each block's
 // ws: device workspace of assemble_workspace_bytes(...) bytes, 256-byte aligned, for
 // the work on s (may be null when that size is 0)
 // err: device (or host-mapped) int set to 1 when a Dirichlet LID lies outside the map (may be null)

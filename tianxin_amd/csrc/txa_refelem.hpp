@@ -1,12 +1,13 @@
 // Reference-element tables (basis values / gradients at the integration points,
 // cubature), computed on the host in FP64 and uploaded once per topology.
 //
-// Bases: Intrepid2 HGRAD_HEX_C1, HGRAD_TET_C1 and HGRAD_HEX_C2 as selected by
+// Bases: Intrepid2 HGRAD_HEX_C1, HGRAD_TET_C1, HGRAD_WEDGE_C1 and HGRAD_HEX_C2 as selected by
 // disc-fe/src/Panzer_IntrepidBasisFactory.hpp:159-184; basis functions are
 // listed in element-DOF (subcell) order: vertices in shards order, then edge
 // midpoints, face centres and the cell centre (dof-mgr/src/Panzer_FieldAggPattern.cpp:202-246).
 // Cubature: tensor Gauss-Legendre (2 or 3 points per axis) on hexes, the
-// 4-point degree-2 rule on tets (disc-fe/src/Panzer_IntegrationValues2.cpp:69-94).
+// 4-point degree-2 rule on tets, and on wedges the 3-point degree-2 triangle rule
+// times 2-point Gauss (disc-fe/src/Panzer_IntegrationValues2.cpp:69-94).
 #pragma once
 
 #include <cmath>
@@ -33,6 +34,32 @@ inline void refelem_build(RefElem &r, int topo) {
   r.topo = topo;
   static const int hexv[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
                                  {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+  if (topo == 4 /* WEDGE6 */) {
+    // shards Wedge<6>: triangle (0,0) (1,0) (0,1) at zeta = -1 (nodes 0-2) and +1 (nodes 3-5);
+    // N = lam_i (1 -+ zeta) / 2, lam = (1 - xi - eta, xi, eta); triangle point fastest
+    static const double tq[3][2] = {{1.0 / 6.0, 1.0 / 6.0}, {2.0 / 3.0, 1.0 / 6.0}, {1.0 / 6.0, 2.0 / 3.0}};
+    static const double dlam[3][2] = {{-1.0, -1.0}, {1.0, 0.0}, {0.0, 1.0}};
+    r.nb = 6; r.nv = 6; r.nq = 6;
+    for (int k = 0; k < 2; ++k)
+      for (int t = 0; t < 3; ++t) {
+        const int q = 3 * k + t;
+        const double x = tq[t][0], y = tq[t][1], z = (k == 0 ? -1.0 : 1.0) / std::sqrt(3.0);
+        const double lam[3] = {1.0 - x - y, x, y};
+        r.qw[q] = 1.0 / 6.0;
+        for (int l = 0; l < 2; ++l) {
+          const double h = l == 0 ? 0.5 * (1.0 - z) : 0.5 * (1.0 + z), dh = l == 0 ? -0.5 : 0.5;
+          for (int i = 0; i < 3; ++i) {
+            const int v = 3 * l + i;
+            r.N[v][q] = lam[i] * h;
+            r.dN[v][q][0] = dlam[i][0] * h;
+            r.dN[v][q][1] = dlam[i][1] * h;
+            r.dN[v][q][2] = lam[i] * dh;
+            for (int d = 0; d < 3; ++d) r.gdN[v][q][d] = r.dN[v][q][d];
+          }
+        }
+      }
+    return;
+  }
   if (topo == 2 /* TET4 */) {
     r.nb = 4; r.nv = 4; r.nq = 4;
     const double a = 0.5854101966249685, b = 0.1381966011250105;

@@ -369,6 +369,8 @@ void mesh_generate(Mesh &m, const double *host_coords, hipStream_t s) {
     for (int d = 0; d < 3; ++d) soa[d * nn + i] = host_coords[3 * i + d];
   m.xyz.alloc(3 * nn);
   TXA_HIP(hipMemcpyAsync(m.xyz.p, soa.data(), sizeof(double) * 3 * nn, hipMemcpyHostToDevice, s));
+  m.nconn_tot = m.nelem * m.nconn;
+  m.blocks.assign(1, MeshBlock{m.topo, m.nconn, m.nvert_elem, m.nelem, 0, 0});
   m.conn.alloc(m.nelem * m.nconn);
   BrickIds b{NX, NY, NZ, nn, ne, nf};
   hipLaunchKernelGGL(k_brick_conn, dim3(grid_for(nc)), dim3(kBlock), 0, s, m.topo, b, nc, m.conn.p);
@@ -376,29 +378,59 @@ void mesh_generate(Mesh &m, const double *host_coords, hipStream_t s) {
   TXA_HIP(hipStreamSynchronize(s));
 }
 
-// A general mesh: connectivity as entity ids (the first nvert_elem of an element are
-// its vertices, in [0, nverts); HEX27 adds its edges, faces and cell as further ids
-// below nentities), vertex coordinates [nverts][3].  Validated on the host.
-void mesh_upload(Mesh &m, int64_t nelem, const int64_t *host_conn, int64_t nentities, int64_t nverts,
-                 const double *host_coords, hipStream_t s) {
-  switch (m.topo) {
-    case TXA_HEX8: m.nconn = 8; m.nvert_elem = 8; break;
-    case TXA_TET4: m.nconn = 4; m.nvert_elem = 4; break;
-    case TXA_HEX27: m.nconn = 27; m.nvert_elem = 8; break;
-    default: throw Error(TXA_ERR_INVALID, "unknown topology " + std::to_string(m.topo));
+// Connectivity width and geometry vertices of an uploadable topology.
+static void topo_sizes(int topo, int *nconn, int *nvert) {
+  switch (topo) {
+    case TXA_HEX8: *nconn = 8; *nvert = 8; break;
+    case TXA_TET4: *nconn = 4; *nvert = 4; break;
+    case TXA_WEDGE6: *nconn = 6; *nvert = 6; break;
+    case TXA_HEX27: *nconn = 27; *nvert = 8; break;
+    default: throw Error(TXA_ERR_INVALID, "unknown topology " + std::to_string(topo));
   }
-  TXA_REQUIRE(nelem >= 1 && nverts >= m.nvert_elem && nentities >= nverts, TXA_ERR_INVALID,
-              "need elements, vertices and nentities >= nverts");
-  for (int64_t e = 0; e < nelem; ++e)
-    for (int k = 0; k < m.nconn; ++k) {
-      const int64_t id = host_conn[e * m.nconn + k];
-      const int64_t hi = k < m.nvert_elem ? nverts : nentities;
-      TXA_REQUIRE(id >= 0 && id < hi, TXA_ERR_INVALID,
-                  "element " + std::to_string(e) + " entry " + std::to_string(k) + ": entity id " +
-                      std::to_string(id) + (k < m.nvert_elem ? " is not a vertex" : " is out of range"));
-    }
+}
+
+// A general mesh of one or several element blocks: connectivity as entity ids (the first
+// nvert of an element are its vertices, in [0, nverts); HEX27 adds its edges, faces and
+// cell as further ids below nentities), vertex coordinates [nverts][3].  Validated on the
+// host; the blocks' connectivity is stored one block after another.
+void mesh_upload_blocks(Mesh &m, int nblocks, const txa_element_block *blocks, int64_t nentities, int64_t nverts,
+                        const double *host_coords, hipStream_t s) {
+  TXA_REQUIRE(nblocks >= 1 && blocks, TXA_ERR_INVALID, "a mesh needs at least one element block");
+  TXA_REQUIRE(nverts >= 1 && nentities >= nverts, TXA_ERR_INVALID, "need vertices and nentities >= nverts");
+  m.blocks.clear();
+  int64_t elem = 0, pos = 0;
+  for (int k = 0; k < nblocks; ++k) {
+    const txa_element_block &in = blocks[k];
+    MeshBlock b;
+    b.topo = in.topo;
+    topo_sizes(in.topo, &b.nconn, &b.nvert_elem);
+    TXA_REQUIRE(in.nelem >= 1 && in.conn, TXA_ERR_INVALID,
+                "block " + std::to_string(k) + ": need elements and their connectivity");
+    TXA_REQUIRE(b.nvert_elem <= nverts, TXA_ERR_INVALID, "fewer vertices than an element has");
+    for (int64_t e = 0; e < in.nelem; ++e)
+      for (int c = 0; c < b.nconn; ++c) {
+        const int64_t id = in.conn[e * b.nconn + c];
+        const int64_t hi = c < b.nvert_elem ? nverts : nentities;
+        TXA_REQUIRE(id >= 0 && id < hi, TXA_ERR_INVALID,
+                    (nblocks > 1 ? "block " + std::to_string(k) + " " : std::string()) + "element " +
+                        std::to_string(e) + " entry " + std::to_string(c) + ": entity id " + std::to_string(id) +
+                        (c < b.nvert_elem ? " is not a vertex" : " is out of range"));
+      }
+    b.nelem = in.nelem;
+    b.elem_begin = elem;
+    b.conn_begin = pos;
+    elem += in.nelem;
+    pos += in.nelem * b.nconn;
+    m.blocks.push_back(b);
+  }
+  bool same = true;
+  for (const MeshBlock &b : m.blocks) same = same && b.topo == m.blocks[0].topo;
+  m.topo = same ? m.blocks[0].topo : 0;
+  m.nconn = same ? m.blocks[0].nconn : 0;
+  m.nvert_elem = same ? m.blocks[0].nvert_elem : 0;
   m.structured = false;
-  m.nelem = nelem;
+  m.nelem = elem;
+  m.nconn_tot = pos;
   m.nverts = nverts;
   m.nentities = nentities;
   std::vector<double> soa(static_cast<size_t>(3 * nverts));
@@ -406,16 +438,24 @@ void mesh_upload(Mesh &m, int64_t nelem, const int64_t *host_conn, int64_t nenti
     for (int d = 0; d < 3; ++d) soa[d * nverts + i] = host_coords[3 * i + d];
   m.xyz.alloc(3 * nverts);
   TXA_HIP(hipMemcpyAsync(m.xyz.p, soa.data(), sizeof(double) * 3 * nverts, hipMemcpyHostToDevice, s));
-  m.conn.alloc(nelem * m.nconn);
-  TXA_HIP(hipMemcpyAsync(m.conn.p, host_conn, sizeof(int64_t) * nelem * m.nconn, hipMemcpyHostToDevice, s));
+  m.conn.alloc(pos);
+  for (const MeshBlock &b : m.blocks)
+    TXA_HIP(hipMemcpyAsync(m.conn.p + b.conn_begin, blocks[&b - m.blocks.data()].conn,
+                           sizeof(int64_t) * b.nelem * b.nconn, hipMemcpyHostToDevice, s));
   TXA_HIP(hipStreamSynchronize(s));
+}
+
+void mesh_upload(Mesh &m, int64_t nelem, const int64_t *host_conn, int64_t nentities, int64_t nverts,
+                 const double *host_coords, hipStream_t s) {
+  txa_element_block b{m.topo, nelem, host_conn};
+  mesh_upload_blocks(m, 1, &b, nentities, nverts, host_coords, s);
 }
 
 void dofmap_build(Mesh &m, int nfields, hipStream_t s) {
   TXA_REQUIRE(nfields >= 1 && nfields <= 3, TXA_ERR_INVALID, "nfields must be 1..3");
   m.nfields = nfields;
   m.ndof_elem = m.nconn * nfields;
-  int64_t ncon_tot = m.nelem * m.nconn;
+  int64_t ncon_tot = m.nconn_tot;
   int64_t npos = ncon_tot * nfields;
   // GUN on one rank: rank of each entity id among the ids present
   DevBuf<int64_t> flag, rank;
@@ -472,7 +512,7 @@ void dofmap_build(Mesh &m, int nfields, hipStream_t s) {
 void graph_build(Mesh &m, hipStream_t s) {
   TXA_REQUIRE(m.ndofs > 0, TXA_ERR_LOGIC, "graph_build before dofmap_build");
   int nd = m.ndof_elem;
-  int64_t npos = m.nelem * nd;
+  int64_t npos = m.npos();
   int64_t nrows = m.ndofs;
   // a brick's rows have at most valence x nd candidates; an uploaded mesh's rows are
   // checked against the capacity while they are sorted
@@ -483,8 +523,14 @@ void graph_build(Mesh &m, hipStream_t s) {
   cnt.alloc(nrows + 1);
   cptr.alloc(nrows + 1);
   TXA_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), s));
-  hipLaunchKernelGGL(k_row_count, dim3(grid_for(npos)), dim3(kBlock), 0, s, m.elem_lid.p, npos, nd, cnt.p);
-  TXA_HIP(hipGetLastError());
+  // candidates: every element row of every block contributes the element's LIDs
+  for (const MeshBlock &b : m.blocks) {
+    const int bnd = b.nconn * m.nfields;
+    const int64_t bn = b.nelem * bnd;
+    hipLaunchKernelGGL(k_row_count, dim3(grid_for(bn)), dim3(kBlock), 0, s, m.elem_lid.p + m.pos_begin(b), bn, bnd,
+                       cnt.p);
+    TXA_HIP(hipGetLastError());
+  }
   exclusive_scan_i64(cnt.p, cptr.p, nrows + 1, s);
   int64_t ncand = 0;
   TXA_HIP(hipMemcpy(&ncand, cptr.p + nrows, sizeof(int64_t), hipMemcpyDeviceToHost));
@@ -493,8 +539,13 @@ void graph_build(Mesh &m, hipStream_t s) {
   TXA_HIP(hipMemcpyAsync(cursor.p, cptr.p, sizeof(int64_t) * (nrows + 1), hipMemcpyDeviceToDevice, s));
   DevBuf<int32_t> cand;
   cand.alloc(ncand);
-  hipLaunchKernelGGL(k_row_fill, dim3(grid_for(npos)), dim3(kBlock), 0, s, m.elem_lid.p, npos, nd, cursor.p, cand.p);
-  TXA_HIP(hipGetLastError());
+  for (const MeshBlock &b : m.blocks) {
+    const int bnd = b.nconn * m.nfields;
+    const int64_t bn = b.nelem * bnd;
+    hipLaunchKernelGGL(k_row_fill, dim3(grid_for(bn)), dim3(kBlock), 0, s, m.elem_lid.p + m.pos_begin(b), bn, bnd,
+                       cursor.p, cand.p);
+    TXA_HIP(hipGetLastError());
+  }
   cursor.release();
   ulen.alloc(nrows + 1);
   TXA_HIP(hipMemsetAsync(ulen.p, 0, ulen.bytes(), s));
@@ -529,7 +580,10 @@ void graph_build(Mesh &m, hipStream_t s) {
   // element rows of each LID for the row gather of uploaded one-field hex8 / tet4 meshes
   m.occ_ptr.release();
   m.occ.release();
-  if (!m.structured && m.nfields == 1 && ((m.topo == TXA_HEX8 && nd == 8) || (m.topo == TXA_TET4 && nd == 4))) {
+  bool lane_blocks = !m.structured && m.nfields == 1;
+  for (const MeshBlock &b : m.blocks)
+    lane_blocks = lane_blocks && (b.topo == TXA_HEX8 || b.topo == TXA_TET4 || b.topo == TXA_WEDGE6);
+  if (lane_blocks) {
     DevBuf<int64_t> ocnt, ocur;
     ocnt.alloc(nrows + 1);
     TXA_HIP(hipMemsetAsync(ocnt.p, 0, ocnt.bytes(), s));
