@@ -205,9 +205,72 @@ typedef struct {
   int n_neumann;
   const int *neumann_sides;     /* [n_neumann] */
   const double *neumann_values; /* [n_neumann][nf] flux per field */
+  /* Element block to assemble: 0 = every block of the mesh, b > 0 = block b - 1 only (one
+   * evaluator per element block, as each block has its own volume field manager). */
+  int block;
+  /* 0: f and vals are overwritten with this call's sums; 1: the sums are added to what f and
+   * vals hold (several blocks or evaluators scattering into one container, which the caller
+   * zeroed, as ScatterResidual_Tpetra adds into a container AssemblyEngine initialised). */
+  int accumulate;
 } txa_assembly_args;
 
 int txa_assemble(txa_ctx ctx, int mesh_id, const txa_assembly_args *args, void *stream);
+
+/* Dirichlet rows alone: f_i = x_i - g_i (g = values or 0) and row i of the CRS values
+ * (vals) becomes the identity row, for the n LIDs in lids (device arrays; f or vals may be
+ * NULL).  The BC step of AssemblyEngine after every block's volume assembly
+ * (disc-fe/src/Panzer_AssemblyEngine_impl.hpp:96-126; TianXin_Dirichlet_impl.hpp:50-82,
+ * Panzer_TpetraLinearObjContainer.hpp:228-237).  LIDs outside the map are reported by txa_sync. */
+int txa_apply_dirichlet(txa_ctx ctx, int mesh_id, const int32_t *lids, const double *values, int64_t n,
+                        const double *x, double *f, double *vals, void *stream);
+
+/* p[i] = value for the n doubles of device array p: zeroing a container's f and matrix
+ * values before an assembly accumulates into them (AssemblyEngine's initialisation of the
+ * ghosted container, Thyra::assign / initializeMatrix in
+ * disc-fe/src/Panzer_ModelEvaluator_impl.hpp:1785-1790). */
+int txa_fill(txa_ctx ctx, double *p, int64_t n, double value, void *stream);
+
+/* y = A x for the CRS values vals on the mesh's graph (device arrays of ndofs / nnz
+ * entries): Tpetra::CrsMatrix::apply, the operator a linear solver applies to W. */
+int txa_spmv(txa_ctx ctx, int mesh_id, const double *vals, const double *x, double *y, void *stream);
+
+/* ModelEvaluator-shaped evaluation: the hand-off to a nonlinear solver.  A model binds a
+ * mesh (with its DOF map and graph), the physics and material of txa_assembly_args and the
+ * Dirichlet rows; txa_model_eval then fills the residual f and the Jacobian W as
+ * panzer::ModelEvaluator::evalModelImpl_basic does
+ * (disc-fe/src/Panzer_ModelEvaluator_impl.hpp:725-780, 1715-1840):
+ *  - a model without transient support always evaluates the steady system (alpha 0, beta 1);
+ *  - with transient support alpha, beta (and gamma2 for the second derivative) come from
+ *    the in-arguments, and the transient terms are evaluated when x_dot is given;
+ *  - f and W are outputs each may leave NULL: f and W, f alone (the Residual evaluation),
+ *    or W alone (the Jacobian evaluation); both NULL does nothing.
+ * W = alpha df/dx_dot + beta df/dx + gamma2 df/dx_dot_dot, the CRS values on the mesh's graph.
+ * The Dirichlet rows are applied last (f_i = x_i - g_i, identity rows). */
+typedef struct {
+  int physics, precision;      /* as txa_assembly_args */
+  const double *coef;          /* device [elements] */
+  double nu, rho;
+  double src[3];
+  double rayleigh_a, rayleigh_b;
+  int transient_support;       /* build_transient_support of the reference's ModelEvaluator */
+  const int32_t *bc_lids;      /* device [n_bc] Dirichlet LIDs, or NULL */
+  const double *bc_vals;       /* device [n_bc] prescribed values, or NULL (= 0) */
+  int64_t n_bc;
+} txa_model_desc;
+typedef struct {
+  const double *x;             /* device [ndofs] */
+  const double *x_dot;         /* device [ndofs] or NULL (steady fill) */
+  const double *x_dot_dot;     /* device [ndofs] or NULL */
+  double alpha, beta, gamma2;
+  double t, step_size;         /* time and step of the evaluation (coefficients are time independent here) */
+} txa_model_in;
+typedef struct {
+  double *f;                   /* device [ndofs] or NULL */
+  double *W;                   /* device [nnz] CRS values or NULL */
+} txa_model_out;
+int txa_model_create(txa_ctx ctx, int mesh_id, const txa_model_desc *desc, int *model_id);
+int txa_model_eval(txa_ctx ctx, int model_id, const txa_model_in *in, const txa_model_out *out, void *stream);
+int txa_model_destroy(txa_ctx ctx, int model_id);
 
 /* txa_assemble without any allocation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight

@@ -51,7 +51,26 @@ class AssemblyArgs(ctypes.Structure):
         ("f", P), ("vals", P),
         ("bc_lids", P), ("bc_vals", P), ("n_bc", I64),
         ("n_neumann", ctypes.c_int), ("neumann_sides", P), ("neumann_values", P),
+        ("block", ctypes.c_int), ("accumulate", ctypes.c_int),
     ]
+
+
+class ModelDesc(ctypes.Structure):
+    _fields_ = [
+        ("physics", ctypes.c_int), ("precision", ctypes.c_int), ("coef", P),
+        ("nu", ctypes.c_double), ("rho", ctypes.c_double), ("src", ctypes.c_double * 3),
+        ("rayleigh_a", ctypes.c_double), ("rayleigh_b", ctypes.c_double), ("transient_support", ctypes.c_int),
+        ("bc_lids", P), ("bc_vals", P), ("n_bc", I64),
+    ]
+
+
+class ModelIn(ctypes.Structure):
+    _fields_ = [("x", P), ("x_dot", P), ("x_dot_dot", P), ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
+                ("gamma2", ctypes.c_double), ("t", ctypes.c_double), ("step_size", ctypes.c_double)]
+
+
+class ModelOut(ctypes.Structure):
+    _fields_ = [("f", P), ("W", P)]
 
 
 class ElementBlock(ctypes.Structure):
@@ -125,6 +144,11 @@ def load_library(path: str = LIB_PATH):
     L.txa_write_matrix_market.argtypes = [P, ctypes.c_int, P, P, P, ctypes.c_char_p, ctypes.c_char_p,
                                           ctypes.c_char_p]
     L.txa_read_vector_mm.argtypes = [P, ctypes.c_char_p, P, I64]
+    L.txa_apply_dirichlet.argtypes = [P, ctypes.c_int, P, P, I64, P, P, P, P]
+    L.txa_spmv.argtypes = [P, ctypes.c_int, P, P, P, P]
+    L.txa_model_create.argtypes = [P, ctypes.c_int, ctypes.POINTER(ModelDesc), ctypes.POINTER(ctypes.c_int)]
+    L.txa_model_eval.argtypes = [P, ctypes.c_int, ctypes.POINTER(ModelIn), ctypes.POINTER(ModelOut), P]
+    L.txa_model_destroy.argtypes = [P, ctypes.c_int]
     _lib = L
     return L
 
@@ -189,7 +213,7 @@ def _stream(stream):
 
 def make_args(*, physics, coef, x, f=None, vals=None, precision=FP64, xdot=None, xddot=None,
               alpha=0.0, beta=1.0, gamma2=0.0, transient=False, nu=0.3, rho=1.0, src=(0.0, 0.0, 0.0),
-              rayleigh=(0.0, 0.0), bc_lids=None, bc_vals=None, neumann=None):
+              rayleigh=(0.0, 0.0), bc_lids=None, bc_vals=None, neumann=None, block=0, accumulate=False):
     """txa_assembly_args from device tensors (see include/txa.h).  neumann: [(side, (flux per field, ...)), ...]
     with sides SIDE_BOTTOM .. SIDE_FRONT."""
     a = AssemblyArgs()
@@ -205,6 +229,7 @@ def make_args(*, physics, coef, x, f=None, vals=None, precision=FP64, xdot=None,
     a.f, a.vals = _dptr(f), _dptr(vals)
     a.bc_lids, a.bc_vals = _dptr(bc_lids), _dptr(bc_vals)
     a.n_bc = 0 if bc_lids is None else int(bc_lids.numel())
+    a.block, a.accumulate = int(block), int(accumulate)
     if neumann:
         sides = (ctypes.c_int * len(neumann))(*[int(s) for s, _ in neumann])
         flat = [float(v) for _, vs in neumann for v in vs]
@@ -359,6 +384,21 @@ class BrickMesh:
         a = make_args(**kw)
         self.eng.check(self.eng.lib.txa_assemble(self.eng.h, self.id, ctypes.byref(a), _stream(stream)))
 
+    def spmv(self, vals, x, y, stream=None):
+        """y = A x for the CRS values vals on this mesh's graph (txa_spmv)."""
+        check_tensors([("vals", vals, "float64", self.nnz, True), ("x", x, "float64", self.ndofs, True),
+                       ("y", y, "float64", self.ndofs, True)], self.eng.device)
+        self.eng.check(self.eng.lib.txa_spmv(self.eng.h, self.id, _dptr(vals), _dptr(x), _dptr(y), _stream(stream)))
+
+    def apply_dirichlet(self, lids, x, f=None, vals=None, values=None, stream=None):
+        """Dirichlet rows alone (txa_apply_dirichlet): f_i = x_i - g_i, identity matrix rows."""
+        check_tensors([("lids", lids, "int32", None, True), ("x", x, "float64", self.ndofs, True),
+                       ("f", f, "float64", self.ndofs, False), ("vals", vals, "float64", self.nnz, False),
+                       ("values", values, "float64", lids.numel(), False)], self.eng.device)
+        self.eng.check(self.eng.lib.txa_apply_dirichlet(self.eng.h, self.id, _dptr(lids), _dptr(values),
+                                                        int(lids.numel()), _dptr(x), _dptr(f), _dptr(vals),
+                                                        _stream(stream)))
+
     def assemble_noalloc(self, *, stream=None, **kw):
         """As assemble, through txa_assemble_noalloc (workspace in the library's static array)."""
         check_assembly_tensors(self, self.eng.device, **kw)
@@ -368,6 +408,51 @@ class BrickMesh:
     def destroy(self):
         if self.id is not None:
             self.eng.check(self.eng.lib.txa_mesh_destroy(self.eng.h, self.id))
+            self.id = None
+
+
+class Model:
+    """ModelEvaluator-shaped evaluation on a mesh (txa_model_*): f and W from x, x_dot, alpha, beta."""
+
+    def __init__(self, mesh, *, physics, coef, precision=FP64, nu=0.3, rho=1.0, src=(0.0, 0.0, 0.0),
+                 rayleigh=(0.0, 0.0), transient_support=False, bc_lids=None, bc_vals=None):
+        eng = mesh.eng
+        check_tensors([("coef", coef, "float64", mesh.nelem, True), ("bc_lids", bc_lids, "int32", None, False),
+                       ("bc_vals", bc_vals, "float64", None if bc_lids is None else bc_lids.numel(), False)],
+                      eng.device)
+        self.eng, self.mesh = eng, mesh
+        d = ModelDesc()
+        d.physics, d.precision, d.coef = physics, precision, _dptr(coef)
+        d.nu, d.rho = nu, rho
+        s = list(src) + [0.0] * 3
+        d.src = (ctypes.c_double * 3)(*s[:3])
+        d.rayleigh_a, d.rayleigh_b = rayleigh
+        d.transient_support = int(transient_support)
+        d.bc_lids, d.bc_vals = _dptr(bc_lids), _dptr(bc_vals)
+        d.n_bc = 0 if bc_lids is None else int(bc_lids.numel())
+        self._keep = (coef, bc_lids, bc_vals)   # the device arrays the model reads at each evaluation
+        mid = ctypes.c_int()
+        eng.check(eng.lib.txa_model_create(eng.h, mesh.id, ctypes.byref(d), ctypes.byref(mid)))
+        self.id = mid.value
+
+    def eval(self, x, *, x_dot=None, x_dot_dot=None, alpha=0.0, beta=1.0, gamma2=0.0, t=0.0, step_size=0.0,
+             f=None, W=None, stream=None):
+        """Fills f [ndofs] and / or W [nnz] (device tensors; None skips one)."""
+        m = self.mesh
+        check_tensors([("x", x, "float64", m.ndofs, True), ("x_dot", x_dot, "float64", m.ndofs, False),
+                       ("x_dot_dot", x_dot_dot, "float64", m.ndofs, False), ("f", f, "float64", m.ndofs, False),
+                       ("W", W, "float64", m.nnz, False)], self.eng.device)
+        i = ModelIn()
+        i.x, i.x_dot, i.x_dot_dot = _dptr(x), _dptr(x_dot), _dptr(x_dot_dot)
+        i.alpha, i.beta, i.gamma2, i.t, i.step_size = alpha, beta, gamma2, t, step_size
+        o = ModelOut()
+        o.f, o.W = _dptr(f), _dptr(W)
+        self.eng.check(self.eng.lib.txa_model_eval(self.eng.h, self.id, ctypes.byref(i), ctypes.byref(o),
+                                                   _stream(stream)))
+
+    def destroy(self):
+        if self.id is not None:
+            self.eng.check(self.eng.lib.txa_model_destroy(self.eng.h, self.id))
             self.id = None
 
 

@@ -398,7 +398,7 @@ __global__ __launch_bounds__(256) void k_lane_gather(int64_t nrows, const int64_
                                                      const int64_t *__restrict__ occ_ptr,
                                                      const int64_t *__restrict__ occ,
                                                      const int32_t *__restrict__ elem_lid,
-                                                     const double *__restrict__ ke, double beta,
+                                                     const double *__restrict__ ke, double beta, int add,
                                                      double *__restrict__ vals) {
   __shared__ __align__(16) double sK[8][64];
   __shared__ __align__(16) int32_t sL[8][64];
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(256) void k_lane_gather(int64_t nrows, const int64_
           for (int u = 0; u < 8; ++u) acc = lv[u] == c ? acc + kv[u] : acc;
         }
       }
-      if (jj < n) vals[r0 + jj] = beta * acc;
+      if (jj < n) vals[r0 + jj] = add ? vals[r0 + jj] + beta * acc : beta * acc;
     }
   }
 }
@@ -479,7 +479,8 @@ __global__ __launch_bounds__(256) void k_lane_gather_blocks(int64_t nrows, const
                                                             const int64_t *__restrict__ occ,
                                                             const int32_t *__restrict__ elem_lid,
                                                             const double *__restrict__ ke, GatherBlocks gb,
-                                                            double beta, double *__restrict__ vals) {
+                                                            int sel, double beta, int add,
+                                                            double *__restrict__ vals) {
   constexpr int NBP = 8;
   __shared__ __align__(16) double sK[8][64];
   __shared__ __align__(16) int32_t sL[8][64];
@@ -508,7 +509,7 @@ __global__ __launch_bounds__(256) void k_lane_gather_blocks(int64_t nrows, const
             const int nb = gb.nb[k];
             const int64_t loc = pe - gb.pos_begin[k], e = loc / nb;
             const int a = static_cast<int>(loc - e * nb), b = static_cast<int>(it % NBP);
-            if (b < nb) {
+            if (b < nb && (sel < 0 || k == sel)) {   // rows of the selected block only (sel >= 0)
               const int lo = a < b ? a : b, hi = a < b ? b : a;
               Li[h] = elem_lid[gb.pos_begin[k] + e * nb + b];
               Ki[h] = ke[gb.ke_begin[k] + e * (nb * (nb + 1) / 2) + lo * nb - (lo * (lo - 1)) / 2 + (hi - lo)];
@@ -540,24 +541,49 @@ __global__ __launch_bounds__(256) void k_lane_gather_blocks(int64_t nrows, const
           for (int u = 0; u < 8; ++u) acc = lv[u] == c ? acc + kv[u] : acc;
         }
       }
-      if (jj < n) vals[r0 + jj] = beta * acc;
+      if (jj < n) vals[r0 + jj] = add ? vals[r0 + jj] + beta * acc : beta * acc;
     }
   }
 }
 
-// Zero fill of an output the atomic kernels add into (a kernel rather than hipMemsetAsync,
-// so the assembly can be recorded in a HIP graph).
-__global__ void k_zero(double *__restrict__ p, int64_t n) {
+// y += x over n entries (accumulating a brick's assembly into the caller's container).
+__global__ void k_add(double *__restrict__ y, const double *__restrict__ x, int64_t n) {
   for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x)
-    p[i] = 0.0;
+    y[i] += x[i];
 }
 
-void zero_async(double *p, int64_t n, hipStream_t s) {
+void add_async(double *y, const double *x, int64_t n, hipStream_t s) {
   const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_zero, dim3(static_cast<unsigned>(std::max<int64_t>(blocks, 1))), dim3(256), 0, s, p, n);
+  hipLaunchKernelGGL(k_add, dim3(static_cast<unsigned>(std::max<int64_t>(blocks, 1))), dim3(256), 0, s, y, x, n);
   TXA_HIP(hipGetLastError());
 }
+
+// y = A x on a CRS matrix: 16 lanes per row, lane-strided over the row, then a fixed-order
+// lane reduction (one wave holds four rows).
+__global__ __launch_bounds__(256) void k_spmv(int64_t nrows, const int64_t *__restrict__ rowptr,
+                                              const int32_t *__restrict__ col, const double *__restrict__ vals,
+                                              const double *__restrict__ x, double *__restrict__ y) {
+  const int l = threadIdx.x % 16;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * (256 / 16);
+  for (int64_t row = (static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x) / 16; row < nrows; row += stride) {
+    double acc = 0.0;
+    for (int64_t k = rowptr[row] + l; k < rowptr[row + 1]; k += 16) acc += vals[k] * x[col[k]];
+#pragma unroll
+    for (int o = 8; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 16);
+    if (l == 0) y[row] = acc;
+  }
+}
+
+// Fill of an output (zero before the atomic kernels add into it; a kernel rather than
+// hipMemsetAsync, so the assembly can be recorded in a HIP graph).
+__global__ void k_fill(double *__restrict__ p, int64_t n, double v) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    p[i] = v;
+}
+
+void zero_async(double *p, int64_t n, hipStream_t s) { fill_async(p, n, 0.0, s); }
 
 // ============================================================================
 // Dirichlet rows: f_i = x_i - g_i; matrix row -> identity row
@@ -701,9 +727,10 @@ static size_t gather_ke_doubles(const Mesh &m, std::vector<int64_t> *begin) {
   return off;
 }
 
-size_t assemble_workspace_bytes(const Mesh &m, int physics, int precision, bool want_f, bool want_vals) {
+// Workspace of the structured kernels' own parts (the elasticity blocks), before the
+// staging arrays of an accumulating call.
+static size_t kernel_workspace_bytes(const Mesh &m, int physics, int precision, bool want_f, bool want_vals) {
   if (lane_gather_mesh(m, physics, precision)) return want_vals ? sizeof(double) * gather_ke_doubles(m, nullptr) : 0;
-
   txa_assembly_args a{};
   a.physics = physics;
   a.precision = precision;
@@ -711,8 +738,73 @@ size_t assemble_workspace_bytes(const Mesh &m, int physics, int precision, bool 
   return elastic_workspace_bytes(m, precision, want_f, want_vals);
 }
 
-void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws, int *err) {
+static size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+// An accumulating call on a brick stages the kernels' rows (they store each row once) in the
+// workspace and adds them to f and vals afterwards.
+static bool stages_rows(const Mesh &m, int physics, int precision, bool accumulate) {
+  txa_assembly_args a{};
+  a.physics = physics;
+  a.precision = precision;
+  return accumulate && (uses_q1_path(m, physics, precision) || elastic_supported(m, a));
+}
+
+size_t assemble_workspace_bytes(const Mesh &m, int physics, int precision, bool want_f, bool want_vals,
+                                bool accumulate) {
+  size_t b = round256(kernel_workspace_bytes(m, physics, precision, want_f, want_vals));
+  if (stages_rows(m, physics, precision, accumulate))
+    b += (want_f ? round256(sizeof(double) * m.ndofs) : 0) + (want_vals ? round256(sizeof(double) * m.nnz) : 0);
+  return b;
+}
+
+void fill_async(double *p, int64_t n, double v, hipStream_t s) {
+  TXA_REQUIRE(n >= 0 && (n == 0 || p), TXA_ERR_INVALID, "null array or negative length");
+  if (n == 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_fill, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, p, n, v);
+  TXA_HIP(hipGetLastError());
+}
+
+void spmv(const Mesh &m, const double *vals, const double *x, double *y, hipStream_t s) {
+  TXA_REQUIRE(m.has_graph, TXA_ERR_LOGIC, "spmv before graph_build");
+  TXA_REQUIRE(vals && x && y, TXA_ERR_INVALID, "null argument");
+  const int64_t blocks = std::min<int64_t>((m.ndofs + 15) / 16, int64_t(1) << 20);
+  hipLaunchKernelGGL(k_spmv, dim3(static_cast<unsigned>(std::max<int64_t>(blocks, 1))), dim3(256), 0, s, m.ndofs,
+                     m.rowptr.p, m.col.p, vals, x, y);
+  TXA_HIP(hipGetLastError());
+}
+
+void apply_dirichlet(const Mesh &m, const int32_t *lids, const double *g, int64_t n, const double *x, double *f,
+                     double *vals, hipStream_t s, int *err) {
+  TXA_REQUIRE(m.has_graph, TXA_ERR_LOGIC, "Dirichlet rows before graph_build");
+  TXA_REQUIRE(n >= 0 && (n == 0 || lids), TXA_ERR_INVALID, "lids missing");
+  TXA_REQUIRE(!f || x, TXA_ERR_INVALID, "x is required for the residual rows");
+  if (n == 0) return;
+  const int64_t per = 4;  // rows per 256-thread block
+  hipLaunchKernelGGL(k_dirichlet, dim3(static_cast<unsigned>((n + per - 1) / per)), dim3(256), 0, s, n, m.ndofs, lids,
+                     g, x, m.rowptr.p, m.col.p, f, vals, err);
+  TXA_HIP(hipGetLastError());
+}
+
+void assemble(Mesh &m, const txa_assembly_args &a_in, hipStream_t s, void *ws, int *err) {
   TXA_REQUIRE(m.has_graph, TXA_ERR_LOGIC, "assemble before graph_build");
+  TXA_REQUIRE(a_in.block >= 0 && a_in.block <= static_cast<int>(m.blocks.size()), TXA_ERR_INVALID,
+              "block " + std::to_string(a_in.block) + ": the mesh has " + std::to_string(m.blocks.size()) +
+                  " element block(s) (0 = all, b = block b - 1)");
+  txa_assembly_args a = a_in;
+  const bool add = a_in.accumulate != 0;
+  // bricks accumulate through staging rows in the workspace, added to f and vals after the kernels
+  const bool stage = stages_rows(m, a.physics, a.precision, add);
+  char *wsc = static_cast<char *>(ws);
+  double *stage_f = nullptr, *stage_v = nullptr;
+  if (stage) {
+    size_t off = round256(kernel_workspace_bytes(m, a.physics, a.precision, a.f != nullptr, a.vals != nullptr));
+    TXA_REQUIRE(wsc != nullptr, TXA_ERR_LOGIC, "an accumulating assembly needs its workspace");
+    if (a.f) { stage_f = reinterpret_cast<double *>(wsc + off); off += round256(sizeof(double) * m.ndofs); }
+    if (a.vals) stage_v = reinterpret_cast<double *>(wsc + off);
+    a.f = stage_f ? stage_f : a.f;
+    a.vals = stage_v ? stage_v : a.vals;
+  }
   TXA_REQUIRE(a.x != nullptr, TXA_ERR_INVALID, "x is required");
   TXA_REQUIRE(a.coef != nullptr, TXA_ERR_INVALID, "coef is required");
   TXA_REQUIRE(a.physics == TXA_DIFFUSION || a.physics == TXA_ELASTICITY, TXA_ERR_INVALID, "unknown physics");
@@ -745,10 +837,11 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws, int 
     const bool gather = a.vals != nullptr && ws != nullptr && lane_gather_mesh(m, a.physics, a.precision);
     std::vector<int64_t> ke_begin;
     if (gather) gather_ke_doubles(m, &ke_begin);
-    if (a.f) zero_async(a.f, m.ndofs, s);
-    if (a.vals && !gather) zero_async(a.vals, m.nnz, s);
+    if (a.f && !add) zero_async(a.f, m.ndofs, s);
+    if (a.vals && !gather && !add) zero_async(a.vals, m.nnz, s);
     // each element block with its own basis and cubature, all adding into the same rows
     for (size_t bi = 0; bi < m.blocks.size(); ++bi) {
+      if (a.block > 0 && static_cast<int>(bi) != a.block - 1) continue;
       const MeshBlock &b = m.blocks[bi];
       TXA_REQUIRE(m.block_ref[b.topo].p != nullptr, TXA_ERR_LOGIC, "reference tables missing");
       RefElem r;
@@ -786,10 +879,10 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws, int 
       const double *ke = static_cast<const double *>(ws);
       if (m.blocks.size() == 1 && m.topo == TXA_HEX8)
         hipLaunchKernelGGL((k_lane_gather<8>), dim3(static_cast<unsigned>(gblocks)), dim3(256), 0, s, m.ndofs,
-                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, ke, gbeta, a.vals);
+                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, ke, gbeta, int(add), a.vals);
       else if (m.blocks.size() == 1 && m.topo == TXA_TET4)
         hipLaunchKernelGGL((k_lane_gather<4>), dim3(static_cast<unsigned>(gblocks)), dim3(256), 0, s, m.ndofs,
-                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, ke, gbeta, a.vals);
+                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, ke, gbeta, int(add), a.vals);
       else {
         GatherBlocks gb{};
         gb.n = static_cast<int>(m.blocks.size());
@@ -799,10 +892,17 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws, int 
           gb.nb[k] = lane_nb(m.blocks[k].topo);
         }
         hipLaunchKernelGGL(k_lane_gather_blocks, dim3(static_cast<unsigned>(gblocks)), dim3(256), 0, s, m.ndofs,
-                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, ke, gb, gbeta, a.vals);
+                           m.rowptr.p, m.col.p, m.occ_ptr.p, m.occ.p, m.elem_lid.p, ke, gb, a.block - 1, gbeta,
+                           int(add), a.vals);
       }
       TXA_HIP(hipGetLastError());
     }
+  }
+  if (stage) {
+    if (stage_f) add_async(a_in.f, stage_f, m.ndofs, s);
+    if (stage_v) add_async(a_in.vals, stage_v, m.nnz, s);
+    a.f = a_in.f;
+    a.vals = a_in.vals;
   }
   // Neumann sides, one launch each (stream order keeps the sums of shared edge points in a fixed order)
   for (int k = 0; a.f && k < a.n_neumann; ++k) {
@@ -823,12 +923,7 @@ void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws, int 
     hipLaunchKernelGGL(k_neumann, dim3(static_cast<unsigned>((npts + 255) / 256)), dim3(256), 0, s, np);
     TXA_HIP(hipGetLastError());
   }
-  if (a.n_bc > 0) {
-    const int64_t per = 4;  // rows per 256-thread block
-    hipLaunchKernelGGL(k_dirichlet, dim3(static_cast<unsigned>((a.n_bc + per - 1) / per)), dim3(256), 0, s, a.n_bc,
-                       m.ndofs, a.bc_lids, a.bc_vals, a.x, m.rowptr.p, m.col.p, a.f, a.vals, err);
-    TXA_HIP(hipGetLastError());
-  }
+  if (a.n_bc > 0) apply_dirichlet(m, a.bc_lids, a.bc_vals, a.n_bc, a.x, a.f, a.vals, s, err);
 }
 
 }  // namespace txa

@@ -285,7 +285,7 @@ int txa_assemble(txa_ctx ctx, int mesh_id, const txa_assembly_args *args, void *
     Mesh &m = ctx->mesh(mesh_id);
     const hipStream_t s = static_cast<hipStream_t>(stream);
     const size_t wsb = txa::assemble_workspace_bytes(m, args->physics, args->precision, args->f != nullptr,
-                                                     args->vals != nullptr);
+                                                     args->vals != nullptr, args->accumulate != 0);
     void *ws = wsb ? ctx->ws_alloc(wsb, s) : nullptr;
     try {
       txa::assemble(m, *args, s, ws, ctx->err.p);
@@ -302,13 +302,106 @@ int txa_assemble_noalloc(txa_ctx ctx, int mesh_id, const txa_assembly_args *args
     TXA_REQUIRE(args, TXA_ERR_INVALID, "null args");
     Mesh &m = ctx->mesh(mesh_id);
     const size_t wsb = txa::assemble_workspace_bytes(m, args->physics, args->precision, args->f != nullptr,
-                                                     args->vals != nullptr);
+                                                     args->vals != nullptr, args->accumulate != 0);
     TXA_REQUIRE(wsb <= kScratchBytes, TXA_ERR_NOMEM,
                 "txa_assemble_noalloc: this assembly needs " + std::to_string(wsb) +
                     " bytes of workspace; the static workspace holds " + std::to_string(kScratchBytes) +
                     " (use txa_assemble)");
     TXA_REQUIRE(ctx->scratch != nullptr, TXA_ERR_LOGIC, "static workspace not set up");
     txa::assemble(m, *args, static_cast<hipStream_t>(stream), wsb ? ctx->scratch : nullptr, ctx->err.p);
+  });
+}
+
+int txa_apply_dirichlet(txa_ctx ctx, int mesh_id, const int32_t *lids, const double *values, int64_t n,
+                        const double *x, double *f, double *vals, void *stream) {
+  return guarded(ctx, [&] {
+    txa::apply_dirichlet(ctx->mesh(mesh_id), lids, values, n, x, f, vals, static_cast<hipStream_t>(stream),
+                         ctx->err.p);
+  });
+}
+
+int txa_fill(txa_ctx ctx, double *p, int64_t n, double value, void *stream) {
+  return guarded(ctx, [&] { txa::fill_async(p, n, value, static_cast<hipStream_t>(stream)); });
+}
+
+int txa_spmv(txa_ctx ctx, int mesh_id, const double *vals, const double *x, double *y, void *stream) {
+  return guarded(ctx, [&] { txa::spmv(ctx->mesh(mesh_id), vals, x, y, static_cast<hipStream_t>(stream)); });
+}
+
+int txa_model_create(txa_ctx ctx, int mesh_id, const txa_model_desc *desc, int *model_id) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(desc && model_id, TXA_ERR_INVALID, "null argument");
+    Mesh &m = ctx->mesh(mesh_id);
+    TXA_REQUIRE(m.has_graph, TXA_ERR_LOGIC, "a model needs the mesh's DOF map and graph");
+    TXA_REQUIRE(desc->coef, TXA_ERR_INVALID, "coef is required");
+    TXA_REQUIRE(desc->n_bc == 0 || desc->bc_lids, TXA_ERR_INVALID, "bc_lids missing");
+    txa::Model md;
+    md.mesh_id = mesh_id;
+    md.desc = *desc;
+    const int id = ctx->next_id++;
+    ctx->models[id] = md;
+    *model_id = id;
+  });
+}
+
+// panzer::ModelEvaluator::evalModelImpl_basic (disc-fe/src/Panzer_ModelEvaluator_impl.hpp:1715-1840)
+// with setupAssemblyInArgs (:725-780): a model without transient support fills the steady
+// system (alpha 0, beta 1); with it, alpha / beta / gamma2 come from the in-arguments and the
+// transient terms are evaluated when x_dot is given.
+int txa_model_eval(txa_ctx ctx, int model_id, const txa_model_in *in, const txa_model_out *out, void *stream) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(in && out, TXA_ERR_INVALID, "null argument");
+    auto it = ctx->models.find(model_id);
+    TXA_REQUIRE(it != ctx->models.end(), TXA_ERR_INVALID, "unknown model id " + std::to_string(model_id));
+    const txa::Model &md = it->second;
+    if (!out->f && !out->W) return;                 // nothing asked for
+    const txa_model_desc &d = md.desc;
+    TXA_REQUIRE(in->x, TXA_ERR_INVALID, "x is required");
+    TXA_REQUIRE(!in->x_dot || d.transient_support, TXA_ERR_INVALID,
+                "x_dot given to a model built without transient support");
+    Mesh &m = ctx->mesh(md.mesh_id);
+    txa_assembly_args a{};
+    a.physics = d.physics;
+    a.precision = d.precision;
+    a.coef = d.coef;
+    a.nu = d.nu;
+    a.rho = d.rho;
+    for (int i = 0; i < 3; ++i) a.src[i] = d.src[i];
+    a.rayleigh_a = d.rayleigh_a;
+    a.rayleigh_b = d.rayleigh_b;
+    a.x = in->x;
+    if (d.transient_support) {
+      a.alpha = in->alpha;
+      a.beta = in->beta;
+      a.gamma2 = in->gamma2;
+      a.transient = in->x_dot != nullptr;
+      a.xdot = in->x_dot;
+      a.xddot = in->x_dot_dot;
+    } else {
+      a.alpha = 0.0;
+      a.beta = 1.0;
+    }
+    a.f = out->f;
+    a.vals = out->W;
+    a.bc_lids = d.bc_lids;
+    a.bc_vals = d.bc_vals;
+    a.n_bc = d.n_bc;
+    const hipStream_t s = static_cast<hipStream_t>(stream);
+    const size_t wsb = txa::assemble_workspace_bytes(m, a.physics, a.precision, a.f != nullptr, a.vals != nullptr);
+    void *ws = wsb ? ctx->ws_alloc(wsb, s) : nullptr;
+    try {
+      txa::assemble(m, a, s, ws, ctx->err.p);
+    } catch (...) {
+      if (ws) (void)hipFreeAsync(ws, s);
+      throw;
+    }
+    if (ws) TXA_HIP(hipFreeAsync(ws, s));
+  });
+}
+
+int txa_model_destroy(txa_ctx ctx, int model_id) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(ctx->models.erase(model_id) == 1, TXA_ERR_INVALID, "unknown model id " + std::to_string(model_id));
   });
 }
 

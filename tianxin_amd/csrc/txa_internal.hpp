@@ -114,10 +114,17 @@ void batch_free(Batch *b);
 struct Server;
 void serve_free(Server *s);
 
+// A ModelEvaluator binding (txa_model_create): the mesh and the physics of its evaluations.
+struct Model {
+  int mesh_id = 0;
+  txa_model_desc desc{};
+};
+
 struct Context {
   int device = 0;
   std::string last_error;
   std::map<int, std::unique_ptr<Mesh>> meshes;
+  std::map<int, Model> models;
   std::map<int, Batch *> batches;
   std::map<int, Server *> servers;
   int next_id = 1;
@@ -192,7 +199,12 @@ void upload_block_refs(Mesh &m, hipStream_t s);   // tables of the mesh's topolo
 // the work on s (may be null when that size is 0)
 // err: device (or host-mapped) int set to 1 when a Dirichlet LID lies outside the map (may be null)
 void assemble(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws, int *err = nullptr);
-size_t assemble_workspace_bytes(const Mesh &m, int physics, int precision, bool want_f, bool want_vals);
+size_t assemble_workspace_bytes(const Mesh &m, int physics, int precision, bool want_f, bool want_vals,
+                                bool accumulate = false);
+void apply_dirichlet(const Mesh &m, const int32_t *lids, const double *g, int64_t n, const double *x, double *f,
+                     double *vals, hipStream_t s, int *err);
+void spmv(const Mesh &m, const double *vals, const double *x, double *y, hipStream_t s);
+void fill_async(double *p, int64_t n, double v, hipStream_t s);
 // element-centric elasticity (txa_elastic.hip)
 void node_perm_build(Mesh &m, hipStream_t s);
 bool elastic_supported(const Mesh &m, const txa_assembly_args &a);

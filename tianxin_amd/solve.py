@@ -1,35 +1,40 @@
-"""Solver hand-off: the assembled CRS as a device sparse matrix, and one Newton step.
+"""Solver hand-off: one Newton step on top of the ModelEvaluator-shaped C ABI.
 
 The reference hands f and the Jacobian W to its nonlinear solvers through the
-ModelEvaluator (disc-fe/src/Panzer_ModelEvaluator_impl.hpp:725-939, evalModel
-with x, x_dot, alpha, beta in and f, W out).  Here the graph and values stay
-on the device: `crs_matrix` wraps them as a sparse CSR tensor (SpMV through
-rocSPARSE), and `newton_step` solves J d = -R with Jacobi-preconditioned
-BiCGStab (the Dirichlet rows make J unsymmetric) and returns x + d.
+ModelEvaluator (disc-fe/src/Panzer_ModelEvaluator_impl.hpp:725-780, 1715-1840:
+evalModel with x, x_dot, alpha, beta in and f, W out).  Here `txa_model_eval`
+(tianxin_amd.Model) fills f and W on the device, `txa_spmv` applies W, and
+`newton_step` solves W d = -f with Jacobi-preconditioned BiCGStab (the Dirichlet
+rows make W unsymmetric) and returns x + d.  The vector updates of the Krylov
+loop run as torch operations on the same device arrays.
 """
 from __future__ import annotations
 
 import torch
 
 
-def crs_matrix(mesh, vals):
-    """Device CSR tensor of the mesh's graph (rows and columns in LID order) with values vals [nnz]."""
+def diagonal(mesh, W):
+    """The diagonal of W (CRS values on the mesh's graph) as a device vector."""
     rp, col = mesh.graph()
-    crow = torch.as_tensor(rp, device=vals.device, dtype=torch.int64)
-    cols = torch.as_tensor(col, device=vals.device, dtype=torch.int64)
-    return torch.sparse_csr_tensor(crow, cols, vals, size=(mesh.ndofs, mesh.ndofs))
+    rows = torch.repeat_interleave(torch.arange(mesh.ndofs, device=W.device),
+                                   torch.as_tensor(rp[1:] - rp[:-1], device=W.device))
+    on = rows == torch.as_tensor(col, device=W.device, dtype=torch.int64)
+    d = torch.zeros(mesh.ndofs, device=W.device, dtype=W.dtype)
+    d.index_add_(0, rows[on], W[on])
+    return d
 
 
-def bicgstab(A, b, tol=1e-12, maxiter=2000):
-    """Solves A x = b (A a device CSR tensor) by BiCGStab with the Jacobi preconditioner.
-    Returns (x, relative residual, iterations)."""
-    d = A.to_sparse_coo().coalesce()
-    idx = d.indices()
-    diag = torch.zeros(A.shape[0], device=b.device, dtype=b.dtype)
-    on = idx[0] == idx[1]
-    diag.index_add_(0, idx[0][on], d.values()[on])
+def bicgstab(mesh, W, b, tol=1e-12, maxiter=2000):
+    """Solves W x = b (W the CRS values on the mesh's graph, applied by txa_spmv) by BiCGStab
+    with the Jacobi preconditioner.  Returns (x, relative residual, iterations)."""
+    diag = diagonal(mesh, W)
     dinv = torch.where(diag != 0, 1.0 / diag, torch.ones_like(diag))
-    mv = lambda v: (A @ v.unsqueeze(1)).squeeze(1)
+
+    def mv(v):
+        y = torch.empty_like(v)
+        mesh.spmv(W, v.contiguous(), y)
+        return y
+
     x = torch.zeros_like(b)
     r = b.clone()
     r0 = r.clone()
@@ -37,6 +42,7 @@ def bicgstab(A, b, tol=1e-12, maxiter=2000):
     v = torch.zeros_like(b)
     p = torch.zeros_like(b)
     nb = float(torch.linalg.vector_norm(b)) or 1.0
+    res = 1.0
     for it in range(1, maxiter + 1):
         rho_new = float(torch.dot(r0, r))
         beta = (rho_new / rho) * (alpha / omega)
@@ -57,7 +63,12 @@ def bicgstab(A, b, tol=1e-12, maxiter=2000):
     return x, res, maxiter
 
 
-def newton_step(mesh, x, f, vals, tol=1e-12, maxiter=2000):
-    """x + d with J d = -R, J the assembled values on the mesh's graph and R the residual f."""
-    d, res, it = bicgstab(crs_matrix(mesh, vals), -f, tol=tol, maxiter=maxiter)
-    return x + d, res, it
+def newton_step(model, x, tol=1e-12, maxiter=2000, **eval_kw):
+    """One Newton step of the model at x: f and W from txa_model_eval, then x + d with
+    W d = -f.  Returns (x + d, f, W, relative linear residual, iterations)."""
+    m = model.mesh
+    f = torch.empty(m.ndofs, device=x.device, dtype=torch.float64)
+    W = torch.empty(m.nnz, device=x.device, dtype=torch.float64)
+    model.eval(x, f=f, W=W, **eval_kw)
+    d, res, it = bicgstab(m, W, -f, tol=tol, maxiter=maxiter)
+    return x + d, f, W, res, it
