@@ -9,6 +9,11 @@ whole mesh (volume kernel + Dirichlet rows) with inputs resident in HBM.
 The timed form integrates elements in FP32 (geometry and the scatter
 accumulation in FP64); the plain FP64 GPU path runs beside it and the max
 relative error of FP32 vs FP64 is reported for R and J.  Prints one JSON line.
+
+After the headline, the default run appends compact summaries of the other
+workloads under "other_workloads" (Q2 hex27 elasticity 64^3, a batch of 512
+Poisson meshes, transient elasticity served at 20 requests/s); --no-extra skips
+them, --workload runs one of them alone with its full line.
 """
 from __future__ import annotations
 
@@ -123,23 +128,60 @@ def _counter_per_dispatch(d, counter, pred, last=None):
     return sum(per[k] for k in keys) / len(keys), len(keys)
 
 
+PROF_STEPS, PROF_WARMUP = 20, 3   # the stats pass: bench.py --steps 20 --warmup 3
+
+
+def _trace_durations(d, kernel):
+    """Durations (us) of the dispatches of `kernel`, in launch order, from a kernel trace."""
+    rows = [r for r in _rows(d, "kernel_trace.csv") if _match(r.get("Kernel_Name", ""), kernel)]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    return [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in rows]
+
+
 def summarize_profiles(stats_dir, fetch_dir, write_dir):
     """Writes profiles/q1_hex8_diffusion_fp32_256_kernel_stats.md and the traffic JSON.
 
-    FETCH_SIZE and WRITE_SIZE are in KiB per dispatch; each is scaled by what the
-    same counter reads for a 1 GiB elementwise kernel (dst = 2 src: COPY_BYTES read,
+    The kernel times come from the kernel trace of the stats pass.  bench.py launches each
+    form's volume kernel in this order: PROF_WARMUP untimed steps, PROF_STEPS timed steps
+    (volume kernel + Dirichlet rows), one untimed and PROF_STEPS timed volume-only steps,
+    and one last full step; the averages below are over the timed dispatches only.
+    FETCH_SIZE and WRITE_SIZE are in KiB per dispatch; each is scaled by what the same
+    counter reads for a 1 GiB elementwise kernel (dst = 2 src: COPY_BYTES read,
     COPY_BYTES written)."""
+    import numpy as np
     os.makedirs(os.path.join(ROOT, "profiles"), exist_ok=True)
     stats = _rows(stats_dir, "kernel_stats.csv")
+    K, W = PROF_STEPS, PROF_WARMUP
     cmd_stats = ("rocprofv3 --kernel-trace --stats --output-format csv -d <out>/prof_stats -- "
-                 "python3 bench.py --steps 5 --warmup 1 --no-cpu")
+                 f"python3 bench.py --steps {K} --warmup {W} --no-cpu --no-extra")
     lines = ["# Kernel statistics: Q1 Poisson hex8 256^3 (bench.py workload)", "",
-             "Command (from the repository root, on one MI355X):", "", "    " + cmd_stats, "",
-             "| kernel | calls | average us | min us | max us | share % |", "|---|---:|---:|---:|---:|---:|"]
+             "Command (from the repository root, on one MI355X):", "", "    " + cmd_stats, ""]
+    timed = {}
+    for label, kern in (("FP32 form", FP32_KERNEL), ("FP64 path", FP64_KERNEL)):
+        d = _trace_durations(stats_dir, kern)
+        if len(d) != W + 2 * K + 2:
+            raise SystemExit(f"{kern}: {len(d)} dispatches in the trace, expected {W + 2 * K + 2}")
+        with_bc, alone = np.array(d[W:W + K]), np.array(d[W + K + 1:W + 2 * K + 1])
+        timed[label] = (kern, with_bc, alone, d)
+    dirich = [t for r in _rows(stats_dir, "kernel_trace.csv") if "k_dirichlet" in r.get("Kernel_Name", "")
+              for t in [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3]]
+    nel = 256 ** 3
+    lines += ["Per-dispatch durations of the volume kernel from the kernel trace (us), timed dispatches only:", "",
+              "| form | kernel | timed steps (with Dirichlet pass) mean / min / max | "
+              "volume-only timed steps mean / min / max | HBM fraction at 408 B/element (volume-only mean) |",
+              "|---|---|---|---|---:|"]
+    for label, (kern, wb, al, _) in timed.items():
+        lines.append(f"| {label} | `{kern}` | {wb.mean():.1f} / {wb.min():.1f} / {wb.max():.1f} | "
+                     f"{al.mean():.1f} / {al.min():.1f} / {al.max():.1f} | "
+                     f"{BYTES_PER_ELEM * nel / (al.mean() / 1e6) / HBM_PEAK:.3f} |")
+    if dirich:
+        lines += ["", f"`k_dirichlet` (the Dirichlet-row pass of each full step): {len(dirich)} calls, "
+                      f"mean {np.mean(dirich):.1f} us"]
+    lines += ["", "All dispatches of the run (setup and checks included), from the stats file:", "",
+              "| kernel | calls | average us | min us | max us | share % |", "|---|---:|---:|---:|---:|---:|"]
     for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:12]:
         lines.append(f"| `{r['Name'][:110]}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.1f} | "
                      f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} |")
-    ring = [r for r in stats if _match(r["Name"], FP32_KERNEL)]
     # torch.mul(src, 2, out=dst), the last three dispatches of the run: reads and writes COPY_BYTES
     copy_pred = lambda n: "MulFunctor<double>" in n
     kf, nf = _counter_per_dispatch(fetch_dir, "FETCH_SIZE", lambda n: _match(n, FP32_KERNEL))
@@ -148,14 +190,14 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir):
     cw, _ = _counter_per_dispatch(write_dir, "WRITE_SIZE", copy_pred, last=3)
     ff, fw = COPY_BYTES / (cf * 1024.0), COPY_BYTES / (cw * 1024.0)
     fetch_b, write_b = kf * 1024.0 * ff, kw * 1024.0 * fw
-    nel = 256 ** 3
+    al = timed["FP32 form"][2]
     tr = {"kernel": FP32_KERNEL, "workload": "Q1 Poisson hex8 256^3, FP32 integration",
           "fetch_kib_per_dispatch": kf, "write_kib_per_dispatch": kw, "dispatches": [nf, nw],
           "copy_fetch_kib": cf, "copy_write_kib": cw, "fetch_factor": ff, "write_factor": fw,
           "fetch_bytes": fetch_b, "write_bytes": write_b, "bytes_per_launch": fetch_b + write_b,
-          "algorithmic_bytes": BYTES_PER_ELEM * nel,
+          "algorithmic_bytes": BYTES_PER_ELEM * nel, "kernel_us_volume_only_timed_mean": float(al.mean()),
           "commands": [f"rocprofv3 --pmc {c} --output-format csv -d <out>/pmc_{c.lower()} -- "
-                       "python3 bench.py --steps 2 --warmup 1 --no-cpu --copy-calibration"
+                       "python3 bench.py --steps 2 --warmup 1 --no-cpu --no-extra --copy-calibration"
                        for c in ("FETCH_SIZE", "WRITE_SIZE")]}
     json.dump(tr, open(TRAFFIC_FILE, "w"), indent=1)
     lines += ["", "## HBM traffic of the FP32 kernel (rocprofv3 --pmc, one pass per counter)", ""]
@@ -167,9 +209,6 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir):
               f"- traffic per launch: {fetch_b / 1e9:.3f} GB read + {write_b / 1e9:.3f} GB written = "
               f"{(fetch_b + write_b) / 1e9:.3f} GB; algorithmic {BYTES_PER_ELEM * nel / 1e9:.3f} GB "
               f"({BYTES_PER_ELEM} B/element x {nel} elements)"]
-    if ring:
-        lines += [f"- `{FP32_KERNEL}` average duration in the stats pass: "
-                  f"{float(ring[0]['AverageNs']) / 1e3:.1f} us over {ring[0]['Calls']} calls"]
     out = os.path.join(ROOT, "profiles", "q1_hex8_diffusion_fp32_256_kernel_stats.md")
     open(out, "w").write("\n".join(lines) + "\n")
     print(out, TRAFFIC_FILE)
@@ -269,7 +308,8 @@ def bench_batch(args):
         tuned_ms = b.tune(args_list, reps=3)
         info = b.info()
         t = time_it(lambda: b.assemble(args_list, use_graph=True), args.steps, 2)
-        out[name] = {"ms_per_step": t, "elements_per_s": nel / (t / 1e3), "tuner_best_ms": tuned_ms,
+        out[name] = {"ms_per_step": t, "elements_per_s": nel / (t / 1e3), "cells_per_s": cells / (t / 1e3),
+                     "tuner_best_ms": tuned_ms,
                      "slab": info["slab"], "grid": info["grid"], "work_items": info["items"]}
         log(f"{name}: {t:.3f} ms (slab {info['slab']}, grid {info['grid']})")
         b.destroy()
@@ -278,14 +318,20 @@ def bench_batch(args):
         for i, m in enumerate(ms):
             m.assemble(precision=txa.FP64, **kw(i, txa.FP64))
     t = time_it(plain, max(2, args.steps // 4), 1)
-    out["fp64_per_mesh"] = {"ms_per_step": t, "elements_per_s": nel / (t / 1e3)}
+    out["fp64_per_mesh"] = {"ms_per_step": t, "elements_per_s": nel / (t / 1e3), "cells_per_s": cells / (t / 1e3)}
     log(f"fp64 one launch per mesh: {t:.3f} ms")
     # FP32 batch vs FP64, per mesh
-    worst = {"R": 0.0, "J": 0.0}
+    worst = {"R": 0.0, "J": 0.0, "R_entrywise_over_1e-12_max": 0.0, "J_entrywise_over_1e-12_max": 0.0}
+
+    def ent(a, b, tau=1e-12):
+        mask = b.abs() > tau * b.abs().max()
+        return float(((a - b).abs()[mask] / b.abs()[mask]).max())
     for i in range(nm):
         (f32, v32), (f64, v64) = per[i]["bufs"][txa.FP32], per[i]["bufs"][txa.FP64]
         worst["R"] = max(worst["R"], float((f32 - f64).abs().max() / f64.abs().max()))
         worst["J"] = max(worst["J"], float((v32 - v64).abs().max() / v64.abs().max()))
+        worst["R_entrywise_over_1e-12_max"] = max(worst["R_entrywise_over_1e-12_max"], ent(f32, f64))
+        worst["J_entrywise_over_1e-12_max"] = max(worst["J_entrywise_over_1e-12_max"], ent(v32, v64))
     value = out["fp32_batch"]["elements_per_s"]
     line = {"metric": "elements/s (residual+Jacobian), batched", "value": value, "unit": "elements/s",
             "n_gpus": 1, "steps": args.steps, "ms_per_step": out["fp32_batch"]["ms_per_step"],
@@ -295,7 +341,9 @@ def bench_batch(args):
                        "cells": cells, "elements": nel, "hex8_meshes": int(np.sum(kinds == txa.HEX8))},
             "forms": out, "max_rel_err_fp32_vs_fp64_per_mesh": worst,
             "speedup_vs_fp64_per_mesh": value / out["fp64_per_mesh"]["elements_per_s"]}
-    print(json.dumps(line), flush=True)
+    for m in ms:
+        m.destroy()
+    return line
 
 
 Q2_FLOP_PER_ELEM = 2 * 81 * 162 * 81 + 2 * 162 * 6 * 81   # dense B^T (W B), B in R^{162 x 81}
@@ -362,6 +410,8 @@ def bench_q2(args):
         mask = b.abs() > tau * b.abs().max()
         return float(((a - b).abs()[mask] / b.abs()[mask]).max())
     errs = {k: {"R_normwise": rel(res[k][0], f64), "J_normwise": rel(res[k][1], v64),
+                "R_entrywise_over_1e-12_max": ent(res[k][0], f64, 1e-12),
+                "J_entrywise_over_1e-12_max": ent(res[k][1], v64, 1e-12),
                 "R_entrywise_over_1e-3_max": ent(res[k][0], f64), "J_entrywise_over_1e-3_max": ent(res[k][1], v64)}
             for k in ("split_bf16", "fp32")}
     del res
@@ -391,17 +441,20 @@ def bench_q2(args):
                        "nu": 0.3, "dirichlet": "u=0 on x=0"},
             "forms": out, "max_rel_err_vs_fp64": errs, "rigid_motion_residual_rel": rigid,
             "speedup_split_vs_fp64": best["elements_per_s"] / out["fp64"]["elements_per_s"]}
-    print(json.dumps(line), flush=True)
+    m.destroy()
+    return line
 
 
 def bench_serve(args):
     """Serving: Newmark steps of transient hex8 elasticity (beta 0.25, gamma 0.5,
     Rayleigh a 0.01, b 0.001, dt 1e-3) on a pool of resident meshes with n^3 cells,
-    n log-uniform in [16, 128].  Phase 1: Poisson arrivals at --qps, p50/p99
-    submit-to-completion latency.  Phase 2: the same requests queued at once
-    (throughput at saturation), beside the plain FP64 path (the same requests one
-    txa_assemble after another on one stream).  Each request carries a seeded state
-    U[-1e-3, 1e-3] generated on the device."""
+    n log-uniform in [16, 128].  For each served precision (FP64, then FP32): phase 1,
+    Poisson arrivals at --qps, p50/p99 submit-to-completion latency; phase 2, the same
+    requests queued at once (throughput at saturation).  Beside them, the plain FP64
+    path: the same requests one txa_assemble after another on one stream.  Each
+    request carries a seeded state U[-1e-3, 1e-3] generated on the device.  The
+    served results of the first 8 requests are checked against the plain FP64 path
+    (normwise and entrywise over |ref| > 1e-12 max|ref|)."""
     import threading
 
     import numpy as np
@@ -410,7 +463,7 @@ def bench_serve(args):
     import tianxin_amd as txa
     from tianxin_amd import synth
 
-    prec = {"fp64": txa.FP64, "fp32": txa.FP32}[args.serve_precision]
+    head = {"fp64": txa.FP64, "fp32": txa.FP32}[args.serve_precision]
     eng = txa.Engine(0)
     sizes = log_uniform_sizes(41, args.pool, 16, 128)
     t0 = time.time()
@@ -437,10 +490,9 @@ def bench_serve(args):
                        for _ in range(3)])
     torch.cuda.synchronize()
     nel = sum(int(pool[i][0].nelem) for i in pick)
-    kw = dict(physics=txa.ELASTICITY, dt=1e-3, newmark=(0.25, 0.5), nu=0.3, rho=1.0, rayleigh=(0.01, 0.001),
-              precision=prec)
+    kw = dict(physics=txa.ELASTICITY, dt=1e-3, newmark=(0.25, 0.5), nu=0.3, rho=1.0, rayleigh=(0.01, 0.001))
 
-    def serve(arrivals):
+    def serve(prec, arrivals):
         srv = txa.Server(eng, [p_[0] for p_ in pool], hbm_budget=int(args.budget_gb * 2 ** 30), slots=args.slots)
         rids = [None] * args.requests
         ready = threading.Semaphore(0)
@@ -465,7 +517,7 @@ def bench_serve(args):
                     time.sleep(delay)
             m, E, bc = pool[pick[r]]
             u, v, a = states[r]
-            rids[r] = srv.submit(m, coef=E, u=u, v=v, a=a, bc_lids=bc, **kw)
+            rids[r] = srv.submit(m, coef=E, u=u, v=v, a=a, bc_lids=bc, precision=prec, **kw)
             ready.release()
         th.join()
         wall = time.perf_counter() - t_start
@@ -473,15 +525,12 @@ def bench_serve(args):
         srv.destroy()
         return st, wall
 
-    lat, wall_lat = serve(True)
-    sat, wall_sat = serve(False)
-    log(f"latency phase: p50 {lat['p50_ms']:.2f} ms p99 {lat['p99_ms']:.2f} ms; saturated: "
-        f"{nel / wall_sat:.3g} elements/s")
     # plain FP64 path: the same requests back to back on one stream
     outs = {}
     for i, (m, E, bc) in enumerate(pool):
         outs[i] = (torch.empty(m.ndofs, device="cuda", dtype=torch.float64),
                    torch.empty(m.nnz, device="cuda", dtype=torch.float64))
+
     def plain(r):
         m, E, bc = pool[pick[r]]
         u, v, a = states[r]
@@ -499,22 +548,64 @@ def bench_serve(args):
         plain(r)
     torch.cuda.synchronize()
     wall_plain = time.perf_counter() - tp
+
+    def ent(a, b, tau):
+        mk = np.abs(b) > tau * np.abs(b).max()
+        return float((np.abs(a - b)[mk] / np.abs(b)[mk]).max())
+
+    def errors(prec, sample=8):
+        """Served results of the first `sample` requests against the plain FP64 path."""
+        worst = dict.fromkeys(("R_normwise", "J_normwise", "R_entrywise_over_1e-12_max", "J_entrywise_over_1e-12_max",
+                               "R_entrywise_over_1e-3_max", "J_entrywise_over_1e-3_max"), 0.0)
+        srv = txa.Server(eng, [p_[0] for p_ in pool], hbm_budget=int(args.budget_gb * 2 ** 30), slots=args.slots)
+        try:
+            for r in range(min(sample, args.requests)):
+                m, E, bc = pool[pick[r]]
+                u, v, a = states[r]
+                plain(r)
+                torch.cuda.synchronize()
+                fr, vr = (t_.cpu().numpy() for t_ in outs[pick[r]])
+                rid = srv.submit(m, coef=E, u=u, v=v, a=a, bc_lids=bc, precision=prec, **kw)
+                fs, vs = srv.result(rid, m)
+                srv.release(rid)
+                for key, x, y in (("R", fs, fr), ("J", vs, vr)):
+                    worst[key + "_normwise"] = max(worst[key + "_normwise"], float(np.abs(x - y).max() / np.abs(y).max()))
+                    for tau, tag in ((1e-12, "1e-12"), (1e-3, "1e-3")):
+                        k_ = f"{key}_entrywise_over_{tag}_max"
+                        worst[k_] = max(worst[k_], ent(x, y, tau))
+        finally:
+            srv.destroy()
+        return worst
+
+    forms = {}
+    for name, prec in (("fp64", txa.FP64), ("fp32", txa.FP32)):
+        lat, _ = serve(prec, True)
+        sat, wall_sat = serve(prec, False)
+        forms[name] = {"latency_ms": {"p50": lat["p50_ms"], "p99": lat["p99_ms"], "mean": lat["mean_ms"],
+                                      "max": lat["max_ms"], "at_qps": args.qps,
+                                      "requests_per_s": lat["requests_per_s"], "peak_inflight": lat["peak_inflight"]},
+                       "saturated": {"elements_per_s": nel / wall_sat, "wall_s": wall_sat, "p50_ms": sat["p50_ms"],
+                                     "p99_ms": sat["p99_ms"], "peak_inflight": sat["peak_inflight"],
+                                     "peak_arena_bytes": sat["peak_bytes"]},
+                       "speedup_vs_plain_fp64": wall_plain / wall_sat,
+                       "max_rel_err_vs_plain_fp64": errors(prec)}
+        log(f"served {name}: p50 {lat['p50_ms']:.2f} ms p99 {lat['p99_ms']:.2f} ms; saturated "
+            f"{nel / wall_sat:.3g} elements/s")
+    hn = "fp64" if head == txa.FP64 else "fp32"
+    h = forms[hn]
     line = {"metric": "serving: p50/p99 latency and elements/s (transient elasticity residual+Jacobian)",
-            "value": nel / wall_sat, "unit": "elements/s", "n_gpus": 1, "higher_is_better": True,
-            "dtype": "f32 element integration, f64 geometry and scatter" if prec == txa.FP32 else "f64",
+            "value": h["saturated"]["elements_per_s"], "unit": "elements/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "f64" if head == txa.FP64 else "f32 element integration, f64 geometry and scatter",
             "data": "synthetic",
             "config": {"workload": "hex8 transient elasticity (Newmark), n^3 cells, n log-uniform in [16, 128]",
                        "pool_n": list(map(int, sizes)), "requests": args.requests, "qps": args.qps,
-                       "slots": args.slots, "hbm_budget_gb": args.budget_gb, "dt": 1e-3},
-            "latency_ms": {"p50": lat["p50_ms"], "p99": lat["p99_ms"], "mean": lat["mean_ms"], "max": lat["max_ms"],
-                           "at_qps": args.qps, "requests_per_s": lat["requests_per_s"],
-                           "peak_inflight": lat["peak_inflight"]},
-            "saturated": {"elements_per_s": nel / wall_sat, "wall_s": wall_sat, "p50_ms": sat["p50_ms"],
-                          "p99_ms": sat["p99_ms"], "peak_inflight": sat["peak_inflight"],
-                          "peak_arena_bytes": sat["peak_bytes"]},
+                       "slots": args.slots, "hbm_budget_gb": args.budget_gb, "dt": 1e-3, "served_precision": hn},
+            "latency_ms": h["latency_ms"], "saturated": h["saturated"], "forms": forms,
             "plain_fp64_sequential": {"elements_per_s": nel / wall_plain, "wall_s": wall_plain},
-            "speedup_vs_plain_fp64": wall_plain / wall_sat, "elements_served": nel}
-    print(json.dumps(line), flush=True)
+            "speedup_vs_plain_fp64": h["speedup_vs_plain_fp64"], "elements_served": nel}
+    for m, _, _ in pool:
+        m.destroy()
+    return line
 
 
 Q2_KERNELS = {"split_bf16": "k_elastic<27, 27, 256, 2>", "fp32": "k_elastic<27, 27, 256, 1>",
@@ -611,7 +702,7 @@ def bench_upload(args):
                        "elements": n ** 3, "quadrature": "2x2x2 Gauss", "k": "lognormal sigma=0.5",
                        "dirichlet": "u=0 on x=0"},
             "orders": out, "checks": checks}
-    print(json.dumps(line), flush=True)
+    return line
 
 
 def summarize_q2(stats_dir, pmc_dir):
@@ -682,6 +773,64 @@ def summarize_q2(stats_dir, pmc_dir):
     print(out)
 
 
+def other_workloads(args):
+    """Compact summaries of the Q2, batch and serving workloads at their own default sizes,
+    appended to the headline line (each run after the headline's buffers are freed; a
+    failure is recorded in place of its summary and leaves the headline intact)."""
+    import copy
+    import torch
+
+    def pick(d, keys):
+        return {k: d[k] for k in keys if k in d}
+
+    out = {}
+    for name, fn, over in (("q2_elasticity_64", bench_q2, dict(n=64, steps=5, warmup=1)),
+                           ("batch_512_meshes", bench_batch, dict(meshes=512, steps=5)),
+                           ("serve_hex8_newmark", bench_serve, dict(requests=48, pool=8))):
+        a = copy.copy(args)
+        for k, v in over.items():
+            setattr(a, k, v)
+        t0 = time.time()
+        try:
+            line = fn(a)
+        except Exception as e:   # keep the headline: report what failed
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"{name}: failed: {e}")
+            continue
+        finally:
+            torch.cuda.synchronize()
+            torch.cuda.empty_cache()
+        s = pick(line, ("metric", "value", "unit", "ms_per_step", "dtype"))
+        s["config"] = line["config"]["workload"]
+        if fn is bench_q2:
+            f = line["forms"]
+            s["forms_ms"] = {k: v["ms_per_step"] for k, v in f.items()}
+            s["max_rel_err_vs_fp64"] = line["max_rel_err_vs_fp64"]
+            s["rigid_motion_residual_rel"] = line["rigid_motion_residual_rel"]
+            s["roofline"] = {"bound": "hbm", "frac": f["split_bf16"]["hbm_frac"],
+                             "bytes_per_element": Q2_BYTES_PER_ELEM,
+                             "dense_bf16_tflops": f["split_bf16"]["dense_tflops"],
+                             "mfma_frac": f["split_bf16"]["dense_tflops"] / 2500.0}
+        elif fn is bench_batch:
+            f = line["forms"]
+            s["forms_ms"] = {k: v["ms_per_step"] for k, v in f.items()}
+            s["cells_per_s"] = f["fp32_batch"]["cells_per_s"]
+            s["max_rel_err_fp32_vs_fp64_per_mesh"] = line["max_rel_err_fp32_vs_fp64_per_mesh"]
+            s["speedup_vs_fp64_per_mesh"] = line["speedup_vs_fp64_per_mesh"]
+        else:
+            s["plain_fp64_elements_per_s"] = line["plain_fp64_sequential"]["elements_per_s"]
+            s["forms"] = {k: {"p50_ms": v["latency_ms"]["p50"], "p99_ms": v["latency_ms"]["p99"],
+                              "saturated_elements_per_s": v["saturated"]["elements_per_s"],
+                              "speedup_vs_plain_fp64": v["speedup_vs_plain_fp64"],
+                              "max_rel_err_vs_plain_fp64": v["max_rel_err_vs_plain_fp64"]}
+                          for k, v in line["forms"].items()}
+            s["latency_ms"] = pick(line["latency_ms"], ("p50", "p99", "at_qps"))
+        s["wall_s"] = time.time() - t0
+        out[name] = s
+        log(f"{name}: {s.get('ms_per_step', s.get('latency_ms'))} ({s['wall_s']:.1f}s)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -689,6 +838,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="headline only: skip the Q2 / batch / serving summaries appended to the default line")
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="after the measurement, run a 1 GiB -> 1 GiB elementwise kernel (dst = 2 src) three "
@@ -705,7 +856,8 @@ def main():
     ap.add_argument("--qps", type=float, default=20.0)
     ap.add_argument("--slots", type=int, default=4)
     ap.add_argument("--budget-gb", type=float, default=64.0)
-    ap.add_argument("--serve-precision", choices=("fp64", "fp32"), default="fp32")
+    ap.add_argument("--serve-precision", choices=("fp64", "fp32"), default="fp64",
+                    help="the served form whose numbers head the serve line (both are measured)")
     ap.add_argument("--summarize-q2", nargs=2, metavar=("STATS_DIR", "PMC_DIR"),
                     help="CPU only: summarise rocprofv3 csv output of --workload q2 into profiles/")
     args = ap.parse_args()
@@ -719,14 +871,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                           "(multi-GPU sharding is a design note, not implemented)"}))
         sys.exit(2)
-    if args.workload == "batch":
-        return bench_batch(args)
-    if args.workload == "serve":
-        return bench_serve(args)
-    if args.workload == "q2":
-        return bench_q2(args)
-    if args.workload == "upload":
-        return bench_upload(args)
+    other = {"batch": bench_batch, "serve": bench_serve, "q2": bench_q2, "upload": bench_upload}
+    if args.workload in other:
+        print(json.dumps(other[args.workload](args)), flush=True)
+        return
 
     import numpy as np
     import torch
@@ -853,6 +1001,11 @@ def main():
         torch.cuda.synchronize()
         log("copy calibration done")
         del src, dst
+    if not args.no_extra:
+        m.destroy()
+        del f64, v64, f32, v32, x, k, bc, isbc
+        torch.cuda.empty_cache()
+        out["other_workloads"] = other_workloads(args)
     if not args.no_cpu:
         t0 = time.time()
         out["cpu_baseline"] = cpu_baseline(args.cpu_n)
