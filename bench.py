@@ -80,8 +80,10 @@ def cpu_baseline(n=128, reps=10, n1=48, reps1=10):
 
 
 COPY_BYTES = 1 << 30
-FP32_KERNEL = "k_q1_hex8_diffusion<1, float, false, false>"   # HEX8, FP32 integration, steady
-FP64_KERNEL = "k_q1_hex8_diffusion<1, double, false, true>"   # HEX8, FP64, steady, beta = 1
+# the unit cube's cells are parallelepipeds: the one-metric hex8 kind (17); jittered cells run kind 1
+FP32_KERNEL = "k_q1_hex8_diffusion<17, float, false, false>"   # FP32 integration, steady
+FP64_KERNEL = "k_q1_hex8_diffusion<17, double, false, true>"   # FP64, steady, beta = 1
+FP32_KERNEL_GENERAL = "k_q1_hex8_diffusion<1, float, false, false>"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "q1_hex8_diffusion_fp32_256_traffic.json")
 
 
@@ -960,6 +962,46 @@ def main():
     ms = float(per32.mean())
     value = nel / (ms / 1e3)
     kt32 = float(kper32.mean())
+    general = None
+    if not args.no_extra:
+        # the same workload on jittered cells (interior vertices moved by U[-0.1h, 0.1h]): the
+        # general kernel, one Jacobian per integration point
+        Xj = synth.brick_coords(n, n, n, jitter=0.1, seed=7)
+        mj = eng.brick(txa.HEX8, n, n, n, Xj)
+        mj.build_dofmap(1)
+        mj.build_graph()
+        bcj = torch.as_tensor(mj.point_lids()[:, 0][ii == 0].astype(np.int32), device="cuda")
+
+        def step_j(prec, f, v, with_bc=True):
+            mj.assemble(physics=txa.DIFFUSION, precision=prec, coef=k, x=x, f=f, vals=v, src=(1.0,),
+                        bc_lids=bcj if with_bc else None)
+        gen = {}
+        for name, prec, f_, v_ in (("fp32", txa.FP32, f32, v32), ("fp64", txa.FP64, f64, v64)):
+            for _ in range(args.warmup):
+                step_j(prec, f_, v_)
+            torch.cuda.synchronize()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+            ev[0].record()
+            for i in range(args.steps):
+                step_j(prec, f_, v_)
+                ev[i + 1].record()
+            torch.cuda.synchronize()
+            per = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)])
+            gen[name] = {"ms_per_step": float(per.mean()), "elements_per_s": nel / (float(per.mean()) / 1e3),
+                         "frac_whole_step": nel / (float(per.mean()) / 1e3) * BYTES_PER_ELEM / HBM_PEAK}
+        step_j(txa.FP32, f32, v32)
+        step_j(txa.FP64, f64, v64)
+        torch.cuda.synchronize()
+        gen["max_rel_err_fp32_vs_fp64"] = {
+            "R_normwise": rel(f32, f64), "J_normwise": rel(v32, v64),
+            "R_entrywise_over_1e-12_max": ent(f32, f64, 1e-12), "J_entrywise_over_1e-12_max": ent(v32, v64, 1e-12),
+            "J_entrywise_over_1e-2_max": ent(v32, v64, 1e-2)}
+        general = {"workload": f"Q1 Poisson hex8 {n}^3, interior vertices jittered by U[-0.1h, 0.1h] "
+                               "(general kernel: one Jacobian per integration point)",
+                   "kernel": FP32_KERNEL_GENERAL, "mesh_affine": mj.affine, **gen}
+        log(f"jittered cells: fp32 {gen['fp32']['ms_per_step']:.3f} ms, fp64 {gen['fp64']['ms_per_step']:.3f} ms")
+        mj.destroy()
+        del bcj
     copy_gbs, fill_gbs = measured_hbm_peaks()
     achieved = BYTES_PER_ELEM * nel / (kt32 / 1e3) / 1e9
     traffic = None
@@ -980,6 +1022,8 @@ def main():
                       "roofline_frac": BYTES_PER_ELEM * nel / (float(kper64.mean()) / 1e3) / HBM_PEAK},
         "max_rel_err_vs_fp64": err,
         "check": {"interior_row_sum_rel_max": rowsum, "rows_all_zero": zero_rows},
+        "mesh_affine": m.affine,
+        "jittered_cells": general,
         "wall_s_timed": wall32,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / (HBM_PEAK / 1e9), "traffic": traffic,

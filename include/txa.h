@@ -127,6 +127,12 @@ int txa_mesh_blocks(txa_ctx ctx, int mesh_id, int64_t *out, int64_t cap);
 int txa_mesh_destroy(txa_ctx ctx, int mesh_id);
 /* out[0]=elements out[1]=connectivity entries/element (0 when the blocks differ) out[2]=vertices */
 int txa_mesh_info(txa_ctx ctx, int mesh_id, int64_t *out);
+/* *affine = 1 when the mesh is a HEX8 brick whose cells are all parallelepipeds (the
+ * bilinear and trilinear terms of every cell's map below 2^-47 of its coordinate
+ * magnitude, i.e. at the rounding of the coordinates), decided once when the brick is
+ * created: its lattice kernels then form one metric per cell instead of one per
+ * integration point.  0 otherwise.  Introspection only; no counterpart in the reference. */
+int txa_mesh_affine(txa_ctx ctx, int mesh_id, int *affine);
 
 /* DOF map for `nfields` fields on every connectivity entity (1 = scalar,
  * 3 = displacement).  GIDs by the GUN rule on one rank, owned_ by first touch

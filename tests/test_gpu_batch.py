@@ -117,6 +117,32 @@ def test_batch_matches_single_and_oracle(engine, items, precision, transient, to
         b.destroy()
 
 
+# every hex8 mesh with parallelepiped cells: the batch runs the one-metric hex8 kernel
+AFFINE_SHAPES = [(txa.HEX8, 1, 1, 1, 0.0), (txa.TET4, 2, 3, 1, 0.1), (txa.HEX8, 40, 9, 7, 0.0),
+                 (txa.HEX8, 5, 5, 70, 0.0), (txa.TET4, 9, 8, 35, 0.05)]
+
+
+@pytest.mark.parametrize("precision,tol", [(txa.FP64, 1e-12), (txa.FP32, 1e-6)])
+def test_batch_of_affine_hex8_meshes(engine, precision, tol):
+    its = [Item(engine, 50 + i, *s_) for i, s_ in enumerate(AFFINE_SHAPES)]
+    try:
+        assert all(it.m.affine for it in its if it.topo == txa.HEX8)
+        b = txa.Batch(engine, [it.m for it in its], precision=precision, transient=True)
+        try:
+            xs = [it.state(700 + i) for i, it in enumerate(its)]
+            got = run_batch(b, its, xs, True, 2.0, use_graph=True)
+            single = run_single(its, xs, precision, True, 2.0)
+            for i, (it, (x, xd), (f, v), (fs, vs)) in enumerate(zip(its, xs, got, single)):
+                fr, vr = it.oracle(x, xd, True, 2.0)
+                assert rel_err(f, fr) <= tol and rel_err(v, vr) <= tol, f"item {i} differs from the oracle"
+                assert rel_err(f, fs) <= SAME[precision] and rel_err(v, vs) <= SAME[precision]
+        finally:
+            b.destroy()
+    finally:
+        for it in its:
+            it.m.destroy()
+
+
 @pytest.mark.parametrize("precision", [txa.FP64, txa.FP32])
 def test_batch_graph_replays(engine, items, precision):
     """Each replay gets new x / f / vals buffers and a new alpha; eager work and a

@@ -33,10 +33,10 @@ def entry_err(got, ref, tau=1e-12):
 
 
 class Case:
-    def __init__(self, engine, topo, nx, ny, nz, nf, jitter=0.0, seed=1):
+    def __init__(self, engine, topo, nx, ny, nz, nf, jitter=0.0, seed=1, X=None):
         print(f"[case] topo={topo} mesh={nx}x{ny}x{nz} nf={nf} jitter={jitter}", flush=True)
         self.topo, self.nf = topo, nf
-        self.X = synth.brick_coords(nx, ny, nz, jitter=jitter, seed=seed)
+        self.X = synth.brick_coords(nx, ny, nz, jitter=jitter, seed=seed) if X is None else X
         self.m = engine.brick(topo, nx, ny, nz, self.X)
         self.ndofs = self.m.build_dofmap(nf)
         self.nnz = self.m.build_graph()
@@ -144,6 +144,56 @@ def test_hex8_diffusion_fp32_unit_cube_bands(engine):
     er, ej = entry_err(f, fr, 1e-12), entry_err(v, vr, 1e-12)
     print(f"unit cube FP32: entrywise R {er:.2e} J {ej:.2e}; normwise R {rel_err(f, fr):.2e} J {rel_err(v, vr):.2e}")
     assert er <= TOL32 and ej <= TOL32
+    c.m.destroy()
+
+
+def sheared(nx, ny, nz):
+    """A parallelepiped brick: the unit-cube lattice under a fixed non-orthogonal linear map."""
+    B = np.array([[1.0, 0.3, -0.2], [0.1, 0.9, 0.25], [-0.15, 0.2, 1.1]])
+    return synth.brick_coords(nx, ny, nz) @ B.T + np.array([0.5, -0.25, 2.0])
+
+
+def test_affine_cells_detected(engine):
+    # parallelepiped cells (unit cube, sheared brick) take the one-metric form; a jittered brick,
+    # a brick with one vertex moved by 1e-9 and tet bricks do not
+    def flag(topo, nx, ny, nz, X):
+        m = engine.brick(topo, nx, ny, nz, X)
+        a = m.affine
+        m.destroy()
+        return a
+    assert flag(txa.HEX8, 7, 5, 6, synth.brick_coords(7, 5, 6))
+    assert flag(txa.HEX8, 7, 5, 6, sheared(7, 5, 6))
+    assert flag(txa.HEX8, 33, 5, 3, synth.brick_coords(33, 5, 3))       # 1/33 spacing: coordinates rounded
+    assert not flag(txa.HEX8, 7, 5, 6, synth.brick_coords(7, 5, 6, jitter=0.1, seed=3))
+    X = sheared(7, 5, 6)
+    X[8 * 5 + 3, 1] += 1e-9
+    assert not flag(txa.HEX8, 7, 5, 6, X)
+    assert not flag(txa.TET4, 7, 5, 6, sheared(7, 5, 6))
+
+
+@pytest.mark.parametrize("shape,transient,bc", [((9, 7, 11), False, True), ((40, 9, 7), True, False),
+                                                ((33, 8, 5), False, False)])
+def test_affine_hex8_diffusion_fp64(engine, shape, transient, bc):
+    c = Case(engine, txa.HEX8, *shape, 1, X=sheared(*shape))
+    assert c.m.affine
+    f, v, fr, vr = run_both(engine, c, txa.DIFFUSION, transient=transient, bc=bc, alpha=2.5, beta=0.75, rho=1.5,
+                            src=(0.7, 0.0, 0.0))
+    assert rel_err(f, fr) <= TOL64
+    assert rel_err(v, vr) <= TOL64
+    c.m.destroy()
+
+
+@pytest.mark.parametrize("transient", [False, True])
+def test_affine_hex8_diffusion_fp32(engine, transient):
+    c = Case(engine, txa.HEX8, 21, 17, 13, 1, X=sheared(21, 17, 13))
+    assert c.m.affine
+    f, v, fr, vr = run_both(engine, c, txa.DIFFUSION, precision=txa.FP32, transient=transient, bc=True, alpha=2.0)
+    er = entry_err(f, fr, 1e-12)
+    print(f"affine FP32: entrywise R {er:.2e}; normwise J {rel_err(v, vr):.2e}; "
+          f"entrywise J (1e-2 band) {entry_err(v, vr, 1e-2):.2e}")
+    assert er <= TOL32
+    assert rel_err(v, vr) <= TOL32
+    assert entry_err(v, vr, 1e-2) <= 1e-5
     c.m.destroy()
 
 

@@ -119,6 +119,7 @@ def load_library(path: str = LIB_PATH):
     L.txa_mesh_destroy.argtypes = [P, ctypes.c_int]
     L.txa_mesh_upload.argtypes = [P, ctypes.c_int, I64, P, I64, I64, P, ctypes.POINTER(ctypes.c_int)]
     L.txa_mesh_info.argtypes = [P, ctypes.c_int, P]
+    L.txa_mesh_affine.argtypes = [P, ctypes.c_int, P]
     L.txa_mesh_upload_blocks.argtypes = [P, ctypes.c_int, P, I64, I64, P, ctypes.POINTER(ctypes.c_int)]
     L.txa_mesh_blocks.argtypes = [P, ctypes.c_int, P, I64]
     L.txa_dofmap_build.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(I64)]
@@ -316,6 +317,10 @@ class BrickMesh:
         info = np.zeros(3, np.int64)
         eng.check(eng.lib.txa_mesh_info(eng.h, self.id, _np(info)))
         self.nelem, self.nconn, self.nverts = (int(v) for v in info)
+        aff = ctypes.c_int()
+        eng.check(eng.lib.txa_mesh_affine(eng.h, self.id, ctypes.byref(aff)))
+        # a HEX8 brick of parallelepiped cells: the lattice kernels use one metric per cell
+        self.affine = bool(aff.value)
         self.ndofs = None
         self.nnz = None
         self.nfields = None

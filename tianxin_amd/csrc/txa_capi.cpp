@@ -208,6 +208,13 @@ int txa_mesh_info(txa_ctx ctx, int mesh_id, int64_t *out) {
   });
 }
 
+int txa_mesh_affine(txa_ctx ctx, int mesh_id, int *affine) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(affine, TXA_ERR_INVALID, "null argument");
+    *affine = ctx->mesh(mesh_id).affine ? 1 : 0;
+  });
+}
+
 int txa_dofmap_build(txa_ctx ctx, int mesh_id, int nfields, int64_t *ndofs) {
   return guarded(ctx, [&] {
     Mesh &m = ctx->mesh(mesh_id);

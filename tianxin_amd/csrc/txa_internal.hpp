@@ -72,6 +72,7 @@ struct MeshBlock {
 struct Mesh {
   int topo = 0, nx = 0, ny = 0, nz = 0;   // topo: that of every block (0 when the blocks differ)
   bool structured = true;  // a brick (txa_mesh_brick): lattice kernels; false: uploaded connectivity
+  bool affine = false;     // HEX8 brick whose cells are all parallelepipeds (brick_cells_affine)
   std::vector<MeshBlock> blocks;   // one for bricks and single-topology uploads
   int64_t nelem = 0;     // elements of all blocks (6 per cell for a TET4 brick)
   int64_t nconn_tot = 0; // connectivity entries of all blocks
@@ -212,6 +213,8 @@ size_t elastic_workspace_bytes(const Mesh &m, int precision, bool want_f, bool w
 void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws);
 // Q1 diffusion fast path and batches (txa_q1_hex8.hip)
 void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s);
+// Whether every cell of a HEX8 brick is a parallelepiped (to within its coordinates' rounding).
+bool brick_cells_affine(const Mesh &m, hipStream_t s);
 Batch *batch_create(const std::vector<Mesh *> &meshes, const std::vector<int> &ids, int precision, bool transient);
 void batch_assemble(Batch &b, const txa_assembly_args *args, bool use_graph, hipStream_t s, int *err);
 double batch_tune(Batch &b, const txa_assembly_args *args, hipStream_t s, int reps, int *err);

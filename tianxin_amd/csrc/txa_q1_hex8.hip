@@ -45,6 +45,9 @@ constexpr int NT = 2 * NE;                     // block: 4 element waves + 4 own
 constexpr double kG = 0.57735026918962576451;   // 1/sqrt(3)
 constexpr double kP = 0.5 * (1.0 + kG);         // P(bit == qbit)
 constexpr double kM = 0.5 * (1.0 - kG);         // P(bit != qbit)
+// element kinds of the lattice kernels: TXA_HEX8 (general trilinear cells), TXA_TET4 (six
+// tets per cell) and hex8 cells that are all parallelepipeds (constant metric per cell)
+constexpr int kHex8Affine = 16 | TXA_HEX8;
 constexpr int NSTAGE = 44;   // per element: 36 packed K, then 8 FP64 R (FP64) or 4 FP64 x-pair sums of R (FP32)
 // Stage slots: FP32 stages are double-buffered (element waves publish layer z in one
 // slot while the owner waves consume layer z-1 from the other, one barrier per
@@ -152,7 +155,7 @@ __device__ __forceinline__ void interp8(double (&v)[8]) {
 // One element.  X: vertex coordinates in binary order v = bx + 2by + 4bz.  mid() runs
 // once the quadrature loop is done (the caller issues its prefetch loads there, where
 // the geometry's registers are free).
-template <typename Real, bool TRANS, bool UNIT, class Mid>
+template <typename Real, bool TRANS, bool UNIT, bool AFF, class Mid>
 __device__ __forceinline__ void hex8_element(const double (&X)[3][8], const double (&u)[8], const double (&ud)[8],
                                              double k, double rhoc, double src, double alpha, double beta,
                                              Hex8Out<Real, TRANS> &o, Mid &&mid) {
@@ -191,160 +194,220 @@ __device__ __forceinline__ void hex8_element(const double (&X)[3][8], const doub
   for (int d = 0; d < 3; ++d)
 #pragma unroll
     for (int v = 1; v < 8; ++v) h[d][v] *= 0.125;
-  // metric tensor at the 8 points, scale 1/4 of the gradient products folded in
-  Real A[8][6];     // xx yy zz xy yz xz
+  // Tables of the element matrix (sum-factorised over the points) and the point weights
+  Real S[3][3][3];                                  // diagonal: S_dd[c1][c2]
+  Real Xxy[2][2][3], Xyz[3][2][2], Xxz[2][3][2];    // cross terms
   double wm[8];
-  double sg = 1.0;
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const double xi = bit(q, 0) ? kG : -kG, eta = bit(q, 1) ? kG : -kG, zeta = bit(q, 2) ? kG : -kG;
-    double J[3][3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      J[d][0] = h[d][1] + h[d][3] * eta + h[d][5] * zeta + h[d][7] * (eta * zeta);
-      J[d][1] = h[d][2] + h[d][3] * xi + h[d][6] * zeta + h[d][7] * (xi * zeta);
-      J[d][2] = h[d][4] + h[d][5] * xi + h[d][6] * eta + h[d][7] * (xi * eta);
-    }
-    double c[3][3];
-    c[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-    c[0][1] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-    c[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-    c[1][0] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
-    c[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
-    c[1][2] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
-    c[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
-    c[2][1] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
-    c[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    const double det = J[0][0] * c[0][0] + J[0][1] * c[0][1] + J[0][2] * c[0][2];
-    if (q == 0) sg = det < 0.0 ? -1.0 : 1.0;   // computeCellMeasure: sign of det at the first point
-    if (!kFlux || TRANS) wm[q] = det * sg;     // Gauss weights are 1
-    const double cw = 0.25 * k * sg * recip(det);   // k wm / det^2, times the 1/4 of (s/2)(s/2)
-    // the metric in FP64, rounded once to Real (six conversions per point instead of ten)
+  // AFF: every cell is a parallelepiped (the bilinear and trilinear coefficients of the map
+  // zero to within the rounding of its coordinates; cell_affine decides it once per mesh),
+  // so there is one Jacobian and the metric, the tables and the flux residual take closed
+  // forms: the point sums of the general form reduce to the weights w(0) = w(1) = 2/3,
+  // w(2) = 1/3 of a pair class and to sum_q s_d(q) s_e(q) = 8 delta_de.
+  if constexpr (AFF) {
+    double c[3][3];   // cofactors of J = [h1 h2 h4]
+    c[0][0] = h[1][2] * h[2][4] - h[1][4] * h[2][2];
+    c[0][1] = h[1][4] * h[2][1] - h[1][1] * h[2][4];
+    c[0][2] = h[1][1] * h[2][2] - h[1][2] * h[2][1];
+    c[1][0] = h[0][4] * h[2][2] - h[0][2] * h[2][4];
+    c[1][1] = h[0][1] * h[2][4] - h[0][4] * h[2][1];
+    c[1][2] = h[0][2] * h[2][1] - h[0][1] * h[2][2];
+    c[2][0] = h[0][2] * h[1][4] - h[0][4] * h[1][2];
+    c[2][1] = h[0][4] * h[1][1] - h[0][1] * h[1][4];
+    c[2][2] = h[0][1] * h[1][2] - h[0][2] * h[1][1];
+    const double det = h[0][1] * c[0][0] + h[0][2] * c[0][1] + h[0][4] * c[0][2];
+    const double sg = det < 0.0 ? -1.0 : 1.0;
+    const double cw = 0.25 * k * sg * recip(det);
     const double mxx = cw * (c[0][0] * c[0][0] + c[1][0] * c[1][0] + c[2][0] * c[2][0]);
     const double myy = cw * (c[0][1] * c[0][1] + c[1][1] * c[1][1] + c[2][1] * c[2][1]);
     const double mzz = cw * (c[0][2] * c[0][2] + c[1][2] * c[1][2] + c[2][2] * c[2][2]);
     const double mxy = cw * (c[0][0] * c[0][1] + c[1][0] * c[1][1] + c[2][0] * c[2][1]);
     const double myz = cw * (c[0][1] * c[0][2] + c[1][1] * c[1][2] + c[2][1] * c[2][2]);
     const double mxz = cw * (c[0][0] * c[0][2] + c[1][0] * c[1][2] + c[2][0] * c[2][2]);
-    A[q][0] = Real(mxx); A[q][1] = Real(myy); A[q][2] = Real(mzz);
-    A[q][3] = Real(mxy); A[q][4] = Real(myz); A[q][5] = Real(mxz);
+    if (!kFlux || TRANS)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) wm[q] = det * sg;
     if constexpr (kFlux) {
-      // reference gradient of u at the point, the flux z = A_q grad u (A_q = G_q / 4), and its
-      // contraction with the gradients of the Walsh basis functions
-      const double s0 = bit(q, 0) ? 1.0 : -1.0, s1 = bit(q, 1) ? 1.0 : -1.0, s2 = bit(q, 2) ? 1.0 : -1.0;
-      const double g0 = uh[1] + s1 * uh[3] + s2 * uh[5] + (s1 * s2) * uh[7];
-      const double g1 = uh[2] + s0 * uh[3] + s2 * uh[6] + (s0 * s2) * uh[7];
-      const double g2 = uh[4] + s0 * uh[5] + s1 * uh[6] + (s0 * s1) * uh[7];
-      const double z0 = mxx * g0 + mxy * g1 + mxz * g2;
-      const double z1 = mxy * g0 + myy * g1 + myz * g2;
-      const double z2 = mxz * g0 + myz * g1 + mzz * g2;
-      rh[1] += z0;
-      rh[2] += z1;
-      rh[4] += z2;
-      rh[3] += s1 * z0 + s0 * z1;
-      rh[5] += s2 * z0 + s0 * z2;
-      rh[6] += s2 * z1 + s1 * z2;
-      rh[7] += (s1 * s2) * z0 + (s0 * s2) * z1 + (s0 * s1) * z2;
-      // source: -F_a = -src sum_q wm_q N_a(q) enters as -(src/4) wm_q phi_s(q), with
-      // phi_s(q) = 3^(-|s|/2) prod sgn, one power of 1/sqrt3 fewer than the later scaling supplies
-      const double ws = -0.25 * src * (det * sg), wg = ws * kG;
-      rh[0] += ws;
-      rh[1] += s0 * wg;
-      rh[2] += s1 * wg;
-      rh[4] += s2 * wg;
-      rh[3] += (s0 * s1) * wg;
-      rh[5] += (s0 * s2) * wg;
-      rh[6] += (s1 * s2) * wg;
-      rh[7] += (s0 * s1 * s2) * wg;
+      rh[0] = -2.0 * src * (det * sg);
+      rh[1] = 8.0 * (mxx * uh[1] + mxy * uh[2] + mxz * uh[4]);
+      rh[2] = 8.0 * (mxy * uh[1] + myy * uh[2] + myz * uh[4]);
+      rh[4] = 8.0 * (mxz * uh[1] + myz * uh[2] + mzz * uh[4]);
+      rh[3] = 8.0 * ((mxx + myy) * uh[3] + mxz * uh[6] + myz * uh[5]);
+      rh[5] = 8.0 * ((mxx + mzz) * uh[5] + mxy * uh[6] + myz * uh[3]);
+      rh[6] = 8.0 * ((myy + mzz) * uh[6] + mxy * uh[5] + mxz * uh[3]);
+      rh[7] = 8.0 * (mxx + myy + mzz) * uh[7];
     }
-  }
-  mid();
-  // ---- diagonal tables S_dd[c1][c2]: sum over the two other axes after summing axis d
-  Real S[3][3][3];
+    mid();
+    const Real a[6] = {Real(mxx), Real(myy), Real(mzz), Real(mxy), Real(myz), Real(mxz)};
+    constexpr double w[3] = {2.0 / 3.0, 2.0 / 3.0, 1.0 / 3.0};
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;   // the two other axes, ascending
-    Real Ab[2][2];
+    for (int d = 0; d < 3; ++d)
 #pragma unroll
-    for (int i1 = 0; i1 < 2; ++i1)
+      for (int c1 = 0; c1 < 3; ++c1)
 #pragma unroll
-      for (int i2 = 0; i2 < 2; ++i2) {
-        const int q0 = (i1 << d1) | (i2 << d2);
-        Ab[i1][i2] = A[q0][d] + A[q0 | (1 << d)][d];
+        for (int c2 = 0; c2 < 3; ++c2) S[d][c1][c2] = Real(2.0 * w[c1] * w[c2]) * a[d];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int c3 = 0; c3 < 3; ++c3) {
+          Xxy[i][j][c3] = Real(w[c3]) * a[3];
+          Xyz[c3][i][j] = Real(w[c3]) * a[4];
+          Xxz[i][c3][j] = Real(w[c3]) * a[5];
+        }
+  } else {
+    // metric tensor at the 8 points, scale 1/4 of the gradient products folded in
+    Real A[8][6];     // xx yy zz xy yz xz
+    double sg = 1.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const double xi = bit(q, 0) ? kG : -kG, eta = bit(q, 1) ? kG : -kG, zeta = bit(q, 2) ? kG : -kG;
+      double J[3][3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        J[d][0] = h[d][1] + h[d][3] * eta + h[d][5] * zeta + h[d][7] * (eta * zeta);
+        J[d][1] = h[d][2] + h[d][3] * xi + h[d][6] * zeta + h[d][7] * (xi * zeta);
+        J[d][2] = h[d][4] + h[d][5] * xi + h[d][6] * eta + h[d][7] * (xi * eta);
       }
-    Real T[2][3];
-#pragma unroll
-    for (int i1 = 0; i1 < 2; ++i1)
-#pragma unroll
-      for (int c2 = 0; c2 < 3; ++c2) T[i1][c2] = Real(Wq(c2, 0)) * Ab[i1][0] + Real(Wq(c2, 1)) * Ab[i1][1];
-#pragma unroll
-    for (int c1 = 0; c1 < 3; ++c1)
-#pragma unroll
-      for (int c2 = 0; c2 < 3; ++c2) S[d][c1][c2] = Real(Wq(c1, 0)) * T[0][c2] + Real(Wq(c1, 1)) * T[1][c2];
-  }
-  // ---- cross tables
-  // Xxy[al][be][cz] = sum_q A_xy P(al,qx) P(be,qy) W(cz,qz)
-  // Xyz[cx][be][ga] = sum_q A_yz W(cx,qx) P(be,qy) P(ga,qz)
-  // Xxz[al][cy][ga] = sum_q A_xz P(al,qx) W(cy,qy) P(ga,qz)
-  Real Xxy[2][2][3], Xyz[3][2][2], Xxz[2][3][2];
-  {
-    Real U[2][2][2];   // [al][qy][qz]
-#pragma unroll
-    for (int al = 0; al < 2; ++al)
-#pragma unroll
-      for (int qy = 0; qy < 2; ++qy)
-#pragma unroll
-        for (int qz = 0; qz < 2; ++qz)
-          U[al][qy][qz] = Real(Pq(al, 0)) * A[0 | (qy << 1) | (qz << 2)][3] +
-                          Real(Pq(al, 1)) * A[1 | (qy << 1) | (qz << 2)][3];
-#pragma unroll
-    for (int al = 0; al < 2; ++al)
-#pragma unroll
-      for (int be = 0; be < 2; ++be) {
-        const Real V0 = Real(Pq(be, 0)) * U[al][0][0] + Real(Pq(be, 1)) * U[al][1][0];
-        const Real V1 = Real(Pq(be, 0)) * U[al][0][1] + Real(Pq(be, 1)) * U[al][1][1];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) Xxy[al][be][c] = Real(Wq(c, 0)) * V0 + Real(Wq(c, 1)) * V1;
+      double c[3][3];
+      c[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+      c[0][1] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+      c[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+      c[1][0] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+      c[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+      c[1][2] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+      c[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+      c[2][1] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+      c[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      const double det = J[0][0] * c[0][0] + J[0][1] * c[0][1] + J[0][2] * c[0][2];
+      if (q == 0) sg = det < 0.0 ? -1.0 : 1.0;   // computeCellMeasure: sign of det at the first point
+      if (!kFlux || TRANS) wm[q] = det * sg;     // Gauss weights are 1
+      const double cw = 0.25 * k * sg * recip(det);   // k wm / det^2, times the 1/4 of (s/2)(s/2)
+      // the metric in FP64, rounded once to Real (six conversions per point instead of ten)
+      const double mxx = cw * (c[0][0] * c[0][0] + c[1][0] * c[1][0] + c[2][0] * c[2][0]);
+      const double myy = cw * (c[0][1] * c[0][1] + c[1][1] * c[1][1] + c[2][1] * c[2][1]);
+      const double mzz = cw * (c[0][2] * c[0][2] + c[1][2] * c[1][2] + c[2][2] * c[2][2]);
+      const double mxy = cw * (c[0][0] * c[0][1] + c[1][0] * c[1][1] + c[2][0] * c[2][1]);
+      const double myz = cw * (c[0][1] * c[0][2] + c[1][1] * c[1][2] + c[2][1] * c[2][2]);
+      const double mxz = cw * (c[0][0] * c[0][2] + c[1][0] * c[1][2] + c[2][0] * c[2][2]);
+      A[q][0] = Real(mxx); A[q][1] = Real(myy); A[q][2] = Real(mzz);
+      A[q][3] = Real(mxy); A[q][4] = Real(myz); A[q][5] = Real(mxz);
+      if constexpr (kFlux) {
+        // reference gradient of u at the point, the flux z = A_q grad u (A_q = G_q / 4), and its
+        // contraction with the gradients of the Walsh basis functions
+        const double s0 = bit(q, 0) ? 1.0 : -1.0, s1 = bit(q, 1) ? 1.0 : -1.0, s2 = bit(q, 2) ? 1.0 : -1.0;
+        const double g0 = uh[1] + s1 * uh[3] + s2 * uh[5] + (s1 * s2) * uh[7];
+        const double g1 = uh[2] + s0 * uh[3] + s2 * uh[6] + (s0 * s2) * uh[7];
+        const double g2 = uh[4] + s0 * uh[5] + s1 * uh[6] + (s0 * s1) * uh[7];
+        const double z0 = mxx * g0 + mxy * g1 + mxz * g2;
+        const double z1 = mxy * g0 + myy * g1 + myz * g2;
+        const double z2 = mxz * g0 + myz * g1 + mzz * g2;
+        rh[1] += z0;
+        rh[2] += z1;
+        rh[4] += z2;
+        rh[3] += s1 * z0 + s0 * z1;
+        rh[5] += s2 * z0 + s0 * z2;
+        rh[6] += s2 * z1 + s1 * z2;
+        rh[7] += (s1 * s2) * z0 + (s0 * s2) * z1 + (s0 * s1) * z2;
+        // source: -F_a = -src sum_q wm_q N_a(q) enters as -(src/4) wm_q phi_s(q), with
+        // phi_s(q) = 3^(-|s|/2) prod sgn, one power of 1/sqrt3 fewer than the later scaling supplies
+        const double ws = -0.25 * src * (det * sg), wg = ws * kG;
+        rh[0] += ws;
+        rh[1] += s0 * wg;
+        rh[2] += s1 * wg;
+        rh[4] += s2 * wg;
+        rh[3] += (s0 * s1) * wg;
+        rh[5] += (s0 * s2) * wg;
+        rh[6] += (s1 * s2) * wg;
+        rh[7] += (s0 * s1 * s2) * wg;
       }
-  }
-  {
-    Real U[2][2][2];   // [ga][qx][qy]: sum over qz with P(ga, qz)
+    }
+    mid();
+    // ---- diagonal tables S_dd[c1][c2]: sum over the two other axes after summing axis d
 #pragma unroll
-    for (int ga = 0; ga < 2; ++ga)
+    for (int d = 0; d < 3; ++d) {
+      const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;   // the two other axes, ascending
+      Real Ab[2][2];
 #pragma unroll
-      for (int qx = 0; qx < 2; ++qx)
+      for (int i1 = 0; i1 < 2; ++i1)
+#pragma unroll
+        for (int i2 = 0; i2 < 2; ++i2) {
+          const int q0 = (i1 << d1) | (i2 << d2);
+          Ab[i1][i2] = A[q0][d] + A[q0 | (1 << d)][d];
+        }
+      Real T[2][3];
+#pragma unroll
+      for (int i1 = 0; i1 < 2; ++i1)
+#pragma unroll
+        for (int c2 = 0; c2 < 3; ++c2) T[i1][c2] = Real(Wq(c2, 0)) * Ab[i1][0] + Real(Wq(c2, 1)) * Ab[i1][1];
+#pragma unroll
+      for (int c1 = 0; c1 < 3; ++c1)
+#pragma unroll
+        for (int c2 = 0; c2 < 3; ++c2) S[d][c1][c2] = Real(Wq(c1, 0)) * T[0][c2] + Real(Wq(c1, 1)) * T[1][c2];
+    }
+    // ---- cross tables
+    // Xxy[al][be][cz] = sum_q A_xy P(al,qx) P(be,qy) W(cz,qz)
+    // Xyz[cx][be][ga] = sum_q A_yz W(cx,qx) P(be,qy) P(ga,qz)
+    // Xxz[al][cy][ga] = sum_q A_xz P(al,qx) W(cy,qy) P(ga,qz)
+    {
+      Real U[2][2][2];   // [al][qy][qz]
+#pragma unroll
+      for (int al = 0; al < 2; ++al)
 #pragma unroll
         for (int qy = 0; qy < 2; ++qy)
-          U[ga][qx][qy] = Real(Pq(ga, 0)) * A[qx | (qy << 1)][4] + Real(Pq(ga, 1)) * A[qx | (qy << 1) | 4][4];
 #pragma unroll
-    for (int ga = 0; ga < 2; ++ga)
+          for (int qz = 0; qz < 2; ++qz)
+            U[al][qy][qz] = Real(Pq(al, 0)) * A[0 | (qy << 1) | (qz << 2)][3] +
+                            Real(Pq(al, 1)) * A[1 | (qy << 1) | (qz << 2)][3];
 #pragma unroll
-      for (int be = 0; be < 2; ++be) {
-        const Real V0 = Real(Pq(be, 0)) * U[ga][0][0] + Real(Pq(be, 1)) * U[ga][0][1];   // qx = 0
-        const Real V1 = Real(Pq(be, 0)) * U[ga][1][0] + Real(Pq(be, 1)) * U[ga][1][1];   // qx = 1
+      for (int al = 0; al < 2; ++al)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) Xyz[c][be][ga] = Real(Wq(c, 0)) * V0 + Real(Wq(c, 1)) * V1;
-      }
-  }
-  {
-    Real U[2][2][2];   // [al][qy][qz]: sum over qx with P(al, qx)
+        for (int be = 0; be < 2; ++be) {
+          const Real V0 = Real(Pq(be, 0)) * U[al][0][0] + Real(Pq(be, 1)) * U[al][1][0];
+          const Real V1 = Real(Pq(be, 0)) * U[al][0][1] + Real(Pq(be, 1)) * U[al][1][1];
 #pragma unroll
-    for (int al = 0; al < 2; ++al)
+          for (int c = 0; c < 3; ++c) Xxy[al][be][c] = Real(Wq(c, 0)) * V0 + Real(Wq(c, 1)) * V1;
+        }
+    }
+    {
+      Real U[2][2][2];   // [ga][qx][qy]: sum over qz with P(ga, qz)
 #pragma unroll
-      for (int qy = 0; qy < 2; ++qy)
+      for (int ga = 0; ga < 2; ++ga)
 #pragma unroll
-        for (int qz = 0; qz < 2; ++qz)
-          U[al][qy][qz] = Real(Pq(al, 0)) * A[0 | (qy << 1) | (qz << 2)][5] +
-                          Real(Pq(al, 1)) * A[1 | (qy << 1) | (qz << 2)][5];
+        for (int qx = 0; qx < 2; ++qx)
 #pragma unroll
-    for (int al = 0; al < 2; ++al)
+          for (int qy = 0; qy < 2; ++qy)
+            U[ga][qx][qy] = Real(Pq(ga, 0)) * A[qx | (qy << 1)][4] + Real(Pq(ga, 1)) * A[qx | (qy << 1) | 4][4];
 #pragma unroll
-      for (int ga = 0; ga < 2; ++ga) {
-        const Real V0 = Real(Pq(ga, 0)) * U[al][0][0] + Real(Pq(ga, 1)) * U[al][0][1];   // qy = 0
-        const Real V1 = Real(Pq(ga, 0)) * U[al][1][0] + Real(Pq(ga, 1)) * U[al][1][1];   // qy = 1
+      for (int ga = 0; ga < 2; ++ga)
 #pragma unroll
-        for (int c = 0; c < 3; ++c) Xxz[al][c][ga] = Real(Wq(c, 0)) * V0 + Real(Wq(c, 1)) * V1;
-      }
+        for (int be = 0; be < 2; ++be) {
+          const Real V0 = Real(Pq(be, 0)) * U[ga][0][0] + Real(Pq(be, 1)) * U[ga][0][1];   // qx = 0
+          const Real V1 = Real(Pq(be, 0)) * U[ga][1][0] + Real(Pq(be, 1)) * U[ga][1][1];   // qx = 1
+#pragma unroll
+          for (int c = 0; c < 3; ++c) Xyz[c][be][ga] = Real(Wq(c, 0)) * V0 + Real(Wq(c, 1)) * V1;
+        }
+    }
+    {
+      Real U[2][2][2];   // [al][qy][qz]: sum over qx with P(al, qx)
+#pragma unroll
+      for (int al = 0; al < 2; ++al)
+#pragma unroll
+        for (int qy = 0; qy < 2; ++qy)
+#pragma unroll
+          for (int qz = 0; qz < 2; ++qz)
+            U[al][qy][qz] = Real(Pq(al, 0)) * A[0 | (qy << 1) | (qz << 2)][5] +
+                            Real(Pq(al, 1)) * A[1 | (qy << 1) | (qz << 2)][5];
+#pragma unroll
+      for (int al = 0; al < 2; ++al)
+#pragma unroll
+        for (int ga = 0; ga < 2; ++ga) {
+          const Real V0 = Real(Pq(ga, 0)) * U[al][0][0] + Real(Pq(ga, 1)) * U[al][0][1];   // qy = 0
+          const Real V1 = Real(Pq(ga, 0)) * U[al][1][0] + Real(Pq(ga, 1)) * U[al][1][1];   // qy = 1
+#pragma unroll
+          for (int c = 0; c < 3; ++c) Xxz[al][c][ga] = Real(Wq(c, 0)) * V0 + Real(Wq(c, 1)) * V1;
+        }
+    }
   }
   // ---- entries
   Real K[36];
@@ -581,7 +644,7 @@ __device__ __forceinline__ void cell_element(const Q1DiffParams &p, const CellCo
                                              const double (&u)[8], const double (&ud)[8], Hex8Out<Real, TRANS> &o,
                                              Mid &&mid) {
   if constexpr (KIND == TXA_TET4) tet6_cell<Real, TRANS, UNIT>(X, u, ud, c.k, p.rhoc, p.src, p.alpha, p.beta, o, mid);
-  else hex8_element<Real, TRANS, UNIT>(X, u, ud, c.k[0], p.rhoc, p.src, p.alpha, p.beta, o, mid);
+  else hex8_element<Real, TRANS, UNIT, KIND == kHex8Affine>(X, u, ud, c.k[0], p.rhoc, p.src, p.alpha, p.beta, o, mid);
 }
 
 // Global-memory view of an output pointer.  Stores through a generic pointer become
@@ -1102,6 +1165,28 @@ __global__ void k_dirichlet_batch(const BcItem *__restrict__ items, int *err) {
   }
 }
 
+// Whether a brick cell is a parallelepiped: the bilinear and trilinear coefficients of its
+// trilinear map (Walsh coefficients 3, 5, 6, 7 of the vertex coordinates in binary order)
+// are below 2^-47 of the cell's coordinate magnitude (its centroid and edge coefficients),
+// i.e. at the level of the coordinates' own rounding.  One thread per cell; any cell that
+// is not clears *flag.
+__global__ void k_cell_affine(int nx, int ny, int nz, int64_t nv, const double *__restrict__ xyz, int *flag) {
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= static_cast<int64_t>(nx) * ny * nz) return;
+  const int64_t i = c % nx, j = (c / nx) % ny, k = c / (static_cast<int64_t>(nx) * ny);
+  double nonaff = 0.0, mag = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    double h[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v)
+      h[v] = xyz[d * nv + (i + bit(v, 0)) + (nx + 1) * ((j + bit(v, 1)) + static_cast<int64_t>(ny + 1) * (k + bit(v, 2)))];
+    walsh8(h);   // h[s] = sum_v prod_{d in s} sgn(v,d) X_v (walsh8 computes hi - lo: the same up to sign)
+    nonaff += 0.125 * (fabs(h[3]) + fabs(h[5]) + fabs(h[6]) + fabs(h[7]));
+    mag += 0.125 * (fabs(h[0]) + fabs(h[1]) + fabs(h[2]) + fabs(h[4]));
+  }
+  if (!(nonaff <= 0x1p-47 * mag)) __hip_atomic_store(flag, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // Element and owner waves split in both precisions (the stage is double-buffered in FP32).
 template <int KIND>
 void launch_kind(const Q1DiffParams &p, dim3 grid, int precision, bool trans, hipStream_t s) {
@@ -1140,6 +1225,22 @@ inline int tiles_y(const Mesh &m) { return (m.ny + 1 + TY - 2) / (TY - 1); }
 
 }  // namespace
 
+bool brick_cells_affine(const Mesh &m, hipStream_t s) {
+  if (!m.structured || m.topo != TXA_HEX8) return false;
+  const int64_t nc = static_cast<int64_t>(m.nx) * m.ny * m.nz;
+  DevBuf<int> flag;
+  flag.alloc(1);
+  const int one = 1;
+  TXA_HIP(hipMemcpyAsync(flag.p, &one, sizeof(int), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_cell_affine, dim3(static_cast<unsigned>((nc + 255) / 256)), dim3(256), 0, s, m.nx, m.ny, m.nz,
+                     m.nverts, m.xyz.p, flag.p);
+  TXA_HIP(hipGetLastError());
+  int h = 0;
+  TXA_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  TXA_HIP(hipStreamSynchronize(s));
+  return h != 0;
+}
+
 void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s) {
   // z slabs: enough workgroups to cover the 256 CUs several times over
   const int gx = tiles_x(m), gy = tiles_y(m);
@@ -1152,6 +1253,7 @@ void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s
   dim3 grid(gx, gy, nchunk);
   const bool trans = a.transient != 0;
   if (m.topo == TXA_TET4) launch_kind<TXA_TET4>(p, grid, a.precision, trans, s);
+  else if (m.affine) launch_kind<kHex8Affine>(p, grid, a.precision, trans, s);
   else launch_kind<TXA_HEX8>(p, grid, a.precision, trans, s);
   TXA_HIP(hipGetLastError());
 }
@@ -1167,6 +1269,7 @@ struct Batch {
   std::vector<int> zc;
   std::vector<int4> items_h;   // hex8 items, then tet4 items
   size_t n_hex = 0;            // items of hex8 meshes (the first n_hex of items_h)
+  bool hex_affine = false;     // every hex8 mesh of the batch has parallelepiped cells
   DevBuf<int4> items;
   DevBuf<Q1DiffParams> prm;
   DevBuf<BcItem> bc;
@@ -1226,6 +1329,9 @@ void batch_build_items(Batch &b, int slab) {
   for (auto &l : lists)
     std::stable_sort(l.begin(), l.end(), [&](const int4 &u, const int4 &v) { return cost(u) > cost(v); });
   b.n_hex = lists[0].size();
+  b.hex_affine = true;
+  for (const Mesh *m : b.meshes)
+    if (m->topo == TXA_HEX8 && !m->affine) b.hex_affine = false;
   b.items_h = lists[0];
   b.items_h.insert(b.items_h.end(), lists[1].begin(), lists[1].end());
   b.items.alloc(b.items_h.size());
@@ -1279,7 +1385,8 @@ void batch_enqueue(Batch &b, hipStream_t s, int *err) {
     if (nk[k] == 0) continue;
     const dim3 g(std::min(b.grid, nk[k]));
     BatchQueue *q = b.q.p + k;
-    if (k == 0) launch_batch<TXA_HEX8>(b, g, it[k], nk[k], q, s);
+    if (k == 0 && b.hex_affine) launch_batch<kHex8Affine>(b, g, it[k], nk[k], q, s);
+    else if (k == 0) launch_batch<TXA_HEX8>(b, g, it[k], nk[k], q, s);
     else launch_batch<TXA_TET4>(b, g, it[k], nk[k], q, s);
   }
   hipLaunchKernelGGL(k_dirichlet_batch, dim3(static_cast<unsigned>(n), 8), dim3(256), 0, s, b.bc.p, err);

@@ -376,6 +376,7 @@ void mesh_generate(Mesh &m, const double *host_coords, hipStream_t s) {
   hipLaunchKernelGGL(k_brick_conn, dim3(grid_for(nc)), dim3(kBlock), 0, s, m.topo, b, nc, m.conn.p);
   TXA_HIP(hipGetLastError());
   TXA_HIP(hipStreamSynchronize(s));
+  m.affine = brick_cells_affine(m, s);
 }
 
 // Connectivity width and geometry vertices of an uploadable topology.
