@@ -73,6 +73,7 @@ __device__ const int8_t kQ2[27][3] = {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 // bf16 with round-to-nearest-even (finite inputs), returned as its bit pattern and its value
 __device__ __forceinline__ short bf16_rne(float x, float &v) {
@@ -128,6 +129,10 @@ struct Dims {
   // MFMA tile on distinct bank pairs (a 64 B stride put them on two banks: 8-way
   // conflicts on every operand read) and keeps rows 16-byte aligned
   static constexpr int KS = 40;
+  // FP64 form: quadrature dimension padded to the 4 of v_mfma_f64_16x16x4f64, and an odd
+  // LDS row stride in doubles (the 16 rows of an operand read fall on distinct bank pairs)
+  static constexpr int KP64 = NB == 8 ? 8 : 28;
+  static constexpr int KS64 = KP64 + 1;
 };
 
 template <int NB, int NQ, int MODE>
@@ -152,15 +157,18 @@ struct Smem {
   // FP64 rows are not padded (81 doubles is already an odd bank stride), which keeps
   // the FP64 form under half of a CU's LDS: two workgroups per CU.
   static constexpr int TP = MODE == MODE_F64 ? D::ND : D::ND + 1;
-  // (the MFMA form uses neither T nor G of `v`: they shrink to one entry there)
+  // (the bf16 MFMA form uses neither T nor G of `v`, the FP64 form no G: they shrink to one entry)
   static constexpr int VR = MODE == MODE_SPLIT ? 1 : D::ND;
   union alignas(16) {
     struct {
       TReal T[VR][TP];
-      TReal G[MODE == MODE_SPLIT ? 1 : NQ * 3 * NB];
+      TReal G[MODE == MODE_F32 ? NQ * 3 * NB : 1];
     } v;
     short ops[3 * D::MP * D::KS];
     float T32[D::ND][D::ND + 1];
+    // FP64 form: the operands of T = A^T B, A = (w det sgn) G and B = G, rows (k, a) padded
+    // to MP, points padded to KP64 (zero); T goes over them once the products are in registers
+    double opd[MODE == MODE_F64 ? 2 * D::MP * D::KS64 : 1];
   } w;
 };
 
@@ -300,6 +308,10 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
         reinterpret_cast<uint4 *>(sm.w.ops + (s * D::MP + ND) * D::KS)[o] = make_uint4(0, 0, 0, 0);
       }
     }
+    if constexpr (MODE == MODE_F64) {
+      constexpr int PADD = (D::MP - ND) * D::KS64;   // doubles per operand
+      for (int i = t; i < 2 * PADD; i += BS) sm.w.opd[((i / PADD) * D::MP + ND) * D::KS64 + i % PADD] = 0.0;
+    }
     __syncthreads();
     // measure w |det J| with the sign of the first point (Intrepid2 computeCellMeasure)
     const double sg = sm.det[0] < 0.0 ? -1.0 : 1.0;
@@ -319,7 +331,7 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
     __syncthreads();
     // ---- physical gradients at the quadrature points (MFMA mode: also the zero columns q >= NQ);
     // q runs fastest over the threads, so the operand writes of a wave are consecutive in LDS
-    constexpr int QG = MODE == MODE_SPLIT ? D::KP : NQ;
+    constexpr int QG = MODE == MODE_SPLIT ? D::KP : (MODE == MODE_F64 ? D::KP64 : NQ);
     // when BS is a multiple of QG every thread keeps one q: its J^-1 and sqrt(w |det J|)
     // are read from LDS once
     constexpr bool QFIX = BS % QG == 0;
@@ -341,6 +353,14 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
           for (int s = 0; s < 3; ++s) sm.w.ops[(s * D::MP + k * NB + a) * D::KS + q] = 0;
         continue;
       }
+      if (MODE == MODE_F64 && q >= NQ) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          sm.w.opd[(0 * D::MP + k * NB + a) * D::KS64 + q] = 0.0;
+          sm.w.opd[(1 * D::MP + k * NB + a) * D::KS64 + q] = 0.0;
+        }
+        continue;
+      }
       const int qx = q % Q1, qy = (q / Q1) % Q1, qz = q / (Q1 * Q1);
       const int ax = sm.off[a][0], ay = sm.off[a][1], az = sm.off[a][2];
       const double lx = sm.bL[ax][qx], ly = sm.bL[ay][qy], lz = sm.bL[az][qz];
@@ -359,6 +379,11 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
           sm.w.ops[(0 * D::MP + m) * D::KS + q] = s0;
           sm.w.ops[(1 * D::MP + m) * D::KS + q] = s1;
           sm.w.ops[(2 * D::MP + m) * D::KS + q] = s2;
+        } else if constexpr (MODE == MODE_F64) {
+          // T[(k,a),(l,b)] = sum_q (sgn w det g_ka) g_lb
+          const int m = k * NB + a;
+          sm.w.opd[(0 * D::MP + m) * D::KS64 + q] = sg * sm.wm[q] * g;
+          sm.w.opd[(1 * D::MP + m) * D::KS64 + q] = g;
         } else {
           sm.w.v.G[(q * 3 + k) * NB + a] = static_cast<TReal>(g);
         }
@@ -416,6 +441,46 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
           if (rr < ND && cc < ND) {
             sm.w.T32[rr][cc] = acc[k][i];
             if (tm != tn) sm.w.T32[cc][rr] = acc[k][i];
+          }
+        }
+      }
+    } else if constexpr (MODE == MODE_F64) {
+      // T = A^T B on v_mfma_f64_16x16x4f64 over the upper-triangle 16x16 tiles: lane l holds
+      // A and B element [row l % 16][point 4 ks + l / 16]; the tile's D[l/16 + 4 i][l % 16]
+      // lands in component i (rows interleaved, unlike the FP32-accumulating 16x16 forms)
+      constexpr int NT = D::MP / 16;
+      constexpr int NTILE = NT * (NT + 1) / 2;
+      constexpr int TPW = (NTILE + BS / 64 - 1) / (BS / 64);
+      const double *A = sm.w.opd, *B = sm.w.opd + D::MP * D::KS64;
+      v4d acc[TPW];
+#pragma unroll
+      for (int k = 0; k < TPW; ++k) {
+        acc[k] = v4d{0.0, 0.0, 0.0, 0.0};
+        const int ti = wave + k * (BS / 64);
+        if (ti >= NTILE) continue;
+        int tm = 0, rem = ti;
+        while (rem >= NT - tm) { rem -= NT - tm; ++tm; }
+        const int tn = tm + rem;
+        const double *ar = A + (tm * 16 + (lane & 15)) * D::KS64 + (lane >> 4);
+        const double *br = B + (tn * 16 + (lane & 15)) * D::KS64 + (lane >> 4);
+#pragma unroll
+        for (int ks = 0; ks < D::KP64 / 4; ++ks)
+          acc[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[4 * ks], br[4 * ks], acc[k], 0, 0, 0);
+      }
+      __syncthreads();   // every wave has read the operands: T goes over them
+#pragma unroll
+      for (int k = 0; k < TPW; ++k) {
+        const int ti = wave + k * (BS / 64);
+        if (ti >= NTILE) continue;
+        int tm = 0, rem = ti;
+        while (rem >= NT - tm) { rem -= NT - tm; ++tm; }
+        const int tn = tm + rem;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int rr = tm * 16 + (lane >> 4) + 4 * i, cc = tn * 16 + (lane & 15);
+          if (rr < ND && cc < ND) {
+            sm.w.v.T[rr][cc] = acc[k][i];
+            if (tm != tn) sm.w.v.T[cc][rr] = acc[k][i];
           }
         }
       }
