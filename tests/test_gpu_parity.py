@@ -197,6 +197,23 @@ def test_affine_hex8_diffusion_fp32(engine, transient):
     c.m.destroy()
 
 
+@pytest.mark.parametrize("shape,transient,precision,tol", [((6, 5, 7), True, txa.FP64, TOL64),
+                                                          ((6, 5, 7), True, txa.FP32, TOL32),
+                                                          ((40, 9, 7), False, txa.FP64, TOL64)])
+def test_affine_hex8_elasticity(engine, shape, transient, precision, tol):
+    """Parallelepiped cells, three fields: the node-centric kernel (closed-form blocks, no
+    workspace) in either precision, against the oracle with Dirichlet rows."""
+    c = Case(engine, txa.HEX8, *shape, 3, X=sheared(*shape))
+    assert c.m.affine
+    f, v, fr, vr = run_both(engine, c, txa.ELASTICITY, precision=precision, transient=transient, bc=True,
+                            src=(0.5, -1.0, 0.25), rho=1.5, rayleigh=(0.01, 0.001), alpha=4.0, gamma2=16.0,
+                            sigma=0.3)
+    print(f"affine hex8 elasticity: R {rel_err(f, fr):.2e} J {rel_err(v, vr):.2e}")
+    assert rel_err(f, fr) <= tol
+    assert rel_err(v, vr) <= tol
+    c.m.destroy()
+
+
 @pytest.mark.parametrize("topo,physics,nf,n", [(txa.TET4, txa.DIFFUSION, 1, 4), (txa.HEX27, txa.DIFFUSION, 1, 3),
                                                (txa.HEX8, txa.ELASTICITY, 3, 3), (txa.HEX27, txa.ELASTICITY, 3, 2),
                                                (txa.TET4, txa.ELASTICITY, 3, 3)])

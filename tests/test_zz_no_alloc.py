@@ -84,10 +84,12 @@ def test_noalloc_elasticity_brick(engine, n, precision):
 
 
 def test_noalloc_refuses_a_workspace_beyond_the_static_array(engine):
-    # hex8 elasticity, FP64: 64^3 elements x (8 x 72 block entries + 24 residual rows) x 8 B ~ 1.26 GB
+    # hex8 elasticity, FP64, on jittered cells (the element kernel and gather; parallelepiped
+    # cells need no workspace): 64^3 elements x (8 x 72 block entries + 24 residual rows) x 8 B ~ 1.26 GB
     n = 64
-    m = engine.brick(txa.HEX8, n, n, n, synth.brick_coords(n, n, n))
+    m = engine.brick(txa.HEX8, n, n, n, synth.brick_coords(n, n, n, jitter=0.1, seed=5))
     try:
+        assert not m.affine
         m.build_dofmap(3)
         m.build_graph()
         E = dev(np.ones(m.nelem))

@@ -764,6 +764,211 @@ __global__ void k_node_perm(int64_t px, int64_t py, int64_t pz, int R, int nf, i
   }
 }
 
+// Affine HEX8 bricks (every cell a parallelepiped), three fields: node-centric assembly with
+// no workspace.  One wave per lattice node.  Lane (c, b) = (cell c of the eight around the
+// node, vertex b of that cell, binary order) forms the 3x3 block of the node's vertex a and b in
+// closed form: with J = dx/dxi constant over the cell and C its cofactor matrix,
+//   T_ab = |det J| J^-T S_ab J^-1 = C S_ab C^T / |det J|,
+//   S_ab[m][n] = sum_q dN_a/dxi_m dN_b/dxi_n   (2x2x2 Gauss)
+//              = s_a^m s_b^n f_d / 4 (m != n, d the third axis),  s_a^m s_b^m f_d1 f_d2 / 2 (m = n),
+// with s the vertex's sign along an axis and f = 2/3 where a and b share it, 1/3 otherwise;
+// the mass entry rho |det J| f_x f_y f_z, the source load src |det J| per vertex.  The rows are
+// summed in LDS in eight steps, one per cell in a fixed order: in step c the cell's eight lanes
+// (eight distinct neighbours) add their blocks at the neighbours' positions (from nperm); one
+// wave's LDS operations execute in order, so every entry is summed in cell order and the result
+// is bitwise reproducible.  The wave then writes the node's three rows once, coalesced.  The
+// general (per-point) form is the element kernel plus the gather.
+struct Elastic8Params {
+  int nx, ny, nz;
+  int64_t npoints;
+  const double *X, *Y, *Z;
+  const int32_t *point_lid;   // [points][3]
+  const int64_t *rowptr;
+  const uint16_t *nperm;
+  int nperm_w;
+  const double *coef;         // Young's modulus per cell
+  double nu, rho, src[3], ra, rb;
+  const double *x, *xdot, *xddot;
+  double alpha, beta, gamma2;
+  int transient;
+  double *f, *vals;
+};
+
+__global__ __launch_bounds__(256, 3) void k_elastic8_affine(Elastic8Params p) {
+  constexpr int MAXLEN = 81;   // entries per row: 27 neighbours x 3 fields
+  __shared__ double rows[4][3 * MAXLEN];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int c = lane >> 3, b = lane & 7;
+  const int ox = c & 1, oy = (c >> 1) & 1, oz = c >> 2;   // cell (I-1+ox, J-1+oy, K-1+oz)
+  const int bxyz[3] = {b & 1, (b >> 1) & 1, b >> 2};
+  const int oxyz[3] = {ox, oy, oz};
+  // the node is vertex a = (1-ox, 1-oy, 1-oz) of cell c; vertex b lies at offset o - 1 + b
+  double S[3][3], mf = 1.0;
+  {
+    double sa[3], sb[3], fd[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      sa[d] = oxyz[d] ? -1.0 : 1.0;
+      sb[d] = bxyz[d] ? 1.0 : -1.0;
+      fd[d] = sa[d] == sb[d] ? 2.0 / 3.0 : 1.0 / 3.0;
+      mf *= fd[d];
+    }
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int n = 0; n < 3; ++n)
+        S[m][n] = m == n ? 0.5 * sa[m] * sb[m] * fd[(m + 1) % 3] * fd[(m + 2) % 3] : 0.25 * sa[m] * sb[n] * fd[3 - m - n];
+  }
+  const int dx = ox - 1 + bxyz[0], dy = oy - 1 + bxyz[1], dz = oz - 1 + bxyz[2];
+  const int slot = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);   // neighbour slot of vertex b
+  // 32-bit lattice arithmetic (the host takes this kernel only below 2^31 nodes)
+  const int sx = p.nx + 1, sxy = sx * (p.ny + 1), np = static_cast<int>(p.npoints);
+  const bool tr = p.transient && p.xdot;
+  const bool tra = tr && p.xddot;
+  const double lamf = p.nu / ((1.0 + p.nu) * (1.0 - 2.0 * p.nu)), muf = 0.5 / (1.0 + p.nu);
+  const int stride = static_cast<int>(gridDim.x) * 4;
+  // Loads come in two rounds per node (the lane's vertex coordinates, LID and the cell's
+  // modulus, the node's LID; then the state by that LID and the node's row pointers and
+  // slot map).  The first round of the next node is issued before this node's arithmetic,
+  // the second round of this node as its iteration starts.
+  struct Round1 {
+    int I, J, K;
+    double X[3];
+    int32_t Lb, L0;
+    double E;
+  };
+  auto round1 = [&](int q, Round1 &r) {
+    const int qc = q < np ? q : np - 1;   // past the end: loaded, never used
+    r.I = qc % sx;
+    r.J = (qc / sx) % (p.ny + 1);
+    r.K = qc / sxy;
+    const int cx = r.I - 1 + ox, cy = r.J - 1 + oy, cz = r.K - 1 + oz;
+    const bool cell_ok = cx >= 0 && cx < p.nx && cy >= 0 && cy < p.ny && cz >= 0 && cz < p.nz;
+    // this lane's vertex (clamped into the mesh for absent cells: loaded, never used)
+    const int vx = min(max(r.I + dx, 0), p.nx), vy = min(max(r.J + dy, 0), p.ny), vz = min(max(r.K + dz, 0), p.nz);
+    const int vi = vx + sx * vy + sxy * vz;
+    r.X[0] = p.X[vi];
+    r.X[1] = p.Y[vi];
+    r.X[2] = p.Z[vi];
+    r.Lb = p.point_lid[3 * vi];
+    r.L0 = p.point_lid[3 * qc];
+    r.E = p.coef[cell_ok ? cx + p.nx * (cy + p.ny * cz) : 0];
+  };
+  int pt = static_cast<int>(blockIdx.x) * 4 + w;
+  Round1 cur, nxt;
+  round1(pt, cur);
+  for (; pt < np; pt += stride) {
+    const int I = cur.I, Jy = cur.J, K = cur.K;
+    const int cx = I - 1 + ox, cy = Jy - 1 + oy, cz = K - 1 + oz;
+    const bool cell_ok = cx >= 0 && cx < p.nx && cy >= 0 && cy < p.ny && cz >= 0 && cz < p.nz;
+    // second round of this node
+    double ub[3], vb[3], ab[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      ub[j] = p.x[cur.Lb + j];
+      vb[j] = tr ? p.xdot[cur.Lb + j] : 0.0;
+      ab[j] = tra ? p.xddot[cur.Lb + j] : 0.0;
+    }
+    const int32_t L0 = cur.L0;
+    const int64_t base = p.rowptr[L0];
+    const int len = min(static_cast<int>(p.rowptr[L0 + 1] - base), MAXLEN);
+    const int node = L0 / 3;
+    const uint16_t pos = p.nperm[static_cast<int64_t>(node) * p.nperm_w + slot];
+    // first round of the next node
+    round1(pt + stride, nxt);
+    __builtin_amdgcn_sched_barrier(0);
+    const double Xb[3] = {cur.X[0], cur.X[1], cur.X[2]};
+    const double E = cur.E;
+    // J[d][j] = (1/8) sum_b s_b^j X_b^d over the cell's eight lanes (DPP: every lane the same bits)
+    double Jm[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) Jm[d][j] = 0.125 * group_sum<8>(bxyz[j] ? Xb[d] : -Xb[d]);
+    double C[3][3];
+    C[0][0] = Jm[1][1] * Jm[2][2] - Jm[1][2] * Jm[2][1];
+    C[0][1] = Jm[1][2] * Jm[2][0] - Jm[1][0] * Jm[2][2];
+    C[0][2] = Jm[1][0] * Jm[2][1] - Jm[1][1] * Jm[2][0];
+    C[1][0] = Jm[0][2] * Jm[2][1] - Jm[0][1] * Jm[2][2];
+    C[1][1] = Jm[0][0] * Jm[2][2] - Jm[0][2] * Jm[2][0];
+    C[1][2] = Jm[0][1] * Jm[2][0] - Jm[0][0] * Jm[2][1];
+    C[2][0] = Jm[0][1] * Jm[1][2] - Jm[0][2] * Jm[1][1];
+    C[2][1] = Jm[0][2] * Jm[1][0] - Jm[0][0] * Jm[1][2];
+    C[2][2] = Jm[0][0] * Jm[1][1] - Jm[0][1] * Jm[1][0];
+    const double det = Jm[0][0] * C[0][0] + Jm[0][1] * C[0][1] + Jm[0][2] * C[0][2];
+    const double adet = fabs(det);
+    // T = C S C^T / |det J|
+    double P[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) P[m][l] = S[m][0] * C[l][0] + S[m][1] * C[l][1] + S[m][2] * C[l][2];
+    const double ia = 1.0 / adet;
+    double T[3][3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int l = 0; l < 3; ++l) T[k][l] = ia * (C[k][0] * P[0][l] + C[k][1] * P[1][l] + C[k][2] * P[2][l]);
+    const double lam = E * lamf, mu = E * muf;
+    const double trs = T[0][0] + T[1][1] + T[2][2];
+    const double Mab = tr ? p.rho * adet * mf : 0.0;
+    double rr[3] = {0.0, 0.0, 0.0};
+    double jb[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double Kv = lam * T[i][j] + mu * T[j][i] + (i == j ? mu * trs : 0.0);
+        const double Mv = i == j ? Mab : 0.0;
+        double jv = p.beta * Kv;
+        rr[i] += Kv * ub[j];
+        if (tr) {
+          const double Cv = p.ra * Mv + p.rb * Kv;
+          rr[i] += Cv * vb[j];
+          jv += p.alpha * Cv;
+          if (tra) {
+            rr[i] += Mv * ab[j];
+            jv += p.gamma2 * Mv;
+          }
+        }
+        jb[3 * i + j] = jv;
+      }
+    // the cell's source load enters once (vertex b = a), F_a = src |det J|
+    const bool ba = b == (1 - ox) + 2 * (1 - oy) + 4 * (1 - oz);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      // fixed-order sum over the wave: each half's total (DPP) ends in lanes 16 and 48
+      const double v = group_sum<32>(cell_ok ? rr[i] - (ba ? p.src[i] * adet : 0.0) : 0.0);
+      const double h0 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 16),
+                                         __builtin_amdgcn_readlane(__double2loint(v), 16));
+      const double h1 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 48),
+                                         __builtin_amdgcn_readlane(__double2loint(v), 48));
+      rr[i] = h0 + h1;
+    }
+    // rows: zero, then the cells' blocks in cell order (every column of the rows is a neighbour
+    // within one cell, so every entry is reached)
+    for (int k = lane; k < 3 * len; k += 64) rows[w][k] = 0.0;
+    const bool add = cell_ok && pos != 0xffffu && pos + 2 < len;
+#pragma unroll
+    for (int cc = 0; cc < 8; ++cc)
+      if (c == cc && add)
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int j = 0; j < 3; ++j) atomicAdd(&rows[w][i * len + pos + j], jb[3 * i + j]);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (p.vals)
+      for (int k = lane; k < 3 * len; k += 64) p.vals[base + k] = rows[w][k];
+    if (p.f && lane < 3) p.f[L0 + lane] = rr[lane];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    cur = nxt;
+  }
+}
+
 template <int NB, int NQ, int BS, int MODE>
 void launch(const ElasticParams &p, hipStream_t s) {
   int dev = 0, ncu = 0;
@@ -784,7 +989,16 @@ constexpr size_t kWsAlign = 256;
 size_t ws_round(size_t b) { return (b + kWsAlign - 1) / kWsAlign * kWsAlign; }
 
 // workspace bytes: residuals [E][ND] FP64 when f is wanted, blocks [E][ND][ND] when vals are
+// affine HEX8 bricks take the node-centric kernel (no workspace) in every precision
+bool affine_hex8(const Mesh &m) {
+  return m.topo == TXA_HEX8 && m.affine && m.structured && m.nfields == 3 && m.npoints < (int64_t(1) << 31);
+}
+
 void ws_parts(const Mesh &m, int precision, bool want_f, bool want_vals, size_t *rbytes, size_t *jbytes) {
+  if (affine_hex8(m)) {
+    *rbytes = *jbytes = 0;
+    return;
+  }
   const size_t nd = static_cast<size_t>(m.ndof_elem), ne = static_cast<size_t>(m.nelem);
   *rbytes = want_f ? ws_round(ne * nd * sizeof(double)) : 0;
   const size_t nb = nd / 3;
@@ -824,6 +1038,33 @@ void node_perm_build(Mesh &m, hipStream_t s) {
 
 void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
   if (!a.f && !a.vals) return;
+  if (affine_hex8(m)) {
+    Elastic8Params q;
+    q.nx = m.nx; q.ny = m.ny; q.nz = m.nz;
+    q.npoints = m.npoints;
+    q.X = m.xyz.p; q.Y = m.xyz.p + m.nverts; q.Z = m.xyz.p + 2 * m.nverts;
+    q.point_lid = m.point_lid.p;
+    q.rowptr = m.rowptr.p;
+    q.nperm = m.nperm.p;
+    q.nperm_w = m.nperm_w;
+    q.coef = a.coef;
+    q.nu = a.nu; q.rho = a.rho;
+    q.src[0] = a.src[0]; q.src[1] = a.src[1]; q.src[2] = a.src[2];
+    q.ra = a.rayleigh_a; q.rb = a.rayleigh_b;
+    q.x = a.x; q.xdot = a.xdot; q.xddot = a.xddot;
+    q.alpha = a.alpha; q.beta = a.beta; q.gamma2 = a.gamma2;
+    q.transient = a.transient;
+    q.f = a.f; q.vals = a.vals;
+    int dev = 0, ncu = 0, per_cu = 0;
+    TXA_HIP(hipGetDevice(&dev));
+    TXA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    TXA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_elastic8_affine, 256, 0));
+    const int64_t want = static_cast<int64_t>(ncu) * std::max(per_cu, 1);
+    const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((m.npoints + 3) / 4, want)));
+    hipLaunchKernelGGL(k_elastic8_affine, dim3(grid), dim3(256), 0, s, q);
+    TXA_HIP(hipGetLastError());
+    return;
+  }
   size_t rbytes = 0, jbytes = 0;
   ws_parts(m, a.precision, a.f != nullptr, a.vals != nullptr, &rbytes, &jbytes);
   char *w = static_cast<char *>(ws);
