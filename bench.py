@@ -216,6 +216,61 @@ def summarize_profiles(stats_dir, fetch_dir, write_dir):
     print(out, TRAFFIC_FILE)
 
 
+UTIL_A = ("SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_ACTIVE_INST_ANY "
+          "SQ_ACTIVE_INST_VALU SQ_INSTS_VALU GRBM_GUI_ACTIVE")
+UTIL_B = ("SQ_INSTS_LDS SQ_INSTS_SALU SQ_INSTS_VMEM_RD SQ_INSTS_VMEM_WR SQ_LDS_BANK_CONFLICT SQ_WAIT_INST_LDS "
+          "SQ_ACTIVE_INST_LDS SQ_INST_LEVEL_LDS GRBM_GUI_ACTIVE")
+
+
+def summarize_counters(dir_a, dir_b):
+    """Writes profiles/q1_hex8_diffusion_fp32_256_counters.md from two --pmc passes (UTIL_A,
+    UTIL_B) of the headline workload: issue counters of the FP32 and FP64 volume kernels."""
+    kernels = (FP32_KERNEL, FP64_KERNEL, FP32_KERNEL_GENERAL)
+    vals = {}
+    for d in (dir_a, dir_b):   # GRBM_GUI_ACTIVE is in both passes, whose dispatch ids overlap: keyed per pass
+        for r in _rows(d, "counter_collection.csv"):
+            for k in kernels:
+                if _match(r.get("Kernel_Name", ""), k):
+                    key = (k, r["Counter_Name"])
+                    did = (d, r["Dispatch_Id"])
+                    vals.setdefault(key, {}).setdefault(did, 0.0)
+                    vals[key][did] += float(r["Counter_Value"])
+
+    def avg(k, c):
+        v = vals.get((k, c), {})
+        return sum(v.values()) / len(v) if v else float("nan")
+    present = [k for k in kernels if (k, "SQ_INSTS_VALU") in vals]
+    cmd = "rocprofv3 --pmc {} --output-format csv -d <out>/{} -- python3 bench.py --steps 2 --warmup 1 --no-cpu --no-extra"
+    lines = ["# Issue counters: Q1 Poisson hex8 256^3 (bench.py workload)", "",
+             "Two `--pmc` passes over the bench command, from the repository root on one MI355X:", "",
+             "    " + cmd.format(UTIL_A, "util_a"), "    " + cmd.format(UTIL_B, "util_b"), "",
+             "Values are per dispatch, averaged over each kernel's dispatches. GRBM_GUI_ACTIVE is summed over",
+             "the 8 XCDs, so the shader clock is GRBM_GUI_ACTIVE / 8 / duration. A wave64 VALU instruction",
+             "occupies a SIMD for 4 cycles; VALU busy = 4 x SQ_INSTS_VALU / (1024 SIMDs x GRBM_GUI_ACTIVE / 8).",
+             "The SQ_WAIT_* and SQ_ACTIVE_* rows are ratios to SQ_WAVE_CYCLES. The unit cube's cells are",
+             "parallelepipeds (one metric per cell: kind 17); the general kind (1) appears when the run",
+             "includes jittered cells.", "",
+             "| | " + " | ".join(f"`{k}`" for k in present) + " |", "|---|" + "---:|" * len(present)]
+
+    def row(label, f):
+        lines.append(f"| {label} | " + " | ".join(f(k) for k in present) + " |")
+    row("SQ_WAVES", lambda k: f"{avg(k, 'SQ_WAVES'):.0f}")
+    row("SQ_INSTS_VALU", lambda k: f"{avg(k, 'SQ_INSTS_VALU'):.4g}")
+    row("VALU busy (of SIMD cycles)",
+        lambda k: f"{4 * avg(k, 'SQ_INSTS_VALU') / (1024 * avg(k, 'GRBM_GUI_ACTIVE') / 8):.3f}")
+    row("VALU wave-instructions per 64 elements (element and owner waves)",
+        lambda k: f"{avg(k, 'SQ_INSTS_VALU') / 256 ** 3 * 64:.0f}")
+    for c in ("SQ_ACTIVE_INST_ANY", "SQ_WAIT_INST_ANY", "SQ_WAIT_ANY", "SQ_WAIT_INST_LDS"):
+        row(f"{c} / SQ_WAVE_CYCLES", lambda k, c=c: f"{avg(k, c) / avg(k, 'SQ_WAVE_CYCLES'):.3f}")
+    row("SQ_INSTS_LDS", lambda k: f"{avg(k, 'SQ_INSTS_LDS'):.4g}")
+    row("SQ_LDS_BANK_CONFLICT (cycles)", lambda k: f"{avg(k, 'SQ_LDS_BANK_CONFLICT'):.3g}")
+    row("SQ_INSTS_VMEM_RD / SQ_INSTS_VMEM_WR", lambda k: f"{avg(k, 'SQ_INSTS_VMEM_RD'):.3g} / {avg(k, 'SQ_INSTS_VMEM_WR'):.3g}")
+    row("SQ_INSTS_SALU", lambda k: f"{avg(k, 'SQ_INSTS_SALU'):.4g}")
+    out = os.path.join(ROOT, "profiles", "q1_hex8_diffusion_fp32_256_counters.md")
+    open(out, "w").write("\n".join(lines) + "\n")
+    print(out)
+
+
 def measured_hbm_peaks(gib=2, reps=10):
     """Achievable HBM bandwidth on this GPU (GB/s): a device-to-device copy of `gib` GiB
     (read + write bytes counted) and a fill of the same size (write bytes), each the
@@ -707,18 +762,83 @@ def bench_upload(args):
     return line
 
 
+def bench_elastic(args):
+    """Transient hex8 linear elasticity, the served physics (Newmark seeds alpha 2000, gamma2 4e6,
+    Rayleigh a 0.01, b 0.001, nu 0.3, E lognormal sigma 0.3 per cell, state U[-1e-3, 1e-3],
+    u = 0 on x = 0) on one n^3 brick (default 128), FP64: parallelepiped cells (the unit cube:
+    the node-centric kernel, no workspace) and jittered cells (U[-0.1h, 0.1h]: the element kernel
+    and the node gather).  Compulsory bytes: the CRS values, f and the Dirichlet rows written once,
+    x, xdot, xddot, coordinates, E and the row pointers read once."""
+    import numpy as np
+    import torch
+
+    import tianxin_amd as txa
+    from tianxin_amd import synth
+
+    n = args.n if args.n != 256 else 128
+    eng = txa.Engine(0)
+    out = {}
+    for name, X in (("parallelepiped_cells", synth.brick_coords(n, n, n)),
+                    ("jittered_cells", synth.brick_coords(n, n, n, jitter=0.1, seed=21))):
+        m = eng.brick(txa.HEX8, n, n, n, X)
+        nd = m.build_dofmap(3)
+        nnz = m.build_graph()
+        pl = m.point_lids()
+        bc = torch.as_tensor(np.unique(pl[np.arange(pl.shape[0]) % (n + 1) == 0].ravel()).astype(np.int32),
+                             device="cuda")
+        E = torch.as_tensor(synth.lognormal(22, m.nelem, 0.3), device="cuda")
+        g = torch.Generator(device="cuda")
+        g.manual_seed(23)
+        u, v, a = (torch.rand(nd, generator=g, device="cuda", dtype=torch.float64) * 2e-3 - 1e-3 for _ in range(3))
+        f = torch.empty(nd, device="cuda", dtype=torch.float64)
+        vals = torch.empty(nnz, device="cuda", dtype=torch.float64)
+
+        def step():
+            m.assemble(physics=txa.ELASTICITY, precision=txa.FP64, coef=E, x=u, xdot=v, xddot=a, f=f, vals=vals,
+                       alpha=2000.0, beta=1.0, gamma2=4e6, transient=True, nu=0.3, rho=1.0, rayleigh=(0.01, 0.001),
+                       bc_lids=bc)
+        for _ in range(args.warmup):
+            step()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+        ev[0].record()
+        for i in range(args.steps):
+            step()
+            ev[i + 1].record()
+        torch.cuda.synchronize()
+        per = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)])
+        ms = float(per.mean())
+        compulsory = 8.0 * nnz + 8.0 * nd * 4 + 24.0 * m.nverts + 8.0 * m.nelem + 8.0 * nd
+        out[name] = {"ms_per_step": ms, "elements_per_s": m.nelem / (ms / 1e3), "mesh_affine": m.affine,
+                     "p50": float(np.percentile(per, 50)), "p99": float(np.percentile(per, 99)),
+                     "compulsory_bytes": compulsory, "hbm_frac": compulsory / (ms / 1e3) / HBM_PEAK}
+        log(f"{name}: {ms:.3f} ms/step")
+        m.destroy()
+        del f, vals, u, v, a
+    best = out["parallelepiped_cells"]
+    return {"metric": "elements/s (residual+Jacobian), transient hex8 elasticity", "value": best["elements_per_s"],
+            "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": best["ms_per_step"], "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"hex8 transient elasticity {n}^3 (Newmark, Rayleigh), FP64",
+                       "elements": n ** 3, "E": "lognormal sigma=0.3", "nu": 0.3, "dirichlet": "u=0 on x=0"},
+            "cells": out, "roofline": {"bound": "hbm", "frac": best["hbm_frac"], "peak": HBM_PEAK / 1e9,
+                                       "unit": "GB/s"}}
+
+
 def summarize_q2(stats_dir, pmc_dir):
     """Writes profiles/q2_hex27_elastic_64_kernel_stats.md from a kernel-stats pass and
     one --pmc pass (SQ_VALU_MFMA_BUSY_CYCLES, GRBM_GUI_ACTIVE, SQ_INSTS_VALU,
-    SQ_INSTS_VALU_MFMA_MOPS_BF16, SQ_INSTS_LDS, SQ_WAVES) of `bench.py --workload q2`."""
+    SQ_INSTS_VALU_MFMA_MOPS_BF16, SQ_INSTS_VALU_MFMA_MOPS_F64, SQ_INSTS_LDS, SQ_WAVES) of
+    `bench.py --workload q2`."""
     stats = _rows(stats_dir, "kernel_stats.csv")
     nel = 64 ** 3
+    pmc_list = ("SQ_VALU_MFMA_BUSY_CYCLES GRBM_GUI_ACTIVE SQ_INSTS_VALU SQ_INSTS_VALU_MFMA_MOPS_BF16 "
+                "SQ_INSTS_VALU_MFMA_MOPS_F64 SQ_INSTS_LDS SQ_WAVES")
     lines = ["# Kernel statistics: Q2 hex27 elasticity 64^3 (bench.py --workload q2)", "",
              "Commands (from the repository root, on one MI355X):", "",
              "    rocprofv3 --kernel-trace --stats --output-format csv -d <out>/prof_q2 -- "
              "python3 bench.py --workload q2 --steps 3 --warmup 1",
-             "    rocprofv3 --pmc SQ_VALU_MFMA_BUSY_CYCLES GRBM_GUI_ACTIVE SQ_INSTS_VALU SQ_INSTS_VALU_MFMA_MOPS_BF16 "
-             "SQ_INSTS_LDS SQ_WAVES --output-format csv -d <out>/pmc_q2 -- "
+             f"    rocprofv3 --pmc {pmc_list} --output-format csv -d <out>/pmc_q2 -- "
              "python3 bench.py --workload q2 --steps 1 --warmup 0", "",
              "Each form is two kernels: the element kernel (geometry, Gram matrix, element block and",
              "residual to a workspace) and the node-centric gather (sums the blocks around each node into",
@@ -751,86 +871,83 @@ def summarize_q2(stats_dir, pmc_dir):
                 d[key] = d.get(key, 0.0) + float(r["Counter_Value"])
     lines += ["", "## Element-kernel counters (one dispatch each)", "",
               "| form | MFMA busy (of all SIMD cycles) | VALU wave-instructions / element | "
-              "MFMA bf16 MOPs / element | LDS wave-instructions / element |", "|---|---:|---:|---:|---:|"]
+              "MFMA bf16 MOPs / element | MFMA f64 MOPs / element | LDS wave-instructions / element |",
+              "|---|---:|---:|---:|---:|---:|"]
     for name, d in per.items():
         tot = {}
         for (disp, c), v in d.items():
             tot.setdefault(c, {})[disp] = v
+
         def avg(c):
             vals = list(tot.get(c, {}).values())
             return sum(vals) / len(vals) if vals else float("nan")
         simds = 256 * 4
         util = avg("SQ_VALU_MFMA_BUSY_CYCLES") / (avg("GRBM_GUI_ACTIVE") * simds) if avg("GRBM_GUI_ACTIVE") else 0
         lines.append(f"| {name} | {100 * util:.1f}% | {avg('SQ_INSTS_VALU') / nel:.0f} | "
-                     f"{avg('SQ_INSTS_VALU_MFMA_MOPS_BF16') / nel:.3g} | {avg('SQ_INSTS_LDS') / nel:.0f} |")
-    lines += ["", "MFMA MOPs count 512 flops each: 4.03e3 MOPs per element is the 2.06 Mflop of the six split",
-              "products (21 upper-triangle 16x16 tiles, 2 k-steps).  MFMA is busy about 1% of the time and",
-              "the VALU work (a few thousand wave-instructions per element over 1024 SIMDs) would take under",
-              "1 ms: the element kernel is bound by per-element latency, one element per workgroup passing",
-              "through a chain of dependent phases (loads, geometry, gradients, Gram matrix, pair blocks,",
-              "block store) separated by barriers, at three workgroups per CU.  The gather moves the",
-              "workspace blocks in and the CRS values out at 3.3-3.9 TB/s."]
+                     f"{avg('SQ_INSTS_VALU_MFMA_MOPS_BF16') / nel:.3g} | {avg('SQ_INSTS_VALU_MFMA_MOPS_F64') / nel:.3g} | "
+                     f"{avg('SQ_INSTS_LDS') / nel:.0f} |")
+    lines += ["", "The split-BF16 form runs six bf16 products per 16x16 tile and k-step (21 upper-triangle",
+              "tiles, 2 k-steps), the FP64 form one v_mfma_f64_16x16x4f64 per tile and k-step (21 tiles,",
+              "7 k-steps).  The element kernels run one element per workgroup through barrier-separated",
+              "phases (loads, geometry, gradients, Gram matrix, pair blocks, block store), so per-element",
+              "latency, not the MFMA rate, sets their time; the gather moves the workspace blocks in and",
+              "the CRS values out."]
     out = os.path.join(ROOT, "profiles", "q2_hex27_elastic_64_kernel_stats.md")
     open(out, "w").write("\n".join(lines) + "\n")
     print(out)
 
 
-def other_workloads(args):
-    """Compact summaries of the Q2, batch and serving workloads at their own default sizes,
-    appended to the headline line (each run after the headline's buffers are freed; a
-    failure is recorded in place of its summary and leaves the headline intact)."""
-    import copy
-    import torch
-
-    def pick(d, keys):
-        return {k: d[k] for k in keys if k in d}
-
-    out = {}
-    for name, fn, over in (("q2_elasticity_64", bench_q2, dict(n=64, steps=5, warmup=1)),
-                           ("batch_512_meshes", bench_batch, dict(meshes=512, steps=5)),
-                           ("serve_hex8_newmark", bench_serve, dict(requests=48, pool=8))):
-        a = copy.copy(args)
-        for k, v in over.items():
-            setattr(a, k, v)
-        t0 = time.time()
-        try:
-            line = fn(a)
-        except Exception as e:   # keep the headline: report what failed
-            out[name] = {"error": f"{type(e).__name__}: {e}"}
-            log(f"{name}: failed: {e}")
-            continue
-        finally:
-            torch.cuda.synchronize()
-            torch.cuda.empty_cache()
-        s = pick(line, ("metric", "value", "unit", "ms_per_step", "dtype"))
-        s["config"] = line["config"]["workload"]
-        if fn is bench_q2:
-            f = line["forms"]
-            s["forms_ms"] = {k: v["ms_per_step"] for k, v in f.items()}
-            s["max_rel_err_vs_fp64"] = line["max_rel_err_vs_fp64"]
-            s["rigid_motion_residual_rel"] = line["rigid_motion_residual_rel"]
-            s["roofline"] = {"bound": "hbm", "frac": f["split_bf16"]["hbm_frac"],
-                             "bytes_per_element": Q2_BYTES_PER_ELEM,
-                             "dense_bf16_tflops": f["split_bf16"]["dense_tflops"],
-                             "mfma_frac": f["split_bf16"]["dense_tflops"] / 2500.0}
-        elif fn is bench_batch:
-            f = line["forms"]
-            s["forms_ms"] = {k: v["ms_per_step"] for k, v in f.items()}
-            s["cells_per_s"] = f["fp32_batch"]["cells_per_s"]
-            s["max_rel_err_fp32_vs_fp64_per_mesh"] = line["max_rel_err_fp32_vs_fp64_per_mesh"]
-            s["speedup_vs_fp64_per_mesh"] = line["speedup_vs_fp64_per_mesh"]
-        else:
-            s["plain_fp64_elements_per_s"] = line["plain_fp64_sequential"]["elements_per_s"]
-            s["forms"] = {k: {"p50_ms": v["latency_ms"]["p50"], "p99_ms": v["latency_ms"]["p99"],
-                              "saturated_elements_per_s": v["saturated"]["elements_per_s"],
-                              "speedup_vs_plain_fp64": v["speedup_vs_plain_fp64"],
-                              "max_rel_err_vs_plain_fp64": v["max_rel_err_vs_plain_fp64"]}
-                          for k, v in line["forms"].items()}
-            s["latency_ms"] = pick(line["latency_ms"], ("p50", "p99", "at_qps"))
-        s["wall_s"] = time.time() - t0
-        out[name] = s
-        log(f"{name}: {s.get('ms_per_step', s.get('latency_ms'))} ({s['wall_s']:.1f}s)")
-    return out
+def summarize_elastic(stats_dir, pmc_dir, bench_json):
+    """Writes profiles/elastic_hex8_128_kernel_stats.md from a kernel-trace pass and one --pmc
+    pass of `bench.py --workload elastic` and the line that workload printed."""
+    import numpy as np
+    line = json.loads(open(bench_json).read().strip().splitlines()[-1])
+    trace = _rows(stats_dir, "kernel_trace.csv")
+    pmc_list = "SQ_INSTS_VALU SQ_INSTS_LDS SQ_INSTS_SALU SQ_INSTS_VMEM_RD SQ_WAVES GRBM_GUI_ACTIVE"
+    lines = ["# Kernel statistics: transient hex8 elasticity 128^3, FP64 (bench.py --workload elastic)", "",
+             "Commands (from the repository root, on one MI355X):", "",
+             "    python3 bench.py --workload elastic --steps 10 --warmup 2",
+             "    rocprofv3 --kernel-trace --stats --output-format csv -d <out>/prof_el -- "
+             "python3 bench.py --workload elastic --steps 5 --warmup 1",
+             f"    rocprofv3 --pmc {pmc_list} --output-format csv -d <out>/pmc_el -- "
+             "python3 bench.py --workload elastic --steps 2 --warmup 0", "",
+             "Workload: " + line["config"]["workload"] + ", E lognormal (sigma 0.3) per cell, Rayleigh damping,",
+             "Newmark seeds (alpha 2000, gamma2 4e6), u = 0 on x = 0; a step is one assembly with the",
+             "Dirichlet rows.  Parallelepiped cells (the unit cube) take the node-centric kernel",
+             "`k_elastic8_affine` (closed-form blocks, no workspace); jittered cells take the element",
+             "kernel `k_elastic<8, 8, 64, 0>` and the gather `k_elastic_gather<8, double>`.", "",
+             "| cells | ms per step (HIP events) | p50 / p99 ms | elements/s | HBM fraction (compulsory bytes) |",
+             "|---|---:|---:|---:|---:|"]
+    for name, c in line["cells"].items():
+        lines.append(f"| {name.replace('_', ' ')} | {c['ms_per_step']:.3f} | {c['p50']:.3f} / {c['p99']:.3f} | "
+                     f"{c['elements_per_s']:.3g} | {c['hbm_frac']:.3f} |")
+    lines += ["", "Per-dispatch durations from the kernel trace (us), all dispatches of the run:", "",
+              "| kernel | calls | mean | min | max |", "|---|---:|---:|---:|---:|"]
+    for k in ("k_elastic8_affine", "k_elastic<8, 8, 64, 0>", "k_elastic_gather<8, double>", "k_dirichlet"):
+        d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in trace
+             if (k in r["Kernel_Name"] if "<" not in k else _match(r["Kernel_Name"], k))]
+        if d:
+            lines.append(f"| `{k}` | {len(d)} | {np.mean(d):.1f} | {np.min(d):.1f} | {np.max(d):.1f} |")
+    tot = {}
+    for r in _rows(pmc_dir, "counter_collection.csv"):
+        if "k_elastic8_affine" in r.get("Kernel_Name", ""):
+            tot.setdefault(r["Counter_Name"], {}).setdefault(r["Dispatch_Id"], 0.0)
+            tot[r["Counter_Name"]][r["Dispatch_Id"]] += float(r["Counter_Value"])
+    if tot:
+        avg = {c: sum(v.values()) / len(v) for c, v in tot.items()}
+        nodes = 129 ** 3
+        lines += ["", "## `k_elastic8_affine` counters (per dispatch, per lattice node)", "",
+                  f"- VALU wave-instructions per node: {avg.get('SQ_INSTS_VALU', 0) / nodes:.0f}; LDS: "
+                  f"{avg.get('SQ_INSTS_LDS', 0) / nodes:.0f}; SALU: {avg.get('SQ_INSTS_SALU', 0) / nodes:.0f}; "
+                  f"vector memory reads: {avg.get('SQ_INSTS_VMEM_RD', 0) / nodes:.0f}",
+                  f"- waves per dispatch: {avg.get('SQ_WAVES', 0):.0f}; GRBM_GUI_ACTIVE (summed over the 8 XCDs): "
+                  f"{avg.get('GRBM_GUI_ACTIVE', 0):.3g}",
+                  "- one wave per node: 8 cells x 8 vertices on the 64 lanes, a 3x3 block per lane; the",
+                  "  arithmetic (the cell's Jacobian by DPP sums, cofactors, C S C^T, the Newmark/Rayleigh",
+                  "  combination) and the eight cell-ordered LDS adds bound it, not HBM."]
+    out = os.path.join(ROOT, "profiles", "elastic_hex8_128_kernel_stats.md")
+    open(out, "w").write("\n".join(lines) + "\n")
+    print(out)
 
 
 def main():
@@ -849,9 +966,10 @@ def main():
     ap.add_argument("--summarize-profiles", nargs=3, metavar=("STATS_DIR", "FETCH_DIR", "WRITE_DIR"),
                     help="CPU only: turn rocprofv3 csv output (kernel stats, FETCH_SIZE pass, WRITE_SIZE pass) "
                          "into the summaries under profiles/")
-    ap.add_argument("--workload", choices=("single", "batch", "serve", "q2", "upload"), default="single",
+    ap.add_argument("--workload", choices=("single", "batch", "serve", "q2", "upload", "elastic"), default="single",
                     help="single: the 256^3 Q1 line (default); batch / serve / q2: the other forms; "
-                         "upload: a 128^3 hex8 brick through txa_mesh_upload")
+                         "upload: a 128^3 hex8 brick through txa_mesh_upload; elastic: 128^3 transient hex8 "
+                         "elasticity")
     ap.add_argument("--meshes", type=int, default=512)
     ap.add_argument("--pool", type=int, default=8)
     ap.add_argument("--requests", type=int, default=48)
@@ -862,9 +980,19 @@ def main():
                     help="the served form whose numbers head the serve line (both are measured)")
     ap.add_argument("--summarize-q2", nargs=2, metavar=("STATS_DIR", "PMC_DIR"),
                     help="CPU only: summarise rocprofv3 csv output of --workload q2 into profiles/")
+    ap.add_argument("--summarize-counters", nargs=2, metavar=("UTIL_A_DIR", "UTIL_B_DIR"),
+                    help="CPU only: summarise the two issue-counter passes of the headline into profiles/")
+    ap.add_argument("--summarize-elastic", nargs=3, metavar=("STATS_DIR", "PMC_DIR", "BENCH_JSON"),
+                    help="CPU only: summarise rocprofv3 csv output of --workload elastic into profiles/")
     args = ap.parse_args()
     if args.summarize_q2:
         summarize_q2(*args.summarize_q2)
+        return
+    if args.summarize_elastic:
+        summarize_elastic(*args.summarize_elastic)
+        return
+    if args.summarize_counters:
+        summarize_counters(*args.summarize_counters)
         return
     if args.summarize_profiles:
         summarize_profiles(*args.summarize_profiles)
@@ -873,7 +1001,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1 "
                           "(multi-GPU sharding is a design note, not implemented)"}))
         sys.exit(2)
-    other = {"batch": bench_batch, "serve": bench_serve, "q2": bench_q2, "upload": bench_upload}
+    other = {"batch": bench_batch, "serve": bench_serve, "q2": bench_q2, "upload": bench_upload,
+             "elastic": bench_elastic}
     if args.workload in other:
         print(json.dumps(other[args.workload](args)), flush=True)
         return
