@@ -10,10 +10,11 @@ The timed form integrates elements in FP32 (geometry and the scatter
 accumulation in FP64); the plain FP64 GPU path runs beside it and the max
 relative error of FP32 vs FP64 is reported for R and J.  Prints one JSON line.
 
-After the headline, the default run appends compact summaries of the other
-workloads under "other_workloads" (Q2 hex27 elasticity 64^3, a batch of 512
-Poisson meshes, transient elasticity served at 20 requests/s); --no-extra skips
-them, --workload runs one of them alone with its full line.
+After the headline, the default run times the same workload on jittered cells
+(the general kernel) and appends compact summaries of the other workloads under
+"other_workloads" (Q2 hex27 elasticity 64^3, a batch of 512 Poisson meshes,
+transient elasticity served at 20 requests/s, transient hex8 elasticity 128^3);
+--no-extra skips them, --workload runs one of them alone with its full line.
 """
 from __future__ import annotations
 
@@ -948,6 +949,69 @@ def summarize_elastic(stats_dir, pmc_dir, bench_json):
     out = os.path.join(ROOT, "profiles", "elastic_hex8_128_kernel_stats.md")
     open(out, "w").write("\n".join(lines) + "\n")
     print(out)
+
+
+def other_workloads(args):
+    """Compact summaries of the Q2, batch and serving workloads at their own default sizes,
+    appended to the headline line (each run after the headline's buffers are freed; a
+    failure is recorded in place of its summary and leaves the headline intact)."""
+    import copy
+    import torch
+
+    def pick(d, keys):
+        return {k: d[k] for k in keys if k in d}
+
+    out = {}
+    for name, fn, over in (("q2_elasticity_64", bench_q2, dict(n=64, steps=5, warmup=1)),
+                           ("batch_512_meshes", bench_batch, dict(meshes=512, steps=5)),
+                           ("serve_hex8_newmark", bench_serve, dict(requests=48, pool=8)),
+                           ("elastic_hex8_128", bench_elastic, dict(n=128, steps=5, warmup=1))):
+        a = copy.copy(args)
+        for k, v in over.items():
+            setattr(a, k, v)
+        t0 = time.time()
+        try:
+            line = fn(a)
+        except Exception as e:   # keep the headline: report what failed
+            out[name] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"{name}: failed: {e}")
+            continue
+        finally:
+            torch.cuda.synchronize()
+            torch.cuda.empty_cache()
+        s = pick(line, ("metric", "value", "unit", "ms_per_step", "dtype"))
+        s["config"] = line["config"]["workload"]
+        if fn is bench_q2:
+            f = line["forms"]
+            s["forms_ms"] = {k: v["ms_per_step"] for k, v in f.items()}
+            s["max_rel_err_vs_fp64"] = line["max_rel_err_vs_fp64"]
+            s["rigid_motion_residual_rel"] = line["rigid_motion_residual_rel"]
+            s["roofline"] = {"bound": "hbm", "frac": f["split_bf16"]["hbm_frac"],
+                             "bytes_per_element": Q2_BYTES_PER_ELEM,
+                             "dense_bf16_tflops": f["split_bf16"]["dense_tflops"],
+                             "mfma_frac": f["split_bf16"]["dense_tflops"] / 2500.0}
+        elif fn is bench_batch:
+            f = line["forms"]
+            s["forms_ms"] = {k: v["ms_per_step"] for k, v in f.items()}
+            s["cells_per_s"] = f["fp32_batch"]["cells_per_s"]
+            s["max_rel_err_fp32_vs_fp64_per_mesh"] = line["max_rel_err_fp32_vs_fp64_per_mesh"]
+            s["speedup_vs_fp64_per_mesh"] = line["speedup_vs_fp64_per_mesh"]
+        elif fn is bench_elastic:
+            s["cells"] = {k: {"ms_per_step": v["ms_per_step"], "hbm_frac": v["hbm_frac"]}
+                          for k, v in line["cells"].items()}
+            s["roofline"] = line["roofline"]
+        else:
+            s["plain_fp64_elements_per_s"] = line["plain_fp64_sequential"]["elements_per_s"]
+            s["forms"] = {k: {"p50_ms": v["latency_ms"]["p50"], "p99_ms": v["latency_ms"]["p99"],
+                              "saturated_elements_per_s": v["saturated"]["elements_per_s"],
+                              "speedup_vs_plain_fp64": v["speedup_vs_plain_fp64"],
+                              "max_rel_err_vs_plain_fp64": v["max_rel_err_vs_plain_fp64"]}
+                          for k, v in line["forms"].items()}
+            s["latency_ms"] = pick(line["latency_ms"], ("p50", "p99", "at_qps"))
+        s["wall_s"] = time.time() - t0
+        out[name] = s
+        log(f"{name}: {s.get('ms_per_step', s.get('latency_ms'))} ({s['wall_s']:.1f}s)")
+    return out
 
 
 def main():
