@@ -71,8 +71,9 @@ int txa_create(int device, txa_ctx *out);
 int txa_destroy(txa_ctx ctx);
 const char *txa_last_error(txa_ctx ctx);
 /* Waits for the context's device work.  Returns TXA_ERR_INVALID when an assembly or
- * batch since the previous txa_sync was given a Dirichlet LID outside its DOF map
- * (that entry touched no row; the reference throws for it), and clears the report. */
+ * batch since the previous txa_sync was given a Dirichlet LID outside its DOF map, or a
+ * txa_neumann_sides call a side that does not exist (those entries touched nothing; the
+ * reference throws for them), and clears the report. */
 int txa_sync(txa_ctx ctx);
 /* Library version string (static). */
 const char *txa_version(void);
@@ -229,6 +230,20 @@ int txa_assemble(txa_ctx ctx, int mesh_id, const txa_assembly_args *args, void *
  * Panzer_TpetraLinearObjContainer.hpp:228-237).  LIDs outside the map are reported by txa_sync. */
 int txa_apply_dirichlet(txa_ctx ctx, int mesh_id, const int32_t *lids, const double *values, int64_t n,
                         const double *x, double *f, double *vals, void *stream);
+
+/* Neumann flux on listed element sides of any mesh (uploaded, multi-block or brick): for side
+ * k, the element elems[k] (mesh element index, blocks in order) and its shards side ordinal
+ * ords[k] (Hexahedron<8>/<27>: 0..5, Tetrahedron<4>: 0..3, Wedge<6>: 0..2 quads, 3..4
+ * triangles), f(node b of the side, field i) += values[k nf + i] * integral over the side of
+ * N_b dS: 2x2 Gauss over the bilinear face map for hex8 and wedge quads, 3x3 with the
+ * biquadratic face basis for hex27, one third of the area per vertex on triangles.  Device
+ * arrays; f gains the loads with FP64 atomics (not bitwise reproducible when sides share
+ * nodes).  A side that does not exist touches nothing and is reported by txa_sync.
+ * Replaces TianXin::Flux::evaluateFields on the side worksets of a side set
+ * (disc-fe/src/evaluators/TianXin_Neumann_impl.hpp:141-160; side worksets of
+ * adapters-stk/src/Panzer_STK_WorksetFactory.cpp:150-245: cells with their local side index). */
+int txa_neumann_sides(txa_ctx ctx, int mesh_id, int64_t n, const int64_t *elems, const int *ords,
+                      const double *values, double *f, void *stream);
 
 /* p[i] = value for the n doubles of device array p: zeroing a container's f and matrix
  * values before an assembly accumulates into them (AssemblyEngine's initialisation of the

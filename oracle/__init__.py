@@ -333,3 +333,78 @@ def neumann(topo, nx, ny, nz, X, ev, elem_lid, nf, side, values, f):
                     for i in range(nf):
                         f[elem_lid[e, k * nf + i]] += values[i] * s
     return f
+
+
+# Side node lists in shards order (Hexahedron<8>, Tetrahedron<4>, Wedge<6>: quads, then
+# triangles); quads in cyclic order, corner i at (xi, eta) = (-1,-1), (1,-1), (1,1), (-1,1).
+HEX_SIDE_NODES = [(0, 1, 5, 4), (1, 2, 6, 5), (2, 3, 7, 6), (0, 4, 7, 3), (0, 3, 2, 1), (4, 5, 6, 7)]
+TET_SIDE_NODES = [(0, 1, 3), (1, 2, 3), (0, 3, 2), (0, 2, 1)]
+WEDGE_SIDE_NODES = [(0, 1, 4, 3), (1, 2, 5, 4), (0, 3, 5, 2), (0, 2, 1), (3, 4, 5)]
+
+
+def side_loads(topo, P, side):
+    """(element node, integral over the side of N_node dS) for one side of one element.
+
+    P: the element's vertex coordinates [nv][3] (shards order).  Restates the side worksets'
+    weighted basis of TianXin::Flux (disc-fe/src/evaluators/TianXin_Neumann_impl.hpp:141-160):
+    N restricted to the side times the face measure, on the side cubature of the volume
+    degree.  Quads (hex8 sides, wedge quad sides): bilinear face map and basis, 2x2 Gauss.
+    Triangles (tet4 sides, wedge end triangles): linear, one third of the area per vertex (the
+    exact integral, which every triangle rule of degree >= 1 gives).  Hex27 sides: the nine
+    face nodes' biquadratic basis over the bilinear map of the four corners, 3x3 Gauss."""
+    if topo == TET4 or (topo == WEDGE6 and side >= 3):
+        tri = TET_SIDE_NODES[side] if topo == TET4 else WEDGE_SIDE_NODES[side]
+        a, b, c = (P[k] for k in tri)
+        w = np.linalg.norm(np.cross(b - a, c - a)) / 6.0
+        return [(k, w) for k in tri]
+    if topo in (HEX8, WEDGE6):
+        quad = HEX_SIDE_NODES[side] if topo == HEX8 else WEDGE_SIDE_NODES[side]
+        C = [P[k] for k in quad]
+        g = 1.0 / np.sqrt(3.0)
+        out = [0.0] * 4
+        for xi in (-g, g):
+            for eta in (-g, g):
+                tx = 0.25 * ((1 - eta) * (C[1] - C[0]) + (1 + eta) * (C[2] - C[3]))
+                ty = 0.25 * ((1 - xi) * (C[3] - C[0]) + (1 + xi) * (C[2] - C[1]))
+                J = np.linalg.norm(np.cross(tx, ty))
+                for i, (sx, sy) in enumerate(((-1, -1), (1, -1), (1, 1), (-1, 1))):
+                    out[i] += 0.25 * (1 + sx * xi) * (1 + sy * eta) * J
+        return list(zip(quad, out))
+    # HEX27
+    d, at_max = SIDES[side]
+    u, v = [k for k in range(3) if k != d]
+    C = {(HEX8_NODES[k][u], HEX8_NODES[k][v]): P[k] for k in range(8) if HEX8_NODES[k][d] == at_max}
+    pts, wts = [-np.sqrt(0.6), 0.0, np.sqrt(0.6)], [5 / 9, 8 / 9, 5 / 9]
+    out = []
+    for k in range(27):
+        if HEX27_NODES[k][d] != 2 * at_max:
+            continue
+        a, b = HEX27_NODES[k][u], HEX27_NODES[k][v]
+        s_ = 0.0
+        for xi, wx in zip(pts, wts):
+            for eta, wy in zip(pts, wts):
+                tx = 0.25 * ((1 - eta) * (C[1, 0] - C[0, 0]) + (1 + eta) * (C[1, 1] - C[0, 1]))
+                ty = 0.25 * ((1 - xi) * (C[0, 1] - C[0, 0]) + (1 + xi) * (C[1, 1] - C[1, 0]))
+                s_ += wx * wy * _lagrange(2, a, xi) * _lagrange(2, b, eta) * np.linalg.norm(np.cross(tx, ty))
+        out.append((k, s_))
+    return out
+
+
+def neumann_sides(blocks, X, elem_lids, nf, elems, ords, values, f):
+    """Adds the Neumann flux of listed element sides into f and returns f.
+
+    blocks: (topology, connectivity [nel][nconn], vertices first) of each element block, in
+    order; elem_lids: each block's element LIDs [nel][nconn * nf]; elems: mesh element index of
+    each side (blocks in order); ords: its shards side ordinal; values [n][nf].  The side set's
+    worksets in order: f(LID of node b, i) += values[k][i] * side_loads."""
+    firsts = np.cumsum([0] + [len(c) for _, c in blocks])
+    for k, (e, o) in enumerate(zip(elems, ords)):
+        b = int(np.searchsorted(firsts, e, side="right")) - 1
+        topo, conn = blocks[b]
+        le = e - firsts[b]
+        nv = {HEX8: 8, TET4: 4, WEDGE6: 6, HEX27: 8}[topo]
+        P = X[conn[le, :nv]]
+        for node, w in side_loads(topo, P, int(o)):
+            for i in range(nf):
+                f[elem_lids[b][le, node * nf + i]] += values[k][i] * w
+    return f

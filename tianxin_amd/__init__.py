@@ -146,6 +146,7 @@ def load_library(path: str = LIB_PATH):
                                           ctypes.c_char_p]
     L.txa_read_vector_mm.argtypes = [P, ctypes.c_char_p, P, I64]
     L.txa_apply_dirichlet.argtypes = [P, ctypes.c_int, P, P, I64, P, P, P, P]
+    L.txa_neumann_sides.argtypes = [P, ctypes.c_int, I64, P, P, P, P, P]
     L.txa_spmv.argtypes = [P, ctypes.c_int, P, P, P, P]
     L.txa_model_create.argtypes = [P, ctypes.c_int, ctypes.POINTER(ModelDesc), ctypes.POINTER(ctypes.c_int)]
     L.txa_model_eval.argtypes = [P, ctypes.c_int, ctypes.POINTER(ModelIn), ctypes.POINTER(ModelOut), P]
@@ -403,6 +404,17 @@ class BrickMesh:
         self.eng.check(self.eng.lib.txa_apply_dirichlet(self.eng.h, self.id, _dptr(lids), _dptr(values),
                                                         int(lids.numel()), _dptr(x), _dptr(f), _dptr(vals),
                                                         _stream(stream)))
+
+    def neumann_sides(self, elems, ords, values, f, stream=None):
+        """Neumann flux on listed element sides (txa_neumann_sides): elems int64 [n] (mesh
+        element index), ords int32 [n] (shards side ordinal), values float64 [n * nfields]
+        (flux per field), added into f float64 [ndofs]."""
+        n = int(elems.numel())
+        check_tensors([("elems", elems, "int64", None, True), ("ords", ords, "int32", n, True),
+                       ("values", values, "float64", n * self.nfields, True), ("f", f, "float64", self.ndofs, True)],
+                      self.eng.device)
+        self.eng.check(self.eng.lib.txa_neumann_sides(self.eng.h, self.id, n, _dptr(elems), _dptr(ords),
+                                                      _dptr(values), _dptr(f), _stream(stream)))
 
     def assemble_noalloc(self, *, stream=None, **kw):
         """As assemble, through txa_assemble_noalloc (workspace in the library's static array)."""

@@ -598,7 +598,7 @@ __global__ void k_dirichlet(int64_t n, int64_t nrows, const int32_t *__restrict_
   if (i >= n) return;
   const int32_t r = lids[i];
   if (r < 0 || r >= nrows) {   // outside the map: no row is touched, the context reports it
-    if (lane == 0 && err) *err = 1;
+    if (lane == 0 && err) atomicOr(err, 1);
     return;
   }
   if (lane == 0 && f) f[r] = x[r] - (g ? g[i] : 0.0);
@@ -676,7 +676,227 @@ __global__ void k_neumann(NeumannParams p) {
   for (int i = 0; i < p.nf; ++i) p.f[p.point_lid[pt * p.nf + i]] += p.val[i] * acc;
 }
 
+// ---- Neumann flux on listed element sides, any mesh (the side worksets of a side set)
+// Side node lists in shards order: Hexahedron<8>, Tetrahedron<4>, Wedge<6> (quads, then the
+// two triangles).  Quads in cyclic order: corner i at (xi, eta) = (-1,-1), (1,-1), (1,1), (-1,1).
+__device__ const int8_t kHexSide[6][4] = {{0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6},
+                                          {0, 4, 7, 3}, {0, 3, 2, 1}, {4, 5, 6, 7}};
+__device__ const int8_t kTetSide[4][3] = {{0, 1, 3}, {1, 2, 3}, {0, 3, 2}, {0, 2, 1}};
+__device__ const int8_t kWedgeQuad[3][4] = {{0, 1, 4, 3}, {1, 2, 5, 4}, {0, 3, 5, 2}};
+__device__ const int8_t kWedgeTri[2][3] = {{0, 2, 1}, {3, 4, 5}};
+// lattice offsets of the shards hex vertices and of the Q2 nodes (subcell order)
+__device__ const int8_t kHexOff[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0},
+                                         {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+__device__ const int8_t kQ2Off[27][3] = {
+    {0, 0, 0}, {2, 0, 0}, {2, 2, 0}, {0, 2, 0}, {0, 0, 2}, {2, 0, 2}, {2, 2, 2}, {0, 2, 2},
+    {1, 0, 0}, {2, 1, 0}, {1, 2, 0}, {0, 1, 0}, {1, 0, 2}, {2, 1, 2}, {1, 2, 2}, {0, 1, 2},
+    {0, 0, 1}, {2, 0, 1}, {2, 2, 1}, {0, 2, 1},
+    {1, 0, 1}, {2, 1, 1}, {1, 2, 1}, {0, 1, 1}, {1, 1, 0}, {1, 1, 2}, {1, 1, 1}};
+__device__ const int8_t kSideAxis[6] = {1, 0, 1, 0, 2, 2}, kSideMax[6] = {0, 1, 1, 0, 0, 1};
+
+struct SideBlocks {
+  int n;
+  int64_t elem_begin[kMaxGatherBlocks + 1];   // n + 1 entries: the last is the element count
+  int64_t conn_begin[kMaxGatherBlocks];
+  int64_t pos_begin[kMaxGatherBlocks];
+  int nconn[kMaxGatherBlocks], topo[kMaxGatherBlocks];
+};
+
+struct SideParams {
+  int64_t n;
+  const int64_t *elems;   // [n] mesh element index (blocks in order)
+  const int *ords;        // [n] shards side ordinal of the element's topology
+  const double *values;   // [n][nf] flux per field
+  int nf;
+  const double *X, *Y, *Z;
+  const int64_t *conn;
+  const int32_t *elem_lid;
+  SideBlocks sb;
+  double *f;
+  int *err;               // bit 1 set for a side that does not exist
+};
+
+// integral over a cyclic quad (corners c[0..3], bilinear map) of each corner's bilinear basis
+// times the face measure, 2x2 Gauss
+__device__ void quad_loads(const double (&c)[4][3], double (&out)[4]) {
+  const double g = 0.57735026918962576451;
+  for (int i = 0; i < 4; ++i) out[i] = 0.0;
+  for (int qe = 0; qe < 2; ++qe)
+    for (int qx = 0; qx < 2; ++qx) {
+      const double xi = qx ? g : -g, eta = qe ? g : -g;
+      double tx[3], ty[3];
+      for (int k = 0; k < 3; ++k) {
+        tx[k] = 0.25 * ((1.0 - eta) * (c[1][k] - c[0][k]) + (1.0 + eta) * (c[2][k] - c[3][k]));
+        ty[k] = 0.25 * ((1.0 - xi) * (c[3][k] - c[0][k]) + (1.0 + xi) * (c[2][k] - c[1][k]));
+      }
+      const double nx_ = tx[1] * ty[2] - tx[2] * ty[1], ny_ = tx[2] * ty[0] - tx[0] * ty[2],
+                   nz_ = tx[0] * ty[1] - tx[1] * ty[0];
+      const double J = sqrt(nx_ * nx_ + ny_ * ny_ + nz_ * nz_);
+      out[0] += 0.25 * (1.0 - xi) * (1.0 - eta) * J;
+      out[1] += 0.25 * (1.0 + xi) * (1.0 - eta) * J;
+      out[2] += 0.25 * (1.0 + xi) * (1.0 + eta) * J;
+      out[3] += 0.25 * (1.0 - xi) * (1.0 + eta) * J;
+    }
+}
+
+// one third of a triangle's area (the integral of each linear vertex basis over it)
+__device__ double tri_third(const double (&a)[3], const double (&b)[3], const double (&c)[3]) {
+  const double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]}, v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  const double n0 = u[1] * v[2] - u[2] * v[1], n1 = u[2] * v[0] - u[0] * v[2], n2 = u[0] * v[1] - u[1] * v[0];
+  return sqrt(n0 * n0 + n1 * n1 + n2 * n2) / 6.0;
+}
+
+// One thread per side; the loads add into f with FP64 atomics (sides share nodes).
+__global__ void k_neumann_sides(SideParams p) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= p.n) return;
+  const int64_t e = p.elems[k];
+  const int ord = p.ords[k];
+  if (e < 0 || e >= p.sb.elem_begin[p.sb.n]) {
+    if (p.err) atomicOr(p.err, 2);
+    return;
+  }
+  int b = 0;
+  while (b + 1 < p.sb.n && p.sb.elem_begin[b + 1] <= e) ++b;
+  const int topo = p.sb.topo[b], nconn = p.sb.nconn[b];
+  const int64_t le = e - p.sb.elem_begin[b];
+  const int64_t *cn = p.conn + p.sb.conn_begin[b] + le * nconn;
+  const int32_t *el = p.elem_lid + p.sb.pos_begin[b] + le * nconn * p.nf;
+  const int nsides = topo == TXA_TET4 ? 4 : (topo == TXA_WEDGE6 ? 5 : 6);
+  if (ord < 0 || ord >= nsides) {
+    if (p.err) atomicOr(p.err, 2);
+    return;
+  }
+  auto vtx = [&](int v, double (&x)[3]) {
+    const int64_t id = cn[v];
+    x[0] = p.X[id]; x[1] = p.Y[id]; x[2] = p.Z[id];
+  };
+  auto add = [&](int node, double w) {
+    for (int i = 0; i < p.nf; ++i) atomicAdd(&p.f[el[node * p.nf + i]], p.values[k * p.nf + i] * w);
+  };
+  const bool tri = topo == TXA_TET4 || (topo == TXA_WEDGE6 && ord >= 3);
+  if (tri) {
+    const int8_t *t = topo == TXA_TET4 ? kTetSide[ord] : kWedgeTri[ord - 3];
+    double a[3], bb[3], c[3];
+    vtx(t[0], a); vtx(t[1], bb); vtx(t[2], c);
+    const double w = tri_third(a, bb, c);
+    for (int i = 0; i < 3; ++i) add(t[i], w);
+  } else if (topo != TXA_HEX27) {
+    const int8_t *q = topo == TXA_WEDGE6 ? kWedgeQuad[ord] : kHexSide[ord];
+    double c[4][3], w[4];
+    for (int i = 0; i < 4; ++i) vtx(q[i], c[i]);
+    quad_loads(c, w);
+    for (int i = 0; i < 4; ++i) add(q[i], w[i]);
+  } else {
+    // 9 face nodes with the biquadratic face basis over the bilinear face map, 3x3 Gauss
+    const int d = kSideAxis[ord], am = kSideMax[ord], u = d == 0 ? 1 : 0, v = d == 2 ? 1 : 2;
+    double C[2][2][3];
+    for (int vv = 0; vv < 8; ++vv)
+      if (kHexOff[vv][d] == am) {
+        double x[3];
+        vtx(vv, x);
+        for (int i = 0; i < 3; ++i) C[kHexOff[vv][u]][kHexOff[vv][v]][i] = x[i];
+      }
+    const double g3 = 0.77459666924148337704;
+    for (int nd = 0; nd < 27; ++nd) {
+      if (kQ2Off[nd][d] != 2 * am) continue;
+      const int a = kQ2Off[nd][u], bq = kQ2Off[nd][v];
+      double acc = 0.0;
+      for (int qv = 0; qv < 3; ++qv)
+        for (int qu = 0; qu < 3; ++qu) {
+          const double xi = (qu - 1) * g3, eta = (qv - 1) * g3;
+          const double wq = (qu == 1 ? 8.0 / 9.0 : 5.0 / 9.0) * (qv == 1 ? 8.0 / 9.0 : 5.0 / 9.0);
+          double tx[3], ty[3];
+          for (int kk = 0; kk < 3; ++kk) {
+            tx[kk] = 0.25 * ((1.0 - eta) * (C[1][0][kk] - C[0][0][kk]) + (1.0 + eta) * (C[1][1][kk] - C[0][1][kk]));
+            ty[kk] = 0.25 * ((1.0 - xi) * (C[0][1][kk] - C[0][0][kk]) + (1.0 + xi) * (C[1][1][kk] - C[1][0][kk]));
+          }
+          const double n0 = tx[1] * ty[2] - tx[2] * ty[1], n1 = tx[2] * ty[0] - tx[0] * ty[2],
+                       n2 = tx[0] * ty[1] - tx[1] * ty[0];
+          acc += wq * lagrange(2, a, xi) * lagrange(2, bq, eta) * sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+        }
+      add(nd, acc);
+    }
+  }
+}
+
+// Brick sides of a TET4 brick: the boundary triangles of the six-tet split.  One thread per
+// (cell on the side, tet); each of the tet's faces with all three vertices on the side's
+// plane adds a third of its area to its vertices' rows (FP64 atomics).
+struct TetSideParams {
+  int nx, ny, nz, nf, axis, at_max;
+  double val[3];
+  const double *X, *Y, *Z;
+  const int64_t *conn;
+  const int32_t *elem_lid;
+  double *f;
+};
+
+__global__ void k_neumann_tet_brick(TetSideParams p) {
+  const int n[3] = {p.nx, p.ny, p.nz};
+  const int d = p.axis, u = d == 0 ? 1 : 0, v = d == 2 ? 1 : 2;
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= static_cast<int64_t>(n[u]) * n[v] * 6) return;
+  const int tet = static_cast<int>(t % 6);
+  const int64_t cf = t / 6;
+  int cc[3];
+  cc[d] = p.at_max ? n[d] - 1 : 0;
+  cc[u] = static_cast<int>(cf % n[u]);
+  cc[v] = static_cast<int>(cf / n[u]);
+  const int64_t cell = cc[0] + static_cast<int64_t>(p.nx) * (cc[1] + static_cast<int64_t>(p.ny) * cc[2]);
+  const int64_t e = 6 * cell + tet;
+  const int64_t *cn = p.conn + e * 4;
+  const int32_t *el = p.elem_lid + e * 4 * p.nf;
+  const int64_t sx = p.nx + 1, sxy = sx * (p.ny + 1);
+  const int64_t plane = p.at_max ? n[d] : 0;
+  bool on[4];
+  for (int i = 0; i < 4; ++i) {
+    const int64_t id = cn[i];
+    const int64_t c = d == 0 ? id % sx : (d == 1 ? (id / sx) % (p.ny + 1) : id / sxy);
+    on[i] = c == plane;
+  }
+  for (int fs = 0; fs < 4; ++fs) {
+    const int8_t *tr = kTetSide[fs];
+    if (!(on[tr[0]] && on[tr[1]] && on[tr[2]])) continue;
+    double x[3][3];
+    for (int i = 0; i < 3; ++i) {
+      const int64_t id = cn[tr[i]];
+      x[i][0] = p.X[id]; x[i][1] = p.Y[id]; x[i][2] = p.Z[id];
+    }
+    const double w = tri_third(x[0], x[1], x[2]);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < p.nf; ++j) atomicAdd(&p.f[el[tr[i] * p.nf + j]], p.val[j] * w);
+  }
+}
+
 }  // namespace
+
+void neumann_sides(const Mesh &m, int64_t n, const int64_t *elems, const int *ords, const double *values, double *f,
+                   hipStream_t s, int *err) {
+  if (n == 0) return;
+  TXA_REQUIRE(m.nfields >= 1 && m.elem_lid.p, TXA_ERR_LOGIC, "DOF map not built");
+  TXA_REQUIRE(elems && ords && values && f, TXA_ERR_INVALID, "null argument");
+  TXA_REQUIRE(static_cast<int>(m.blocks.size()) <= kMaxGatherBlocks, TXA_ERR_UNSUPPORTED,
+              "Neumann sides on meshes of more than 8 element blocks");
+  SideParams p;
+  p.n = n; p.elems = elems; p.ords = ords; p.values = values; p.nf = m.nfields;
+  p.X = m.xyz.p; p.Y = m.xyz.p + m.nverts; p.Z = m.xyz.p + 2 * m.nverts;
+  p.conn = m.conn.p; p.elem_lid = m.elem_lid.p;
+  p.sb.n = static_cast<int>(m.blocks.size());
+  for (int b = 0; b < p.sb.n; ++b) {
+    const MeshBlock &mb = m.blocks[b];
+    p.sb.elem_begin[b] = mb.elem_begin;
+    p.sb.conn_begin[b] = mb.conn_begin;
+    p.sb.pos_begin[b] = m.pos_begin(mb);
+    p.sb.nconn[b] = mb.nconn;
+    p.sb.topo[b] = mb.topo;
+  }
+  p.sb.elem_begin[p.sb.n] = m.nelem;
+  p.f = f;
+  p.err = err;
+  hipLaunchKernelGGL(k_neumann_sides, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s, p);
+  TXA_HIP(hipGetLastError());
+}
 
 void upload_refelem(int topo, DevBuf<double> &tab, hipStream_t s) {
   RefElem r;
@@ -818,8 +1038,8 @@ void assemble(Mesh &m, const txa_assembly_args &a_in, hipStream_t s, void *ws, i
   TXA_REQUIRE(a.n_neumann >= 0 && (a.n_neumann == 0 || (a.neumann_sides && a.neumann_values)), TXA_ERR_INVALID,
               "neumann_sides / neumann_values missing");
   if (a.n_neumann > 0) {
-    TXA_REQUIRE(m.structured && (m.topo == TXA_HEX8 || m.topo == TXA_HEX27), TXA_ERR_UNSUPPORTED,
-                "Neumann sides are implemented for hex8 and hex27 bricks");
+    TXA_REQUIRE(m.structured, TXA_ERR_UNSUPPORTED,
+                "brick sides (TXA_SIDE_*) need a brick; other meshes list their sides (txa_neumann_sides)");
     for (int k = 0; k < a.n_neumann; ++k)
       TXA_REQUIRE(a.neumann_sides[k] >= 0 && a.neumann_sides[k] <= 5, TXA_ERR_INVALID,
                   "side " + std::to_string(a.neumann_sides[k]) + " is not a side of the brick (0..5)");
@@ -905,7 +1125,25 @@ void assemble(Mesh &m, const txa_assembly_args &a_in, hipStream_t s, void *ws, i
     a.vals = a_in.vals;
   }
   // Neumann sides, one launch each (stream order keeps the sums of shared edge points in a fixed order)
-  for (int k = 0; a.f && k < a.n_neumann; ++k) {
+  for (int k = 0; a.f && k < a.n_neumann && m.topo == TXA_TET4; ++k) {
+    TetSideParams tp;
+    tp.nx = m.nx; tp.ny = m.ny; tp.nz = m.nz; tp.nf = m.nfields;
+    const int side = a.neumann_sides[k];
+    static const int kAx[6] = {1, 0, 1, 0, 2, 2}, kMx[6] = {0, 1, 1, 0, 0, 1};
+    tp.axis = kAx[side];
+    tp.at_max = kMx[side];
+    for (int i = 0; i < 3; ++i) tp.val[i] = i < m.nfields ? a.neumann_values[k * m.nfields + i] : 0.0;
+    tp.X = m.xyz.p; tp.Y = m.xyz.p + m.nverts; tp.Z = m.xyz.p + 2 * m.nverts;
+    tp.conn = m.conn.p;
+    tp.elem_lid = m.elem_lid.p;
+    tp.f = a.f;
+    const int n3[3] = {m.nx, m.ny, m.nz};
+    const int u = tp.axis == 0 ? 1 : 0, v = tp.axis == 2 ? 1 : 2;
+    const int64_t nt = static_cast<int64_t>(n3[u]) * n3[v] * 6;
+    hipLaunchKernelGGL(k_neumann_tet_brick, dim3(static_cast<unsigned>((nt + 255) / 256)), dim3(256), 0, s, tp);
+    TXA_HIP(hipGetLastError());
+  }
+  for (int k = 0; a.f && k < a.n_neumann && m.topo != TXA_TET4; ++k) {
     NeumannParams np;
     np.step = m.topo == TXA_HEX27 ? 2 : 1;
     np.nx = m.nx; np.ny = m.ny; np.nz = m.nz; np.nf = m.nfields;

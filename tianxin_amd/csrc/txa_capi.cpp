@@ -124,9 +124,12 @@ int txa_sync(txa_ctx ctx) {
     TXA_HIP(hipMemcpy(&flag, ctx->err.p, sizeof(int), hipMemcpyDeviceToHost));
     if (flag) {
       TXA_HIP(hipMemset(ctx->err.p, 0, sizeof(int)));
-      throw Error(TXA_ERR_INVALID,
-                  "a Dirichlet LID of an assembly since the last txa_sync lies outside the DOF map "
-                  "(its row was not applied)");
+      std::string msg;
+      if (flag & 1) msg = "a Dirichlet LID of an assembly since the last txa_sync lies outside the DOF map "
+                          "(its row was not applied)";
+      if (flag & 2) msg += std::string(msg.empty() ? "" : "; ") + "a Neumann side since the last txa_sync does not "
+                           "exist (element index or side ordinal out of range; it was not applied)";
+      throw Error(TXA_ERR_INVALID, msg);
     }
   });
 }
@@ -324,6 +327,15 @@ int txa_apply_dirichlet(txa_ctx ctx, int mesh_id, const int32_t *lids, const dou
   return guarded(ctx, [&] {
     txa::apply_dirichlet(ctx->mesh(mesh_id), lids, values, n, x, f, vals, static_cast<hipStream_t>(stream),
                          ctx->err.p);
+  });
+}
+
+int txa_neumann_sides(txa_ctx ctx, int mesh_id, int64_t n, const int64_t *elems, const int *ords,
+                      const double *values, double *f, void *stream) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(n >= 0, TXA_ERR_INVALID, "negative side count");
+    Mesh &m = ctx->mesh(mesh_id);
+    txa::neumann_sides(m, n, elems, ords, values, f, static_cast<hipStream_t>(stream), ctx->err.p);
   });
 }
 

@@ -213,6 +213,9 @@ size_t elastic_workspace_bytes(const Mesh &m, int precision, bool want_f, bool w
 void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws);
 // Q1 diffusion fast path and batches (txa_q1_hex8.hip)
 void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s);
+// Neumann flux on listed element sides (device arrays); a side that does not exist sets bit 1 of *err.
+void neumann_sides(const Mesh &m, int64_t n, const int64_t *elems, const int *ords, const double *values, double *f,
+                   hipStream_t s, int *err);
 // Whether every cell of a HEX8 brick is a parallelepiped (to within its coordinates' rounding).
 bool brick_cells_affine(const Mesh &m, hipStream_t s);
 Batch *batch_create(const std::vector<Mesh *> &meshes, const std::vector<int> &ids, int precision, bool transient);

@@ -1156,7 +1156,7 @@ __global__ void k_dirichlet_batch(const BcItem *__restrict__ items, int *err) {
        i += static_cast<int64_t>(gridDim.y) * (blockDim.x / 64)) {
     const int32_t r = b.lids[i];
     if (r < 0 || r >= b.nrows) {
-      if (lane == 0 && err) *err = 1;
+      if (lane == 0 && err) atomicOr(err, 1);
       continue;
     }
     if (lane == 0 && b.f) b.f[r] = b.x[r] - (b.g ? b.g[i] : 0.0);
