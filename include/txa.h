@@ -134,6 +134,13 @@ int txa_mesh_info(txa_ctx ctx, int mesh_id, int64_t *out);
  * created: its lattice kernels then form one metric per cell instead of one per
  * integration point.  0 otherwise.  Introspection only; no counterpart in the reference. */
 int txa_mesh_affine(txa_ctx ctx, int mesh_id, int *affine);
+/* Node patches of an uploaded one-field HEX8 / TET4 mesh of one block, built with its graph:
+ * out[0] = rows per patch, out[1] = patches, out[2] = element instances (elements counted
+ * once per patch they touch), out[3] = element colours, out[4] = longest row.  All zero
+ * when the mesh has none (bricks, several blocks, other topologies or fields, or more than
+ * 64 colours): its assembly then takes the element kernels.  Introspection only; no
+ * counterpart in the reference. */
+int txa_mesh_patches(txa_ctx ctx, int mesh_id, int64_t *out);
 
 /* DOF map for `nfields` fields on every connectivity entity (1 = scalar,
  * 3 = displacement).  GIDs by the GUN rule on one rank, owned_ by first touch

@@ -3,10 +3,12 @@
 A brick's own connectivity with its elements shuffled and its vertices
 renumbered, and the reference's Exodus fixtures (tests/golden), are uploaded as
 unstructured meshes.  The DOF map and graph must equal the oracle's on the same
-arrays bit for bit; the assembly (the generic kernels, FP64: one workgroup per
-element, or one element per lane with the Jacobian gathered per row) must equal
-the oracle's within 1e-12 normwise, Dirichlet rows included.  Lattice-only
-features (reduced precision, Neumann sides, batches) must be refused.
+arrays bit for bit; the assembly (one-field hex8 / tet4 diffusion: the node-patch
+kernels, FP64 and FP32; other physics and topologies: the element kernels, FP64)
+must equal the oracle's within 1e-12 normwise in FP64, Dirichlet rows included, and
+in the FP32 form R entrywise over the 1e-12 band and J normwise within 1e-6.
+Lattice-only features (split-BF16 diffusion, brick side sets, batches) must be
+refused.
 """
 import os
 
@@ -16,7 +18,7 @@ import torch
 
 import oracle as O
 import tianxin_amd as txa
-from test_gpu_parity import TOL64, dev, rel_err
+from test_gpu_parity import TOL32, TOL64, dev, entry_err, rel_err
 from tianxin_amd import exodus, synth
 
 pytestmark = pytest.mark.gpu
@@ -91,13 +93,16 @@ def test_renumbered_brick_many_blocks(engine, topo):
 
 @pytest.mark.parametrize("topo", [txa.HEX8, txa.TET4])
 def test_uploaded_jacobian_row_gather(engine, topo):
-    # the lane kernel's Jacobian is gathered per row (no atomics): repeated calls agree bit
-    # for bit, and a Jacobian-only call gives the full call's values
+    # the node-patch kernel sums every row in LDS in a fixed (colour, element) order and stores
+    # it once: repeated calls agree bit for bit in f and J, and a Jacobian-only call gives the
+    # full call's values
     conn, ev, X, nent = renumbered_brick(topo, 9, seed=80 + topo)
     m = engine.upload(topo, conn, X, nentities=nent)
     try:
         nd = m.build_dofmap(1)
         nnz = m.build_graph()
+        pt = m.patches()
+        assert pt is not None and pt["patches"] >= 2 and pt["instances"] >= m.nelem
         coef = dev(synth.lognormal(81, conn.shape[0], 0.5))
         x = dev(synth.uniform(82, nd, -1, 1))
 
@@ -106,12 +111,65 @@ def test_uploaded_jacobian_row_gather(engine, topo):
             v = torch.full((nnz,), 7.0, device="cuda", dtype=torch.float64)
             m.assemble(physics=txa.DIFFUSION, coef=coef, x=x, f=f, vals=v, src=(1.0,))
             torch.cuda.synchronize()
-            return v.cpu().numpy()
+            return (f.cpu().numpy() if want_f else None), v.cpu().numpy()
 
-        v1, v2, v3 = run(True), run(True), run(False)
-        assert np.array_equal(v1, v2)
+        (f1, v1), (f2, v2), (_, v3) = run(True), run(True), run(False)
+        assert np.array_equal(f1, f2) and np.array_equal(v1, v2)
         assert np.array_equal(v1, v3)
         assert np.count_nonzero(v1) > 0.9 * nnz
+    finally:
+        m.destroy()
+
+
+@pytest.mark.parametrize("topo", [txa.HEX8, txa.TET4])
+@pytest.mark.parametrize("transient", [False, True])
+def test_uploaded_fp32_form(engine, topo, transient):
+    # FP32 element matrices, FP64 geometry and residual (the flux form): R entrywise over the
+    # whole 1e-12 band, J normwise, against the FP64 oracle
+    conn, ev, X, nent = renumbered_brick(topo, 8, seed=100 + topo)
+    m = engine.upload(topo, conn, X, nentities=nent)
+    try:
+        nd = m.build_dofmap(1)
+        nnz = m.build_graph()
+        assert m.patches() is not None
+        gids, owned, el = O.dofmap(conn, 1)
+        rp, col = O.graph(el, owned.size)
+        coef = synth.lognormal(101, conn.shape[0], 0.5)
+        x, xd = synth.uniform(102, nd, -1, 1), synth.uniform(103, nd, -1, 1)
+        kw = dict(alpha=2.0, beta=0.8, gamma2=0.0) if transient else dict(alpha=0.0, beta=1.0, gamma2=0.0)
+        A = O.Assembler(OTOPO[topo], txa.DIFFUSION, ev, X, el, coef, rp, col, nu=0.3, rho=1.3, src=(1.0, 0.0, 0.0))
+        fr, vr = A.evaluate(x, xd if transient else None, None, transient=transient, mode=O.MODE_EXPLICIT, **kw)
+        f = torch.empty(nd, device="cuda", dtype=torch.float64)
+        v = torch.empty(nnz, device="cuda", dtype=torch.float64)
+        m.assemble(physics=txa.DIFFUSION, precision=txa.FP32, coef=dev(coef), x=dev(x),
+                   xdot=dev(xd) if transient else None, f=f, vals=v, rho=1.3, src=(1.0,), transient=transient, **kw)
+        torch.cuda.synchronize()
+        er = entry_err(f.cpu().numpy(), fr, 1e-12)
+        ev_ = rel_err(v.cpu().numpy(), vr)
+        print(f"FP32 form on an uploaded mesh: R entrywise {er:.2e}, J normwise {ev_:.2e}")
+        assert er <= TOL32 and ev_ <= TOL32
+    finally:
+        m.destroy()
+
+
+@pytest.mark.parametrize("topo", [txa.HEX8, txa.TET4])
+def test_uploaded_accumulate(engine, topo):
+    # accumulate = 1 adds the patch rows to what f and vals hold: twice into zeros is twice one call
+    conn, ev, X, nent = renumbered_brick(topo, 5, seed=110 + topo)
+    m = engine.upload(topo, conn, X, nentities=nent)
+    try:
+        nd = m.build_dofmap(1)
+        nnz = m.build_graph()
+        kw = dict(physics=txa.DIFFUSION, coef=dev(synth.lognormal(111, conn.shape[0], 0.5)),
+                  x=dev(synth.uniform(112, nd, -1, 1)), src=(1.0,))
+        f1 = torch.empty(nd, device="cuda", dtype=torch.float64)
+        v1 = torch.empty(nnz, device="cuda", dtype=torch.float64)
+        m.assemble(f=f1, vals=v1, **kw)
+        f2, v2 = torch.zeros_like(f1), torch.zeros_like(v1)
+        for _ in range(2):
+            m.assemble(f=f2, vals=v2, accumulate=True, **kw)
+        torch.cuda.synchronize()
+        assert torch.equal(f2, 2 * f1) and torch.equal(v2, 2 * v1)
     finally:
         m.destroy()
 
@@ -170,7 +228,7 @@ def test_upload_rejections(engine):
         k = torch.ones(m.nelem, device="cuda", dtype=torch.float64)
         x = torch.zeros(nd, device="cuda", dtype=torch.float64)
         f = torch.empty(nd, device="cuda", dtype=torch.float64)
-        for kw in (dict(precision=txa.FP32), dict(neumann=[(txa.SIDE_LEFT, (1.0,))])):
+        for kw in (dict(precision=txa.SPLIT_BF16), dict(neumann=[(txa.SIDE_LEFT, (1.0,))])):
             with pytest.raises(txa.TxaError) as e:
                 m.assemble(physics=txa.DIFFUSION, coef=k, x=x, f=f, **kw)
             assert e.value.code == txa.TXA_ERR_UNSUPPORTED

@@ -2,8 +2,9 @@
 txa_assemble on the same inputs, through the C ABI.
 
 Both kernels that take a workspace are covered: the uploaded-mesh row gather
-(one-field hex8 diffusion, element matrices in the workspace) and the two-phase
-elasticity kernels (element blocks in the workspace).  Each case runs twice in a
+(one-field hex8 diffusion on a mesh of two element blocks, element matrices in the
+workspace; a single-block mesh takes the node-patch kernel, which needs none) and the
+two-phase elasticity kernels (element blocks in the workspace).  Each case runs twice in a
 row, the second time on a smaller mesh, so the second call reuses the static
 workspace the first call left behind.  Matrix values must be equal bit for bit
 (both gathers sum every row in a fixed order); residuals equal bit for bit on the
@@ -48,10 +49,12 @@ def both(m, **kw):
 def test_noalloc_uploaded_hex8_diffusion(engine, n):
     print(f"[noalloc] uploaded hex8 {n}x{n + 1}x{n + 2}", flush=True)
     conn, X = shuffled_hex8(n, 70 + n)
-    m = engine.upload(txa.HEX8, conn, X)
+    h = conn.shape[0] // 2
+    m = engine.upload_blocks([(txa.HEX8, conn[:h]), (txa.HEX8, conn[h:])], X)
     try:
         m.build_dofmap(1)
         m.build_graph()
+        assert m.patches() is None
         k = dev(synth.lognormal(71, m.nelem, 0.5))
         x = dev(synth.uniform(72, m.ndofs, -1, 1))
         bc = dev(np.arange(0, m.ndofs, 7, dtype=np.int32), torch.int32)

@@ -120,6 +120,7 @@ def load_library(path: str = LIB_PATH):
     L.txa_mesh_upload.argtypes = [P, ctypes.c_int, I64, P, I64, I64, P, ctypes.POINTER(ctypes.c_int)]
     L.txa_mesh_info.argtypes = [P, ctypes.c_int, P]
     L.txa_mesh_affine.argtypes = [P, ctypes.c_int, P]
+    L.txa_mesh_patches.argtypes = [P, ctypes.c_int, P]
     L.txa_mesh_upload_blocks.argtypes = [P, ctypes.c_int, P, I64, I64, P, ctypes.POINTER(ctypes.c_int)]
     L.txa_mesh_blocks.argtypes = [P, ctypes.c_int, P, I64]
     L.txa_dofmap_build.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.POINTER(I64)]
@@ -365,6 +366,15 @@ class BrickMesh:
         self.eng.check(self.eng.lib.txa_graph_build(self.eng.h, self.id, ctypes.byref(n)))
         self.nnz = int(n.value)
         return self.nnz
+
+    def patches(self):
+        """Node patches of an uploaded one-field hex8 / tet4 mesh (after build_graph): rows per
+        patch, patches, element instances, colours, longest row; None when the mesh has none."""
+        out = np.zeros(5, np.int64)
+        self.eng.check(self.eng.lib.txa_mesh_patches(self.eng.h, self.id, _np(out)))
+        if out[1] == 0:
+            return None
+        return dict(zip(("rows_per_patch", "patches", "instances", "colours", "longest_row"), map(int, out)))
 
     def graph(self, with_cols=True):
         rp = np.zeros(self.ndofs + 1, np.int64)

@@ -12,7 +12,7 @@ ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["txa_capi.cpp", "txa_setup.hip", "txa_assemble.hip", "txa_q1_hex8.hip", "txa_elastic.hip", "txa_serve.cpp",
-           "txa_io.cpp"]
+           "txa_io.cpp", "txa_patch.hip"]
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          # packed FP32 instructions stay off: ordinary float code only
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

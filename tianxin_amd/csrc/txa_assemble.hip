@@ -950,6 +950,7 @@ static size_t gather_ke_doubles(const Mesh &m, std::vector<int64_t> *begin) {
 // Workspace of the structured kernels' own parts (the elasticity blocks), before the
 // staging arrays of an accumulating call.
 static size_t kernel_workspace_bytes(const Mesh &m, int physics, int precision, bool want_f, bool want_vals) {
+  if (patch_supported(m, physics, precision)) return 0;
   if (lane_gather_mesh(m, physics, precision)) return want_vals ? sizeof(double) * gather_ke_doubles(m, nullptr) : 0;
   txa_assembly_args a{};
   a.physics = physics;
@@ -1049,10 +1050,12 @@ void assemble(Mesh &m, const txa_assembly_args &a_in, hipStream_t s, void *ws, i
     launch_q1_hex8_diffusion(m, a, s);
   } else if (elastic_supported(m, a)) {
     launch_elastic(m, a, s, ws);
+  } else if (patch_supported(m, a.physics, a.precision)) {
+    launch_patch(m, a, s);
   } else {
     TXA_REQUIRE(a.precision == TXA_FP64, TXA_ERR_UNSUPPORTED,
-                "reduced-precision integration is implemented for hex8 / tet4 diffusion and 3-field elasticity "
-                "on bricks");
+                "reduced-precision integration is implemented for hex8 / tet4 diffusion (bricks and single-block "
+                "uploaded meshes) and 3-field elasticity on bricks");
     // the Jacobian of the lane kernels by the row gather: element matrices in ws, rows stored whole
     const bool gather = a.vals != nullptr && ws != nullptr && lane_gather_mesh(m, a.physics, a.precision);
     std::vector<int64_t> ke_begin;

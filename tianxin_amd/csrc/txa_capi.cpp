@@ -218,6 +218,19 @@ int txa_mesh_affine(txa_ctx ctx, int mesh_id, int *affine) {
   });
 }
 
+int txa_mesh_patches(txa_ctx ctx, int mesh_id, int64_t *out) {
+  return guarded(ctx, [&] {
+    TXA_REQUIRE(out, TXA_ERR_INVALID, "null argument");
+    const Mesh &m = ctx->mesh(mesh_id);
+    const bool on = m.npatch > 0;
+    out[0] = on ? m.patch_P : 0;
+    out[1] = m.npatch;
+    out[2] = on ? m.patch_ninst : 0;
+    out[3] = on ? m.patch_NC : 0;
+    out[4] = on ? m.patch_RS : 0;
+  });
+}
+
 int txa_dofmap_build(txa_ctx ctx, int mesh_id, int nfields, int64_t *ndofs) {
   return guarded(ctx, [&] {
     Mesh &m = ctx->mesh(mesh_id);

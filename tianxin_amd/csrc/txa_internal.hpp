@@ -103,6 +103,16 @@ struct Mesh {
   // in elem_lid (element e of a block, row a: block's position base + e*ndof + a), ascending
   // (occ[occ_ptr[r] .. occ_ptr[r+1]) for row r), for the row gather of the assembly
   DevBuf<int64_t> occ_ptr, occ;
+  // node patches of uploaded one-field HEX8 / TET4 meshes (txa_patch.hip): rows along a Morton
+  // curve cut into patches of patch_P, the elements touching each patch by (colour, element)
+  int patch_P = 0, patch_RS = 0, patch_NC = 0;   // rows per patch, LDS row stride, colours
+  int64_t npatch = 0, patch_ninst = 0;
+  DevBuf<int64_t> patch_cstart;   // [npatch*NC + 1] first instance of each (patch, colour)
+  DevBuf<int32_t> patch_inst;     // element of each instance
+  DevBuf<uint8_t> patch_icol;     // colour of each instance
+  DevBuf<int32_t> patch_rows;     // [npatch*P] row at each curve position, -1 past the last row
+  DevBuf<double> patch_rec;       // [rows][4] vertex x, y, z and the row's curve position (int64 bits)
+  DevBuf<uint8_t> elem_slot;      // [nelem][nb][nb] position of LID b in the CRS row of LID a
   DevBuf<double> ref;         // reference-element tables of the topology (single-topology meshes)
   DevBuf<double> block_ref[5];   // reference-element tables by topology (TXA_HEX8 .. TXA_WEDGE6)
   bool mixed() const { return blocks.size() > 1 && topo == 0; }
@@ -213,6 +223,10 @@ size_t elastic_workspace_bytes(const Mesh &m, int precision, bool want_f, bool w
 void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws);
 // Q1 diffusion fast path and batches (txa_q1_hex8.hip)
 void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s);
+// uploaded one-field HEX8 / TET4 diffusion by node patches (txa_patch.hip)
+void patch_build(Mesh &m, hipStream_t s);
+bool patch_supported(const Mesh &m, int physics, int precision);
+void launch_patch(const Mesh &m, const txa_assembly_args &a, hipStream_t s);
 // Neumann flux on listed element sides (device arrays); a side that does not exist sets bit 1 of *err.
 void neumann_sides(const Mesh &m, int64_t n, const int64_t *elems, const int *ords, const double *values, double *f,
                    hipStream_t s, int *err);

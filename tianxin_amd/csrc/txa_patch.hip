@@ -1,0 +1,561 @@
+// Uploaded one-field hex8 / tet4 diffusion by node patches.
+//
+// Replaces, for meshes that arrive as connectivity (a ConnManager's elements), the same
+// chain as the lattice kernels: Gather -> DOF/DOFGradient -> Integrator_GradBasisDotVector
+// (k grad u), Integrator_BasisTimesScalar (source, transient rho_c udot) ->
+// ScatterResidual_Tpetra (disc-fe/src/evaluators/Panzer_GatherSolution_Tpetra_impl.hpp:549-650,
+// Panzer_Integrator_GradBasisDotVector_impl.hpp:225-294, Panzer_Integrator_BasisTimesScalar_impl.hpp:215-270,
+// Panzer_ScatterResidual_Tpetra_impl.hpp:375-520), with the element loop of
+// Panzer_AssemblyEngine_impl.hpp:160-172 over worksets of any element order.
+//
+// Layout, built once per mesh with its graph:
+// - the rows (LIDs; one field, so a row is a vertex) sorted along a Morton curve of their
+//   vertex coordinates and cut into patches of P consecutive rows;
+// - for each patch, its instances: every element with a vertex in the patch, listed by
+//   colour, then element;
+// - a colouring of the elements (Jones-Plassmann with hashed priorities): elements that
+//   share a vertex differ in colour;
+// - the slot map: for each element, row a and column b, the position of LID b in LID a's
+//   CRS row.
+// One workgroup assembles one patch.  Each lane integrates one instance in registers
+// (the sum-factorised hex8 routine of the lattice kernels, or the constant-gradient tet),
+// then adds the element rows of the patch's own vertices into the patch's rows in LDS.
+// The adds go colour by colour with a barrier between colours, so a row receives at
+// most one add per step: the sums run in a fixed order (colour, element) and the
+// results do not depend on scheduling.  Finally each row is stored once, whole, with
+// its residual entry.  No workspace, no global atomics, no dependence on the order in
+// which the elements were uploaded beyond the colouring's priorities.  Elements on a
+// patch border are integrated once per patch they touch.
+#include "txa_internal.hpp"
+#include "txa_q1_elem.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+
+namespace txa {
+namespace {
+using namespace q1elem;
+
+constexpr int kPatchThreads = 256;
+constexpr int kMaxColours = 64;         // colour masks are 64-bit
+constexpr int kPatchLdsBytes = 65536;   // row accumulators of one patch
+constexpr int kSetupBlock = 256;
+
+inline unsigned setup_grid(int64_t n) { return static_cast<unsigned>((n + kSetupBlock - 1) / kSetupBlock); }
+
+// ---------------------------------------------------------------------------------------
+// setup
+
+// Vertex coordinates by row (one field: the row of an element position is its vertex's LID).
+__global__ void k_node_xyz(int64_t npos, const int64_t *__restrict__ conn, const int32_t *__restrict__ elem_lid,
+                           const double *__restrict__ xyz, int64_t nverts, double *__restrict__ rec) {
+  const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= npos) return;
+  const int64_t v = conn[p];
+  const int64_t r = elem_lid[p];
+  rec[4 * r + 0] = xyz[v];   // every element position of a vertex writes the same values
+  rec[4 * r + 1] = xyz[nverts + v];
+  rec[4 * r + 2] = xyz[2 * nverts + v];
+}
+
+__device__ __forceinline__ uint64_t spread3(uint64_t x) {   // 21 bits to every third bit
+  x &= 0x1fffffull;
+  x = (x | (x << 32)) & 0x1f00000000ffffull;
+  x = (x | (x << 16)) & 0x1f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+struct Box {
+  double lo[3], scale[3];
+};
+
+__global__ void k_morton(int64_t nrows, const double *__restrict__ rec, Box b, uint64_t *__restrict__ key,
+                         int32_t *__restrict__ row) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  uint64_t q[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double t = (rec[4 * r + d] - b.lo[d]) * b.scale[d];
+    t = t < 0.0 ? 0.0 : (t > 2097151.0 ? 2097151.0 : t);
+    q[d] = static_cast<uint64_t>(t);
+  }
+  key[r] = spread3(q[0]) | (spread3(q[1]) << 1) | (spread3(q[2]) << 2);
+  row[r] = static_cast<int32_t>(r);
+}
+
+// patch_rows[pos] = row (padded with -1), and the row's position in the w slot of its record
+__global__ void k_place_rows(int64_t nrows, int64_t npad, const int32_t *__restrict__ sorted, int32_t *__restrict__ prow,
+                             double *__restrict__ rec) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= npad) return;
+  const int32_t r = i < nrows ? sorted[i] : -1;
+  prow[i] = r;
+  if (r >= 0) rec[4 * static_cast<int64_t>(r) + 3] = __longlong_as_double(i);
+}
+
+__device__ __forceinline__ uint64_t prio(uint64_t e) {   // splitmix64 finaliser: hashed priorities
+  uint64_t z = e + 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// One Jones-Plassmann round: an uncoloured element whose priority exceeds that of every
+// uncoloured element sharing a vertex takes the smallest colour no coloured neighbour has.
+// Reads cin (the colours before the round), writes cout (a copy of cin on entry).
+__global__ void k_jp_round(int64_t nelem, int nb, const int32_t *__restrict__ elem_lid,
+                           const int64_t *__restrict__ occ_ptr, const int64_t *__restrict__ occ,
+                           const int32_t *__restrict__ cin, int32_t *__restrict__ cout, int *__restrict__ done,
+                           int *__restrict__ maxc) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= nelem || cin[e] >= 0) return;
+  const uint64_t pe = prio(static_cast<uint64_t>(e));
+  uint64_t used = 0;
+  for (int v = 0; v < nb; ++v) {
+    const int32_t r = elem_lid[e * nb + v];
+    for (int64_t k = occ_ptr[r]; k < occ_ptr[r + 1]; ++k) {
+      const int64_t e2 = occ[k] / nb;
+      if (e2 == e) continue;
+      const int32_t c2 = cin[e2];
+      if (c2 >= 0) {
+        used |= 1ull << c2;
+      } else {
+        const uint64_t p2 = prio(static_cast<uint64_t>(e2));
+        if (p2 > pe || (p2 == pe && e2 > e)) return;   // a neighbour goes first
+      }
+    }
+  }
+  const int c = used == ~0ull ? kMaxColours : __builtin_ctzll(~used);
+  cout[e] = c < kMaxColours ? c : kMaxColours;   // kMaxColours: out of colours (the patch path is refused)
+  atomicAdd(done, 1);
+  atomicMax(maxc, c);
+}
+
+// The distinct patches of each element, as keys patch << 38 | colour << 32 | element; the
+// other slots hold ~0 (sorted last).
+__global__ void k_instance_keys(int64_t nelem, int nb, int P, const int32_t *__restrict__ elem_lid,
+                                const double *__restrict__ rec, const int32_t *__restrict__ colour,
+                                uint64_t *__restrict__ key, unsigned long long *__restrict__ count) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  int64_t pt[8];
+  int n = 0;
+  for (int v = 0; v < nb; ++v) {
+    const int64_t pos = __double_as_longlong(rec[4 * static_cast<int64_t>(elem_lid[e * nb + v]) + 3]);
+    const int64_t p = pos / P;
+    bool seen = false;
+    for (int j = 0; j < n; ++j) seen = seen || pt[j] == p;
+    if (!seen) pt[n++] = p;
+  }
+  const uint64_t tail = (static_cast<uint64_t>(colour[e]) << 32) | static_cast<uint64_t>(e);
+  for (int v = 0; v < nb; ++v) key[e * nb + v] = v < n ? (static_cast<uint64_t>(pt[v]) << 38) | tail : ~0ull;
+  atomicAdd(count, static_cast<unsigned long long>(n));
+}
+
+// Instances from the sorted keys, and the count of each (patch, colour) pair.
+__global__ void k_instances(int64_t ninst, int nc, const uint64_t *__restrict__ key, int32_t *__restrict__ inst,
+                            uint8_t *__restrict__ icol, unsigned long long *__restrict__ cnt) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= ninst) return;
+  const uint64_t k = key[i];
+  const int64_t p = static_cast<int64_t>(k >> 38);
+  const int c = static_cast<int>((k >> 32) & 63);
+  inst[i] = static_cast<int32_t>(k & 0xffffffffull);
+  icol[i] = static_cast<uint8_t>(c);
+  atomicAdd(cnt + p * nc + c, 1ull);
+}
+
+// Slot map: position of LID b in the CRS row of LID a, for every element (uint8; 255: absent).
+__global__ void k_elem_slots(int64_t nelem, int nb, const int32_t *__restrict__ elem_lid,
+                             const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                             uint8_t *__restrict__ slot) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nelem * nb) return;
+  const int64_t e = t / nb;
+  const int32_t ra = elem_lid[t];
+  const int64_t r0 = rowptr[ra], n = rowptr[ra + 1] - r0;
+  for (int b = 0; b < nb; ++b) {
+    const int32_t cb = elem_lid[e * nb + b];
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (col[r0 + mid] < cb) lo = mid + 1; else hi = mid;
+    }
+    slot[t * nb + b] = lo < n && col[r0 + lo] == cb ? static_cast<uint8_t>(lo) : uint8_t(255);
+  }
+}
+
+__global__ void k_max_row(int64_t nrows, const int64_t *__restrict__ rowptr, int *__restrict__ out) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= nrows) return;
+  atomicMax(out, static_cast<int>(rowptr[r + 1] - rowptr[r]));
+}
+
+// ---------------------------------------------------------------------------------------
+// assembly
+
+struct PatchParams {
+  int P, RS, NC;
+  const int64_t *cstart;       // [npatch*NC + 1] first instance of each (patch, colour)
+  const int32_t *inst;         // element of each instance
+  const uint8_t *icol;         // colour of each instance
+  const int32_t *prow;         // [npatch*P] row at each curve position, -1 past the last
+  const double4 *rec;          // [rows] vertex x, y, z; curve position (int64 bits) in w
+  const int32_t *elem_lid;     // [nelem][NB]
+  const uint8_t *slot;         // [nelem][NB][NB]
+  const int64_t *rowptr;
+  const double *coef, *x, *xdot;
+  double alpha, beta, rhoc, src;
+  double *f, *vals;
+  int add;
+};
+
+// shards hex order -> binary order v = bx + 2by + 4bz (an involution)
+__device__ __forceinline__ constexpr int hex_bin(int s) { return s ^ ((s >> 1) & 1); }
+
+// One linear tet, 4-point degree-2 rule in closed form: K_ab = k |T| grad N_a . grad N_b,
+// M_ab = rho_c |T| (1 + delta_ab) / 20, F_a = src |T| / 4 (the rule integrates these
+// exactly; IntegrationValues2 weights carry |det J|, computeCellMeasure's sign).  R is FP64
+// in both precisions: the flux k |T| grad u . grad N_a from the FP64 cofactors, then
+// M udot - F; the matrix (Real) is beta K + alpha M.
+template <typename Real, bool TRANS, bool UNIT>
+__device__ __forceinline__ void tet4_element(const double (&X)[3][4], const double (&u)[4], const double (&ud)[4],
+                                             double k, double rhoc, double src, double alpha, double beta,
+                                             Real (&K)[10], double (&R)[4]) {
+  double J[3][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) J[d][j] = X[d][j + 1] - X[d][0];
+  double c[3][3];
+  c[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  c[0][1] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  c[0][2] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  c[1][0] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+  c[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+  c[1][2] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+  c[2][0] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+  c[2][1] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+  c[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  const double det = J[0][0] * c[0][0] + J[0][1] * c[0][1] + J[0][2] * c[0][2];
+  const double sg = det < 0.0 ? -1.0 : 1.0;
+  const double vol = det * sg / 6.0;
+  const double cw = k * sg * recip(det) / 6.0;   // k |T| / det^2
+  // grad N_{j+1} = c[:, j] / det, grad N_0 = -(sum); flux in FP64
+  const double du0 = u[1] - u[0], du1 = u[2] - u[0], du2 = u[3] - u[0];
+  double tt[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) tt[i] = cw * (c[i][0] * du0 + c[i][1] * du1 + c[i][2] * du2);
+  double rs = 0.0;
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    R[j + 1] = c[0][j] * tt[0] + c[1][j] * tt[1] + c[2][j] * tt[2];
+    rs += R[j + 1];
+  }
+  R[0] = -rs;
+  const double fs = src * vol * 0.25;
+  const double mw = rhoc * vol * 0.05;
+  double sud = 0.0;
+  if constexpr (TRANS) sud = ud[0] + ud[1] + ud[2] + ud[3];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) R[a] = (TRANS ? R[a] + mw * (sud + ud[a]) : R[a]) - fs;
+  double g[4][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    g[1][i] = c[i][0]; g[2][i] = c[i][1]; g[3][i] = c[i][2];
+    g[0][i] = -(c[i][0] + c[i][1] + c[i][2]);
+  }
+  int n = 0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = a; b < 4; ++b, ++n) {
+      double v = cw * (g[a][0] * g[b][0] + g[a][1] * g[b][1] + g[a][2] * g[b][2]);
+      if (!TRANS && UNIT) { K[n] = Real(v); continue; }
+      v *= beta;
+      if (TRANS) v += alpha * mw * (a == b ? 2.0 : 1.0);
+      K[n] = Real(v);
+    }
+}
+
+__device__ __forceinline__ constexpr int sym4(int a, int b) {
+  return a <= b ? a * 4 - (a * (a - 1)) / 2 + (b - a) : b * 4 - (b * (b - 1)) / 2 + (a - b);
+}
+
+template <int TOPO, typename Real, bool TRANS, bool UNIT>
+__global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
+  constexpr int NB = TOPO == TXA_HEX8 ? 8 : 4;
+  extern __shared__ double sacc[];   // [P][RS] row entries, then [P] residual entries
+  const int P = p.P, RS = p.RS, NC = p.NC;
+  double *sres = sacc + P * RS;
+  const int tid = threadIdx.x;
+  const int64_t pb = blockIdx.x;
+  for (int i = tid; i < P * (RS + 1); i += kPatchThreads) sacc[i] = 0.0;
+  const int64_t i_begin = p.cstart[pb * NC], i_end = p.cstart[(pb + 1) * NC];
+  const int64_t pos0 = pb * P;
+  __syncthreads();
+  for (int64_t c0 = i_begin; c0 < i_end; c0 += kPatchThreads) {
+    const int64_t i = c0 + tid;
+    const bool act = i < i_end;
+    const int64_t ilast = (c0 + kPatchThreads < i_end ? c0 + kPatchThreads : i_end) - 1;
+    const int cfirst = p.icol[c0], clast = p.icol[ilast];
+    int mycol = -1;
+    int lrow[NB];
+    uint32_t w[NB * NB / 4];   // slot map of the element, four slots a word
+    Real Ke[NB * (NB + 1) / 2];
+    double Re[NB];
+    if (act) {
+      const int64_t e = p.inst[i];
+      mycol = p.icol[i];
+      int32_t L[NB];
+#pragma unroll
+      for (int v = 0; v < NB; ++v) L[v] = p.elem_lid[e * NB + v];
+      double X[3][NB], u[NB], ud[NB];
+#pragma unroll
+      for (int v = 0; v < NB; ++v) {
+        const double4 r = p.rec[L[v]];
+        X[0][v] = r.x; X[1][v] = r.y; X[2][v] = r.z;
+        const int64_t lp = __double_as_longlong(r.w) - pos0;
+        lrow[v] = lp >= 0 && lp < P ? static_cast<int>(lp) : -1;
+        u[v] = p.x[L[v]];
+        ud[v] = TRANS ? p.xdot[L[v]] : 0.0;
+      }
+      // the slot map is loaded once the element's geometry registers are free
+      auto load_slots = [&] {
+        if (!p.vals) return;
+        const uint4 *s4 = reinterpret_cast<const uint4 *>(p.slot + e * NB * NB);
+#pragma unroll
+        for (int q = 0; q < NB * NB / 16; ++q) {
+          const uint4 t = s4[q];
+          w[4 * q] = t.x; w[4 * q + 1] = t.y; w[4 * q + 2] = t.z; w[4 * q + 3] = t.w;
+        }
+      };
+      const double k = p.coef[e];
+      if constexpr (TOPO == TXA_HEX8) {
+        double Xb[3][8], ub[8], udb[8];
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+          Xb[0][hex_bin(s)] = X[0][s]; Xb[1][hex_bin(s)] = X[1][s]; Xb[2][hex_bin(s)] = X[2][s];
+          ub[hex_bin(s)] = u[s];
+          udb[hex_bin(s)] = ud[s];
+        }
+        Hex8Out<Real, TRANS> o;
+        hex8_element<Real, TRANS, UNIT, false>(Xb, ub, udb, k, p.rhoc, p.src, p.alpha, p.beta, o, load_slots);
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+          Re[a] = o.R[hex_bin(a)];
+#pragma unroll
+          for (int b = a; b < 8; ++b) Ke[sym8(a, b)] = o.K[sym8(hex_bin(a), hex_bin(b))];
+        }
+      } else {
+        load_slots();
+        tet4_element<Real, TRANS, UNIT>(X, u, ud, k, p.rhoc, p.src, p.alpha, p.beta, Ke, Re);
+      }
+    }
+    // colour by colour: within one colour no two instances share a vertex
+    for (int c = cfirst; c <= clast; ++c) {
+      __syncthreads();
+      if (mycol == c) {
+#pragma unroll
+        for (int a = 0; a < NB; ++a) {
+          if (lrow[a] < 0) continue;
+          if (p.f) __hip_atomic_fetch_add(sres + lrow[a], Re[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+          if (p.vals) {
+            double *row = sacc + lrow[a] * RS;
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+              const double kv = static_cast<double>(NB == 8 ? Ke[sym8(a, b)] : Ke[sym4(a, b)]);
+              // one add per address and colour: the atomic is a single LDS instruction, not a race
+              const uint32_t sab = (w[(a * NB + b) >> 2] >> (8 * ((a * NB + b) & 3))) & 0xffu;
+              __hip_atomic_fetch_add(row + sab, kv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // store the patch's rows whole: 32 lanes per row
+  const int lane = tid & 31;
+  for (int r = tid >> 5; r < P; r += kPatchThreads / 32) {
+    const int32_t L = p.prow[pos0 + r];
+    if (L < 0) continue;
+    if (p.vals) {
+      const int64_t r0 = p.rowptr[L];
+      const int n = static_cast<int>(p.rowptr[L + 1] - r0);
+      for (int s = lane; s < n; s += 32) {
+        const double v = sacc[r * RS + s];
+        p.vals[r0 + s] = p.add ? p.vals[r0 + s] + v : v;
+      }
+    }
+    if (p.f && lane == 0) p.f[L] = p.add ? p.f[L] + sres[r] : sres[r];
+  }
+}
+
+template <int TOPO, typename Real>
+void launch_topo(const Mesh &m, const PatchParams &pp, bool tr, bool unit, hipStream_t s) {
+  const dim3 grid(static_cast<unsigned>(m.npatch)), block(kPatchThreads);
+  const size_t lds = sizeof(double) * static_cast<size_t>(pp.P) * (pp.RS + 1);
+  if (tr) hipLaunchKernelGGL((k_patch<TOPO, Real, true, false>), grid, block, lds, s, pp);
+  else if (unit) hipLaunchKernelGGL((k_patch<TOPO, Real, false, true>), grid, block, lds, s, pp);
+  else hipLaunchKernelGGL((k_patch<TOPO, Real, false, false>), grid, block, lds, s, pp);
+  TXA_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void patch_build(Mesh &m, hipStream_t s) {
+  m.npatch = 0;
+  m.patch_P = 0;
+  const bool fits = !m.structured && m.nfields == 1 && m.blocks.size() == 1 &&
+                    (m.topo == TXA_HEX8 || m.topo == TXA_TET4) && m.occ.p && m.nelem < (int64_t(1) << 31);
+  if (!fits) return;
+  const int nb = m.topo == TXA_HEX8 ? 8 : 4;
+  const int64_t nrows = m.ndofs, nelem = m.nelem, npos = nelem * nb;
+  // row stride in LDS: the longest row; patch size: the largest power of two whose rows fit
+  DevBuf<int> dmax;
+  dmax.alloc(2);
+  TXA_HIP(hipMemsetAsync(dmax.p, 0, 2 * sizeof(int), s));
+  hipLaunchKernelGGL(k_max_row, dim3(setup_grid(nrows)), dim3(kSetupBlock), 0, s, nrows, m.rowptr.p, dmax.p);
+  TXA_HIP(hipGetLastError());
+  int rs = 0;
+  TXA_HIP(hipMemcpy(&rs, dmax.p, sizeof(int), hipMemcpyDeviceToHost));
+  int P = 1024;
+  while (P >= 32 && static_cast<size_t>(P) * (rs + 1) * sizeof(double) > static_cast<size_t>(kPatchLdsBytes)) P /= 2;
+  if (P < 32 || rs > 255) return;
+  // colouring
+  DevBuf<int32_t> col_a, col_b;
+  col_a.alloc(nelem);
+  col_b.alloc(nelem);
+  TXA_HIP(hipMemsetAsync(col_a.p, 0xff, sizeof(int32_t) * nelem, s));
+  TXA_HIP(hipMemsetAsync(col_b.p, 0xff, sizeof(int32_t) * nelem, s));
+  DevBuf<int> cnt;
+  cnt.alloc(2);
+  TXA_HIP(hipMemsetAsync(cnt.p, 0, 2 * sizeof(int), s));
+  int64_t coloured = 0;
+  for (int round = 0; coloured < nelem; ++round) {
+    TXA_REQUIRE(round < 100000, TXA_ERR_LOGIC, "element colouring did not finish");
+    hipLaunchKernelGGL(k_jp_round, dim3(setup_grid(nelem)), dim3(kSetupBlock), 0, s, nelem, nb, m.elem_lid.p,
+                       m.occ_ptr.p, m.occ.p, col_a.p, col_b.p, cnt.p, cnt.p + 1);
+    TXA_HIP(hipGetLastError());
+    TXA_HIP(hipMemcpyAsync(col_a.p, col_b.p, sizeof(int32_t) * nelem, hipMemcpyDeviceToDevice, s));
+    int h[2];
+    TXA_HIP(hipMemcpyAsync(h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    TXA_HIP(hipStreamSynchronize(s));
+    TXA_REQUIRE(h[0] > coloured || coloured == nelem, TXA_ERR_LOGIC, "element colouring made no progress");
+    coloured = h[0];
+    if (h[1] >= kMaxColours) return;   // more colours than the masks hold: no patch path
+  }
+  int nc = 0;
+  TXA_HIP(hipMemcpy(&nc, cnt.p + 1, sizeof(int), hipMemcpyDeviceToHost));
+  nc += 1;
+  col_b.release();
+  // rows along the Morton curve of their vertices
+  m.patch_rec.alloc(4 * nrows);
+  hipLaunchKernelGGL(k_node_xyz, dim3(setup_grid(npos)), dim3(kSetupBlock), 0, s, npos, m.conn.p, m.elem_lid.p,
+                     m.xyz.p, m.nverts, m.patch_rec.p);
+  TXA_HIP(hipGetLastError());
+  Box box;
+  {
+    std::vector<double> h(static_cast<size_t>(3 * m.nverts));
+    TXA_HIP(hipMemcpyAsync(h.data(), m.xyz.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+    TXA_HIP(hipStreamSynchronize(s));
+    for (int d = 0; d < 3; ++d) {
+      const double *b = h.data() + d * m.nverts;
+      const auto mm = std::minmax_element(b, b + m.nverts);
+      box.lo[d] = *mm.first;
+      const double ext = *mm.second - *mm.first;
+      box.scale[d] = ext > 0.0 ? 2097151.0 / ext : 0.0;
+    }
+  }
+  DevBuf<uint64_t> key, key_s;
+  DevBuf<int32_t> row, row_s;
+  key.alloc(nrows); key_s.alloc(nrows); row.alloc(nrows); row_s.alloc(nrows);
+  hipLaunchKernelGGL(k_morton, dim3(setup_grid(nrows)), dim3(kSetupBlock), 0, s, nrows, m.patch_rec.p, box, key.p,
+                     row.p);
+  TXA_HIP(hipGetLastError());
+  size_t tmp_bytes = 0;
+  TXA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, key.p, key_s.p, row.p, row_s.p,
+                                             static_cast<int>(nrows), 0, 63, s));
+  DevBuf<char> tmp;
+  tmp.alloc(tmp_bytes);
+  TXA_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, key.p, key_s.p, row.p, row_s.p,
+                                             static_cast<int>(nrows), 0, 63, s));
+  const int64_t npatch = (nrows + P - 1) / P;
+  m.patch_rows.alloc(npatch * P);
+  hipLaunchKernelGGL(k_place_rows, dim3(setup_grid(npatch * P)), dim3(kSetupBlock), 0, s, nrows, npatch * P,
+                     row_s.p, m.patch_rows.p, m.patch_rec.p);
+  TXA_HIP(hipGetLastError());
+  key.release(); key_s.release(); row.release(); row_s.release(); tmp.release();
+  // instances: (patch, colour, element), sorted
+  DevBuf<uint64_t> ik, ik_s;
+  ik.alloc(npos);
+  ik_s.alloc(npos);
+  DevBuf<unsigned long long> nin;
+  nin.alloc(1);
+  TXA_HIP(hipMemsetAsync(nin.p, 0, sizeof(unsigned long long), s));
+  hipLaunchKernelGGL(k_instance_keys, dim3(setup_grid(nelem)), dim3(kSetupBlock), 0, s, nelem, nb, P, m.elem_lid.p,
+                     m.patch_rec.p, col_a.p, ik.p, nin.p);
+  TXA_HIP(hipGetLastError());
+  tmp_bytes = 0;
+  TXA_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, ik.p, ik_s.p, static_cast<int>(npos), 0, 64, s));
+  tmp.alloc(tmp_bytes);
+  TXA_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tmp_bytes, ik.p, ik_s.p, static_cast<int>(npos), 0, 64, s));
+  unsigned long long ninst = 0;
+  TXA_HIP(hipMemcpyAsync(&ninst, nin.p, sizeof(ninst), hipMemcpyDeviceToHost, s));
+  TXA_HIP(hipStreamSynchronize(s));
+  ik.release();
+  tmp.release();
+  m.patch_inst.alloc(ninst);
+  m.patch_icol.alloc(ninst);
+  DevBuf<unsigned long long> pc;
+  pc.alloc(npatch * nc + 1);
+  TXA_HIP(hipMemsetAsync(pc.p, 0, sizeof(unsigned long long) * (npatch * nc + 1), s));
+  hipLaunchKernelGGL(k_instances, dim3(setup_grid(static_cast<int64_t>(ninst))), dim3(kSetupBlock), 0, s,
+                     static_cast<int64_t>(ninst), nc, ik_s.p, m.patch_inst.p, m.patch_icol.p, pc.p);
+  TXA_HIP(hipGetLastError());
+  m.patch_cstart.alloc(npatch * nc + 1);
+  exclusive_scan_i64(reinterpret_cast<const int64_t *>(pc.p), m.patch_cstart.p, npatch * nc + 1, s);
+  // slot map
+  m.elem_slot.alloc(npos * nb);
+  hipLaunchKernelGGL(k_elem_slots, dim3(setup_grid(npos)), dim3(kSetupBlock), 0, s, nelem, nb, m.elem_lid.p,
+                     m.rowptr.p, m.col.p, m.elem_slot.p);
+  TXA_HIP(hipGetLastError());
+  TXA_HIP(hipStreamSynchronize(s));
+  m.patch_P = P;
+  m.patch_RS = rs;
+  m.patch_NC = nc;
+  m.npatch = npatch;
+  m.patch_ninst = static_cast<int64_t>(ninst);
+}
+
+bool patch_supported(const Mesh &m, int physics, int precision) {
+  return m.npatch > 0 && physics == TXA_DIFFUSION && (precision == TXA_FP64 || precision == TXA_FP32);
+}
+
+void launch_patch(const Mesh &m, const txa_assembly_args &a, hipStream_t s) {
+  PatchParams pp;
+  pp.P = m.patch_P; pp.RS = m.patch_RS; pp.NC = m.patch_NC;
+  pp.cstart = m.patch_cstart.p; pp.inst = m.patch_inst.p; pp.icol = m.patch_icol.p;
+  pp.prow = m.patch_rows.p;
+  pp.rec = reinterpret_cast<const double4 *>(m.patch_rec.p);
+  pp.elem_lid = m.elem_lid.p; pp.slot = m.elem_slot.p; pp.rowptr = m.rowptr.p;
+  pp.coef = a.coef; pp.x = a.x; pp.xdot = a.xdot;
+  pp.alpha = a.alpha; pp.beta = a.beta; pp.rhoc = a.rho; pp.src = a.src[0];
+  pp.f = a.f; pp.vals = a.vals; pp.add = a.accumulate != 0;
+  const bool tr = a.transient != 0, unit = !tr && a.beta == 1.0;
+  if (m.topo == TXA_HEX8) {
+    if (a.precision == TXA_FP32) launch_topo<TXA_HEX8, float>(m, pp, tr, unit, s);
+    else launch_topo<TXA_HEX8, double>(m, pp, tr, unit, s);
+  } else {
+    if (a.precision == TXA_FP32) launch_topo<TXA_TET4, float>(m, pp, tr, unit, s);
+    else launch_topo<TXA_TET4, double>(m, pp, tr, unit, s);
+  }
+}
+
+}  // namespace txa

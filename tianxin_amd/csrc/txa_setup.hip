@@ -602,6 +602,7 @@ void graph_build(Mesh &m, hipStream_t s) {
     TXA_HIP(hipGetLastError());
     TXA_HIP(hipStreamSynchronize(s));   // ocur is freed at scope end
   }
+  patch_build(m, s);   // node patches of single-block hex8 / tet4 meshes
   TXA_HIP(hipStreamSynchronize(s));
   m.has_graph = true;
 }
