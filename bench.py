@@ -675,9 +675,9 @@ Q2_GATHER = {"split_bf16": "k_elastic_gather<27, float>", "fp32": "k_elastic_gat
 
 def bench_upload(args):
     """Uploaded-mesh path (txa_mesh_upload, the connectivity a ConnManager hands over):
-    an n^3 hex8 brick (default 128) uploaded as unstructured connectivity, FP64, steady
-    Poisson (k lognormal sigma 0.5 per element, f = 1, u ~ U[-1, 1], u = 0 on x = 0).
-    Times the cells in brick order and in a seeded random order.  Checks: in brick order
+    an n^3 hex8 brick (default 128) uploaded as unstructured connectivity, FP64 (and the
+    FP32 form beside it), steady Poisson (k lognormal sigma 0.5 per element, f = 1,
+    u ~ U[-1, 1], u = 0 on x = 0).  Times the cells in brick order and in a seeded random order.  Checks: in brick order
     the upload numbers LIDs as the lattice mesh does, so f and J must equal the lattice
     FP64 path's to rounding; Poisson rows off the Dirichlet set sum to zero."""
     import numpy as np
@@ -709,26 +709,41 @@ def bench_upload(args):
                                  device="cuda")
         else:
             kk = k
-        log(f"{name}: setup {time.time() - t0:.1f}s, {m.nelem} elements, {nd} dofs, {nnz} nnz")
+        log(f"{name}: setup {time.time() - t0:.1f}s, {m.nelem} elements, {nd} dofs, {nnz} nnz, patches {m.patches()}")
+
+        def timed(precision, f, v):
+            def step():
+                m.assemble(physics=txa.DIFFUSION, precision=precision, coef=kk, x=x, f=f, vals=v, src=(1.0,),
+                           bc_lids=bc)
+            for _ in range(args.warmup):
+                step()
+            torch.cuda.synchronize()
+            ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+            ev[0].record()
+            for i in range(args.steps):
+                step()
+                ev[i + 1].record()
+            torch.cuda.synchronize()
+            return np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)])
+
         f = torch.empty(nd, device="cuda", dtype=torch.float64)
         v = torch.empty(nnz, device="cuda", dtype=torch.float64)
-
-        def step():
-            m.assemble(physics=txa.DIFFUSION, precision=txa.FP64, coef=kk, x=x, f=f, vals=v, src=(1.0,), bc_lids=bc)
-        for _ in range(args.warmup):
-            step()
-        torch.cuda.synchronize()
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
-        ev[0].record()
-        for i in range(args.steps):
-            step()
-            ev[i + 1].record()
-        torch.cuda.synchronize()
-        per = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)])
+        per = timed(txa.FP64, f, v)
         ms = float(per.mean())
         out[name] = {"ms_per_step": ms, "elements_per_s": m.nelem / (ms / 1e3),
-                     "p50": float(np.percentile(per, 50)), "p99": float(np.percentile(per, 99))}
+                     "p50": float(np.percentile(per, 50)), "p99": float(np.percentile(per, 99)),
+                     "patches": m.patches()}
         log(f"{name}: {ms:.3f} ms/step")
+        # the FP32 form (FP32 element matrices, FP64 geometry and residual) against this FP64 result
+        f32 = torch.empty_like(f)
+        v32 = torch.empty_like(v)
+        per32 = timed(txa.FP32, f32, v32)
+        band = f.abs() > 1e-12 * f.abs().max()
+        out[name]["fp32"] = {"ms_per_step": float(per32.mean()),
+                             "R_entrywise_over_1e-12_max": float(((f32 - f).abs()[band] / f.abs()[band]).max()),
+                             "J_normwise": float((v32 - v).abs().max() / v.abs().max())}
+        log(f"{name}: FP32 form {float(per32.mean()):.3f} ms/step")
+        del f32, v32
         # Poisson rows off the Dirichlet set sum to zero
         rp, _ = m.graph(with_cols=False)
         rows = torch.repeat_interleave(torch.arange(nd, device="cuda"), torch.as_tensor(np.diff(rp), device="cuda"))

@@ -32,6 +32,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cmath>
 
 namespace txa {
 namespace {
@@ -43,6 +44,10 @@ constexpr int kPatchLdsBytes = 65536;   // row accumulators of one patch
 constexpr int kSetupBlock = 256;
 
 inline unsigned setup_grid(int64_t n) { return static_cast<unsigned>((n + kSetupBlock - 1) / kSetupBlock); }
+
+// LDS of one patch: [P][RS] row entries and [P] residual entries (double), then each row's
+// first CRS position (int64), LID and length (int32)
+inline size_t patch_lds_bytes(int P, int RS) { return static_cast<size_t>(P) * ((RS + 1) * sizeof(double) + 16); }
 
 // ---------------------------------------------------------------------------------------
 // setup
@@ -80,7 +85,7 @@ __global__ void k_morton(int64_t nrows, const double *__restrict__ rec, Box b, u
   uint64_t q[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    double t = (rec[4 * r + d] - b.lo[d]) * b.scale[d];
+    double t = (rec[4 * r + d] - b.lo[d]) * b.scale[d] + 0.5;
     t = t < 0.0 ? 0.0 : (t > 2097151.0 ? 2097151.0 : t);
     q[d] = static_cast<uint64_t>(t);
   }
@@ -293,28 +298,46 @@ __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
   extern __shared__ double sacc[];   // [P][RS] row entries, then [P] residual entries
   const int P = p.P, RS = p.RS, NC = p.NC;
   double *sres = sacc + P * RS;
+  int64_t *srow0 = reinterpret_cast<int64_t *>(sres + P);   // first CRS position of each row
+  int32_t *slid = reinterpret_cast<int32_t *>(srow0 + P);   // LID of each row (-1: none)
+  int32_t *slen = slid + P;                                  // length of each row
   const int tid = threadIdx.x;
   const int64_t pb = blockIdx.x;
-  for (int i = tid; i < P * (RS + 1); i += kPatchThreads) sacc[i] = 0.0;
   const int64_t i_begin = p.cstart[pb * NC], i_end = p.cstart[(pb + 1) * NC];
   const int64_t pos0 = pb * P;
+  // the first chunk's elements and LIDs, then the rows' CRS ranges for the final stores
+  int64_t e_cur = i_begin + tid < i_end ? p.inst[i_begin + tid] : -1;
+  int32_t Lc[NB];
+#pragma unroll
+  for (int v = 0; v < NB; ++v) Lc[v] = e_cur >= 0 ? p.elem_lid[e_cur * NB + v] : 0;
+  for (int r = tid; r < P; r += kPatchThreads) {
+    const int32_t L = p.prow[pos0 + r];
+    slid[r] = L;
+    if (L >= 0) {
+      const int64_t r0 = p.rowptr[L];
+      srow0[r] = r0;
+      slen[r] = static_cast<int>(p.rowptr[L + 1] - r0);
+    }
+  }
+  for (int i = tid; i < P * (RS + 1); i += kPatchThreads) sacc[i] = 0.0;
   __syncthreads();
   for (int64_t c0 = i_begin; c0 < i_end; c0 += kPatchThreads) {
     const int64_t i = c0 + tid;
     const bool act = i < i_end;
     const int64_t ilast = (c0 + kPatchThreads < i_end ? c0 + kPatchThreads : i_end) - 1;
     const int cfirst = p.icol[c0], clast = p.icol[ilast];
+    const int64_t e_nx = i + kPatchThreads < i_end ? p.inst[i + kPatchThreads] : -1;   // the next chunk's element
     int mycol = -1;
     int lrow[NB];
     uint32_t w[NB * NB / 4];   // slot map of the element, four slots a word
     Real Ke[NB * (NB + 1) / 2];
     double Re[NB];
     if (act) {
-      const int64_t e = p.inst[i];
+      const int64_t e = e_cur;
       mycol = p.icol[i];
       int32_t L[NB];
 #pragma unroll
-      for (int v = 0; v < NB; ++v) L[v] = p.elem_lid[e * NB + v];
+      for (int v = 0; v < NB; ++v) L[v] = Lc[v];
       double X[3][NB], u[NB], ud[NB];
 #pragma unroll
       for (int v = 0; v < NB; ++v) {
@@ -357,6 +380,10 @@ __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
         tet4_element<Real, TRANS, UNIT>(X, u, ud, k, p.rhoc, p.src, p.alpha, p.beta, Ke, Re);
       }
     }
+    // the next chunk's LIDs load while this chunk adds its rows
+    e_cur = e_nx;
+#pragma unroll
+    for (int v = 0; v < NB; ++v) Lc[v] = e_nx >= 0 ? p.elem_lid[e_nx * NB + v] : 0;
     // colour by colour: within one colour no two instances share a vertex
     for (int c = cfirst; c <= clast; ++c) {
       __syncthreads();
@@ -380,27 +407,32 @@ __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
     }
     __syncthreads();
   }
-  // store the patch's rows whole: 32 lanes per row
+  // store the patch's rows whole: 32 lanes per row (values written once, not re-read here:
+  // non-temporal stores)
   const int lane = tid & 31;
-  for (int r = tid >> 5; r < P; r += kPatchThreads / 32) {
-    const int32_t L = p.prow[pos0 + r];
-    if (L < 0) continue;
-    if (p.vals) {
-      const int64_t r0 = p.rowptr[L];
-      const int n = static_cast<int>(p.rowptr[L + 1] - r0);
-      for (int s = lane; s < n; s += 32) {
-        const double v = sacc[r * RS + s];
-        p.vals[r0 + s] = p.add ? p.vals[r0 + s] + v : v;
-      }
+  if (p.add) {
+    for (int r = tid >> 5; r < P; r += kPatchThreads / 32) {
+      const int32_t L = slid[r];
+      if (L < 0) continue;
+      if (p.vals)
+        for (int s = lane; s < slen[r]; s += 32) p.vals[srow0[r] + s] += sacc[r * RS + s];
+      if (p.f && lane == 0) p.f[L] += sres[r];
     }
-    if (p.f && lane == 0) p.f[L] = p.add ? p.f[L] + sres[r] : sres[r];
+    return;
+  }
+  for (int r = tid >> 5; r < P; r += kPatchThreads / 32) {
+    const int32_t L = slid[r];
+    if (L < 0) continue;
+    if (p.vals)
+      for (int s = lane; s < slen[r]; s += 32) __builtin_nontemporal_store(sacc[r * RS + s], p.vals + srow0[r] + s);
+    if (p.f && lane == 0) __builtin_nontemporal_store(sres[r], p.f + L);
   }
 }
 
 template <int TOPO, typename Real>
 void launch_topo(const Mesh &m, const PatchParams &pp, bool tr, bool unit, hipStream_t s) {
   const dim3 grid(static_cast<unsigned>(m.npatch)), block(kPatchThreads);
-  const size_t lds = sizeof(double) * static_cast<size_t>(pp.P) * (pp.RS + 1);
+  const size_t lds = patch_lds_bytes(pp.P, pp.RS);
   if (tr) hipLaunchKernelGGL((k_patch<TOPO, Real, true, false>), grid, block, lds, s, pp);
   else if (unit) hipLaunchKernelGGL((k_patch<TOPO, Real, false, true>), grid, block, lds, s, pp);
   else hipLaunchKernelGGL((k_patch<TOPO, Real, false, false>), grid, block, lds, s, pp);
@@ -426,7 +458,7 @@ void patch_build(Mesh &m, hipStream_t s) {
   int rs = 0;
   TXA_HIP(hipMemcpy(&rs, dmax.p, sizeof(int), hipMemcpyDeviceToHost));
   int P = 1024;
-  while (P >= 32 && static_cast<size_t>(P) * (rs + 1) * sizeof(double) > static_cast<size_t>(kPatchLdsBytes)) P /= 2;
+  while (P >= 32 && patch_lds_bytes(P, rs) > static_cast<size_t>(kPatchLdsBytes)) P /= 2;
   if (P < 32 || rs > 255) return;
   // colouring
   DevBuf<int32_t> col_a, col_b;
@@ -460,18 +492,24 @@ void patch_build(Mesh &m, hipStream_t s) {
   hipLaunchKernelGGL(k_node_xyz, dim3(setup_grid(npos)), dim3(kSetupBlock), 0, s, npos, m.conn.p, m.elem_lid.p,
                      m.xyz.p, m.nverts, m.patch_rec.p);
   TXA_HIP(hipGetLastError());
+  // integer lattice of about one vertex per unit cell (the mean spacing h of the rows over the
+  // bounding box), so that aligned Morton blocks of 2^k points are compact blocks of vertices
   Box box;
   {
     std::vector<double> h(static_cast<size_t>(3 * m.nverts));
     TXA_HIP(hipMemcpyAsync(h.data(), m.xyz.p, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
     TXA_HIP(hipStreamSynchronize(s));
+    double ext[3], vol = 1.0;
+    int dims = 0;
     for (int d = 0; d < 3; ++d) {
       const double *b = h.data() + d * m.nverts;
       const auto mm = std::minmax_element(b, b + m.nverts);
       box.lo[d] = *mm.first;
-      const double ext = *mm.second - *mm.first;
-      box.scale[d] = ext > 0.0 ? 2097151.0 / ext : 0.0;
+      ext[d] = *mm.second - *mm.first;
+      if (ext[d] > 0.0) { vol *= ext[d]; ++dims; }
     }
+    const double hs = dims ? std::pow(vol / static_cast<double>(nrows), 1.0 / dims) : 1.0;
+    for (int d = 0; d < 3; ++d) box.scale[d] = ext[d] > 0.0 ? std::min(1.0 / hs, 2097150.0 / ext[d]) : 0.0;
   }
   DevBuf<uint64_t> key, key_s;
   DevBuf<int32_t> row, row_s;
