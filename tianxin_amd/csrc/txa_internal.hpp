@@ -112,7 +112,8 @@ struct Mesh {
   DevBuf<uint8_t> patch_icol;     // colour of each instance
   DevBuf<int32_t> patch_rows;     // [npatch*P] row at each curve position, -1 past the last row
   DevBuf<double> patch_rec;       // [rows][4] vertex x, y, z and the row's curve position (int64 bits)
-  DevBuf<uint8_t> elem_slot;      // [nelem][nb][nb] position of LID b in the CRS row of LID a
+  DevBuf<int32_t> patch_ilid;     // [instances][nb] LIDs of each instance's element
+  DevBuf<uint8_t> patch_islot;    // [instances][nb][nb] position of LID b in the CRS row of LID a
   DevBuf<double> ref;         // reference-element tables of the topology (single-topology meshes)
   DevBuf<double> block_ref[5];   // reference-element tables by topology (TXA_HEX8 .. TXA_WEDGE6)
   bool mixed() const { return blocks.size() > 1 && topo == 0; }

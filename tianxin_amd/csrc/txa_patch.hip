@@ -15,8 +15,9 @@
 //   colour, then element;
 // - a colouring of the elements (Jones-Plassmann with hashed priorities): elements that
 //   share a vertex differ in colour;
-// - the slot map: for each element, row a and column b, the position of LID b in LID a's
-//   CRS row.
+// - instance records in instance order: the element's LIDs and its slot map (for row a and
+//   column b, the position of LID b in LID a's CRS row), so that a patch streams its
+//   instances contiguously whatever order the elements were uploaded in.
 // One workgroup assembles one patch.  Each lane integrates one instance in registers
 // (the sum-factorised hex8 routine of the lattice kernels, or the constant-gradient tet),
 // then adds the element rows of the patch's own vertices into the patch's rows in LDS.
@@ -175,14 +176,19 @@ __global__ void k_instances(int64_t ninst, int nc, const uint64_t *__restrict__ 
   atomicAdd(cnt + p * nc + c, 1ull);
 }
 
-// Slot map: position of LID b in the CRS row of LID a, for every element (uint8; 255: absent).
-__global__ void k_elem_slots(int64_t nelem, int nb, const int32_t *__restrict__ elem_lid,
-                             const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
-                             uint8_t *__restrict__ slot) {
+// Instance records: the LIDs of each instance's element and its slot map, the position of
+// LID b in the CRS row of LID a (uint8; 255: absent), stored in instance order so that a
+// patch reads its instances as one contiguous stream whatever the element order.
+__global__ void k_inst_records(int64_t ninst, int nb, const int32_t *__restrict__ inst,
+                               const int32_t *__restrict__ elem_lid, const int64_t *__restrict__ rowptr,
+                               const int32_t *__restrict__ col, int32_t *__restrict__ ilid,
+                               uint8_t *__restrict__ slot) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= nelem * nb) return;
-  const int64_t e = t / nb;
-  const int32_t ra = elem_lid[t];
+  if (t >= ninst * nb) return;
+  const int64_t i = t / nb, e = inst[i];
+  const int a = static_cast<int>(t - i * nb);
+  const int32_t ra = elem_lid[e * nb + a];
+  ilid[t] = ra;
   const int64_t r0 = rowptr[ra], n = rowptr[ra + 1] - r0;
   for (int b = 0; b < nb; ++b) {
     const int32_t cb = elem_lid[e * nb + b];
@@ -211,13 +217,14 @@ struct PatchParams {
   const uint8_t *icol;         // colour of each instance
   const int32_t *prow;         // [npatch*P] row at each curve position, -1 past the last
   const double4 *rec;          // [rows] vertex x, y, z; curve position (int64 bits) in w
-  const int32_t *elem_lid;     // [nelem][NB]
-  const uint8_t *slot;         // [nelem][NB][NB]
+  const int32_t *ilid;         // [instances][NB] LIDs of each instance's element
+  const uint8_t *islot;        // [instances][NB][NB] slot of LID b in the row of LID a
   const int64_t *rowptr;
   const double *coef, *x, *xdot;
   double alpha, beta, rhoc, src;
   double *f, *vals;
   int add;
+  int64_t npatch;
 };
 
 // shards hex order -> binary order v = bx + 2by + 4bz (an involution)
@@ -292,6 +299,54 @@ __device__ __forceinline__ constexpr int sym4(int a, int b) {
   return a <= b ? a * 4 - (a * (a - 1)) / 2 + (b - a) : b * 4 - (b * (b - 1)) / 2 + (a - b);
 }
 
+// One lane's instance of a chunk: geometry, state and coefficient, the rows' curve positions
+// and the slot map, gathered through the instance's LIDs (which are loaded a chunk ahead).
+template <int NB, bool TRANS>
+struct InstIn {
+  double X[3][NB], u[NB], ud[TRANS ? NB : 1], k;
+  int32_t pos[NB];
+  uint32_t w[NB * NB / 4];   // slot map, four slots a word
+};
+
+template <int NB>
+__device__ __forceinline__ void load_lids(const PatchParams &p, int64_t i, int64_t i_end, int32_t (&L)[NB]) {
+#pragma unroll
+  for (int v = 0; v < NB; ++v) L[v] = 0;
+  if (i >= i_end) return;
+  const int4 *l4 = reinterpret_cast<const int4 *>(p.ilid + i * NB);
+#pragma unroll
+  for (int q = 0; q < NB / 4; ++q) {
+    const int4 t = l4[q];
+    L[4 * q] = t.x; L[4 * q + 1] = t.y; L[4 * q + 2] = t.z; L[4 * q + 3] = t.w;
+  }
+}
+
+template <int NB, bool TRANS>
+__device__ __forceinline__ void load_inst(const PatchParams &p, int64_t i, const int32_t (&L)[NB],
+                                          InstIn<NB, TRANS> &in) {
+#pragma unroll
+  for (int v = 0; v < NB; ++v) {
+    const double4 r = p.rec[L[v]];
+    in.X[0][v] = r.x; in.X[1][v] = r.y; in.X[2][v] = r.z;
+    in.pos[v] = static_cast<int32_t>(__double_as_longlong(r.w));
+    in.u[v] = p.x[L[v]];
+    if constexpr (TRANS) in.ud[v] = p.xdot[L[v]];
+  }
+  in.k = p.coef[p.inst[i]];
+  if (p.vals) {
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(p.islot + i * NB * NB);
+#pragma unroll
+    for (int q = 0; q < NB * NB / 16; ++q) {
+      const uint4 t = s4[q];
+      in.w[4 * q] = t.x; in.w[4 * q + 1] = t.y; in.w[4 * q + 2] = t.z; in.w[4 * q + 3] = t.w;
+    }
+  }
+}
+
+constexpr int kMaxRowsPerThread = 4;   // patches of at most 4 x 256 rows
+
+// One workgroup per patch, its instances in chunks of one per lane.  A chunk's LIDs load
+// while the previous chunk adds its rows.
 template <int TOPO, typename Real, bool TRANS, bool UNIT>
 __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
   constexpr int NB = TOPO == TXA_HEX8 ? 8 : 4;
@@ -305,11 +360,8 @@ __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
   const int64_t pb = blockIdx.x;
   const int64_t i_begin = p.cstart[pb * NC], i_end = p.cstart[(pb + 1) * NC];
   const int64_t pos0 = pb * P;
-  // the first chunk's elements and LIDs, then the rows' CRS ranges for the final stores
-  int64_t e_cur = i_begin + tid < i_end ? p.inst[i_begin + tid] : -1;
   int32_t Lc[NB];
-#pragma unroll
-  for (int v = 0; v < NB; ++v) Lc[v] = e_cur >= 0 ? p.elem_lid[e_cur * NB + v] : 0;
+  load_lids<NB>(p, i_begin + tid, i_end, Lc);
   for (int r = tid; r < P; r += kPatchThreads) {
     const int32_t L = p.prow[pos0 + r];
     slid[r] = L;
@@ -322,53 +374,34 @@ __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
   for (int i = tid; i < P * (RS + 1); i += kPatchThreads) sacc[i] = 0.0;
   __syncthreads();
   for (int64_t c0 = i_begin; c0 < i_end; c0 += kPatchThreads) {
-    const int64_t i = c0 + tid;
-    const bool act = i < i_end;
     const int64_t ilast = (c0 + kPatchThreads < i_end ? c0 + kPatchThreads : i_end) - 1;
     const int cfirst = p.icol[c0], clast = p.icol[ilast];
-    const int64_t e_nx = i + kPatchThreads < i_end ? p.inst[i + kPatchThreads] : -1;   // the next chunk's element
-    int mycol = -1;
+    const int64_t i = c0 + tid;
+    const int mycol = i < i_end ? p.icol[i] : -1;
     int lrow[NB];
-    uint32_t w[NB * NB / 4];   // slot map of the element, four slots a word
+    uint32_t w[NB * NB / 4];
     Real Ke[NB * (NB + 1) / 2];
     double Re[NB];
-    if (act) {
-      const int64_t e = e_cur;
-      mycol = p.icol[i];
-      int32_t L[NB];
-#pragma unroll
-      for (int v = 0; v < NB; ++v) L[v] = Lc[v];
-      double X[3][NB], u[NB], ud[NB];
+    if (mycol >= 0) {
+      InstIn<NB, TRANS> nx;
+      load_inst(p, i, Lc, nx);
 #pragma unroll
       for (int v = 0; v < NB; ++v) {
-        const double4 r = p.rec[L[v]];
-        X[0][v] = r.x; X[1][v] = r.y; X[2][v] = r.z;
-        const int64_t lp = __double_as_longlong(r.w) - pos0;
-        lrow[v] = lp >= 0 && lp < P ? static_cast<int>(lp) : -1;
-        u[v] = p.x[L[v]];
-        ud[v] = TRANS ? p.xdot[L[v]] : 0.0;
+        const int lp = nx.pos[v] - static_cast<int>(pos0);
+        lrow[v] = lp >= 0 && lp < P ? lp : -1;
       }
-      // the slot map is loaded once the element's geometry registers are free
-      auto load_slots = [&] {
-        if (!p.vals) return;
-        const uint4 *s4 = reinterpret_cast<const uint4 *>(p.slot + e * NB * NB);
 #pragma unroll
-        for (int q = 0; q < NB * NB / 16; ++q) {
-          const uint4 t = s4[q];
-          w[4 * q] = t.x; w[4 * q + 1] = t.y; w[4 * q + 2] = t.z; w[4 * q + 3] = t.w;
-        }
-      };
-      const double k = p.coef[e];
+      for (int q = 0; q < NB * NB / 4; ++q) w[q] = nx.w[q];
       if constexpr (TOPO == TXA_HEX8) {
         double Xb[3][8], ub[8], udb[8];
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
-          Xb[0][hex_bin(s)] = X[0][s]; Xb[1][hex_bin(s)] = X[1][s]; Xb[2][hex_bin(s)] = X[2][s];
-          ub[hex_bin(s)] = u[s];
-          udb[hex_bin(s)] = ud[s];
+          Xb[0][hex_bin(s)] = nx.X[0][s]; Xb[1][hex_bin(s)] = nx.X[1][s]; Xb[2][hex_bin(s)] = nx.X[2][s];
+          ub[hex_bin(s)] = nx.u[s];
+          udb[hex_bin(s)] = TRANS ? nx.ud[s] : 0.0;
         }
         Hex8Out<Real, TRANS> o;
-        hex8_element<Real, TRANS, UNIT, false>(Xb, ub, udb, k, p.rhoc, p.src, p.alpha, p.beta, o, load_slots);
+        hex8_element<Real, TRANS, UNIT, false>(Xb, ub, udb, nx.k, p.rhoc, p.src, p.alpha, p.beta, o, [] {});
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
           Re[a] = o.R[hex_bin(a)];
@@ -376,14 +409,14 @@ __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
           for (int b = a; b < 8; ++b) Ke[sym8(a, b)] = o.K[sym8(hex_bin(a), hex_bin(b))];
         }
       } else {
-        load_slots();
-        tet4_element<Real, TRANS, UNIT>(X, u, ud, k, p.rhoc, p.src, p.alpha, p.beta, Ke, Re);
+        double ud[NB];
+#pragma unroll
+        for (int v = 0; v < NB; ++v) ud[v] = TRANS ? nx.ud[v] : 0.0;
+        tet4_element<Real, TRANS, UNIT>(nx.X, nx.u, ud, nx.k, p.rhoc, p.src, p.alpha, p.beta, Ke, Re);
       }
     }
-    // the next chunk's LIDs load while this chunk adds its rows
-    e_cur = e_nx;
-#pragma unroll
-    for (int v = 0; v < NB; ++v) Lc[v] = e_nx >= 0 ? p.elem_lid[e_nx * NB + v] : 0;
+    // the next chunk's LIDs
+    load_lids<NB>(p, i + kPatchThreads, i_end, Lc);
     // colour by colour: within one colour no two instances share a vertex
     for (int c = cfirst; c <= clast; ++c) {
       __syncthreads();
@@ -457,7 +490,7 @@ void patch_build(Mesh &m, hipStream_t s) {
   TXA_HIP(hipGetLastError());
   int rs = 0;
   TXA_HIP(hipMemcpy(&rs, dmax.p, sizeof(int), hipMemcpyDeviceToHost));
-  int P = 1024;
+  int P = kMaxRowsPerThread * kPatchThreads;
   while (P >= 32 && patch_lds_bytes(P, rs) > static_cast<size_t>(kPatchLdsBytes)) P /= 2;
   if (P < 32 || rs > 255) return;
   // colouring
@@ -559,10 +592,12 @@ void patch_build(Mesh &m, hipStream_t s) {
   TXA_HIP(hipGetLastError());
   m.patch_cstart.alloc(npatch * nc + 1);
   exclusive_scan_i64(reinterpret_cast<const int64_t *>(pc.p), m.patch_cstart.p, npatch * nc + 1, s);
-  // slot map
-  m.elem_slot.alloc(npos * nb);
-  hipLaunchKernelGGL(k_elem_slots, dim3(setup_grid(npos)), dim3(kSetupBlock), 0, s, nelem, nb, m.elem_lid.p,
-                     m.rowptr.p, m.col.p, m.elem_slot.p);
+  // instance records
+  m.patch_ilid.alloc(ninst * nb);
+  m.patch_islot.alloc(ninst * nb * nb);
+  hipLaunchKernelGGL(k_inst_records, dim3(setup_grid(static_cast<int64_t>(ninst) * nb)), dim3(kSetupBlock), 0, s,
+                     static_cast<int64_t>(ninst), nb, m.patch_inst.p, m.elem_lid.p, m.rowptr.p, m.col.p,
+                     m.patch_ilid.p, m.patch_islot.p);
   TXA_HIP(hipGetLastError());
   TXA_HIP(hipStreamSynchronize(s));
   m.patch_P = P;
@@ -582,10 +617,11 @@ void launch_patch(const Mesh &m, const txa_assembly_args &a, hipStream_t s) {
   pp.cstart = m.patch_cstart.p; pp.inst = m.patch_inst.p; pp.icol = m.patch_icol.p;
   pp.prow = m.patch_rows.p;
   pp.rec = reinterpret_cast<const double4 *>(m.patch_rec.p);
-  pp.elem_lid = m.elem_lid.p; pp.slot = m.elem_slot.p; pp.rowptr = m.rowptr.p;
+  pp.ilid = m.patch_ilid.p; pp.islot = m.patch_islot.p; pp.rowptr = m.rowptr.p;
   pp.coef = a.coef; pp.x = a.x; pp.xdot = a.xdot;
   pp.alpha = a.alpha; pp.beta = a.beta; pp.rhoc = a.rho; pp.src = a.src[0];
   pp.f = a.f; pp.vals = a.vals; pp.add = a.accumulate != 0;
+  pp.npatch = m.npatch;
   const bool tr = a.transient != 0, unit = !tr && a.beta == 1.0;
   if (m.topo == TXA_HEX8) {
     if (a.precision == TXA_FP32) launch_topo<TXA_HEX8, float>(m, pp, tr, unit, s);
