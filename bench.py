@@ -966,6 +966,101 @@ def summarize_elastic(stats_dir, pmc_dir, bench_json):
     print(out)
 
 
+def summarize_upload(stats_dir, fetch_dir, write_dir, util_dir, bench_json):
+    """Writes profiles/upload_hex8_128_kernel_stats.md from a kernel-trace pass, the FETCH_SIZE,
+    WRITE_SIZE and issue-counter passes of `bench.py --workload upload`, and the line it printed."""
+    import numpy as np
+    line = json.loads(open(bench_json).read().strip().splitlines()[-1])
+    trace = sorted(_rows(stats_dir, "kernel_trace.csv"), key=lambda r: int(r["Start_Timestamp"]))
+    n = int(round(line["config"]["elements"] ** (1 / 3)))
+    rows_, nnz = (n + 1) ** 3, (3 * n + 1) ** 3   # a vertex row couples 2 or 3 vertices along each axis
+    ne = n ** 3
+    alg = 8 * nnz + 8 * rows_ + 8 * rows_ + 24 * rows_ + 8 * ne + 32 * ne + 8 * rows_
+    util = ("SQ_WAVES SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_WAIT_ANY SQ_WAIT_INST_ANY SQ_ACTIVE_INST_ANY "
+            "SQ_ACTIVE_INST_VALU SQ_INSTS_VALU GRBM_GUI_ACTIVE")
+    cmd = "    rocprofv3 --pmc {} --output-format csv -d <out>/{} -o up -- python3 bench.py --workload upload " \
+          "--steps 2 --warmup 0 --no-cpu --no-extra"
+    o = line["orders"]
+    pt = o["brick_order"]["patches"]
+    lines = [f"# Kernel statistics: uploaded hex8 Poisson brick {n}^3, node patches (bench.py --workload upload)", "",
+             "Commands (from the repository root, on one MI355X):", "",
+             "    python3 bench.py --workload upload --steps 20 --warmup 3",
+             "    rocprofv3 --kernel-trace --stats --output-format csv -d <out>/prof_up -o up -- python3 bench.py "
+             "--workload upload --steps 5 --warmup 1 --no-cpu --no-extra",
+             cmd.format("FETCH_SIZE", "pmc_f"), cmd.format("WRITE_SIZE", "pmc_w"), cmd.format(util, "pmc_u"), "",
+             f"Workload: {line['config']['workload']} ({ne} elements, {rows_} rows, {nnz} nnz), steady Poisson,",
+             "k lognormal sigma 0.5 per element, u = 0 on x = 0; the cells in brick order, then in a seeded",
+             "random order (the LIDs, numbered by first touch, follow the element order).  The assembly is",
+             f"one launch of `k_patch` (one workgroup per patch of {pt['rows_per_patch']} rows along the Morton "
+             "curve) and the", "Dirichlet rows.", "",
+             f"Patches: {pt['patches']}, element instances {pt['instances']} ({pt['instances'] / ne:.2f} per element: "
+             f"an element is integrated once per patch it touches), {pt['colours']} colours, rows of at most "
+             f"{pt['longest_row']} entries.", "",
+             "| order | form | ms per step (HIP events) | p50 / p99 ms | elements/s |", "|---|---|---:|---:|---:|"]
+    for name in ("brick_order", "shuffled"):
+        c = o[name]
+        lines.append(f"| {name.replace('_', ' ')} | FP64 | {c['ms_per_step']:.3f} | {c['p50']:.3f} / {c['p99']:.3f} | "
+                     f"{c['elements_per_s']:.3g} |")
+        lines.append(f"| {name.replace('_', ' ')} | FP32 | {c['fp32']['ms_per_step']:.3f} | | "
+                     f"{ne / c['fp32']['ms_per_step'] * 1e3:.3g} |")
+    lines += ["", "Per-dispatch durations from the kernel trace (us); each order runs the FP64 form (1 + 5",
+              "dispatches), then the FP32 form (1 + 5):", "",
+              "| kernel | order | calls | mean | min | max |", "|---|---|---:|---:|---:|---:|"]
+    for k in ("k_patch<1, double, false, true>", "k_patch<1, float, false, true>"):
+        d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in trace if _match(r["Kernel_Name"], k)]
+        h = len(d) // 2
+        for name, part in (("brick order", d[:h]), ("shuffled", d[h:])):
+            if part:
+                lines.append(f"| `{k}` | {name} | {len(part)} | {np.mean(part):.1f} | {np.min(part):.1f} | "
+                             f"{np.max(part):.1f} |")
+    kern = "k_patch<1, double, false, true>"
+
+    def per_dispatch(d, counter):
+        tot = {}
+        for r in _rows(d, "counter_collection.csv"):
+            if r["Counter_Name"] == counter and _match(r["Kernel_Name"], kern):
+                tot[int(r["Dispatch_Id"])] = tot.get(int(r["Dispatch_Id"]), 0.0) + float(r["Counter_Value"])
+        return [tot[k] for k in sorted(tot)]
+    f_, w_ = per_dispatch(fetch_dir, "FETCH_SIZE"), per_dispatch(write_dir, "WRITE_SIZE")
+    brick_us = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in trace if _match(r["Kernel_Name"], kern)]
+    brick_us = float(np.mean(brick_us[1:len(brick_us) // 2])) if len(brick_us) >= 4 else float("nan")
+    if f_ and w_:
+        h = len(f_) // 2
+        rd = [2.0 * v * 1024 / 1e9 for v in f_]   # KiB, times the copy-kernel calibration factors
+        wr = [1.0 * v * 1024 / 1e9 for v in w_]
+        lines += ["", f"## HBM traffic of `{kern}` per dispatch", "",
+                  "FETCH_SIZE and WRITE_SIZE (KiB) scaled by the factors a copy kernel of known size gives on",
+                  "this GPU (fetch 2.000, write 1.000; `q1_hex8_diffusion_fp32_256_kernel_stats.md`):", "",
+                  "| order | HBM read GB | HBM written GB |", "|---|---:|---:|",
+                  f"| brick order | {np.mean(rd[:h]):.3f} | {np.mean(wr[:h]):.3f} |",
+                  f"| shuffled | {np.mean(rd[h:]):.3f} | {np.mean(wr[h:]):.3f} |", "",
+                  f"Algorithmic bytes per launch: {alg / 1e9:.3f} GB (values 8 B x nnz, f, x, vertex coordinates",
+                  "24 B, rowptr per row; coefficient and 8 LIDs per element).  At the brick-order kernel mean of",
+                  f"{brick_us:.0f} us (warm dispatches) that is {alg / (brick_us * 1e-6) / 1e12:.2f} TB/s, "
+                  f"{alg / (brick_us * 1e-6) / 8e12:.3f} of 8 TB/s; the actual traffic "
+                  f"({np.mean(rd[:h]) + np.mean(wr[:h]):.2f} GB) is about "
+                  f"{(np.mean(rd[:h]) + np.mean(wr[:h])) / (alg / 1e9):.1f}x the algorithmic bytes (the instance "
+                  "records, 96 B per", "instance, and the row position of each vertex record)."]
+    u = {c: per_dispatch(util_dir, c) for c in ("SQ_INSTS_VALU", "GRBM_GUI_ACTIVE", "SQ_WAVE_CYCLES", "SQ_WAIT_ANY",
+                                               "SQ_ACTIVE_INST_ANY", "SQ_WAVES")}
+    if all(u.values()):
+        h = len(u["SQ_INSTS_VALU"]) // 2
+        m = {c: float(np.mean(v[:h])) for c, v in u.items()}
+        lines += ["", f"## Issue counters of `{kern}`, brick order (per dispatch)", "",
+                  f"- waves {m['SQ_WAVES']:.0f}; VALU wave-instructions {m['SQ_INSTS_VALU']:.4g} "
+                  f"({m['SQ_INSTS_VALU'] / pt['instances'] * 64:.0f} per instance);",
+                  f"- VALU busy 4 x SQ_INSTS_VALU / (1024 SIMDs x GRBM_GUI_ACTIVE / 8) = "
+                  f"{4 * m['SQ_INSTS_VALU'] / (1024 * m['GRBM_GUI_ACTIVE'] / 8):.3f};",
+                  f"- SQ_ACTIVE_INST_ANY / SQ_WAVE_CYCLES = {m['SQ_ACTIVE_INST_ANY'] / m['SQ_WAVE_CYCLES']:.3f}, "
+                  f"SQ_WAIT_ANY / SQ_WAVE_CYCLES = {m['SQ_WAIT_ANY'] / m['SQ_WAVE_CYCLES']:.3f}.", "",
+                  "Two waves per SIMD (the FP64 hex8 element routine holds about 210 VGPRs; two 61 KB patches",
+                  "per CU) leave the waits of the gathers, the colour rounds' barriers and the row stores",
+                  "mostly exposed: the kernel is bound by latency and issue, not by HBM."]
+    out = os.path.join(ROOT, "profiles", "upload_hex8_128_kernel_stats.md")
+    open(out, "w").write("\n".join(lines) + "\n")
+    print(out)
+
+
 def other_workloads(args):
     """Compact summaries of the Q2, batch and serving workloads at their own default sizes,
     appended to the headline line (each run after the headline's buffers are freed; a
@@ -1063,7 +1158,13 @@ def main():
                     help="CPU only: summarise the two issue-counter passes of the headline into profiles/")
     ap.add_argument("--summarize-elastic", nargs=3, metavar=("STATS_DIR", "PMC_DIR", "BENCH_JSON"),
                     help="CPU only: summarise rocprofv3 csv output of --workload elastic into profiles/")
+    ap.add_argument("--summarize-upload", nargs=5,
+                    metavar=("STATS_DIR", "FETCH_DIR", "WRITE_DIR", "UTIL_DIR", "BENCH_JSON"),
+                    help="CPU only: summarise rocprofv3 csv output of --workload upload into profiles/")
     args = ap.parse_args()
+    if args.summarize_upload:
+        summarize_upload(*args.summarize_upload)
+        return
     if args.summarize_q2:
         summarize_q2(*args.summarize_q2)
         return
