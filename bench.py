@@ -782,8 +782,9 @@ def bench_elastic(args):
     """Transient hex8 linear elasticity, the served physics (Newmark seeds alpha 2000, gamma2 4e6,
     Rayleigh a 0.01, b 0.001, nu 0.3, E lognormal sigma 0.3 per cell, state U[-1e-3, 1e-3],
     u = 0 on x = 0) on one n^3 brick (default 128), FP64: parallelepiped cells (the unit cube:
-    the node-centric kernel, no workspace) and jittered cells (U[-0.1h, 0.1h]: the element kernel
-    and the node gather).  Compulsory bytes: the CRS values, f and the Dirichlet rows written once,
+    the node-centric kernel, no workspace), jittered cells (U[-0.1h, 0.1h]: the element kernel
+    and the node gather), and the jittered brick uploaded as connectivity (the element kernel and
+    the general node gather; cells in brick order and shuffled).  Compulsory bytes: the CRS values, f and the Dirichlet rows written once,
     x, xdot, xddot, coordinates, E and the row pointers read once."""
     import numpy as np
     import torch
@@ -829,8 +830,55 @@ def bench_elastic(args):
                      "p50": float(np.percentile(per, 50)), "p99": float(np.percentile(per, 99)),
                      "compulsory_bytes": compulsory, "hbm_frac": compulsory / (ms / 1e3) / HBM_PEAK}
         log(f"{name}: {ms:.3f} ms/step")
+        if name == "jittered_cells":
+            ref = (f.clone(), vals.clone(), u, v, a, bc, E, X)
         m.destroy()
-        del f, vals, u, v, a
+        del f, vals
+    # the jittered brick uploaded as unstructured connectivity (txa_mesh_upload): the element
+    # kernel and the general node gather, cells in brick order (same LIDs as the brick: checked
+    # against the lattice result) and in a seeded random order
+    fr, vr, u, v, a, bc, E, X = ref
+    up = {}
+    for order, seed in (("brick_order", None), ("shuffled", 24)):
+        conn = synth.brick_hex8_conn(n, n, n, shuffle_seed=seed)
+        perm = np.argsort(synth.splitmix64(seed, n ** 3), kind="stable") if seed is not None else None
+        m = eng.upload(txa.HEX8, conn, X)
+        nd = m.build_dofmap(3)
+        nnz = m.build_graph()
+        owned, _ = m.dofmap()
+        lid_of_gid = np.empty(nd, np.int64)
+        lid_of_gid[owned] = np.arange(nd)
+        vx = (np.arange((n + 1) ** 2) * (n + 1)).astype(np.int64)   # vertices with i = 0 (GID = 3 vertex + field)
+        ubc = torch.as_tensor(np.sort(lid_of_gid[(3 * vx[:, None] + np.arange(3)).ravel()]).astype(np.int32),
+                              device="cuda")
+        Eu = E if perm is None else E[torch.as_tensor(perm, device="cuda")]
+        f = torch.empty(nd, device="cuda", dtype=torch.float64)
+        vals = torch.empty(nnz, device="cuda", dtype=torch.float64)
+        uu, vv, aa = (t if perm is None else torch.rand(nd, device="cuda", dtype=torch.float64) * 2e-3 - 1e-3
+                      for t in (u, v, a))
+
+        def ustep():
+            m.assemble(physics=txa.ELASTICITY, precision=txa.FP64, coef=Eu, x=uu, xdot=vv, xddot=aa, f=f,
+                       vals=vals, alpha=2000.0, beta=1.0, gamma2=4e6, transient=True, nu=0.3, rho=1.0,
+                       rayleigh=(0.01, 0.001), bc_lids=ubc)
+        for _ in range(args.warmup):
+            ustep()
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+        ev[0].record()
+        for i in range(args.steps):
+            ustep()
+            ev[i + 1].record()
+        torch.cuda.synchronize()
+        per = np.array([ev[i].elapsed_time(ev[i + 1]) for i in range(args.steps)])
+        up[order] = {"ms_per_step": float(per.mean()), "elements_per_s": m.nelem / (float(per.mean()) / 1e3)}
+        if perm is None:
+            up[order]["vs_brick_normwise"] = {"R": float((f - fr).abs().max() / fr.abs().max()),
+                                              "J": float((vals - vr).abs().max() / vr.abs().max())}
+        log(f"uploaded jittered cells, {order}: {per.mean():.3f} ms/step")
+        m.destroy()
+        del f, vals
+    out["jittered_cells_uploaded"] = up
     best = out["parallelepiped_cells"]
     return {"metric": "elements/s (residual+Jacobian), transient hex8 elasticity", "value": best["elements_per_s"],
             "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -935,11 +983,22 @@ def summarize_elastic(stats_dir, pmc_dir, bench_json):
              "| cells | ms per step (HIP events) | p50 / p99 ms | elements/s | HBM fraction (compulsory bytes) |",
              "|---|---:|---:|---:|---:|"]
     for name, c in line["cells"].items():
+        if "p50" not in c:
+            continue
         lines.append(f"| {name.replace('_', ' ')} | {c['ms_per_step']:.3f} | {c['p50']:.3f} / {c['p99']:.3f} | "
                      f"{c['elements_per_s']:.3g} | {c['hbm_frac']:.3f} |")
+    for order, c in line["cells"].get("jittered_cells_uploaded", {}).items():
+        lines.append(f"| jittered cells uploaded (`txa_mesh_upload`), {order.replace('_', ' ')} | "
+                     f"{c['ms_per_step']:.3f} | | {c['elements_per_s']:.3g} | |")
+    up = line["cells"].get("jittered_cells_uploaded", {}).get("brick_order", {}).get("vs_brick_normwise")
+    if up:
+        lines += ["", "The uploaded mesh (element kernel, then `k_elastic_gather_u`: each node's element instances",
+                  "from its list, slot maps per instance) in brick order against the brick's lattice gather: "
+                  f"R {up['R']:.1e}, J {up['J']:.1e} normwise."]
     lines += ["", "Per-dispatch durations from the kernel trace (us), all dispatches of the run:", "",
               "| kernel | calls | mean | min | max |", "|---|---:|---:|---:|---:|"]
-    for k in ("k_elastic8_affine", "k_elastic<8, 8, 64, 0>", "k_elastic_gather<8, double>", "k_dirichlet"):
+    for k in ("k_elastic8_affine", "k_elastic<8, 8, 64, 0>", "k_elastic_gather<8, double>",
+              "k_elastic_gather_u<8, double>", "k_dirichlet"):
         d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in trace
              if (k in r["Kernel_Name"] if "<" not in k else _match(r["Kernel_Name"], k))]
         if d:
@@ -1107,7 +1166,8 @@ def other_workloads(args):
             s["max_rel_err_fp32_vs_fp64_per_mesh"] = line["max_rel_err_fp32_vs_fp64_per_mesh"]
             s["speedup_vs_fp64_per_mesh"] = line["speedup_vs_fp64_per_mesh"]
         elif fn is bench_elastic:
-            s["cells"] = {k: {"ms_per_step": v["ms_per_step"], "hbm_frac": v["hbm_frac"]}
+            s["cells"] = {k: ({"ms_per_step": v["ms_per_step"], "hbm_frac": v["hbm_frac"]} if "hbm_frac" in v
+                              else {o: c["ms_per_step"] for o, c in v.items()})
                           for k, v in line["cells"].items()}
             s["roofline"] = line["roofline"]
         else:

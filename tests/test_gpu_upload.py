@@ -4,7 +4,8 @@ A brick's own connectivity with its elements shuffled and its vertices
 renumbered, and the reference's Exodus fixtures (tests/golden), are uploaded as
 unstructured meshes.  The DOF map and graph must equal the oracle's on the same
 arrays bit for bit; the assembly (one-field hex8 / tet4 diffusion: the node-patch
-kernels, FP64 and FP32; other physics and topologies: the element kernels, FP64)
+kernels, FP64 and FP32; 3-field hex8 / hex27 elasticity: the element kernels and a
+general node gather, every precision; the rest: the workgroup-per-element kernel, FP64)
 must equal the oracle's within 1e-12 normwise in FP64, Dirichlet rows included, and
 in the FP32 form R entrywise over the 1e-12 band and J normwise within 1e-6.
 Lattice-only features (split-BF16 diffusion, brick side sets, batches) must be
@@ -197,6 +198,45 @@ def test_uploaded_transient_diffusion(engine, topo):
         torch.cuda.synchronize()
         assert rel_err(f.cpu().numpy(), fr) <= TOL64
         assert rel_err(v.cpu().numpy(), vr) <= TOL64
+    finally:
+        m.destroy()
+
+
+@pytest.mark.parametrize("topo,precision,tol", [(txa.HEX8, txa.FP64, TOL64), (txa.HEX8, txa.FP32, TOL32),
+                                                (txa.HEX27, txa.FP64, TOL64), (txa.HEX27, txa.FP32, TOL32),
+                                                (txa.HEX27, txa.SPLIT_BF16, TOL32)])
+def test_uploaded_elasticity_transient(engine, topo, precision, tol):
+    # uploaded 3-field hex8 / hex27 meshes take the element kernels and the general node gather
+    # (each node's instances from its list, slot maps per instance): Newmark seeds, Rayleigh
+    # damping, body force, against the oracle (reduced precision normwise); two calls agree bit
+    # for bit (every row and residual entry is summed in a fixed order)
+    n = 4 if topo == txa.HEX8 else 2
+    conn, ev, X, nent = renumbered_brick(topo, n, seed=120 + topo)
+    m = engine.upload(topo, conn, X, nentities=nent)
+    try:
+        nd = m.build_dofmap(3)
+        nnz = m.build_graph()
+        gids, owned, el = O.dofmap(conn, 3)
+        rp, col = O.graph(el, owned.size)
+        E = synth.lognormal(121, conn.shape[0], 0.3)
+        x, xd, xdd = (synth.uniform(122 + i, nd, -1, 1) for i in range(3))
+        kw = dict(alpha=2000.0, beta=1.0, gamma2=4.0e6)
+        A = O.Assembler(OTOPO[topo], txa.ELASTICITY, ev, X, el, E, rp, col, nu=0.3, rho=2.0, src=(0.5, -1.0, 0.25),
+                        rayleigh=(0.01, 0.001))
+        fr, vr = A.evaluate(x, xd, xdd, transient=True, mode=O.MODE_EXPLICIT, **kw)
+        out = []
+        for _ in range(2):
+            f = torch.empty(nd, device="cuda", dtype=torch.float64)
+            v = torch.empty(nnz, device="cuda", dtype=torch.float64)
+            m.assemble(physics=txa.ELASTICITY, precision=precision, coef=dev(E), x=dev(x), xdot=dev(xd),
+                       xddot=dev(xdd), f=f, vals=v, nu=0.3, rho=2.0, src=(0.5, -1.0, 0.25), rayleigh=(0.01, 0.001),
+                       transient=True, **kw)
+            torch.cuda.synchronize()
+            out.append((f.cpu().numpy(), v.cpu().numpy()))
+        ef, ev_ = rel_err(out[0][0], fr), rel_err(out[0][1], vr)
+        print(f"uploaded elasticity topo {topo} precision {precision}: R {ef:.2e} J {ev_:.2e}")
+        assert ef <= tol and ev_ <= tol
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     finally:
         m.destroy()
 

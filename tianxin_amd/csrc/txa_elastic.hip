@@ -734,6 +734,139 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
   }
 }
 
+// The same gather on an uploaded hex8 / hex27 mesh of one block: node n (rows 3n, 3n + 1, 3n + 2:
+// the first-touch numbering gives a node's three fields consecutive LIDs) takes its element
+// instances (e, a) from the node's list, ascending, eight at a time, and the row position of
+// each element node's field 0 from the instance's slot map.
+struct GatherUParams {
+  int64_t nnodes;
+  const int64_t *occ_ptr;   // [nnodes + 1]
+  const int32_t *occ;       // e * NB + a per instance
+  const uint16_t *slot;     // [instances][NB] position of (element node b, field 0) in row 3n
+  const int64_t *rowptr;
+  const void *jbuf;
+  const double *rbuf;
+  double *f, *vals;
+};
+
+template <int NB, typename JT>
+__global__ __launch_bounds__(256) void k_elastic_gather_u(GatherUParams g) {
+  constexpr int ND = 3 * NB;
+  constexpr int R = NB == 8 ? 1 : 2, S = 2 * R + 1;
+  constexpr int MAXLEN = 3 * S * S * S;        // entries per row (checked when the lists are built)
+  __shared__ double buf[4][3 * MAXLEN];
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  constexpr int NR = (3 * ND + 63) / 64;
+  constexpr int NP = block_stride(NB);
+  double *b = buf[w];
+  const JT *J = static_cast<const JT *>(g.jbuf);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * 4;
+  for (int64_t n = static_cast<int64_t>(blockIdx.x) * 4 + w; n < g.nnodes; n += stride) {
+    const int32_t L0 = static_cast<int32_t>(3 * n);
+    const int64_t base = g.rowptr[L0];
+    const int len = min(static_cast<int>(g.rowptr[L0 + 1] - base), MAXLEN);
+    for (int k = lane; k < 3 * len; k += 64) b[k] = 0.0;
+    gather_lds_sync();
+    const int64_t o0 = g.occ_ptr[n], o1 = g.occ_ptr[n + 1];
+    double racc = 0.0;
+    for (int64_t k0 = o0; k0 < o1; k0 += 8) {
+      const int ne = static_cast<int>(o1 - k0 < 8 ? o1 - k0 : 8);
+      JT val[8][NR];
+      uint16_t pos[8][NR];
+      double rr[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        rr[c] = 0.0;
+        const int32_t v = c < ne ? g.occ[k0 + c] : 0;
+        const int64_t e = v / NB;
+        const int a = v - static_cast<int>(e) * NB;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          val[c][r] = JT(0);
+          pos[c][r] = 0xffffu;
+          const int idx = lane + 64 * r;
+          if (c < ne && idx < 3 * ND && J) {
+            val[c][r] = J[(e * NB + a) * NP + idx];
+            pos[c][r] = g.slot[(k0 + c) * NB + (idx % ND) % NB];
+          }
+        }
+        if (c < ne && lane < 3 && g.rbuf) rr[c] = g.rbuf[e * ND + 3 * a + lane];
+      }
+#pragma unroll
+      for (int c = 0; c < 8; ++c)
+#pragma unroll
+        for (int r = 0; r < NR; ++r) asm volatile("" : "+v"(val[c][r]));
+      // one wave's LDS operations execute in order and no two lanes of one instruction share
+      // an entry: every entry is summed in instance order
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        if (c < ne) {
+#pragma unroll
+          for (int r = 0; r < NR; ++r) {
+            const int idx = lane + 64 * r;
+            const int i = idx / ND, j = (idx % ND) / NB;
+            const uint16_t ps = pos[c][r];
+            if (idx < 3 * ND && ps != 0xffffu && ps + j < len)
+              atomicAdd(&b[i * len + ps + j], static_cast<double>(val[c][r]));
+          }
+          racc += rr[c];
+        }
+      }
+    }
+    gather_lds_sync();
+    if (g.vals)
+      for (int k = lane; k < 3 * len; k += 64) g.vals[base + k] = b[k];
+    if (g.f && lane < 3) g.f[L0 + lane] = racc;
+    gather_lds_sync();
+  }
+}
+
+// Element instances of each node of an uploaded 3-field mesh (field-0 positions of elem_lid)
+__global__ void k_uocc_count(int64_t npos0, int nb, const int32_t *__restrict__ elem_lid,
+                             unsigned long long *__restrict__ cnt) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= npos0) return;   // t = e * nb + a
+  atomicAdd(cnt + elem_lid[3 * t] / 3, 1ull);
+}
+
+__global__ void k_uocc_fill(int64_t npos0, int nb, const int32_t *__restrict__ elem_lid,
+                            unsigned long long *__restrict__ cursor, int32_t *__restrict__ occ) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= npos0) return;
+  occ[atomicAdd(cursor + elem_lid[3 * t] / 3, 1ull)] = static_cast<int32_t>(t);
+}
+
+// Sorts each node's list (ascending e * nb + a: a fixed summation order), the slot map of each
+// instance, and the longest row and list (out[0], out[1]).
+__global__ void k_uocc_finish(int64_t nnodes, int nb, const int64_t *__restrict__ optr, int32_t *__restrict__ occ,
+                              const int32_t *__restrict__ elem_lid, const int64_t *__restrict__ rowptr,
+                              const int32_t *__restrict__ col, uint16_t *__restrict__ slot, int *__restrict__ out) {
+  const int64_t n = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (n >= nnodes) return;
+  const int64_t lo = optr[n], hi = optr[n + 1];
+  for (int64_t i = lo + 1; i < hi; ++i) {
+    const int32_t v = occ[i];
+    int64_t j = i;
+    for (; j > lo && occ[j - 1] > v; --j) occ[j] = occ[j - 1];
+    occ[j] = v;
+  }
+  const int64_t r0 = rowptr[3 * n], len = rowptr[3 * n + 1] - r0;
+  atomicMax(out, static_cast<int>(len));
+  atomicMax(out + 1, static_cast<int>(hi - lo));
+  for (int64_t i = lo; i < hi; ++i) {
+    const int64_t e = occ[i] / nb;
+    for (int b = 0; b < nb; ++b) {
+      const int32_t key = elem_lid[(e * nb + b) * 3];
+      int64_t l = 0, h = len;
+      while (l < h) {
+        const int64_t m = (l + h) >> 1;
+        if (col[r0 + m] < key) l = m + 1; else h = m;
+      }
+      slot[i * nb + b] = l < len && col[r0 + l] == key && l < 0xffff ? static_cast<uint16_t>(l) : uint16_t(0xffff);
+    }
+  }
+}
+
 // Row position of (neighbour, field 0) for every node and lattice offset in
 // [-R, R]^3 (0xffff: not a neighbour), by binary search in the node's first row.
 __global__ void k_node_perm(int64_t px, int64_t py, int64_t pz, int R, int nf, int W,
@@ -1010,7 +1143,7 @@ void ws_parts(const Mesh &m, int precision, bool want_f, bool want_vals, size_t 
 }  // namespace
 
 bool elastic_supported(const Mesh &m, const txa_assembly_args &a) {
-  if (a.physics != TXA_ELASTICITY || m.nfields != 3 || !m.nperm.p) return false;
+  if (a.physics != TXA_ELASTICITY || m.nfields != 3 || !(m.nperm.p || m.uocc.p)) return false;
   if (m.topo == TXA_HEX8) return a.precision == TXA_FP64 || a.precision == TXA_FP32;
   return m.topo == TXA_HEX27;
 }
@@ -1022,6 +1155,44 @@ size_t elastic_workspace_bytes(const Mesh &m, int precision, bool want_f, bool w
 }
 
 void node_perm_build(Mesh &m, hipStream_t s) {
+  m.uocc_ptr.release();
+  m.uocc.release();
+  m.uslot.release();
+  if (!m.structured && m.nfields == 3 && m.blocks.size() == 1 && (m.topo == TXA_HEX8 || m.topo == TXA_HEX27)) {
+    // uploaded mesh: element instances of each node and their slot maps, for the general gather
+    const int nb = m.nconn;
+    const int64_t nnodes = m.ndofs / 3, npos0 = m.nelem * nb;
+    if (npos0 >= (int64_t(1) << 31)) return;
+    DevBuf<unsigned long long> cnt;
+    cnt.alloc(nnodes + 1);
+    TXA_HIP(hipMemsetAsync(cnt.p, 0, cnt.bytes(), s));
+    const unsigned gp = static_cast<unsigned>((npos0 + 255) / 256), gn = static_cast<unsigned>((nnodes + 255) / 256);
+    hipLaunchKernelGGL(k_uocc_count, dim3(gp), dim3(256), 0, s, npos0, nb, m.elem_lid.p, cnt.p);
+    TXA_HIP(hipGetLastError());
+    m.uocc_ptr.alloc(nnodes + 1);
+    exclusive_scan_i64(reinterpret_cast<const int64_t *>(cnt.p), m.uocc_ptr.p, nnodes + 1, s);
+    TXA_HIP(hipMemcpyAsync(cnt.p, m.uocc_ptr.p, sizeof(int64_t) * (nnodes + 1), hipMemcpyDeviceToDevice, s));
+    m.uocc.alloc(npos0);
+    hipLaunchKernelGGL(k_uocc_fill, dim3(gp), dim3(256), 0, s, npos0, nb, m.elem_lid.p, cnt.p, m.uocc.p);
+    TXA_HIP(hipGetLastError());
+    m.uslot.alloc(npos0 * nb);
+    DevBuf<int> mx;
+    mx.alloc(2);
+    TXA_HIP(hipMemsetAsync(mx.p, 0, 2 * sizeof(int), s));
+    hipLaunchKernelGGL(k_uocc_finish, dim3(gn), dim3(256), 0, s, nnodes, nb, m.uocc_ptr.p, m.uocc.p, m.elem_lid.p,
+                       m.rowptr.p, m.col.p, m.uslot.p, mx.p);
+    TXA_HIP(hipGetLastError());
+    int h[2];
+    TXA_HIP(hipMemcpyAsync(h, mx.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    TXA_HIP(hipStreamSynchronize(s));
+    const int S = nb == 8 ? 3 : 5;
+    if (h[0] > 3 * S * S * S) {   // a row longer than the gather's LDS rows: the generic kernel
+      m.uocc_ptr.release();
+      m.uocc.release();
+      m.uslot.release();
+    }
+    return;
+  }
   if (!m.structured || m.nfields != 3 || (m.topo != TXA_HEX8 && m.topo != TXA_HEX27)) return;
   const int R = m.topo == TXA_HEX8 ? 1 : 2;
   const int64_t px = m.topo == TXA_HEX8 ? m.nx + 1 : 2 * static_cast<int64_t>(m.nx) + 1;
@@ -1093,6 +1264,32 @@ void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws
     else if (a.precision == TXA_FP32) launch<27, 27, 256, MODE_F32>(p, s);
     else launch<27, 27, 256, MODE_F64>(p, s);
   }
+  int dev = 0, ncu = 0;
+  TXA_HIP(hipGetDevice(&dev));
+  TXA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  const bool wide = a.precision == TXA_FP64;
+  if (!m.structured) {
+    GatherUParams u;
+    u.nnodes = m.ndofs / 3;
+    u.occ_ptr = m.uocc_ptr.p;
+    u.occ = m.uocc.p;
+    u.slot = m.uslot.p;
+    u.rowptr = m.rowptr.p;
+    u.jbuf = p.jbuf;
+    u.rbuf = p.rbuf;
+    u.f = a.f;
+    u.vals = a.vals;
+    const unsigned ug = static_cast<unsigned>(std::min<int64_t>((u.nnodes + 3) / 4, static_cast<int64_t>(ncu) * 16));
+    if (m.topo == TXA_HEX8) {
+      if (wide) hipLaunchKernelGGL((k_elastic_gather_u<8, double>), dim3(ug), dim3(256), 0, s, u);
+      else hipLaunchKernelGGL((k_elastic_gather_u<8, float>), dim3(ug), dim3(256), 0, s, u);
+    } else {
+      if (wide) hipLaunchKernelGGL((k_elastic_gather_u<27, double>), dim3(ug), dim3(256), 0, s, u);
+      else hipLaunchKernelGGL((k_elastic_gather_u<27, float>), dim3(ug), dim3(256), 0, s, u);
+    }
+    TXA_HIP(hipGetLastError());
+    return;
+  }
   GatherParams g;
   g.nx = m.nx; g.ny = m.ny; g.nz = m.nz;
   const int64_t st = m.topo == TXA_HEX8 ? 1 : 2;
@@ -1106,11 +1303,7 @@ void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws
   g.rbuf = p.rbuf;
   g.f = a.f;
   g.vals = a.vals;
-  int dev = 0, ncu = 0;
-  TXA_HIP(hipGetDevice(&dev));
-  TXA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   const unsigned grid = static_cast<unsigned>(std::min<int64_t>((m.npoints + 3) / 4, static_cast<int64_t>(ncu) * 16));
-  const bool wide = a.precision == TXA_FP64;
   if (m.topo == TXA_HEX8) {
     if (wide) hipLaunchKernelGGL((k_elastic_gather<8, double>), dim3(grid), dim3(256), 0, s, g);
     else hipLaunchKernelGGL((k_elastic_gather<8, float>), dim3(grid), dim3(256), 0, s, g);

@@ -99,6 +99,11 @@ struct Mesh {
   DevBuf<uint8_t> perm;       // [ndofs][perm_width] position of each stencil neighbour in the row, 255 = absent
   int nperm_w = 0;            // entries per node of nperm
   DevBuf<uint16_t> nperm;     // 3-field HEX8 / HEX27: [nodes][nperm_w] position of (neighbour, field 0) in the node's rows
+  // uploaded 3-field HEX8 / HEX27 meshes of one block: the element instances (e * nb + a) of each
+  // node, ascending, and each instance's slot map (position of element node b's field 0 in row 3n)
+  DevBuf<int64_t> uocc_ptr;
+  DevBuf<int32_t> uocc;
+  DevBuf<uint16_t> uslot;
   // uploaded one-field HEX8 / TET4 / WEDGE6 blocks: the element rows of each LID, as positions p
   // in elem_lid (element e of a block, row a: block's position base + e*ndof + a), ascending
   // (occ[occ_ptr[r] .. occ_ptr[r+1]) for row r), for the row gather of the assembly
