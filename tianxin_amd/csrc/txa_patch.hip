@@ -300,7 +300,8 @@ __device__ __forceinline__ constexpr int sym4(int a, int b) {
 }
 
 // One lane's instance of a chunk: geometry, state and coefficient, the rows' curve positions
-// and the slot map, gathered through the instance's LIDs (which are loaded a chunk ahead).
+// and the slot map, gathered through the instance's LIDs (which, with its element, are loaded a
+// chunk ahead).
 template <int NB, bool TRANS>
 struct InstIn {
   double X[3][NB], u[NB], ud[TRANS ? NB : 1], k;
@@ -309,10 +310,13 @@ struct InstIn {
 };
 
 template <int NB>
-__device__ __forceinline__ void load_lids(const PatchParams &p, int64_t i, int64_t i_end, int32_t (&L)[NB]) {
+__device__ __forceinline__ void load_lids(const PatchParams &p, int64_t i, int64_t i_end, int32_t (&L)[NB],
+                                          int32_t &e) {
 #pragma unroll
   for (int v = 0; v < NB; ++v) L[v] = 0;
+  e = 0;
   if (i >= i_end) return;
+  e = p.inst[i];
   const int4 *l4 = reinterpret_cast<const int4 *>(p.ilid + i * NB);
 #pragma unroll
   for (int q = 0; q < NB / 4; ++q) {
@@ -322,7 +326,7 @@ __device__ __forceinline__ void load_lids(const PatchParams &p, int64_t i, int64
 }
 
 template <int NB, bool TRANS>
-__device__ __forceinline__ void load_inst(const PatchParams &p, int64_t i, const int32_t (&L)[NB],
+__device__ __forceinline__ void load_inst(const PatchParams &p, int64_t i, const int32_t (&L)[NB], int32_t e,
                                           InstIn<NB, TRANS> &in) {
 #pragma unroll
   for (int v = 0; v < NB; ++v) {
@@ -332,7 +336,7 @@ __device__ __forceinline__ void load_inst(const PatchParams &p, int64_t i, const
     in.u[v] = p.x[L[v]];
     if constexpr (TRANS) in.ud[v] = p.xdot[L[v]];
   }
-  in.k = p.coef[p.inst[i]];
+  in.k = p.coef[e];
   if (p.vals) {
     const uint4 *s4 = reinterpret_cast<const uint4 *>(p.islot + i * NB * NB);
 #pragma unroll
@@ -361,7 +365,8 @@ __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
   const int64_t i_begin = p.cstart[pb * NC], i_end = p.cstart[(pb + 1) * NC];
   const int64_t pos0 = pb * P;
   int32_t Lc[NB];
-  load_lids<NB>(p, i_begin + tid, i_end, Lc);
+  int32_t Ec;
+  load_lids<NB>(p, i_begin + tid, i_end, Lc, Ec);
   for (int r = tid; r < P; r += kPatchThreads) {
     const int32_t L = p.prow[pos0 + r];
     slid[r] = L;
@@ -384,7 +389,7 @@ __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
     double Re[NB];
     if (mycol >= 0) {
       InstIn<NB, TRANS> nx;
-      load_inst(p, i, Lc, nx);
+      load_inst(p, i, Lc, Ec, nx);
 #pragma unroll
       for (int v = 0; v < NB; ++v) {
         const int lp = nx.pos[v] - static_cast<int>(pos0);
@@ -416,7 +421,7 @@ __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
       }
     }
     // the next chunk's LIDs
-    load_lids<NB>(p, i + kPatchThreads, i_end, Lc);
+    load_lids<NB>(p, i + kPatchThreads, i_end, Lc, Ec);
     // colour by colour: within one colour no two instances share a vertex
     for (int c = cfirst; c <= clast; ++c) {
       __syncthreads();
@@ -525,8 +530,9 @@ void patch_build(Mesh &m, hipStream_t s) {
   hipLaunchKernelGGL(k_node_xyz, dim3(setup_grid(npos)), dim3(kSetupBlock), 0, s, npos, m.conn.p, m.elem_lid.p,
                      m.xyz.p, m.nverts, m.patch_rec.p);
   TXA_HIP(hipGetLastError());
-  // integer lattice of about one vertex per unit cell (the mean spacing h of the rows over the
-  // bounding box), so that aligned Morton blocks of 2^k points are compact blocks of vertices
+  // integer lattice of about one vertex per unit cell (the spacing h of a lattice of as many
+  // points spanning the bounding box), so that aligned Morton blocks of 2^k points are compact
+  // blocks of vertices (on a brick, exactly its lattice)
   Box box;
   {
     std::vector<double> h(static_cast<size_t>(3 * m.nverts));
@@ -541,7 +547,9 @@ void patch_build(Mesh &m, hipStream_t s) {
       ext[d] = *mm.second - *mm.first;
       if (ext[d] > 0.0) { vol *= ext[d]; ++dims; }
     }
-    const double hs = dims ? std::pow(vol / static_cast<double>(nrows), 1.0 / dims) : 1.0;
+    // the spacing of a lattice of nrows points spanning the box (n^(1/dims) points per axis)
+    const double per_axis = dims ? std::pow(static_cast<double>(nrows), 1.0 / dims) : 1.0;
+    const double hs = dims && per_axis > 1.5 ? std::pow(vol, 1.0 / dims) / (per_axis - 1.0) : 1.0;
     for (int d = 0; d < 3; ++d) box.scale[d] = ext[d] > 0.0 ? std::min(1.0 / hs, 2097150.0 / ext[d]) : 0.0;
   }
   DevBuf<uint64_t> key, key_s;
