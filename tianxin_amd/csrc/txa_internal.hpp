@@ -109,7 +109,7 @@ struct Mesh {
   // (occ[occ_ptr[r] .. occ_ptr[r+1]) for row r), for the row gather of the assembly
   DevBuf<int64_t> occ_ptr, occ;
   // node patches of uploaded one-field HEX8 / TET4 meshes (txa_patch.hip): rows along a Morton
-  // curve cut into patches of patch_P, the elements touching each patch by (colour, element)
+  // curve cut into patches of patch_P, the elements touching each patch by (colour, sweep order)
   int patch_P = 0, patch_RS = 0, patch_NC = 0;   // rows per patch, LDS row stride, colours
   int64_t npatch = 0, patch_ninst = 0;
   DevBuf<int64_t> patch_cstart;   // [npatch*NC + 1] first instance of each (patch, colour)

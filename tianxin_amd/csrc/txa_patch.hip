@@ -12,9 +12,9 @@
 // - the rows (LIDs; one field, so a row is a vertex) sorted along a Morton curve of their
 //   vertex coordinates and cut into patches of P consecutive rows;
 // - for each patch, its instances: every element with a vertex in the patch, listed by
-//   colour, then element;
-// - a colouring of the elements (Jones-Plassmann with hashed priorities): elements that
-//   share a vertex differ in colour;
+//   colour, then sweep order;
+// - a colouring of each patch's instances, greedy in sweep order (element centroids by z,
+//   then y, then x): instances that share one of the patch's own vertices differ in colour;
 // - instance records in instance order: the element's LIDs and its slot map (for row a and
 //   column b, the position of LID b in LID a's CRS row), so that a patch streams its
 //   instances contiguously whatever order the elements were uploaded in.
@@ -22,7 +22,7 @@
 // (the sum-factorised hex8 routine of the lattice kernels, or the constant-gradient tet),
 // then adds the element rows of the patch's own vertices into the patch's rows in LDS.
 // The adds go colour by colour with a barrier between colours, so a row receives at
-// most one add per step: the sums run in a fixed order (colour, element) and the
+// most one add per step: the sums run in a fixed order (colour, sweep order) and the
 // results do not depend on scheduling.  Finally each row is stored once, whole, with
 // its residual entry.  No workspace, no global atomics, no dependence on the order in
 // which the elements were uploaded beyond the colouring's priorities.  Elements on a
@@ -104,49 +104,38 @@ __global__ void k_place_rows(int64_t nrows, int64_t npad, const int32_t *__restr
   if (r >= 0) rec[4 * static_cast<int64_t>(r) + 3] = __longlong_as_double(i);
 }
 
-__device__ __forceinline__ uint64_t prio(uint64_t e) {   // splitmix64 finaliser: hashed priorities
-  uint64_t z = e + 0x9e3779b97f4a7c15ull;
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-  return z ^ (z >> 31);
-}
-
-// One Jones-Plassmann round: an uncoloured element whose priority exceeds that of every
-// uncoloured element sharing a vertex takes the smallest colour no coloured neighbour has.
-// Reads cin (the colours before the round), writes cout (a copy of cin on entry).
-__global__ void k_jp_round(int64_t nelem, int nb, const int32_t *__restrict__ elem_lid,
-                           const int64_t *__restrict__ occ_ptr, const int64_t *__restrict__ occ,
-                           const int32_t *__restrict__ cin, int32_t *__restrict__ cout, int *__restrict__ done,
-                           int *__restrict__ maxc) {
+// Sweep key of each element: its centroid on the Morton lattice, z then y then x (lexicographic),
+// so that a greedy colouring in this order colours like a structured sweep.
+__global__ void k_sweep_keys(int64_t nelem, int nb, const int32_t *__restrict__ elem_lid,
+                             const double *__restrict__ rec, Box b, uint64_t *__restrict__ key,
+                             int32_t *__restrict__ elem) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (e >= nelem || cin[e] >= 0) return;
-  const uint64_t pe = prio(static_cast<uint64_t>(e));
-  uint64_t used = 0;
-  for (int v = 0; v < nb; ++v) {
-    const int32_t r = elem_lid[e * nb + v];
-    for (int64_t k = occ_ptr[r]; k < occ_ptr[r + 1]; ++k) {
-      const int64_t e2 = occ[k] / nb;
-      if (e2 == e) continue;
-      const int32_t c2 = cin[e2];
-      if (c2 >= 0) {
-        used |= 1ull << c2;
-      } else {
-        const uint64_t p2 = prio(static_cast<uint64_t>(e2));
-        if (p2 > pe || (p2 == pe && e2 > e)) return;   // a neighbour goes first
-      }
-    }
+  if (e >= nelem) return;
+  uint64_t q[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    double c = 0.0;
+    for (int v = 0; v < nb; ++v) c += rec[4 * static_cast<int64_t>(elem_lid[e * nb + v]) + d];
+    double t = (c / nb - b.lo[d]) * b.scale[d];
+    t = t < 0.0 ? 0.0 : (t > 2097151.0 ? 2097151.0 : t);
+    q[d] = static_cast<uint64_t>(t);
   }
-  const int c = used == ~0ull ? kMaxColours : __builtin_ctzll(~used);
-  cout[e] = c < kMaxColours ? c : kMaxColours;   // kMaxColours: out of colours (the patch path is refused)
-  atomicAdd(done, 1);
-  atomicMax(maxc, c);
+  key[e] = (q[2] << 42) | (q[1] << 21) | q[0];
+  elem[e] = static_cast<int32_t>(e);
 }
 
-// The distinct patches of each element, as keys patch << 38 | colour << 32 | element; the
-// other slots hold ~0 (sorted last).
+__global__ void k_rank_of(int64_t nelem, const int32_t *__restrict__ by_rank, int32_t *__restrict__ rank) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= nelem) return;
+  rank[by_rank[r]] = static_cast<int32_t>(r);
+}
+
+// The distinct patches of each element, as keys patch << 32 | sweep rank of the element; the
+// other slots hold ~0 (sorted last).  Counts the instances of each patch.
 __global__ void k_instance_keys(int64_t nelem, int nb, int P, const int32_t *__restrict__ elem_lid,
-                                const double *__restrict__ rec, const int32_t *__restrict__ colour,
-                                uint64_t *__restrict__ key, unsigned long long *__restrict__ count) {
+                                const double *__restrict__ rec, const int32_t *__restrict__ rank,
+                                uint64_t *__restrict__ key, unsigned long long *__restrict__ count,
+                                unsigned long long *__restrict__ per_patch) {
   const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= nelem) return;
   int64_t pt[8];
@@ -158,20 +147,61 @@ __global__ void k_instance_keys(int64_t nelem, int nb, int P, const int32_t *__r
     for (int j = 0; j < n; ++j) seen = seen || pt[j] == p;
     if (!seen) pt[n++] = p;
   }
-  const uint64_t tail = (static_cast<uint64_t>(colour[e]) << 32) | static_cast<uint64_t>(e);
-  for (int v = 0; v < nb; ++v) key[e * nb + v] = v < n ? (static_cast<uint64_t>(pt[v]) << 38) | tail : ~0ull;
+  for (int v = 0; v < nb; ++v)
+    key[e * nb + v] = v < n ? (static_cast<uint64_t>(pt[v]) << 32) | static_cast<uint32_t>(rank[e]) : ~0ull;
+  for (int j = 0; j < n; ++j) atomicAdd(per_patch + pt[j], 1ull);
   atomicAdd(count, static_cast<unsigned long long>(n));
 }
 
-// Instances from the sorted keys, and the count of each (patch, colour) pair.
-__global__ void k_instances(int64_t ninst, int nc, const uint64_t *__restrict__ key, int32_t *__restrict__ inst,
+// Greedy colouring of each patch's instances in sweep order (one thread per patch): an instance
+// takes the smallest colour that no earlier instance of the patch sharing one of the patch's
+// own vertices has.  Only the patch's rows are summed in its workgroup, so only those vertices
+// conflict.  rowmask: [npatch * P] colours taken at each row, zero on entry.
+__global__ void k_patch_colour(int64_t npatch, int nb, int P, const int64_t *__restrict__ pstart,
+                               const uint64_t *__restrict__ key, const int32_t *__restrict__ by_rank,
+                               const int32_t *__restrict__ elem_lid, const double *__restrict__ rec,
+                               unsigned long long *__restrict__ rowmask, uint8_t *__restrict__ colour,
+                               int *__restrict__ maxc) {
+  const int64_t pb = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (pb >= npatch) return;
+  int cmax = 0;
+  for (int64_t i = pstart[pb]; i < pstart[pb + 1]; ++i) {
+    const int64_t e = by_rank[static_cast<uint32_t>(key[i])];
+    int64_t own[8];
+    unsigned long long used = 0;
+    for (int v = 0; v < nb; ++v) {
+      const int64_t pos = __double_as_longlong(rec[4 * static_cast<int64_t>(elem_lid[e * nb + v]) + 3]);
+      own[v] = pos / P == pb ? pos : -1;
+      if (own[v] >= 0) used |= rowmask[own[v]];
+    }
+    const int c = used == ~0ull ? kMaxColours : __builtin_ctzll(~used);
+    colour[i] = static_cast<uint8_t>(c < kMaxColours ? c : kMaxColours - 1);
+    cmax = c > cmax ? c : cmax;
+    if (c < kMaxColours)
+      for (int v = 0; v < nb; ++v)
+        if (own[v] >= 0) rowmask[own[v]] |= 1ull << c;
+  }
+  atomicMax(maxc, cmax);
+}
+
+// Final keys: patch << 38 | colour << 32 | sweep rank.
+__global__ void k_final_keys(int64_t ninst, const uint64_t *__restrict__ key, const uint8_t *__restrict__ colour,
+                             uint64_t *__restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= ninst) return;
+  out[i] = ((key[i] >> 32) << 38) | (static_cast<uint64_t>(colour[i]) << 32) | (key[i] & 0xffffffffull);
+}
+
+// Instances from the sorted final keys, and the count of each (patch, colour) pair.
+__global__ void k_instances(int64_t ninst, int nc, const uint64_t *__restrict__ key,
+                            const int32_t *__restrict__ by_rank, int32_t *__restrict__ inst,
                             uint8_t *__restrict__ icol, unsigned long long *__restrict__ cnt) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= ninst) return;
   const uint64_t k = key[i];
   const int64_t p = static_cast<int64_t>(k >> 38);
   const int c = static_cast<int>((k >> 32) & 63);
-  inst[i] = static_cast<int32_t>(k & 0xffffffffull);
+  inst[i] = by_rank[static_cast<uint32_t>(k)];
   icol[i] = static_cast<uint8_t>(c);
   atomicAdd(cnt + p * nc + c, 1ull);
 }
@@ -294,6 +324,10 @@ __device__ __forceinline__ void tet4_element(const double (&X)[3][4], const doub
       K[n] = Real(v);
     }
 }
+
+// LDS row of a patch row: rows 32 apart along the curve (the same bank for an odd row stride)
+// are spread over the banks by folding bits 5..9 of the row into bits 0..4
+__device__ __forceinline__ int lds_row(int r) { return r ^ ((r >> 5) & 31); }
 
 __device__ __forceinline__ constexpr int sym4(int a, int b) {
   return a <= b ? a * 4 - (a * (a - 1)) / 2 + (b - a) : b * 4 - (b * (b - 1)) / 2 + (a - b);
@@ -431,7 +465,7 @@ __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
           if (lrow[a] < 0) continue;
           if (p.f) __hip_atomic_fetch_add(sres + lrow[a], Re[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
           if (p.vals) {
-            double *row = sacc + lrow[a] * RS;
+            double *row = sacc + lds_row(lrow[a]) * RS;
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
               const double kv = static_cast<double>(NB == 8 ? Ke[sym8(a, b)] : Ke[sym4(a, b)]);
@@ -453,7 +487,7 @@ __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
       const int32_t L = slid[r];
       if (L < 0) continue;
       if (p.vals)
-        for (int s = lane; s < slen[r]; s += 32) p.vals[srow0[r] + s] += sacc[r * RS + s];
+        for (int s = lane; s < slen[r]; s += 32) p.vals[srow0[r] + s] += sacc[lds_row(r) * RS + s];
       if (p.f && lane == 0) p.f[L] += sres[r];
     }
     return;
@@ -462,7 +496,8 @@ __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
     const int32_t L = slid[r];
     if (L < 0) continue;
     if (p.vals)
-      for (int s = lane; s < slen[r]; s += 32) __builtin_nontemporal_store(sacc[r * RS + s], p.vals + srow0[r] + s);
+      for (int s = lane; s < slen[r]; s += 32)
+        __builtin_nontemporal_store(sacc[lds_row(r) * RS + s], p.vals + srow0[r] + s);
     if (p.f && lane == 0) __builtin_nontemporal_store(sres[r], p.f + L);
   }
 }
@@ -498,33 +533,6 @@ void patch_build(Mesh &m, hipStream_t s) {
   int P = kMaxRowsPerThread * kPatchThreads;
   while (P >= 32 && patch_lds_bytes(P, rs) > static_cast<size_t>(kPatchLdsBytes)) P /= 2;
   if (P < 32 || rs > 255) return;
-  // colouring
-  DevBuf<int32_t> col_a, col_b;
-  col_a.alloc(nelem);
-  col_b.alloc(nelem);
-  TXA_HIP(hipMemsetAsync(col_a.p, 0xff, sizeof(int32_t) * nelem, s));
-  TXA_HIP(hipMemsetAsync(col_b.p, 0xff, sizeof(int32_t) * nelem, s));
-  DevBuf<int> cnt;
-  cnt.alloc(2);
-  TXA_HIP(hipMemsetAsync(cnt.p, 0, 2 * sizeof(int), s));
-  int64_t coloured = 0;
-  for (int round = 0; coloured < nelem; ++round) {
-    TXA_REQUIRE(round < 100000, TXA_ERR_LOGIC, "element colouring did not finish");
-    hipLaunchKernelGGL(k_jp_round, dim3(setup_grid(nelem)), dim3(kSetupBlock), 0, s, nelem, nb, m.elem_lid.p,
-                       m.occ_ptr.p, m.occ.p, col_a.p, col_b.p, cnt.p, cnt.p + 1);
-    TXA_HIP(hipGetLastError());
-    TXA_HIP(hipMemcpyAsync(col_a.p, col_b.p, sizeof(int32_t) * nelem, hipMemcpyDeviceToDevice, s));
-    int h[2];
-    TXA_HIP(hipMemcpyAsync(h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, s));
-    TXA_HIP(hipStreamSynchronize(s));
-    TXA_REQUIRE(h[0] > coloured || coloured == nelem, TXA_ERR_LOGIC, "element colouring made no progress");
-    coloured = h[0];
-    if (h[1] >= kMaxColours) return;   // more colours than the masks hold: no patch path
-  }
-  int nc = 0;
-  TXA_HIP(hipMemcpy(&nc, cnt.p + 1, sizeof(int), hipMemcpyDeviceToHost));
-  nc += 1;
-  col_b.release();
   // rows along the Morton curve of their vertices
   m.patch_rec.alloc(4 * nrows);
   hipLaunchKernelGGL(k_node_xyz, dim3(setup_grid(npos)), dim3(kSetupBlock), 0, s, npos, m.conn.p, m.elem_lid.p,
@@ -571,15 +579,37 @@ void patch_build(Mesh &m, hipStream_t s) {
                      row_s.p, m.patch_rows.p, m.patch_rec.p);
   TXA_HIP(hipGetLastError());
   key.release(); key_s.release(); row.release(); row_s.release(); tmp.release();
-  // instances: (patch, colour, element), sorted
+  // elements in sweep order (their centroids on the lattice, z, y, x)
+  DevBuf<int32_t> rank, by_rank;
+  {
+    DevBuf<uint64_t> ek, ek_s;
+    DevBuf<int32_t> el;
+    ek.alloc(nelem); ek_s.alloc(nelem); el.alloc(nelem); by_rank.alloc(nelem); rank.alloc(nelem);
+    hipLaunchKernelGGL(k_sweep_keys, dim3(setup_grid(nelem)), dim3(kSetupBlock), 0, s, nelem, nb, m.elem_lid.p,
+                       m.patch_rec.p, box, ek.p, el.p);
+    TXA_HIP(hipGetLastError());
+    tmp_bytes = 0;
+    TXA_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, ek.p, ek_s.p, el.p, by_rank.p,
+                                               static_cast<int>(nelem), 0, 63, s));
+    tmp.alloc(tmp_bytes);
+    TXA_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, tmp_bytes, ek.p, ek_s.p, el.p, by_rank.p,
+                                               static_cast<int>(nelem), 0, 63, s));
+    hipLaunchKernelGGL(k_rank_of, dim3(setup_grid(nelem)), dim3(kSetupBlock), 0, s, nelem, by_rank.p, rank.p);
+    TXA_HIP(hipGetLastError());
+    TXA_HIP(hipStreamSynchronize(s));
+    tmp.release();
+  }
+  // instances: (patch, sweep rank), sorted; then coloured patch by patch
   DevBuf<uint64_t> ik, ik_s;
   ik.alloc(npos);
   ik_s.alloc(npos);
-  DevBuf<unsigned long long> nin;
+  DevBuf<unsigned long long> nin, per;
   nin.alloc(1);
+  per.alloc(npatch + 1);
   TXA_HIP(hipMemsetAsync(nin.p, 0, sizeof(unsigned long long), s));
+  TXA_HIP(hipMemsetAsync(per.p, 0, sizeof(unsigned long long) * (npatch + 1), s));
   hipLaunchKernelGGL(k_instance_keys, dim3(setup_grid(nelem)), dim3(kSetupBlock), 0, s, nelem, nb, P, m.elem_lid.p,
-                     m.patch_rec.p, col_a.p, ik.p, nin.p);
+                     m.patch_rec.p, rank.p, ik.p, nin.p, per.p);
   TXA_HIP(hipGetLastError());
   tmp_bytes = 0;
   TXA_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, tmp_bytes, ik.p, ik_s.p, static_cast<int>(npos), 0, 64, s));
@@ -588,6 +618,35 @@ void patch_build(Mesh &m, hipStream_t s) {
   unsigned long long ninst = 0;
   TXA_HIP(hipMemcpyAsync(&ninst, nin.p, sizeof(ninst), hipMemcpyDeviceToHost, s));
   TXA_HIP(hipStreamSynchronize(s));
+  DevBuf<int64_t> pstart;
+  pstart.alloc(npatch + 1);
+  exclusive_scan_i64(reinterpret_cast<const int64_t *>(per.p), pstart.p, npatch + 1, s);
+  DevBuf<unsigned long long> rowmask;
+  rowmask.alloc(npatch * P);
+  TXA_HIP(hipMemsetAsync(rowmask.p, 0, rowmask.bytes(), s));
+  DevBuf<uint8_t> icol;
+  icol.alloc(ninst);
+  DevBuf<int> maxc;
+  maxc.alloc(1);
+  TXA_HIP(hipMemsetAsync(maxc.p, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_patch_colour, dim3(setup_grid(npatch)), dim3(kSetupBlock), 0, s, npatch, nb, P, pstart.p,
+                     ik_s.p, by_rank.p, m.elem_lid.p, m.patch_rec.p, rowmask.p, icol.p, maxc.p);
+  TXA_HIP(hipGetLastError());
+  int nc = 0;
+  TXA_HIP(hipMemcpyAsync(&nc, maxc.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  TXA_HIP(hipStreamSynchronize(s));
+  if (nc >= kMaxColours) {   // more colours than the masks hold: no patch path
+    m.patch_rows.release();
+    m.patch_rec.release();
+    return;
+  }
+  nc += 1;
+  rowmask.release();
+  // final order: (patch, colour, sweep rank)
+  hipLaunchKernelGGL(k_final_keys, dim3(setup_grid(static_cast<int64_t>(ninst))), dim3(kSetupBlock), 0, s,
+                     static_cast<int64_t>(ninst), ik_s.p, icol.p, ik.p);
+  TXA_HIP(hipGetLastError());
+  TXA_HIP(hipcub::DeviceRadixSort::SortKeys(tmp.p, tmp_bytes, ik.p, ik_s.p, static_cast<int>(ninst), 0, 64, s));
   ik.release();
   tmp.release();
   m.patch_inst.alloc(ninst);
@@ -596,7 +655,7 @@ void patch_build(Mesh &m, hipStream_t s) {
   pc.alloc(npatch * nc + 1);
   TXA_HIP(hipMemsetAsync(pc.p, 0, sizeof(unsigned long long) * (npatch * nc + 1), s));
   hipLaunchKernelGGL(k_instances, dim3(setup_grid(static_cast<int64_t>(ninst))), dim3(kSetupBlock), 0, s,
-                     static_cast<int64_t>(ninst), nc, ik_s.p, m.patch_inst.p, m.patch_icol.p, pc.p);
+                     static_cast<int64_t>(ninst), nc, ik_s.p, by_rank.p, m.patch_inst.p, m.patch_icol.p, pc.p);
   TXA_HIP(hipGetLastError());
   m.patch_cstart.alloc(npatch * nc + 1);
   exclusive_scan_i64(reinterpret_cast<const int64_t *>(pc.p), m.patch_cstart.p, npatch * nc + 1, s);
