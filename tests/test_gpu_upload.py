@@ -154,6 +154,64 @@ def test_uploaded_fp32_form(engine, topo, transient):
 
 
 @pytest.mark.parametrize("topo", [txa.HEX8, txa.TET4])
+def test_uploaded_brick_order_reads_by_lid(engine, topo):
+    # a brick uploaded in its own element order numbers its LIDs along the lattice, so the patch
+    # kernel reads the state by LID (no gather pass); the result equals the oracle's
+    n = 9
+    conn, ev, _ = O.brick_mesh(OTOPO[topo], n, n + 1, n + 2)
+    X = synth.brick_coords(n, n + 1, n + 2, jitter=0.1, seed=130 + topo)
+    m = engine.upload(topo, conn, X)
+    try:
+        nd = m.build_dofmap(1)
+        nnz = m.build_graph()
+        pt = m.patches()
+        assert pt is not None and pt["state_in_curve_order"] == 0
+        gids, owned, el = O.dofmap(conn, 1)
+        rp, col = O.graph(el, owned.size)
+        coef = synth.lognormal(131, conn.shape[0], 0.5)
+        x, xd = synth.uniform(132, nd, -1, 1), synth.uniform(133, nd, -1, 1)
+        A = O.Assembler(OTOPO[topo], txa.DIFFUSION, ev, X, el, coef, rp, col, rho=1.5, src=(1.0, 0.0, 0.0))
+        fr, vr = A.evaluate(x, xd, None, alpha=1.5, beta=0.5, gamma2=0.0, transient=True, mode=O.MODE_EXPLICIT)
+        f = torch.empty(nd, device="cuda", dtype=torch.float64)
+        v = torch.empty(nnz, device="cuda", dtype=torch.float64)
+        m.assemble(physics=txa.DIFFUSION, coef=dev(coef), x=dev(x), xdot=dev(xd), f=f, vals=v, rho=1.5, src=(1.0,),
+                   transient=True, alpha=1.5, beta=0.5)
+        torch.cuda.synchronize()
+        assert rel_err(f.cpu().numpy(), fr) <= TOL64 and rel_err(v.cpu().numpy(), vr) <= TOL64
+    finally:
+        m.destroy()
+
+
+@pytest.mark.parametrize("topo", [txa.HEX8, txa.TET4])
+def test_uploaded_state_in_curve_order(engine, topo):
+    # shuffled elements of a larger brick: the first-touch LIDs scatter along the curve, so the
+    # patch kernel gathers the state into curve order before each assembly and reads coordinates
+    # by curve position; transient FP64 against the oracle
+    n = 20 if topo == txa.HEX8 else 16
+    conn, ev, X, nent = renumbered_brick(topo, n, seed=140 + topo)
+    m = engine.upload(topo, conn, X, nentities=nent)
+    try:
+        nd = m.build_dofmap(1)
+        nnz = m.build_graph()
+        pt = m.patches()
+        assert pt is not None and pt["state_in_curve_order"] == 1
+        gids, owned, el = O.dofmap(conn, 1)
+        rp, col = O.graph(el, owned.size)
+        coef = synth.lognormal(141, conn.shape[0], 0.5)
+        x, xd = synth.uniform(142, nd, -1, 1), synth.uniform(143, nd, -1, 1)
+        A = O.Assembler(OTOPO[topo], txa.DIFFUSION, ev, X, el, coef, rp, col, rho=0.7, src=(1.0, 0.0, 0.0))
+        fr, vr = A.evaluate(x, xd, None, alpha=2.0, beta=1.0, gamma2=0.0, transient=True, mode=O.MODE_EXPLICIT)
+        f = torch.empty(nd, device="cuda", dtype=torch.float64)
+        v = torch.empty(nnz, device="cuda", dtype=torch.float64)
+        m.assemble(physics=txa.DIFFUSION, coef=dev(coef), x=dev(x), xdot=dev(xd), f=f, vals=v, rho=0.7, src=(1.0,),
+                   transient=True, alpha=2.0, beta=1.0)
+        torch.cuda.synchronize()
+        assert rel_err(f.cpu().numpy(), fr) <= TOL64 and rel_err(v.cpu().numpy(), vr) <= TOL64
+    finally:
+        m.destroy()
+
+
+@pytest.mark.parametrize("topo", [txa.HEX8, txa.TET4])
 def test_uploaded_accumulate(engine, topo):
     # accumulate = 1 adds the patch rows to what f and vals hold: twice into zeros is twice one call
     conn, ev, X, nent = renumbered_brick(topo, 5, seed=110 + topo)

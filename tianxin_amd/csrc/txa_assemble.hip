@@ -950,7 +950,7 @@ static size_t gather_ke_doubles(const Mesh &m, std::vector<int64_t> *begin) {
 // Workspace of the structured kernels' own parts (the elasticity blocks), before the
 // staging arrays of an accumulating call.
 static size_t kernel_workspace_bytes(const Mesh &m, int physics, int precision, bool want_f, bool want_vals) {
-  if (patch_supported(m, physics, precision)) return 0;
+  if (patch_supported(m, physics, precision)) return patch_workspace_bytes(m);
   if (lane_gather_mesh(m, physics, precision)) return want_vals ? sizeof(double) * gather_ke_doubles(m, nullptr) : 0;
   txa_assembly_args a{};
   a.physics = physics;
@@ -1051,7 +1051,7 @@ void assemble(Mesh &m, const txa_assembly_args &a_in, hipStream_t s, void *ws, i
   } else if (elastic_supported(m, a)) {
     launch_elastic(m, a, s, ws);
   } else if (patch_supported(m, a.physics, a.precision)) {
-    launch_patch(m, a, s);
+    launch_patch(m, a, s, ws);
   } else {
     TXA_REQUIRE(a.precision == TXA_FP64, TXA_ERR_UNSUPPORTED,
                 "reduced-precision integration is implemented for hex8 / tet4 diffusion (bricks and single-block "

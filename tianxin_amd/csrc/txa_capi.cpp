@@ -228,6 +228,7 @@ int txa_mesh_patches(txa_ctx ctx, int mesh_id, int64_t *out) {
     out[2] = on ? m.patch_ninst : 0;
     out[3] = on ? m.patch_NC : 0;
     out[4] = on ? m.patch_RS : 0;
+    out[5] = on && m.patch_bypos ? 1 : 0;
   });
 }
 

@@ -116,8 +116,13 @@ struct Mesh {
   DevBuf<int32_t> patch_inst;     // element of each instance
   DevBuf<uint8_t> patch_icol;     // colour of each instance
   DevBuf<int32_t> patch_rows;     // [npatch*P] row at each curve position, -1 past the last row
-  DevBuf<double> patch_rec;       // [rows][4] vertex x, y, z and the row's curve position (int64 bits)
-  DevBuf<int32_t> patch_ilid;     // [instances][nb] LIDs of each instance's element
+  DevBuf<double> patch_rec;       // [rows][4] vertex x, y, z and the row's curve position, by LID
+  bool patch_bypos = false;       // the LIDs do not follow the curve: state and coordinates read in curve order
+  DevBuf<int32_t> patch_ivtx;     // [instances][nb] vertices of each instance's element: curve positions
+                                  // (patch_bypos) or LIDs
+  DevBuf<double> patch_xyz;       // [npatch*P][4] vertex x, y, z in curve order (patch_bypos)
+  DevBuf<int64_t> patch_r0;       // [npatch*P] first CRS position of the row at each curve position
+  DevBuf<int32_t> patch_len;      // [npatch*P] its length
   DevBuf<uint8_t> patch_islot;    // [instances][nb][nb] position of LID b in the CRS row of LID a
   DevBuf<double> ref;         // reference-element tables of the topology (single-topology meshes)
   DevBuf<double> block_ref[5];   // reference-element tables by topology (TXA_HEX8 .. TXA_WEDGE6)
@@ -232,7 +237,8 @@ void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s
 // uploaded one-field HEX8 / TET4 diffusion by node patches (txa_patch.hip)
 void patch_build(Mesh &m, hipStream_t s);
 bool patch_supported(const Mesh &m, int physics, int precision);
-void launch_patch(const Mesh &m, const txa_assembly_args &a, hipStream_t s);
+size_t patch_workspace_bytes(const Mesh &m);   // the state in curve order
+void launch_patch(const Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws);
 // Neumann flux on listed element sides (device arrays); a side that does not exist sets bit 1 of *err.
 void neumann_sides(const Mesh &m, int64_t n, const int64_t *elems, const int *ords, const double *values, double *f,
                    hipStream_t s, int *err);

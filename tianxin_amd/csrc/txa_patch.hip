@@ -94,14 +94,45 @@ __global__ void k_morton(int64_t nrows, const double *__restrict__ rec, Box b, u
   row[r] = static_cast<int32_t>(r);
 }
 
-// patch_rows[pos] = row (padded with -1), and the row's position in the w slot of its record
+// patch_rows[pos] = row (padded with -1), the row's position in the w slot of its record, and
+// in curve order: the vertex coordinates (xyzp) and the row's CRS range (r0p, lenp)
 __global__ void k_place_rows(int64_t nrows, int64_t npad, const int32_t *__restrict__ sorted, int32_t *__restrict__ prow,
-                             double *__restrict__ rec) {
+                             double *__restrict__ rec, double *__restrict__ xyzp, const int64_t *__restrict__ rowptr,
+                             int64_t *__restrict__ r0p, int32_t *__restrict__ lenp) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= npad) return;
   const int32_t r = i < nrows ? sorted[i] : -1;
   prow[i] = r;
+  for (int d = 0; d < 3; ++d) xyzp[4 * i + d] = r >= 0 ? rec[4 * static_cast<int64_t>(r) + d] : 0.0;
+  xyzp[4 * i + 3] = 0.0;
+  r0p[i] = r >= 0 ? rowptr[r] : 0;
+  lenp[i] = r >= 0 ? static_cast<int32_t>(rowptr[r + 1] - rowptr[r]) : 0;
   if (r >= 0) rec[4 * static_cast<int64_t>(r) + 3] = __longlong_as_double(i);
+}
+
+// Keys (patch, 16-entry line of a by-LID FP64 array) of every row: the distinct keys count the
+// 128-byte lines a patch touches when it reads its state by LID; padding rows get ~0.
+__global__ void k_line_keys(int64_t npad, int P, const int32_t *__restrict__ prow, uint64_t *__restrict__ key) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= npad) return;
+  const int32_t r = prow[i];
+  key[i] = r >= 0 ? (static_cast<uint64_t>(i / P) << 32) | static_cast<uint32_t>(r >> 4) : ~0ull;
+}
+
+__global__ void k_count_distinct(int64_t n, const uint64_t *__restrict__ key, unsigned long long *__restrict__ out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n || key[i] == ~0ull) return;
+  if (i == 0 || key[i - 1] != key[i]) atomicAdd(out, 1ull);
+}
+
+// The solution (and its time derivative) in curve order, before each assembly: xp[pos] = x[row].
+__global__ void k_state_by_pos(int64_t npad, const int32_t *__restrict__ prow, const double *__restrict__ x,
+                               const double *__restrict__ xdot, double *__restrict__ xp, double *__restrict__ xdp) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= npad) return;
+  const int32_t r = prow[i];
+  xp[i] = r >= 0 ? x[r] : 0.0;
+  if (xdp) xdp[i] = r >= 0 ? xdot[r] : 0.0;
 }
 
 // Sweep key of each element: its centroid on the Morton lattice, z then y then x (lexicographic),
@@ -206,19 +237,20 @@ __global__ void k_instances(int64_t ninst, int nc, const uint64_t *__restrict__ 
   atomicAdd(cnt + p * nc + c, 1ull);
 }
 
-// Instance records: the LIDs of each instance's element and its slot map, the position of
-// LID b in the CRS row of LID a (uint8; 255: absent), stored in instance order so that a
-// patch reads its instances as one contiguous stream whatever the element order.
+// Instance records: the vertices of each instance's element (curve positions or LIDs) and its
+// slot map,
+// the position of LID b in the CRS row of LID a (uint8; 255: absent), stored in instance order
+// so that a patch reads its instances as one contiguous stream whatever the element order.
 __global__ void k_inst_records(int64_t ninst, int nb, const int32_t *__restrict__ inst,
                                const int32_t *__restrict__ elem_lid, const int64_t *__restrict__ rowptr,
-                               const int32_t *__restrict__ col, int32_t *__restrict__ ilid,
-                               uint8_t *__restrict__ slot) {
+                               const int32_t *__restrict__ col, const double *__restrict__ rec, int bypos,
+                               int32_t *__restrict__ ivtx, uint8_t *__restrict__ slot) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= ninst * nb) return;
   const int64_t i = t / nb, e = inst[i];
   const int a = static_cast<int>(t - i * nb);
   const int32_t ra = elem_lid[e * nb + a];
-  ilid[t] = ra;
+  ivtx[t] = bypos ? static_cast<int32_t>(__double_as_longlong(rec[4 * static_cast<int64_t>(ra) + 3])) : ra;
   const int64_t r0 = rowptr[ra], n = rowptr[ra + 1] - r0;
   for (int b = 0; b < nb; ++b) {
     const int32_t cb = elem_lid[e * nb + b];
@@ -246,11 +278,19 @@ struct PatchParams {
   const int32_t *inst;         // element of each instance
   const uint8_t *icol;         // colour of each instance
   const int32_t *prow;         // [npatch*P] row at each curve position, -1 past the last
-  const double4 *rec;          // [rows] vertex x, y, z; curve position (int64 bits) in w
-  const int32_t *ilid;         // [instances][NB] LIDs of each instance's element
+  const int64_t *r0p;          // [npatch*P] first CRS position of each row, in curve order
+  const int32_t *lenp;         // [npatch*P] its length
+  // vertices of each instance ([instances][NB]): curve positions when the state is read in curve
+  // order (bypos: coordinates xyzp, state xp / xdp by position), else LIDs (coordinates and
+  // position from rec, state x / xdot by LID)
+  int bypos;
+  const int32_t *ivtx;
+  const double4 *xyzp;         // [npatch*P] vertex x, y, z in curve order (bypos)
+  const double4 *rec;          // [rows] vertex x, y, z, curve position (int64 bits) by LID (otherwise)
   const uint8_t *islot;        // [instances][NB][NB] slot of LID b in the row of LID a
-  const int64_t *rowptr;
-  const double *coef, *x, *xdot;
+  const double *coef;
+  const double *x, *xdot;      // by LID
+  const double *xp, *xdp;      // [npatch*P] in curve order (k_state_by_pos; bypos)
   double alpha, beta, rhoc, src;
   double *f, *vals;
   int add;
@@ -351,7 +391,7 @@ __device__ __forceinline__ void load_lids(const PatchParams &p, int64_t i, int64
   e = 0;
   if (i >= i_end) return;
   e = p.inst[i];
-  const int4 *l4 = reinterpret_cast<const int4 *>(p.ilid + i * NB);
+  const int4 *l4 = reinterpret_cast<const int4 *>(p.ivtx + i * NB);
 #pragma unroll
   for (int q = 0; q < NB / 4; ++q) {
     const int4 t = l4[q];
@@ -364,11 +404,19 @@ __device__ __forceinline__ void load_inst(const PatchParams &p, int64_t i, const
                                           InstIn<NB, TRANS> &in) {
 #pragma unroll
   for (int v = 0; v < NB; ++v) {
-    const double4 r = p.rec[L[v]];
-    in.X[0][v] = r.x; in.X[1][v] = r.y; in.X[2][v] = r.z;
-    in.pos[v] = static_cast<int32_t>(__double_as_longlong(r.w));
-    in.u[v] = p.x[L[v]];
-    if constexpr (TRANS) in.ud[v] = p.xdot[L[v]];
+    if (p.bypos) {   // L: curve positions
+      const double4 r = p.xyzp[L[v]];
+      in.X[0][v] = r.x; in.X[1][v] = r.y; in.X[2][v] = r.z;
+      in.pos[v] = L[v];
+      in.u[v] = p.xp[L[v]];
+      if constexpr (TRANS) in.ud[v] = p.xdp[L[v]];
+    } else {         // L: LIDs
+      const double4 r = p.rec[L[v]];
+      in.X[0][v] = r.x; in.X[1][v] = r.y; in.X[2][v] = r.z;
+      in.pos[v] = static_cast<int32_t>(__double_as_longlong(r.w));
+      in.u[v] = p.x[L[v]];
+      if constexpr (TRANS) in.ud[v] = p.xdot[L[v]];
+    }
   }
   in.k = p.coef[e];
   if (p.vals) {
@@ -383,8 +431,8 @@ __device__ __forceinline__ void load_inst(const PatchParams &p, int64_t i, const
 
 constexpr int kMaxRowsPerThread = 4;   // patches of at most 4 x 256 rows
 
-// One workgroup per patch, its instances in chunks of one per lane.  A chunk's LIDs load
-// while the previous chunk adds its rows.
+// One workgroup per patch, its instances in chunks of one per lane.  A chunk's row positions
+// load while the previous chunk adds its rows; everything is read in curve order.
 template <int TOPO, typename Real, bool TRANS, bool UNIT>
 __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
   constexpr int NB = TOPO == TXA_HEX8 ? 8 : 4;
@@ -402,13 +450,9 @@ __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
   int32_t Ec;
   load_lids<NB>(p, i_begin + tid, i_end, Lc, Ec);
   for (int r = tid; r < P; r += kPatchThreads) {
-    const int32_t L = p.prow[pos0 + r];
-    slid[r] = L;
-    if (L >= 0) {
-      const int64_t r0 = p.rowptr[L];
-      srow0[r] = r0;
-      slen[r] = static_cast<int>(p.rowptr[L + 1] - r0);
-    }
+    slid[r] = p.prow[pos0 + r];
+    srow0[r] = p.r0p[pos0 + r];
+    slen[r] = p.lenp[pos0 + r];
   }
   for (int i = tid; i < P * (RS + 1); i += kPatchThreads) sacc[i] = 0.0;
   __syncthreads();
@@ -575,10 +619,39 @@ void patch_build(Mesh &m, hipStream_t s) {
                                              static_cast<int>(nrows), 0, 63, s));
   const int64_t npatch = (nrows + P - 1) / P;
   m.patch_rows.alloc(npatch * P);
+  m.patch_xyz.alloc(4 * npatch * P);
+  m.patch_r0.alloc(npatch * P);
+  m.patch_len.alloc(npatch * P);
   hipLaunchKernelGGL(k_place_rows, dim3(setup_grid(npatch * P)), dim3(kSetupBlock), 0, s, nrows, npatch * P,
-                     row_s.p, m.patch_rows.p, m.patch_rec.p);
+                     row_s.p, m.patch_rows.p, m.patch_rec.p, m.patch_xyz.p, m.rowptr.p, m.patch_r0.p,
+                     m.patch_len.p);
   TXA_HIP(hipGetLastError());
   key.release(); key_s.release(); row.release(); row_s.release(); tmp.release();
+  {   // read the state in curve order when the patches' rows scatter over the by-LID arrays (more
+      // than 0.3 128-byte lines per row; a brick in its own order touches about 0.19, the same brick
+      // with its elements shuffled about 0.4)
+    const int64_t npad = npatch * P;
+    DevBuf<uint64_t> lk, lk_s;
+    lk.alloc(npad);
+    lk_s.alloc(npad);
+    hipLaunchKernelGGL(k_line_keys, dim3(setup_grid(npad)), dim3(kSetupBlock), 0, s, npad, P, m.patch_rows.p, lk.p);
+    TXA_HIP(hipGetLastError());
+    size_t lb = 0;
+    TXA_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, lb, lk.p, lk_s.p, static_cast<int>(npad), 0, 64, s));
+    DevBuf<char> lt;
+    lt.alloc(lb);
+    TXA_HIP(hipcub::DeviceRadixSort::SortKeys(lt.p, lb, lk.p, lk_s.p, static_cast<int>(npad), 0, 64, s));
+    DevBuf<unsigned long long> nl;
+    nl.alloc(1);
+    TXA_HIP(hipMemsetAsync(nl.p, 0, sizeof(unsigned long long), s));
+    hipLaunchKernelGGL(k_count_distinct, dim3(setup_grid(npad)), dim3(kSetupBlock), 0, s, npad, lk_s.p, nl.p);
+    TXA_HIP(hipGetLastError());
+    unsigned long long h = 0;
+    TXA_HIP(hipMemcpyAsync(&h, nl.p, sizeof(h), hipMemcpyDeviceToHost, s));
+    TXA_HIP(hipStreamSynchronize(s));
+    m.patch_bypos = 10 * static_cast<int64_t>(h) > 3 * nrows;
+  }
+  if (!m.patch_bypos) m.patch_xyz.release();
   // elements in sweep order (their centroids on the lattice, z, y, x)
   DevBuf<int32_t> rank, by_rank;
   {
@@ -660,13 +733,14 @@ void patch_build(Mesh &m, hipStream_t s) {
   m.patch_cstart.alloc(npatch * nc + 1);
   exclusive_scan_i64(reinterpret_cast<const int64_t *>(pc.p), m.patch_cstart.p, npatch * nc + 1, s);
   // instance records
-  m.patch_ilid.alloc(ninst * nb);
+  m.patch_ivtx.alloc(ninst * nb);
   m.patch_islot.alloc(ninst * nb * nb);
   hipLaunchKernelGGL(k_inst_records, dim3(setup_grid(static_cast<int64_t>(ninst) * nb)), dim3(kSetupBlock), 0, s,
                      static_cast<int64_t>(ninst), nb, m.patch_inst.p, m.elem_lid.p, m.rowptr.p, m.col.p,
-                     m.patch_ilid.p, m.patch_islot.p);
+                     m.patch_rec.p, int(m.patch_bypos), m.patch_ivtx.p, m.patch_islot.p);
   TXA_HIP(hipGetLastError());
   TXA_HIP(hipStreamSynchronize(s));
+  if (m.patch_bypos) m.patch_rec.release();   // read by LID only when the state is
   m.patch_P = P;
   m.patch_RS = rs;
   m.patch_NC = nc;
@@ -678,14 +752,31 @@ bool patch_supported(const Mesh &m, int physics, int precision) {
   return m.npatch > 0 && physics == TXA_DIFFUSION && (precision == TXA_FP64 || precision == TXA_FP32);
 }
 
-void launch_patch(const Mesh &m, const txa_assembly_args &a, hipStream_t s) {
+size_t patch_workspace_bytes(const Mesh &m) {
+  return m.npatch > 0 && m.patch_bypos ? 2 * sizeof(double) * static_cast<size_t>(m.npatch) * m.patch_P : 0;
+}
+
+void launch_patch(const Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
   PatchParams pp;
   pp.P = m.patch_P; pp.RS = m.patch_RS; pp.NC = m.patch_NC;
   pp.cstart = m.patch_cstart.p; pp.inst = m.patch_inst.p; pp.icol = m.patch_icol.p;
   pp.prow = m.patch_rows.p;
+  pp.r0p = m.patch_r0.p; pp.lenp = m.patch_len.p;
+  pp.bypos = m.patch_bypos;
+  pp.ivtx = m.patch_ivtx.p; pp.islot = m.patch_islot.p;
+  pp.xyzp = reinterpret_cast<const double4 *>(m.patch_xyz.p);
   pp.rec = reinterpret_cast<const double4 *>(m.patch_rec.p);
-  pp.ilid = m.patch_ilid.p; pp.islot = m.patch_islot.p; pp.rowptr = m.rowptr.p;
   pp.coef = a.coef; pp.x = a.x; pp.xdot = a.xdot;
+  pp.xp = pp.xdp = nullptr;
+  if (m.patch_bypos) {   // the state in curve order, in the workspace
+    const int64_t npad = m.npatch * m.patch_P;
+    TXA_REQUIRE(ws != nullptr, TXA_ERR_LOGIC, "the node-patch assembly needs its workspace");
+    double *xp = static_cast<double *>(ws), *xdp = a.transient ? xp + npad : nullptr;
+    hipLaunchKernelGGL(k_state_by_pos, dim3(setup_grid(npad)), dim3(kSetupBlock), 0, s, npad, m.patch_rows.p, a.x,
+                       a.xdot, xp, xdp);
+    TXA_HIP(hipGetLastError());
+    pp.xp = xp; pp.xdp = xdp;
+  }
   pp.alpha = a.alpha; pp.beta = a.beta; pp.rhoc = a.rho; pp.src = a.src[0];
   pp.f = a.f; pp.vals = a.vals; pp.add = a.accumulate != 0;
   pp.npatch = m.npatch;
