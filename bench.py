@@ -1072,6 +1072,13 @@ def summarize_upload(stats_dir, fetch_dir, write_dir, util_dir, bench_json):
             if part:
                 lines.append(f"| `{k}` | {name} | {len(part)} | {np.mean(part):.1f} | {np.min(part):.1f} | "
                              f"{np.max(part):.1f} |")
+    d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in trace if "k_state_by_pos" in r["Kernel_Name"]]
+    if d:
+        lines.append(f"| `k_state_by_pos` (the state in curve order, before each assembly) | shuffled | {len(d)} | "
+                     f"{np.mean(d):.1f} | {np.min(d):.1f} | {np.max(d):.1f} |")
+    lines += ["", "The brick in its own order reads the state by LID (its rows touch 0.19 lines of 128 bytes",
+              "per row along the curve); with shuffled elements the LIDs scatter (0.41 lines per row) and",
+              "the state is gathered into curve order first (`txa_mesh_patches` reports the choice)."]
     kern = "k_patch<1, double, false, true>"
 
     def per_dispatch(d, counter):
