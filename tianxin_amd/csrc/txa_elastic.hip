@@ -133,6 +133,10 @@ struct Dims {
   // LDS row stride in doubles (the 16 rows of an operand read fall on distinct bank pairs)
   static constexpr int KP64 = NB == 8 ? 8 : 28;
   static constexpr int KS64 = KP64 + 1;
+  // FP32 form: the same padding for v_mfma_f32_16x16x4f32; a row stride in floats whose 16
+  // multiples are distinct multiples of 4 modulo the 64 banks, so the 16 rows x 4 points of an
+  // operand read fall on 64 distinct banks
+  static constexpr int KS32 = NB == 8 ? 12 : 36;
 };
 
 template <int NB, int NQ, int MODE>
@@ -162,13 +166,15 @@ struct Smem {
   union alignas(16) {
     struct {
       TReal T[VR][TP];
-      TReal G[MODE == MODE_F32 ? NQ * 3 * NB : 1];
+      TReal G[1];
     } v;
     short ops[3 * D::MP * D::KS];
     float T32[D::ND][D::ND + 1];
     // FP64 form: the operands of T = A^T B, A = (w det sgn) G and B = G, rows (k, a) padded
     // to MP, points padded to KP64 (zero); T goes over them once the products are in registers
     double opd[MODE == MODE_F64 ? 2 * D::MP * D::KS64 : 1];
+    // FP32 form: the same operands in FP32 for v_mfma_f32_16x16x4f32
+    float opf[MODE == MODE_F32 ? 2 * D::MP * D::KS32 : 1];
   } w;
 };
 
@@ -312,6 +318,10 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
       constexpr int PADD = (D::MP - ND) * D::KS64;   // doubles per operand
       for (int i = t; i < 2 * PADD; i += BS) sm.w.opd[((i / PADD) * D::MP + ND) * D::KS64 + i % PADD] = 0.0;
     }
+    if constexpr (MODE == MODE_F32) {
+      constexpr int PADF = (D::MP - ND) * D::KS32;   // floats per operand
+      for (int i = t; i < 2 * PADF; i += BS) sm.w.opf[((i / PADF) * D::MP + ND) * D::KS32 + i % PADF] = 0.0f;
+    }
     __syncthreads();
     // measure w |det J| with the sign of the first point (Intrepid2 computeCellMeasure)
     const double sg = sm.det[0] < 0.0 ? -1.0 : 1.0;
@@ -331,7 +341,7 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
     __syncthreads();
     // ---- physical gradients at the quadrature points (MFMA mode: also the zero columns q >= NQ);
     // q runs fastest over the threads, so the operand writes of a wave are consecutive in LDS
-    constexpr int QG = MODE == MODE_SPLIT ? D::KP : (MODE == MODE_F64 ? D::KP64 : NQ);
+    constexpr int QG = MODE == MODE_SPLIT ? D::KP : D::KP64;
     // when BS is a multiple of QG every thread keeps one q: its J^-1 and sqrt(w |det J|)
     // are read from LDS once
     constexpr bool QFIX = BS % QG == 0;
@@ -361,6 +371,14 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
         }
         continue;
       }
+      if (MODE == MODE_F32 && q >= NQ) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          sm.w.opf[(0 * D::MP + k * NB + a) * D::KS32 + q] = 0.0f;
+          sm.w.opf[(1 * D::MP + k * NB + a) * D::KS32 + q] = 0.0f;
+        }
+        continue;
+      }
       const int qx = q % Q1, qy = (q / Q1) % Q1, qz = q / (Q1 * Q1);
       const int ax = sm.off[a][0], ay = sm.off[a][1], az = sm.off[a][2];
       const double lx = sm.bL[ax][qx], ly = sm.bL[ay][qy], lz = sm.bL[az][qz];
@@ -385,7 +403,10 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
           sm.w.opd[(0 * D::MP + m) * D::KS64 + q] = sg * sm.wm[q] * g;
           sm.w.opd[(1 * D::MP + m) * D::KS64 + q] = g;
         } else {
-          sm.w.v.G[(q * 3 + k) * NB + a] = static_cast<TReal>(g);
+          // T[(k,a),(l,b)] = sum_q (sgn w det g_ka) g_lb, operands rounded to FP32 once
+          const int m = k * NB + a;
+          sm.w.opf[(0 * D::MP + m) * D::KS32 + q] = static_cast<float>(sg * sm.wm[q] * g);
+          sm.w.opf[(1 * D::MP + m) * D::KS32 + q] = static_cast<float>(g);
         }
       }
     }
@@ -485,13 +506,43 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
         }
       }
     } else {
-      for (int i = t; i < ND * ND; i += BS) {
-        const int rr = i / ND, cc = i % ND;
-        const int kr = rr / NB, ar = rr % NB, kc = cc / NB, ac = cc % NB;
-        TReal s = 0;
-        for (int q = 0; q < NQ; ++q)
-          s += static_cast<TReal>(sm.wm[q]) * sm.w.v.G[(q * 3 + kr) * NB + ar] * sm.w.v.G[(q * 3 + kc) * NB + ac];
-        sm.w.v.T[rr][cc] = static_cast<TReal>(sg) * s;
+      // T = A^T B on v_mfma_f32_16x16x4f32 over the upper-triangle 16x16 tiles: lane l holds A
+      // and B element [row l % 16][point 4 ks + l / 16]; D[4 (l / 16) + i][l % 16] in component i
+      constexpr int NT = D::MP / 16;
+      constexpr int NTILE = NT * (NT + 1) / 2;
+      constexpr int TPW = (NTILE + BS / 64 - 1) / (BS / 64);
+      const float *A = sm.w.opf, *B = sm.w.opf + D::MP * D::KS32;
+      v4f acc[TPW];
+#pragma unroll
+      for (int k = 0; k < TPW; ++k) {
+        acc[k] = v4f{0.f, 0.f, 0.f, 0.f};
+        const int ti = wave + k * (BS / 64);
+        if (ti >= NTILE) continue;
+        int tm = 0, rem = ti;
+        while (rem >= NT - tm) { rem -= NT - tm; ++tm; }
+        const int tn = tm + rem;
+        const float *ar = A + (tm * 16 + (lane & 15)) * D::KS32 + (lane >> 4);
+        const float *br = B + (tn * 16 + (lane & 15)) * D::KS32 + (lane >> 4);
+#pragma unroll
+        for (int ks = 0; ks < D::KP64 / 4; ++ks)
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[4 * ks], br[4 * ks], acc[k], 0, 0, 0);
+      }
+      __syncthreads();   // every wave has read the operands: T goes over them
+#pragma unroll
+      for (int k = 0; k < TPW; ++k) {
+        const int ti = wave + k * (BS / 64);
+        if (ti >= NTILE) continue;
+        int tm = 0, rem = ti;
+        while (rem >= NT - tm) { rem -= NT - tm; ++tm; }
+        const int tn = tm + rem;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int rr = tm * 16 + 4 * (lane >> 4) + i, cc = tn * 16 + (lane & 15);
+          if (rr < ND && cc < ND) {
+            sm.w.v.T[rr][cc] = acc[k][i];
+            if (tm != tn) sm.w.v.T[cc][rr] = acc[k][i];
+          }
+        }
       }
     }
     __syncthreads();
