@@ -72,6 +72,8 @@ __device__ const int8_t kQ2[27][3] = {
     {1, 0, 1}, {2, 1, 1}, {1, 2, 1}, {0, 1, 1}, {1, 1, 0}, {1, 1, 2}, {1, 1, 1}};
 
 typedef short v4s __attribute__((ext_vector_type(4)));
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 
@@ -426,23 +428,26 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
         int tm = 0, rem = ti;
         while (rem >= NT - tm) { rem -= NT - tm; ++tm; }
         const int tn = tm + rem;
+        // gfx950's K = 32 bf16 product: lane l brings eight points of row l % 16, the eighth
+        // 8 (l / 16) .. 8 (l / 16) + 7, of A and of B alike (whatever order the instruction gives
+        // the eight, the sum over all 32 points is the same)
 #pragma unroll
-        for (int ks = 0; ks < D::KP / 16; ++ks) {
-          const int k0 = ks * 16 + 4 * (lane >> 4);
+        for (int ks = 0; ks < D::KP / 32; ++ks) {
+          const int k0 = ks * 32 + 8 * (lane >> 4);
           const int ma = tm * 16 + (lane & 15), nb = tn * 16 + (lane & 15);
-          v4s a_[3], b_[3];
+          v8bf a_[3], b_[3];
 #pragma unroll
           for (int s = 0; s < 3; ++s) {
-            a_[s] = *reinterpret_cast<const v4s *>(O + (s * D::MP + ma) * D::KS + k0);
-            b_[s] = *reinterpret_cast<const v4s *>(O + (s * D::MP + nb) * D::KS + k0);
+            a_[s] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v8s *>(O + (s * D::MP + ma) * D::KS + k0));
+            b_[s] = __builtin_bit_cast(v8bf, *reinterpret_cast<const v8s *>(O + (s * D::MP + nb) * D::KS + k0));
           }
           // products of order 2^-16, 2^-8 and 1 (smallest first)
-          acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[2], b_[0], acc[k], 0, 0, 0);
-          acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[1], b_[1], acc[k], 0, 0, 0);
-          acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[0], b_[2], acc[k], 0, 0, 0);
-          acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[1], b_[0], acc[k], 0, 0, 0);
-          acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[0], b_[1], acc[k], 0, 0, 0);
-          acc[k] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a_[0], b_[0], acc[k], 0, 0, 0);
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_[2], b_[0], acc[k], 0, 0, 0);
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_[1], b_[1], acc[k], 0, 0, 0);
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_[0], b_[2], acc[k], 0, 0, 0);
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_[1], b_[0], acc[k], 0, 0, 0);
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_[0], b_[1], acc[k], 0, 0, 0);
+          acc[k] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_[0], b_[0], acc[k], 0, 0, 0);
         }
       }
       __syncthreads();   // every wave has read the operands: T goes over them
