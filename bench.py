@@ -950,10 +950,11 @@ def summarize_q2(stats_dir, pmc_dir):
         lines.append(f"| {name} | {100 * util:.1f}% | {avg('SQ_INSTS_VALU') / nel:.0f} | "
                      f"{avg('SQ_INSTS_VALU_MFMA_MOPS_BF16') / nel:.3g} | {avg('SQ_INSTS_VALU_MFMA_MOPS_F64') / nel:.3g} | "
                      f"{avg('SQ_INSTS_LDS') / nel:.0f} |")
-    lines += ["", "The split-BF16 form runs six bf16 products per 16x16 tile and k-step (21 upper-triangle",
-              "tiles, 2 k-steps), the FP64 form one v_mfma_f64_16x16x4f64 and the FP32 form one",
-              "v_mfma_f32_16x16x4f32 per tile and k-step (21 tiles, 7 k-steps; its MOPs are counted by",
-              "neither column).  The element kernels run one element per workgroup through barrier-separated",
+    lines += ["", "The split-BF16 form runs six v_mfma_f32_16x16x32_bf16 products per 16x16 tile (21",
+              "upper-triangle tiles, one K = 32 step over the 27 points), the FP64 form one",
+              "v_mfma_f64_16x16x4f64 and the FP32 form one v_mfma_f32_16x16x4f32 per tile and k-step (21",
+              "tiles, 7 k-steps; the FP32 MOPs are counted by neither column).  The element kernels run one",
+              "element per workgroup through barrier-separated",
               "phases (loads, geometry, gradients, Gram matrix, pair blocks, block store), so per-element",
               "latency, not the MFMA rate, sets their time; the gather moves the workspace blocks in and",
               "the CRS values out."]
