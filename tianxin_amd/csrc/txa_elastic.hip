@@ -776,7 +776,7 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
     }
     gather_lds_sync();   // all additions done before the rows are read
     if (g.vals)
-      for (int k = lane; k < 3 * len; k += 64) g.vals[base + k] = b[k];
+      for (int k = lane; k < 3 * len; k += 64) __builtin_nontemporal_store(b[k], g.vals + base + k);
     if (g.f && lane < 3) g.f[L0 + lane] = racc;
     gather_lds_sync();   // the buffer is reused by the next node
     Lc = Ln;
@@ -871,7 +871,7 @@ __global__ __launch_bounds__(256) void k_elastic_gather_u(GatherUParams g) {
     }
     gather_lds_sync();
     if (g.vals)
-      for (int k = lane; k < 3 * len; k += 64) g.vals[base + k] = b[k];
+      for (int k = lane; k < 3 * len; k += 64) __builtin_nontemporal_store(b[k], g.vals + base + k);
     if (g.f && lane < 3) g.f[L0 + lane] = racc;
     gather_lds_sync();
   }

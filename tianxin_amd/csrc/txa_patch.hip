@@ -562,7 +562,8 @@ void patch_build(Mesh &m, hipStream_t s) {
   m.npatch = 0;
   m.patch_P = 0;
   const bool fits = !m.structured && m.nfields == 1 && m.blocks.size() == 1 &&
-                    (m.topo == TXA_HEX8 || m.topo == TXA_TET4) && m.occ.p && m.nelem < (int64_t(1) << 31);
+                    (m.topo == TXA_HEX8 || m.topo == TXA_TET4) && m.occ.p &&
+                    m.nelem * (m.topo == TXA_HEX8 ? 8 : 4) < (int64_t(1) << 31);   // sort lengths are int
   if (!fits) return;
   const int nb = m.topo == TXA_HEX8 ? 8 : 4;
   const int64_t nrows = m.ndofs, nelem = m.nelem, npos = nelem * nb;
