@@ -95,8 +95,11 @@ int txa_mesh_brick(txa_ctx ctx, int topo, int nx, int ny, int nz, const double *
  * as in the brick).  Vertex entries are vertex ids in [0, nverts); the others
  * any ids below nentities (nentities = nverts for vertex-only topologies).
  * vertex_coords host [nverts][3].  The DOF map and graph follow the same rules as
- * for a brick; the element work runs on the generic kernel in FP64 (the lattice
- * kernels, Neumann sides and batches need a brick).
+ * for a brick.  The assembly of a single-block mesh takes the node-patch kernels
+ * (one-field hex8 / tet4 diffusion, FP64 or FP32; see txa_mesh_patches) or the
+ * element kernels with a general node gather (3-field hex8 / hex27 elasticity, every
+ * precision); other meshes take the generic kernels in FP64.  Brick side sets
+ * (TXA_SIDE_*) and batches need a brick; listed sides (txa_neumann_sides) do not.
  * Replaces the connectivity STKConnManager / ConnManager hands to the DOF manager
  * (adapters-stk/src/Panzer_STKConnManager.cpp:243-300). */
 int txa_mesh_upload(txa_ctx ctx, int topo, int64_t nelem, const int64_t *conn, int64_t nentities, int64_t nverts,
@@ -305,8 +308,9 @@ int txa_model_destroy(txa_ctx ctx, int model_id);
 /* txa_assemble without any allocation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc).  The assembly workspace (the
- * element blocks of the two-phase elasticity kernels and the element matrices of
- * the uploaded-mesh row gather) comes from one static device array of 128 MiB per
+ * element blocks of the two-phase elasticity kernels, the element matrices of the
+ * multi-block row gather, the curve-ordered state of a scattered node-patch mesh)
+ * comes from one static device array of 128 MiB per
  * device instead of the context's pool; an assembly that needs more is refused with
  * TXA_ERR_NOMEM before anything is launched.  Results equal txa_assemble's.  Every
  * call on one device shares that array: calls whose work overlaps on one GPU (other
