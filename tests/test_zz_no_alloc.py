@@ -1,10 +1,11 @@
 """GPU: txa_assemble_noalloc (the allocation-free variant of txa_assemble) against
 txa_assemble on the same inputs, through the C ABI.
 
-Both kernels that take a workspace are covered: the uploaded-mesh row gather
+Every kernel that takes a workspace is covered: the uploaded-mesh row gather
 (one-field hex8 diffusion on a mesh of two element blocks, element matrices in the
-workspace; a single-block mesh takes the node-patch kernel, which needs none) and the
-two-phase elasticity kernels (element blocks in the workspace).  Each case runs twice in a
+workspace), the node-patch kernel on a shuffled single-block mesh (the state gathered
+into curve order in the workspace) and the two-phase elasticity kernels (element
+blocks in the workspace).  Each case runs twice in a
 row, the second time on a smaller mesh, so the second call reuses the static
 workspace the first call left behind.  Matrix values must be equal bit for bit
 (both gathers sum every row in a fixed order); residuals equal bit for bit on the
@@ -61,6 +62,30 @@ def test_noalloc_uploaded_hex8_diffusion(engine, n):
         (f0, v0), (f1, v1) = both(m, physics=txa.DIFFUSION, coef=k, x=x, src=(1.0,), bc_lids=bc)
         assert np.isfinite(v1).all() and np.array_equal(v0, v1)
         assert np.max(np.abs(f1 - f0)) <= 1e-14 * np.max(np.abs(f0))
+    finally:
+        m.destroy()
+
+
+@pytest.mark.parametrize("n", [20, 16])
+def test_noalloc_node_patches_in_curve_order(engine, n):
+    # a shuffled single-block hex8 mesh: the node-patch kernel gathers the state into curve order
+    # in the workspace first; both entry points sum every row in the same fixed order
+    print(f"[noalloc] node patches, shuffled hex8 {n}x{n + 1}x{n + 2}", flush=True)
+    conn, X = shuffled_hex8(n, 90 + n)
+    m = engine.upload(txa.HEX8, conn, X)
+    try:
+        m.build_dofmap(1)
+        m.build_graph()
+        pt = m.patches()
+        assert pt is not None and pt["state_in_curve_order"] == 1
+        k = dev(synth.lognormal(91, m.nelem, 0.5))
+        x = dev(synth.uniform(92, m.ndofs, -1, 1))
+        xd = dev(synth.uniform(93, m.ndofs, -1, 1))
+        bc = dev(np.arange(0, m.ndofs, 5, dtype=np.int32), torch.int32)
+        (f0, v0), (f1, v1) = both(m, physics=txa.DIFFUSION, coef=k, x=x, xdot=xd, transient=True, alpha=2.0,
+                                  rho=1.5, src=(1.0,), bc_lids=bc)
+        assert np.isfinite(f1).all() and np.isfinite(v1).all()
+        assert np.array_equal(f0, f1) and np.array_equal(v0, v1)
     finally:
         m.destroy()
 
