@@ -1130,9 +1130,9 @@ def summarize_upload(stats_dir, fetch_dir, write_dir, util_dir, bench_json):
 
 
 def other_workloads(args):
-    """Compact summaries of the Q2, batch and serving workloads at their own default sizes,
-    appended to the headline line (each run after the headline's buffers are freed; a
-    failure is recorded in place of its summary and leaves the headline intact)."""
+    """Compact summaries of the Q2, batch, serving, hex8 elasticity and uploaded-mesh workloads at
+    their own default sizes, appended to the headline line (each run after the headline's buffers
+    are freed; a failure is recorded in place of its summary and leaves the headline intact)."""
     import copy
     import torch
 
@@ -1143,7 +1143,8 @@ def other_workloads(args):
     for name, fn, over in (("q2_elasticity_64", bench_q2, dict(n=64, steps=5, warmup=1)),
                            ("batch_512_meshes", bench_batch, dict(meshes=512, steps=5)),
                            ("serve_hex8_newmark", bench_serve, dict(requests=48, pool=8)),
-                           ("elastic_hex8_128", bench_elastic, dict(n=128, steps=5, warmup=1))):
+                           ("elastic_hex8_128", bench_elastic, dict(n=128, steps=5, warmup=1)),
+                           ("upload_hex8_128", bench_upload, dict(n=128, steps=10, warmup=2))):
         a = copy.copy(args)
         for k, v in over.items():
             setattr(a, k, v)
@@ -1174,6 +1175,10 @@ def other_workloads(args):
             s["cells_per_s"] = f["fp32_batch"]["cells_per_s"]
             s["max_rel_err_fp32_vs_fp64_per_mesh"] = line["max_rel_err_fp32_vs_fp64_per_mesh"]
             s["speedup_vs_fp64_per_mesh"] = line["speedup_vs_fp64_per_mesh"]
+        elif fn is bench_upload:
+            s["orders"] = {k: {"ms_per_step": v["ms_per_step"], "fp32_ms_per_step": v["fp32"]["ms_per_step"],
+                               "patches": v["patches"]} for k, v in line["orders"].items()}
+            s["checks"] = line["checks"]
         elif fn is bench_elastic:
             s["cells"] = {k: ({"ms_per_step": v["ms_per_step"], "hbm_frac": v["hbm_frac"]} if "hbm_frac" in v
                               else {o: c["ms_per_step"] for o, c in v.items()})
