@@ -970,11 +970,11 @@ __global__ void k_node_perm(int64_t px, int64_t py, int64_t pz, int R, int nf, i
 struct Elastic8Params {
   int nx, ny, nz;
   int64_t npoints;
-  const double *X, *Y, *Z;
   const int32_t *point_lid;   // [points][3]
   const int64_t *rowptr;
   const uint16_t *nperm;
   int nperm_w;
+  const double *geom;         // [cells][10]: cofactors of J scaled by |det J|^-1/2, then |det J|
   const double *coef;         // Young's modulus per cell
   double nu, rho, src[3], ra, rb;
   const double *x, *xdot, *xddot;
@@ -983,10 +983,55 @@ struct Elastic8Params {
   double *f, *vals;
 };
 
+// Geometry of each parallelepiped cell, once per mesh: its constant Jacobian
+// J[d][j] = (1/8) sum_b s_b^j X_b^d (s_b^j = -1 / +1 for vertex b's j-th bit 0 / 1), the
+// cofactor matrix C (adj J = C^T) and |det J|, stored as C |det J|^-1/2 and |det J|, so that
+// C S C^T / |det J| = Chat S Chat^T in the node kernel.  One thread per cell.
+__global__ void k_cell_geom(int nx, int ny, int nz, int64_t nv, const double *__restrict__ xyz,
+                            double *__restrict__ geom) {
+  const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= static_cast<int64_t>(nx) * ny * nz) return;
+  const int64_t i = c % nx, j = (c / nx) % ny, k = c / (static_cast<int64_t>(nx) * ny);
+  double Jm[3][3] = {};
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
+    const int64_t v = (i + bx) + (nx + 1) * ((j + by) + static_cast<int64_t>(ny + 1) * (k + bz));
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double X = xyz[d * nv + v];
+      Jm[d][0] += bx ? X : -X;
+      Jm[d][1] += by ? X : -X;
+      Jm[d][2] += bz ? X : -X;
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int jj = 0; jj < 3; ++jj) Jm[d][jj] *= 0.125;
+  double C[9];
+  C[0] = Jm[1][1] * Jm[2][2] - Jm[1][2] * Jm[2][1];
+  C[1] = Jm[1][2] * Jm[2][0] - Jm[1][0] * Jm[2][2];
+  C[2] = Jm[1][0] * Jm[2][1] - Jm[1][1] * Jm[2][0];
+  C[3] = Jm[0][2] * Jm[2][1] - Jm[0][1] * Jm[2][2];
+  C[4] = Jm[0][0] * Jm[2][2] - Jm[0][2] * Jm[2][0];
+  C[5] = Jm[0][1] * Jm[2][0] - Jm[0][0] * Jm[2][1];
+  C[6] = Jm[0][1] * Jm[1][2] - Jm[0][2] * Jm[1][1];
+  C[7] = Jm[0][2] * Jm[1][0] - Jm[0][0] * Jm[1][2];
+  C[8] = Jm[0][0] * Jm[1][1] - Jm[0][1] * Jm[1][0];
+  const double adet = fabs(Jm[0][0] * C[0] + Jm[0][1] * C[1] + Jm[0][2] * C[2]);
+  const double sc = 1.0 / sqrt(adet);
+  double *g = geom + 10 * c;
+#pragma unroll
+  for (int t = 0; t < 9; ++t) g[t] = C[t] * sc;
+  g[9] = adet;
+}
+
 __global__ __launch_bounds__(256, 3) void k_elastic8_affine(Elastic8Params p) {
   constexpr int MAXLEN = 81;   // entries per row: 27 neighbours x 3 fields
   __shared__ double rows[4][3 * MAXLEN];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // the wave index is uniform: the node's lattice arithmetic runs on the scalar unit
+  const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int c = lane >> 3, b = lane & 7;
   const int ox = c & 1, oy = (c >> 1) & 1, oz = c >> 2;   // cell (I-1+ox, J-1+oy, K-1+oz)
   const int bxyz[3] = {b & 1, (b >> 1) & 1, b >> 2};
@@ -1016,13 +1061,13 @@ __global__ __launch_bounds__(256, 3) void k_elastic8_affine(Elastic8Params p) {
   const bool tra = tr && p.xddot;
   const double lamf = p.nu / ((1.0 + p.nu) * (1.0 - 2.0 * p.nu)), muf = 0.5 / (1.0 + p.nu);
   const int stride = static_cast<int>(gridDim.x) * 4;
-  // Loads come in two rounds per node (the lane's vertex coordinates, LID and the cell's
+  // Loads come in two rounds per node (the lane's vertex LID, its cell's geometry and
   // modulus, the node's LID; then the state by that LID and the node's row pointers and
   // slot map).  The first round of the next node is issued before this node's arithmetic,
   // the second round of this node as its iteration starts.
   struct Round1 {
     int I, J, K;
-    double X[3];
+    double2 G[5];
     int32_t Lb, L0;
     double E;
   };
@@ -1036,12 +1081,13 @@ __global__ __launch_bounds__(256, 3) void k_elastic8_affine(Elastic8Params p) {
     // this lane's vertex (clamped into the mesh for absent cells: loaded, never used)
     const int vx = min(max(r.I + dx, 0), p.nx), vy = min(max(r.J + dy, 0), p.ny), vz = min(max(r.K + dz, 0), p.nz);
     const int vi = vx + sx * vy + sxy * vz;
-    r.X[0] = p.X[vi];
-    r.X[1] = p.Y[vi];
-    r.X[2] = p.Z[vi];
     r.Lb = p.point_lid[3 * vi];
     r.L0 = p.point_lid[3 * qc];
-    r.E = p.coef[cell_ok ? cx + p.nx * (cy + p.ny * cz) : 0];
+    const int cell = cell_ok ? cx + p.nx * (cy + p.ny * cz) : 0;
+    r.E = p.coef[cell];
+    const double2 *g = reinterpret_cast<const double2 *>(p.geom + 10 * static_cast<int64_t>(cell));
+#pragma unroll
+    for (int t = 0; t < 5; ++t) r.G[t] = g[t];
   };
   int pt = static_cast<int>(blockIdx.x) * 4 + w;
   Round1 cur, nxt;
@@ -1066,38 +1112,22 @@ __global__ __launch_bounds__(256, 3) void k_elastic8_affine(Elastic8Params p) {
     // first round of the next node
     round1(pt + stride, nxt);
     __builtin_amdgcn_sched_barrier(0);
-    const double Xb[3] = {cur.X[0], cur.X[1], cur.X[2]};
     const double E = cur.E;
-    // J[d][j] = (1/8) sum_b s_b^j X_b^d over the cell's eight lanes (DPP: every lane the same bits)
-    double Jm[3][3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) Jm[d][j] = 0.125 * group_sum<8>(bxyz[j] ? Xb[d] : -Xb[d]);
-    double C[3][3];
-    C[0][0] = Jm[1][1] * Jm[2][2] - Jm[1][2] * Jm[2][1];
-    C[0][1] = Jm[1][2] * Jm[2][0] - Jm[1][0] * Jm[2][2];
-    C[0][2] = Jm[1][0] * Jm[2][1] - Jm[1][1] * Jm[2][0];
-    C[1][0] = Jm[0][2] * Jm[2][1] - Jm[0][1] * Jm[2][2];
-    C[1][1] = Jm[0][0] * Jm[2][2] - Jm[0][2] * Jm[2][0];
-    C[1][2] = Jm[0][1] * Jm[2][0] - Jm[0][0] * Jm[2][1];
-    C[2][0] = Jm[0][1] * Jm[1][2] - Jm[0][2] * Jm[1][1];
-    C[2][1] = Jm[0][2] * Jm[1][0] - Jm[0][0] * Jm[1][2];
-    C[2][2] = Jm[0][0] * Jm[1][1] - Jm[0][1] * Jm[1][0];
-    const double det = Jm[0][0] * C[0][0] + Jm[0][1] * C[0][1] + Jm[0][2] * C[0][2];
-    const double adet = fabs(det);
-    // T = C S C^T / |det J|
+    // T = C S C^T / |det J| = Chat S Chat^T with the cell's scaled cofactors
+    const double Ch[3][3] = {{cur.G[0].x, cur.G[0].y, cur.G[1].x},
+                             {cur.G[1].y, cur.G[2].x, cur.G[2].y},
+                             {cur.G[3].x, cur.G[3].y, cur.G[4].x}};
+    const double adet = cur.G[4].y;
     double P[3][3];
 #pragma unroll
     for (int m = 0; m < 3; ++m)
 #pragma unroll
-      for (int l = 0; l < 3; ++l) P[m][l] = S[m][0] * C[l][0] + S[m][1] * C[l][1] + S[m][2] * C[l][2];
-    const double ia = 1.0 / adet;
+      for (int l = 0; l < 3; ++l) P[m][l] = S[m][0] * Ch[l][0] + S[m][1] * Ch[l][1] + S[m][2] * Ch[l][2];
     double T[3][3];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
-      for (int l = 0; l < 3; ++l) T[k][l] = ia * (C[k][0] * P[0][l] + C[k][1] * P[1][l] + C[k][2] * P[2][l]);
+      for (int l = 0; l < 3; ++l) T[k][l] = Ch[k][0] * P[0][l] + Ch[k][1] * P[1][l] + Ch[k][2] * P[2][l];
     const double lam = E * lamf, mu = E * muf;
     const double trs = T[0][0] + T[1][1] + T[2][2];
     const double Mab = tr ? p.rho * adet * mf : 0.0;
@@ -1149,7 +1179,7 @@ __global__ __launch_bounds__(256, 3) void k_elastic8_affine(Elastic8Params p) {
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (p.vals)
-      for (int k = lane; k < 3 * len; k += 64) p.vals[base + k] = rows[w][k];
+      for (int k = lane; k < 3 * len; k += 64) __builtin_nontemporal_store(rows[w][k], &p.vals[base + k]);
     if (p.f && lane < 3) p.f[L0 + lane] = rr[lane];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1211,6 +1241,7 @@ size_t elastic_workspace_bytes(const Mesh &m, int precision, bool want_f, bool w
 }
 
 void node_perm_build(Mesh &m, hipStream_t s) {
+  m.cell_geom.release();
   m.uocc_ptr.release();
   m.uocc.release();
   m.uslot.release();
@@ -1261,19 +1292,27 @@ void node_perm_build(Mesh &m, hipStream_t s) {
   hipLaunchKernelGGL(k_node_perm, dim3(static_cast<unsigned>((nt + 255) / 256)), dim3(256), 0, s, px, py, pz, R,
                      m.nfields, m.nperm_w, m.point_lid.p, m.rowptr.p, m.col.p, m.nperm.p);
   TXA_HIP(hipGetLastError());
+  if (affine_hex8(m)) {
+    const int64_t nc = static_cast<int64_t>(m.nx) * m.ny * m.nz;
+    m.cell_geom.alloc(static_cast<size_t>(nc) * 10);
+    hipLaunchKernelGGL(k_cell_geom, dim3(static_cast<unsigned>((nc + 255) / 256)), dim3(256), 0, s, m.nx, m.ny, m.nz,
+                       m.nverts, m.xyz.p, m.cell_geom.p);
+    TXA_HIP(hipGetLastError());
+  }
 }
 
 void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws) {
   if (!a.f && !a.vals) return;
   if (affine_hex8(m)) {
+    TXA_REQUIRE(m.cell_geom.p != nullptr, TXA_ERR_LOGIC, "cell geometry missing (graph_build)");
     Elastic8Params q;
     q.nx = m.nx; q.ny = m.ny; q.nz = m.nz;
     q.npoints = m.npoints;
-    q.X = m.xyz.p; q.Y = m.xyz.p + m.nverts; q.Z = m.xyz.p + 2 * m.nverts;
     q.point_lid = m.point_lid.p;
     q.rowptr = m.rowptr.p;
     q.nperm = m.nperm.p;
     q.nperm_w = m.nperm_w;
+    q.geom = m.cell_geom.p;
     q.coef = a.coef;
     q.nu = a.nu; q.rho = a.rho;
     q.src[0] = a.src[0]; q.src[1] = a.src[1]; q.src[2] = a.src[2];

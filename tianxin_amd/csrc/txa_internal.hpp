@@ -93,12 +93,14 @@ struct Mesh {
   // graph
   bool has_graph = false;
   int64_t nnz = 0;
+  int max_row = 0;            // entries in the longest CRS row
   DevBuf<int64_t> rowptr;
   DevBuf<int32_t> col;
   int perm_width = 0;         // bytes per row of the stencil slot map (27*nfields rounded up to 16)
   DevBuf<uint8_t> perm;       // [ndofs][perm_width] position of each stencil neighbour in the row, 255 = absent
   int nperm_w = 0;            // entries per node of nperm
   DevBuf<uint16_t> nperm;     // 3-field HEX8 / HEX27: [nodes][nperm_w] position of (neighbour, field 0) in the node's rows
+  DevBuf<double> cell_geom;   // 3-field HEX8 bricks of parallelepiped cells: [cells][10] scaled cofactors, |det J|
   // uploaded 3-field HEX8 / HEX27 meshes of one block: the element instances (e * nb + a) of each
   // node, ascending, and each instance's slot map (position of element node b's field 0 in row 3n)
   DevBuf<int64_t> uocc_ptr;

@@ -263,12 +263,6 @@ __global__ void k_inst_records(int64_t ninst, int nb, const int32_t *__restrict_
   }
 }
 
-__global__ void k_max_row(int64_t nrows, const int64_t *__restrict__ rowptr, int *__restrict__ out) {
-  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  atomicMax(out, static_cast<int>(rowptr[r + 1] - rowptr[r]));
-}
-
 // ---------------------------------------------------------------------------------------
 // assembly
 
@@ -568,13 +562,7 @@ void patch_build(Mesh &m, hipStream_t s) {
   const int nb = m.topo == TXA_HEX8 ? 8 : 4;
   const int64_t nrows = m.ndofs, nelem = m.nelem, npos = nelem * nb;
   // row stride in LDS: the longest row; patch size: the largest power of two whose rows fit
-  DevBuf<int> dmax;
-  dmax.alloc(2);
-  TXA_HIP(hipMemsetAsync(dmax.p, 0, 2 * sizeof(int), s));
-  hipLaunchKernelGGL(k_max_row, dim3(setup_grid(nrows)), dim3(kSetupBlock), 0, s, nrows, m.rowptr.p, dmax.p);
-  TXA_HIP(hipGetLastError());
-  int rs = 0;
-  TXA_HIP(hipMemcpy(&rs, dmax.p, sizeof(int), hipMemcpyDeviceToHost));
+  const int rs = m.max_row;
   int P = kMaxRowsPerThread * kPatchThreads;
   while (P >= 32 && patch_lds_bytes(P, rs) > static_cast<size_t>(kPatchLdsBytes)) P /= 2;
   if (P < 32 || rs > 255) return;

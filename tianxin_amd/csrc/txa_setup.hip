@@ -264,6 +264,16 @@ __global__ __launch_bounds__(64) void k_row_sort_unique(int64_t nrows, const int
   if (lane == 63) ulen[row] = off;
 }
 
+// longest row: each wave's maximum by shuffles, one atomic per wave
+__global__ void k_len_max(int64_t n, const int64_t *__restrict__ len, int *__restrict__ out) {
+  int v = 0;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    v = max(v, static_cast<int>(len[i]));
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, v);
+}
+
 __global__ void k_row_copy(int64_t nrows, const int64_t *__restrict__ cptr, const int32_t *__restrict__ cand,
                            const int64_t *__restrict__ rowptr, int32_t *__restrict__ col) {
   int64_t row = blockIdx.x;
@@ -560,6 +570,10 @@ void graph_build(Mesh &m, hipStream_t s) {
   int ovf = 0;
   TXA_HIP(hipMemcpy(&ovf, overflow.p, sizeof(int), hipMemcpyDeviceToHost));
   TXA_REQUIRE(ovf == 0, TXA_ERR_UNSUPPORTED, "a graph row has more than 1024 candidate columns (element valence)");
+  TXA_HIP(hipMemsetAsync(overflow.p, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_len_max, dim3(std::min(grid_for(nrows), 2048u)), dim3(kBlock), 0, s, nrows, ulen.p, overflow.p);
+  TXA_HIP(hipGetLastError());
+  TXA_HIP(hipMemcpy(&m.max_row, overflow.p, sizeof(int), hipMemcpyDeviceToHost));
   m.rowptr.alloc(nrows + 1);
   exclusive_scan_i64(ulen.p, m.rowptr.p, nrows + 1, s);
   TXA_HIP(hipMemcpy(&m.nnz, m.rowptr.p + nrows, sizeof(int64_t), hipMemcpyDeviceToHost));
