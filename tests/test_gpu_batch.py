@@ -122,18 +122,25 @@ AFFINE_SHAPES = [(txa.HEX8, 1, 1, 1, 0.0), (txa.TET4, 2, 3, 1, 0.1), (txa.HEX8, 
                  (txa.HEX8, 5, 5, 70, 0.0), (txa.TET4, 9, 8, 35, 0.05)]
 
 
-@pytest.mark.parametrize("precision,tol", [(txa.FP64, 1e-12), (txa.FP32, 1e-6)])
-def test_batch_of_affine_hex8_meshes(engine, precision, tol):
+@pytest.mark.parametrize("precision,transient,tol", [(txa.FP64, True, 1e-12), (txa.FP32, True, 1e-6),
+                                                     (txa.FP32, False, 1e-6)])
+def test_batch_of_affine_hex8_meshes(engine, precision, transient, tol):
+    """Parallelepiped hex8 meshes with tet4 meshes: the one-metric hex8 body; FP32 steady runs
+    both kinds in one persistent launch over one queue (graph replays included)."""
     its = [Item(engine, 50 + i, *s_) for i, s_ in enumerate(AFFINE_SHAPES)]
     try:
         assert all(it.m.affine for it in its if it.topo == txa.HEX8)
-        b = txa.Batch(engine, [it.m for it in its], precision=precision, transient=True)
+        b = txa.Batch(engine, [it.m for it in its], precision=precision, transient=transient)
         try:
-            xs = [it.state(700 + i) for i, it in enumerate(its)]
-            got = run_batch(b, its, xs, True, 2.0, use_graph=True)
-            single = run_single(its, xs, precision, True, 2.0)
+            for rep in range(2):
+                xs = [it.state(700 + 10 * rep + i) for i, it in enumerate(its)]
+                got = run_batch(b, its, xs, transient, 2.0, use_graph=True)
+                eager = run_batch(b, its, xs, transient, 2.0, use_graph=False)
+                for i, ((fg, vg), (fe, ve)) in enumerate(zip(got, eager)):
+                    assert np.array_equal(fg, fe) and np.array_equal(vg, ve), f"replay {rep} item {i} differs"
+            single = run_single(its, xs, precision, transient, 2.0)
             for i, (it, (x, xd), (f, v), (fs, vs)) in enumerate(zip(its, xs, got, single)):
-                fr, vr = it.oracle(x, xd, True, 2.0)
+                fr, vr = it.oracle(x, xd, transient, 2.0)
                 assert rel_err(f, fr) <= tol and rel_err(v, vr) <= tol, f"item {i} differs from the oracle"
                 assert rel_err(f, fs) <= SAME[precision] and rel_err(v, vs) <= SAME[precision]
         finally:
@@ -162,6 +169,38 @@ def test_batch_graph_replays(engine, items, precision):
         fr, vr = items[3].oracle(*xs[3], True, alpha)
         tol = 1e-12 if precision == txa.FP64 else 1e-6
         assert rel_err(g[3][0], fr) <= tol and rel_err(g[3][1], vr) <= tol
+    finally:
+        b.destroy()
+
+
+def test_batch_dirichlet_lists_change_between_replays(engine, items):
+    """The Dirichlet pass spreads each mesh's list over blocks of consecutive entries; lists
+    whose lengths change between calls (full, a prefix, empty for some meshes, full again)
+    rebuild the captured graph, and every call equals the eager single-mesh assembly.  The
+    packed argument blocks (Batch.pack) give the same values as the dicts."""
+    b = txa.Batch(engine, [it.m for it in items], precision=txa.FP64, transient=False)
+    try:
+        xs = [it.state(900 + i) for i, it in enumerate(items)]
+        cuts = [lambda n: n, lambda n: (n + 1) // 2, lambda n: 0 if n % 2 else n, lambda n: n]
+        for rep, cut in enumerate(cuts):
+            outs = [it.outputs() for it in items]
+            ref = []
+            kws = []
+            for it, (x, xd), (f, v) in zip(items, xs, outs):
+                k = cut(it.bc.size)
+                kw = it.kwargs(dev(x), dev(xd), f, v, False, 0.0)
+                kw["bc_lids"], kw["bc_vals"] = (it.d_bc[:k], it.d_bcv[:k]) if k else (None, None)
+                kws.append(kw)
+                fs, vs = it.outputs()
+                it.m.assemble(precision=txa.FP64, **dict(kw, f=fs, vals=vs))
+                ref.append((fs, vs))
+            packed = b.pack(kws) if rep % 2 else kws
+            b.assemble(packed, use_graph=True)
+            torch.cuda.synchronize()
+            for i, ((f, v), (fs, vs)) in enumerate(zip(outs, ref)):
+                f, v, fs, vs = f.cpu().numpy(), v.cpu().numpy(), fs.cpu().numpy(), vs.cpu().numpy()
+                assert rel_err(f, fs) <= SAME[txa.FP64] and rel_err(v, vs) <= SAME[txa.FP64], \
+                    f"call {rep} item {i} differs from txa_assemble"
     finally:
         b.destroy()
 

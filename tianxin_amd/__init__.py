@@ -498,7 +498,16 @@ class Batch:
         self.precision = precision
         self.transient = transient
 
+    def pack(self, per_mesh):
+        """Checks and packs one make_args keyword dict per mesh (batch order) into the argument
+        blocks a call passes.  The result can be passed to assemble / tune in place of the
+        dicts, so a loop that reuses the same device arrays pays the per-mesh checks once; it
+        keeps those arrays alive."""
+        return self._pack(per_mesh)
+
     def _pack(self, per_mesh):
+        if getattr(per_mesh, "_batch", None) is self:
+            return per_mesh
         if len(per_mesh) != len(self.meshes):
             raise ValueError("one argument set per mesh")
         arr = (AssemblyArgs * len(per_mesh))()
@@ -511,10 +520,11 @@ class Batch:
             made.append(make_args(**kw))
             arr[i] = made[-1]
         arr._keep = made   # host arrays the copied argument blocks point to
+        arr._batch = self
         return arr
 
     def assemble(self, per_mesh, use_graph=True, stream=None):
-        """per_mesh: one make_args keyword dict per mesh, in batch order."""
+        """per_mesh: one make_args keyword dict per mesh, in batch order, or their pack()."""
         arr = self._pack(per_mesh)
         self.eng.check(self.eng.lib.txa_batch_assemble(self.eng.h, self.id, ctypes.cast(arr, P), int(use_graph),
                                                        _stream(stream)))

@@ -20,6 +20,7 @@
 // and every other combination k_generic (one workgroup per element, FP64 atomic
 // scatter into a zeroed CRS).
 #include "txa_internal.hpp"
+#include "txa_dirichlet.hpp"
 #include "txa_refelem.hpp"
 
 #include <algorithm>
@@ -589,47 +590,14 @@ void zero_async(double *p, int64_t n, hipStream_t s) { fill_async(p, n, 0.0, s);
 // Dirichlet rows: f_i = x_i - g_i; matrix row -> identity row
 // (disc-fe/src/lof/Panzer_TpetraLinearObjContainer.hpp:228-237, 300-318)
 // ============================================================================
-// G lanes per row (64/G rows per wave), kDirU passes per wave whose loads (LID, row range, the
-// first G columns) are all issued before any store, so a wave has 64/G * kDirU rows in flight.
-constexpr int kDirU = 4;
-
+// one wave per kDirU * 64/G consecutive list entries
 template <int G>
 __global__ __launch_bounds__(256) void k_dirichlet(int64_t n, int64_t nrows, const int32_t *__restrict__ lids,
                                                    const double *__restrict__ g, const double *__restrict__ x,
                                                    const int64_t *__restrict__ rowptr, const int32_t *__restrict__ col,
                                                    double *__restrict__ f, double *__restrict__ vals, int *err) {
-  constexpr int RPW = 64 / G;
-  const int lane = threadIdx.x & 63, gl = lane % G;
   const int64_t wave = static_cast<int64_t>(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64;
-  int64_t idx[kDirU], b[kDirU], e[kDirU];
-  int32_t r[kDirU];
-  bool ok[kDirU];
-#pragma unroll
-  for (int u = 0; u < kDirU; ++u) {
-    idx[u] = (wave * kDirU + u) * RPW + lane / G;
-    r[u] = idx[u] < n ? lids[idx[u]] : -1;
-  }
-#pragma unroll
-  for (int u = 0; u < kDirU; ++u) {
-    ok[u] = idx[u] < n && r[u] >= 0 && r[u] < nrows;
-    // outside the map: no row is touched, the context reports it
-    if (idx[u] < n && !ok[u] && gl == 0 && err) atomicOr(err, 1);
-    b[u] = ok[u] ? rowptr[r[u]] : 0;
-    e[u] = ok[u] ? rowptr[r[u] + 1] : 0;
-  }
-  if (f)
-#pragma unroll
-    for (int u = 0; u < kDirU; ++u)
-      if (ok[u] && gl == 0) f[r[u]] = x[r[u]] - (g ? g[idx[u]] : 0.0);
-  if (!vals) return;
-  int32_t c0[kDirU];
-#pragma unroll
-  for (int u = 0; u < kDirU; ++u) c0[u] = b[u] + gl < e[u] ? col[b[u] + gl] : -1;
-#pragma unroll
-  for (int u = 0; u < kDirU; ++u) {
-    if (b[u] + gl < e[u]) vals[b[u] + gl] = c0[u] == r[u] ? 1.0 : 0.0;
-    for (int64_t k = b[u] + gl + G; k < e[u]; k += G) vals[k] = col[k] == r[u] ? 1.0 : 0.0;
-  }
+  dirichlet_wave<G>(wave * kDirU * (64 / G), n, nrows, lids, g, x, rowptr, col, f, vals, err);
 }
 
 // Neumann flux on one side of the brick: one thread per lattice point of the
@@ -1027,8 +995,7 @@ void apply_dirichlet(const Mesh &m, const int32_t *lids, const double *g, int64_
   TXA_REQUIRE(n >= 0 && (n == 0 || lids), TXA_ERR_INVALID, "lids missing");
   TXA_REQUIRE(!f || x, TXA_ERR_INVALID, "x is required for the residual rows");
   if (n == 0) return;
-  // lanes per row: the longest row rounded up to a power of two, 8 to 64
-  const int G = m.max_row <= 8 ? 8 : m.max_row <= 16 ? 16 : m.max_row <= 32 ? 32 : 64;
+  const int G = dirichlet_lanes(m.max_row);
   const int64_t per = 4 * (64 / G) * kDirU;   // rows per 256-thread block
   const dim3 grid(static_cast<unsigned>((n + per - 1) / per));
   if (G == 8) hipLaunchKernelGGL(k_dirichlet<8>, grid, dim3(256), 0, s, n, m.ndofs, lids, g, x, m.rowptr.p, m.col.p, f, vals, err);

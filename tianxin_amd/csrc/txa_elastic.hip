@@ -610,7 +610,11 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
       constexpr int NP = block_stride(NB);
       for (int idx = t; idx < NB * NP; idx += BS) {
         const int a = idx / NP, q = idx % NP;
-        static_cast<TReal *>(p.jbuf)[e * (NB * NP) + idx] = q < 3 * ND ? Tref((q / ND) * NB + a, q % ND) : TReal(0);
+        const TReal v = q < 3 * ND ? TReal(Tref((q / ND) * NB + a, q % ND)) : TReal(0);
+        TReal *dst = &static_cast<TReal *>(p.jbuf)[e * (NB * NP) + idx];
+        // hex27 blocks stream past the caches (measured faster); hex8 blocks as plain stores
+        if (NB == 27) __builtin_nontemporal_store(v, dst);
+        else *dst = v;
       }
     }
     __syncthreads();

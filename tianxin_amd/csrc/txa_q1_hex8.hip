@@ -12,6 +12,7 @@
 //
 // The element routine (sum-factorised stiffness, FP64 flux residual of the FP32 form) is
 // in txa_q1_elem.hpp.
+#include "txa_dirichlet.hpp"
 #include "txa_internal.hpp"
 #include "txa_q1_elem.hpp"
 
@@ -660,8 +661,9 @@ __device__ __forceinline__ void leave_queue(BatchQueue *q) {
   }
 }
 
-// One element kind per launch: a kernel holding both bodies needs the register
-// budget of the larger one plus spills.
+// One element kind per launch (FP64, transient or general-cell batches, and batches of one
+// kind): a kernel holding both bodies spills there.  FP32 steady batches of both kinds with
+// parallelepiped hex8 cells take k_q1_batch_mixed.
 template <int KIND, typename Real, bool TRANS>
 __global__ __launch_bounds__(NT) void k_q1_batch_spec(const Q1DiffParams *__restrict__ prm,
                                                       const int4 *__restrict__ items, int nitems, BatchQueue *q) {
@@ -681,6 +683,31 @@ __global__ __launch_bounds__(NT) void k_q1_batch_spec(const Q1DiffParams *__rest
   leave_queue(q);
 }
 
+// Both element kinds in one persistent launch: one queue of (mesh | kMixedTet for tet4
+// meshes, tile x, tile y, slab) items over hex8 and tet4 meshes, longest first.
+constexpr int kMixedTet = 1 << 30;
+
+template <int HEXKIND, typename Real, bool TRANS>
+__global__ __launch_bounds__(NT) void k_q1_batch_mixed(const Q1DiffParams *__restrict__ prm,
+                                                       const int4 *__restrict__ items, int nitems, BatchQueue *q) {
+  __shared__ __align__(16) Real stage[stage_slots<Real>() * NSTAGE * NE];
+  __shared__ double rowbuf[(TY - 1) * ROWBUF];
+  __shared__ uint32_t meta[stage_slots<Real>() * MSLOT];
+  __shared__ int slot;
+  for (;;) {
+    const int it = fetch_item(q, &slot);
+    if (it >= nitems) break;
+    const int4 w = items[it];
+    const bool tet = (w.x & kMixedTet) != 0;
+    const Q1DiffParams &p = prm[w.x & (kMixedTet - 1)];
+    if (w.w * p.zc > p.nz) continue;   // slab past the mesh: nothing to do (uniform over the block)
+    if (tet) spec_block<TXA_TET4, Real, TRANS, false>(p, w.y, w.z, w.w, stage, rowbuf, meta);
+    else spec_block<HEXKIND, Real, TRANS, false>(p, w.y, w.z, w.w, stage, rowbuf, meta);
+    __syncthreads();
+  }
+  leave_queue(q);
+}
+
 struct BcItem {
   const int32_t *lids;
   const double *g;
@@ -690,24 +717,24 @@ struct BcItem {
   double *f;
   double *vals;
   int64_t n, nrows;
+  int64_t blk0;   // first block of this mesh's list entries
 };
 
-// Dirichlet rows of every mesh of a batch: blockIdx.x = mesh, blockIdx.y strides its rows.
-// A LID outside a mesh's map touches no row and sets *err.
-__global__ void k_dirichlet_batch(const BcItem *__restrict__ items, int *err) {
-  const BcItem b = items[blockIdx.x];
-  const int lane = threadIdx.x % 64;
-  for (int64_t i = static_cast<int64_t>(blockIdx.y) * (blockDim.x / 64) + threadIdx.x / 64; i < b.n;
-       i += static_cast<int64_t>(gridDim.y) * (blockDim.x / 64)) {
-    const int32_t r = b.lids[i];
-    if (r < 0 || r >= b.nrows) {
-      if (lane == 0 && err) atomicOr(err, 1);
-      continue;
-    }
-    if (lane == 0 && b.f) b.f[r] = b.x[r] - (b.g ? b.g[i] : 0.0);
-    if (b.vals)
-      for (int64_t k = b.rowptr[r] + lane; k < b.rowptr[r + 1]; k += 64) b.vals[k] = b.col[k] == r ? 1.0 : 0.0;
-  }
+// list entries of the batch Dirichlet pass per 256-thread block
+template <int G>
+constexpr int64_t bc_rows_per_block() { return 4 * kDirU * (64 / G); }
+
+// Dirichlet rows of every mesh of a batch: each block takes bc_rows_per_block consecutive
+// entries of one mesh's list (blk_mesh: the mesh of each block), so the largest list is
+// spread over as many blocks as it needs.  A LID outside a mesh's map touches no row and
+// sets *err.
+template <int G>
+__global__ __launch_bounds__(256) void k_dirichlet_batch(const BcItem *__restrict__ items,
+                                                         const int32_t *__restrict__ blk_mesh, int *err) {
+  const BcItem &b = items[blk_mesh[blockIdx.x]];
+  const int64_t i0 = (static_cast<int64_t>(blockIdx.x) - b.blk0) * bc_rows_per_block<G>() +
+                     static_cast<int64_t>(threadIdx.x / 64) * kDirU * (64 / G);
+  dirichlet_wave<G>(i0, b.n, b.nrows, b.lids, b.g, b.x, b.rowptr, b.col, b.f, b.vals, err);
 }
 
 // Whether a brick cell is a parallelepiped: the bilinear and trilinear coefficients of its
@@ -814,6 +841,10 @@ struct Batch {
   std::vector<int> zc;
   std::vector<int4> items_h;   // hex8 items, then tet4 items
   size_t n_hex = 0;            // items of hex8 meshes (the first n_hex of items_h)
+  // hex8 and tet4 meshes in one persistent launch over one queue (items of tet4 meshes marked
+  // kMixedTet), where the combined body fits the register file unspilled: FP32 steady, every
+  // hex8 mesh of parallelepiped cells
+  bool mixed = false;
   bool hex_affine = false;     // every hex8 mesh of the batch has parallelepiped cells
   DevBuf<int4> items;
   DevBuf<Q1DiffParams> prm;
@@ -821,6 +852,11 @@ struct Batch {
   DevBuf<BatchQueue> q;        // [2]: hex8 queue, tet4 queue
   Q1DiffParams *prm_h = nullptr;   // pinned: a graph's copy node reads it at replay
   BcItem *bc_h = nullptr;
+  int bc_lanes = 32;              // lanes per Dirichlet row (the meshes' longest row)
+  int64_t bc_blocks = 0;          // blocks of the Dirichlet pass for the current lists
+  size_t bc_map_cap = 0;
+  int32_t *bc_map_h = nullptr;    // pinned: the mesh of each Dirichlet block
+  DevBuf<int32_t> bc_map;
   hipStream_t cap = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -833,6 +869,7 @@ struct Batch {
     if (cap) (void)hipStreamDestroy(cap);
     if (prm_h) (void)hipHostFree(prm_h);
     if (bc_h) (void)hipHostFree(bc_h);
+    if (bc_map_h) (void)hipHostFree(bc_map_h);
   }
 };
 
@@ -878,7 +915,17 @@ void batch_build_items(Batch &b, int slab) {
   for (const Mesh *m : b.meshes)
     if (m->topo == TXA_HEX8 && !m->affine) b.hex_affine = false;
   b.items_h = lists[0];
-  b.items_h.insert(b.items_h.end(), lists[1].begin(), lists[1].end());
+  if (b.mixed) {
+    // one list over both kinds, longest first (ties: hex8 before tet4, then build order)
+    std::vector<int4> all(lists[0]);
+    all.insert(all.end(), lists[1].begin(), lists[1].end());
+    std::stable_sort(all.begin(), all.end(), [&](const int4 &u, const int4 &v) { return cost(u) > cost(v); });
+    for (int4 &w : all)
+      if (b.meshes[w.x]->topo == TXA_TET4) w.x |= kMixedTet;
+    b.items_h = all;
+  } else {
+    b.items_h.insert(b.items_h.end(), lists[1].begin(), lists[1].end());
+  }
   b.items.alloc(b.items_h.size());
   TXA_HIP(hipMemcpy(b.items.p, b.items_h.data(), sizeof(int4) * b.items_h.size(), hipMemcpyHostToDevice));
   batch_drop_graph(b);
@@ -899,6 +946,30 @@ void batch_fill(Batch &b, const txa_assembly_args *args) {
     BcItem &c = b.bc_h[i];
     c.lids = a.bc_lids; c.g = a.bc_vals; c.x = a.x; c.rowptr = m.rowptr.p; c.col = m.col.p;
     c.f = a.f; c.vals = a.vals; c.n = a.n_bc; c.nrows = m.ndofs;
+  }
+  // Dirichlet blocks: each mesh's list in runs of bc_rows_per_block entries.  A change in the
+  // block count changes the pass's grid and copy size: the captured graph is rebuilt.
+  const int64_t rpb = b.bc_lanes == 8 ? bc_rows_per_block<8>() : b.bc_lanes == 16 ? bc_rows_per_block<16>()
+                    : b.bc_lanes == 32 ? bc_rows_per_block<32>() : bc_rows_per_block<64>();
+  int64_t nblk = 0;
+  for (size_t i = 0; i < b.meshes.size(); ++i) {
+    b.bc_h[i].blk0 = nblk;
+    nblk += (b.bc_h[i].n + rpb - 1) / rpb;
+  }
+  TXA_REQUIRE(nblk < (int64_t(1) << 31), TXA_ERR_UNSUPPORTED, "too many Dirichlet rows for one batch pass");
+  if (nblk != b.bc_blocks) batch_drop_graph(b);
+  b.bc_blocks = nblk;
+  if (static_cast<size_t>(nblk) > b.bc_map_cap) {
+    if (b.bc_map_h) TXA_HIP(hipHostFree(b.bc_map_h));
+    b.bc_map_h = nullptr;
+    b.bc_map_cap = 0;
+    TXA_HIP(hipHostMalloc(reinterpret_cast<void **>(&b.bc_map_h), sizeof(int32_t) * nblk, hipHostMallocDefault));
+    b.bc_map_cap = static_cast<size_t>(nblk);
+    b.bc_map.alloc(nblk);
+  }
+  for (size_t i = 0; i < b.meshes.size(); ++i) {
+    const int64_t e = i + 1 < b.meshes.size() ? b.bc_h[i + 1].blk0 : nblk;
+    for (int64_t k = b.bc_h[i].blk0; k < e; ++k) b.bc_map_h[k] = static_cast<int32_t>(i);
   }
 }
 
@@ -926,7 +997,13 @@ void batch_enqueue(Batch &b, hipStream_t s, int *err) {
   TXA_HIP(hipMemcpyAsync(b.bc.p, b.bc_h, sizeof(BcItem) * n, hipMemcpyHostToDevice, s));
   const int nk[2] = {static_cast<int>(b.n_hex), static_cast<int>(b.items_h.size() - b.n_hex)};
   const int4 *it[2] = {b.items.p, b.items.p + b.n_hex};
-  for (int k = 0; k < 2; ++k) {
+  if (b.mixed) {
+    const int nall = static_cast<int>(b.items_h.size());
+    const dim3 g(std::min(b.grid, nall));
+    hipLaunchKernelGGL((k_q1_batch_mixed<kHex8Affine, float, false>), g, dim3(NT), 0, s, b.prm.p, b.items.p, nall, b.q.p);
+    TXA_HIP(hipGetLastError());
+  }
+  for (int k = 0; k < 2 && !b.mixed; ++k) {
     if (nk[k] == 0) continue;
     const dim3 g(std::min(b.grid, nk[k]));
     BatchQueue *q = b.q.p + k;
@@ -934,8 +1011,15 @@ void batch_enqueue(Batch &b, hipStream_t s, int *err) {
     else if (k == 0) launch_batch<TXA_HEX8>(b, g, it[k], nk[k], q, s);
     else launch_batch<TXA_TET4>(b, g, it[k], nk[k], q, s);
   }
-  hipLaunchKernelGGL(k_dirichlet_batch, dim3(static_cast<unsigned>(n), 8), dim3(256), 0, s, b.bc.p, err);
-  TXA_HIP(hipGetLastError());
+  if (b.bc_blocks > 0) {
+    TXA_HIP(hipMemcpyAsync(b.bc_map.p, b.bc_map_h, sizeof(int32_t) * b.bc_blocks, hipMemcpyHostToDevice, s));
+    const dim3 g(static_cast<unsigned>(b.bc_blocks));
+    if (b.bc_lanes == 8) hipLaunchKernelGGL(k_dirichlet_batch<8>, g, dim3(256), 0, s, b.bc.p, b.bc_map.p, err);
+    else if (b.bc_lanes == 16) hipLaunchKernelGGL(k_dirichlet_batch<16>, g, dim3(256), 0, s, b.bc.p, b.bc_map.p, err);
+    else if (b.bc_lanes == 32) hipLaunchKernelGGL(k_dirichlet_batch<32>, g, dim3(256), 0, s, b.bc.p, b.bc_map.p, err);
+    else hipLaunchKernelGGL(k_dirichlet_batch<64>, g, dim3(256), 0, s, b.bc.p, b.bc_map.p, err);
+    TXA_HIP(hipGetLastError());
+  }
 }
 
 }  // namespace
@@ -954,6 +1038,15 @@ Batch *batch_create(const std::vector<Mesh *> &meshes, const std::vector<int> &i
     TXA_REQUIRE(m->structured, TXA_ERR_UNSUPPORTED, "batched assembly covers bricks (txa_mesh_brick)");
     TXA_REQUIRE(m->has_graph && m->nfields == 1, TXA_ERR_LOGIC, "batch meshes need a scalar DOF map and a graph");
   }
+  int max_row = 0, nhex = 0;
+  bool affine = true;
+  for (const Mesh *m : meshes) {
+    max_row = std::max(max_row, m->max_row);
+    nhex += m->topo == TXA_HEX8;
+    affine = affine && (m->topo != TXA_HEX8 || m->affine);
+  }
+  b->mixed = precision == TXA_FP32 && !transient && affine && nhex > 0 && nhex < static_cast<int>(meshes.size());
+  b->bc_lanes = dirichlet_lanes(max_row);
   b->prm.alloc(meshes.size());
   b->bc.alloc(meshes.size());
   b->q.alloc(2);
