@@ -340,6 +340,8 @@ def bench_batch(args):
         per.append(dict(coef=k, x=x, bc=bc, bufs=bufs))
     nel = sum(m.nelem for m in ms)
     cells = int(np.sum(sizes.astype(np.int64) ** 3))
+    # compulsory HBM bytes of one batch call: CRS values and f written once, x and k read
+    cbytes = sum(8 * m.nnz + 16 * m.ndofs + 8 * m.nelem for m in ms)
     log(f"setup {time.time() - t0:.1f}s: {nm} meshes, {cells} cells, {nel} elements")
 
     def kw(i, prec):
@@ -369,9 +371,11 @@ def bench_batch(args):
         info = b.info()
         t = time_it(lambda: b.assemble(args_list, use_graph=True), args.steps, 2)
         out[name] = {"ms_per_step": t, "elements_per_s": nel / (t / 1e3), "cells_per_s": cells / (t / 1e3),
+                     "hbm_frac": cbytes / (t / 1e3) / HBM_PEAK,
                      "tuner_best_ms": tuned_ms,
-                     "slab": info["slab"], "grid": info["grid"], "work_items": info["items"]}
-        log(f"{name}: {t:.3f} ms (slab {info['slab']}, grid {info['grid']})")
+                     "slab": info["slab"], "grid": info["grid"], "work_items": info["items"],
+                     "tile": f"{info['tile_x']}x{256 // info['tile_x']}", "launches": info["launches"]}
+        log(f"{name}: {t:.3f} ms (tile {info['tile_x']}, slab {info['slab']}, grid {info['grid']})")
         b.destroy()
 
     def plain():
@@ -399,7 +403,7 @@ def bench_batch(args):
             "data": "synthetic",
             "config": {"workload": f"{nm} Poisson meshes, n^3 cells with n log-uniform in [8, 64], hex8/tet4 50/50",
                        "cells": cells, "elements": nel, "hex8_meshes": int(np.sum(kinds == txa.HEX8))},
-            "forms": out, "max_rel_err_fp32_vs_fp64_per_mesh": worst,
+            "forms": out, "max_rel_err_fp32_vs_fp64_per_mesh": worst, "compulsory_bytes": cbytes,
             "speedup_vs_fp64_per_mesh": value / out["fp64_per_mesh"]["elements_per_s"]}
     for m in ms:
         m.destroy()
@@ -1175,6 +1179,9 @@ def other_workloads(args):
             f = line["forms"]
             s["forms_ms"] = {k: v["ms_per_step"] for k, v in f.items()}
             s["cells_per_s"] = f["fp32_batch"]["cells_per_s"]
+            s["roofline"] = {"bound": "hbm", "frac": f["fp32_batch"]["hbm_frac"],
+                             "compulsory_bytes": line["compulsory_bytes"]}
+            s["fp32_batch_shape"] = {k: f["fp32_batch"][k] for k in ("tile", "slab", "grid", "launches")}
             s["max_rel_err_fp32_vs_fp64_per_mesh"] = line["max_rel_err_fp32_vs_fp64_per_mesh"]
             s["speedup_vs_fp64_per_mesh"] = line["speedup_vs_fp64_per_mesh"]
         elif fn is bench_upload:

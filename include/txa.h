@@ -334,15 +334,24 @@ int txa_batch_destroy(txa_ctx ctx, int batch_id);
  * as for txa_assemble).  Host-blocking only while a previous launch of the
  * same batch still reads its argument blocks. */
 int txa_batch_assemble(txa_ctx ctx, int batch_id, const txa_assembly_args *args, int use_graph, void *stream);
-/* Autotuner: times `reps` launches for each slab height (vertex layers per
- * work item: 8, 16, 32, 64, whole column) and persistent grid (1, 2, 4
- * workgroups per CU), keeps the fastest and reports its time per launch in
+/* Autotuner: times `reps` launches for each tile shape (32 x 8 or 16 x 16 vertex
+ * columns per workgroup), slab height (vertex layers per work item: 8, 16, 32,
+ * 64, whole column) and persistent grid (1, 2, 4 workgroups per CU), keeps the
+ * fastest and reports its time per launch in
  * best_ms.  The counterpart of the workset-size sweep in
  * adapters-stk/example/MixedPoissonExample/analyze_performance.py:57-61; it
  * writes f / vals as an assembly does. */
 int txa_batch_tune(txa_ctx ctx, int batch_id, const txa_assembly_args *args, int reps, void *stream,
                    double *best_ms);
-/* out[0]=meshes out[1]=work items out[2]=slab height (0 = whole) out[3]=persistent workgroups */
+/* Sets the work shape by hand instead of txa_batch_tune: tile width (16 or 32 vertex
+ * columns along x; the tile is tile_x x 256/tile_x), slab height (vertex layers per work
+ * item, 0 = whole column) and persistent workgroups.  The counterpart of the "Workset
+ * Size" entry of the assembly parameters
+ * (adapters-stk/src/Panzer_STK_ModelEvaluatorFactory_impl.hpp:294). */
+int txa_batch_shape(txa_ctx ctx, int batch_id, int tile_x, int slab, int grid);
+/* out[0]=meshes out[1]=work items out[2]=slab height (0 = whole) out[3]=persistent workgroups
+ * out[4]=tile width (vertex columns along x; the tile is out[4] x 256/out[4])
+ * out[5]=kernel launches per call besides the Dirichlet pass (1: both element kinds in one) */
 int txa_batch_info(txa_ctx ctx, int batch_id, int64_t *out);
 
 /* Serving: a worker thread inside the library admits assembly requests for

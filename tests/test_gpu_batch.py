@@ -99,11 +99,17 @@ def run_single(its, xs, precision, transient, alpha):
     return res
 
 
+@pytest.mark.parametrize("tile", [32, 16])
 @pytest.mark.parametrize("precision,transient,tol", [(txa.FP64, True, 1e-12), (txa.FP64, False, 1e-12),
                                                      (txa.FP32, True, 1e-6), (txa.FP32, False, 1e-6)])
-def test_batch_matches_single_and_oracle(engine, items, precision, transient, tol):
+def test_batch_matches_single_and_oracle(engine, items, precision, transient, tol, tile):
+    """Both tile shapes (32 x 8 and 16 x 16 vertex columns per workgroup), with slabs of 8
+    layers, against txa_assemble and the oracle."""
     b = txa.Batch(engine, [it.m for it in items], precision=precision, transient=transient)
     try:
+        b.shape(tile_x=tile, slab=8, grid=64)
+        info = b.info()
+        assert info["tile_x"] == tile and info["slab"] == 8 and info["grid"] == 64
         xs = [it.state(400 + i) for i, it in enumerate(items)]
         got = run_batch(b, items, xs, transient, 3.0, use_graph=False)
         single = run_single(items, xs, precision, transient, 3.0)
@@ -122,9 +128,9 @@ AFFINE_SHAPES = [(txa.HEX8, 1, 1, 1, 0.0), (txa.TET4, 2, 3, 1, 0.1), (txa.HEX8, 
                  (txa.HEX8, 5, 5, 70, 0.0), (txa.TET4, 9, 8, 35, 0.05)]
 
 
-@pytest.mark.parametrize("precision,transient,tol", [(txa.FP64, True, 1e-12), (txa.FP32, True, 1e-6),
-                                                     (txa.FP32, False, 1e-6)])
-def test_batch_of_affine_hex8_meshes(engine, precision, transient, tol):
+@pytest.mark.parametrize("precision,transient,tol,tile", [(txa.FP64, True, 1e-12, 32), (txa.FP32, True, 1e-6, 32),
+                                                          (txa.FP32, False, 1e-6, 32), (txa.FP32, False, 1e-6, 16)])
+def test_batch_of_affine_hex8_meshes(engine, precision, transient, tol, tile):
     """Parallelepiped hex8 meshes with tet4 meshes: the one-metric hex8 body; FP32 steady runs
     both kinds in one persistent launch over one queue (graph replays included)."""
     its = [Item(engine, 50 + i, *s_) for i, s_ in enumerate(AFFINE_SHAPES)]
@@ -132,6 +138,8 @@ def test_batch_of_affine_hex8_meshes(engine, precision, transient, tol):
         assert all(it.m.affine for it in its if it.topo == txa.HEX8)
         b = txa.Batch(engine, [it.m for it in its], precision=precision, transient=transient)
         try:
+            b.shape(tile_x=tile, slab=16, grid=128)
+            assert b.info()["launches"] == (1 if precision == txa.FP32 and not transient else 2)
             for rep in range(2):
                 xs = [it.state(700 + 10 * rep + i) for i, it in enumerate(its)]
                 got = run_batch(b, its, xs, transient, 2.0, use_graph=True)
@@ -215,6 +223,7 @@ def test_batch_autotune(engine, items):
         info = b.info()
         assert ms > 0.0
         assert info["slab"] in (8, 16, 32, 64, 0) and info["grid"] > 0 and info["meshes"] == len(items)
+        assert info["tile_x"] in (16, 32)
         got = run_batch(b, items, xs, False, 0.0, use_graph=True)
         single = run_single(items, xs, txa.FP32, False, 0.0)
         for (f, v), (fs, vs) in zip(got, single):
@@ -226,6 +235,10 @@ def test_batch_autotune(engine, items):
 def test_batch_rejections(engine, items):
     b = txa.Batch(engine, [items[0].m, items[1].m], precision=txa.FP64)
     try:
+        for bad in (dict(tile_x=8), dict(slab=-1), dict(grid=0)):
+            with pytest.raises(txa.TxaError) as e:
+                b.shape(**bad)
+            assert e.value.code == txa.TXA_ERR_INVALID
         with pytest.raises(txa.TxaError) as e:
             items[0].m.destroy()
         assert e.value.code == txa.TXA_ERR_LOGIC

@@ -136,6 +136,7 @@ def load_library(path: str = LIB_PATH):
     L.txa_batch_assemble.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P]
     L.txa_batch_tune.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P, ctypes.POINTER(ctypes.c_double)]
     L.txa_batch_info.argtypes = [P, ctypes.c_int, P]
+    L.txa_batch_shape.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.txa_serve_create.argtypes = [P, P, ctypes.c_int, I64, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     L.txa_serve_destroy.argtypes = [P, ctypes.c_int]
     L.txa_serve_submit.argtypes = [P, ctypes.c_int, ctypes.POINTER(ServeRequest), ctypes.POINTER(I64)]
@@ -530,17 +531,22 @@ class Batch:
                                                        _stream(stream)))
 
     def tune(self, per_mesh, reps=3, stream=None):
-        """Autotune slab height and persistent grid; returns the best time per launch (ms)."""
+        """Autotune tile shape, slab height and persistent grid; returns the best time per launch (ms)."""
         arr = self._pack(per_mesh)
         ms = ctypes.c_double()
         self.eng.check(self.eng.lib.txa_batch_tune(self.eng.h, self.id, ctypes.cast(arr, P), reps, _stream(stream),
                                                    ctypes.byref(ms)))
         return ms.value
 
+    def shape(self, tile_x=32, slab=0, grid=256):
+        """Sets tile width (16 or 32), slab height (0: whole column) and persistent grid by hand."""
+        self.eng.check(self.eng.lib.txa_batch_shape(self.eng.h, self.id, int(tile_x), int(slab), int(grid)))
+
     def info(self):
-        out = np.zeros(4, np.int64)
+        out = np.zeros(6, np.int64)
         self.eng.check(self.eng.lib.txa_batch_info(self.eng.h, self.id, _np(out)))
-        return dict(meshes=int(out[0]), items=int(out[1]), slab=int(out[2]), grid=int(out[3]))
+        return dict(meshes=int(out[0]), items=int(out[1]), slab=int(out[2]), grid=int(out[3]), tile_x=int(out[4]),
+                    launches=int(out[5]))
 
     def destroy(self):
         if self.id is not None:

@@ -478,6 +478,10 @@ int txa_batch_tune(txa_ctx ctx, int batch_id, const txa_assembly_args *args, int
   });
 }
 
+int txa_batch_shape(txa_ctx ctx, int batch_id, int tile_x, int slab, int grid) {
+  return guarded(ctx, [&] { txa::batch_shape(ctx->batch(batch_id), tile_x, slab, grid); });
+}
+
 int txa_batch_info(txa_ctx ctx, int batch_id, int64_t *out) {
   return guarded(ctx, [&] {
     TXA_REQUIRE(out, TXA_ERR_INVALID, "null argument");

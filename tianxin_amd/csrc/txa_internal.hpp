@@ -250,6 +250,7 @@ Batch *batch_create(const std::vector<Mesh *> &meshes, const std::vector<int> &i
 void batch_assemble(Batch &b, const txa_assembly_args *args, bool use_graph, hipStream_t s, int *err);
 double batch_tune(Batch &b, const txa_assembly_args *args, hipStream_t s, int reps, int *err);
 void batch_info(const Batch &b, int64_t *out);
+void batch_shape(Batch &b, int tile_x, int slab, int grid);
 const std::vector<int> &batch_mesh_ids(const Batch &b);
 // serving loop (txa_serve.cpp)
 Server *serve_create(int device, const std::vector<Mesh *> &meshes, const std::vector<int> &ids, int64_t budget,

@@ -22,9 +22,23 @@
 namespace txa {
 namespace {
 
-// element tile 32 x 8 (a 16 x 16 tile recomputes fewer halo elements but ran 1.5% slower at 256^3)
-constexpr int TX = 32, TY = 256 / TX, NE = TX * TY;
+// Element tiles: TX vertex columns along x by TY along y, one element lane and one owner
+// lane per column.  Single meshes take 32 x 8 (a 16 x 16 tile recomputes fewer halo
+// elements but ran 1.5% slower at 256^3); batches take 32 x 8 or 16 x 16, whichever the
+// autotuner times faster (16 x 16 covers meshes 16-64 cells wide with fewer idle lanes).
+constexpr int NE = 256;                        // element lanes = owner lanes
 constexpr int NT = 2 * NE;                     // block: 4 element waves + 4 owner waves
+constexpr int kTX = 32;                        // tile width of the single-mesh kernels
+template <int TXV>
+struct Tile {
+  static_assert(TXV == 16 || TXV == 32, "tile width");
+  static constexpr int TX = TXV, TY = NE / TXV;
+  // owner groups (the TX lanes of one x line of owners, TX-1 of them own rows): doubles per
+  // group's row buffer, TX-1 rows of at most 27 entries, then a spare slot for the entries a
+  // row does not have
+  static constexpr int ROWBUF = 27 * TX, ROWBUF_SPARE = 27 * (TX - 1);
+  static constexpr int NOWN = TX * (TY - 1);   // owner lanes with rows
+};
 using namespace q1elem;
 // element kinds of the lattice kernels: TXA_HEX8 (general trilinear cells), TXA_TET4 (six
 // tets per cell) and hex8 cells that are all parallelepipeds (constant metric per cell)
@@ -207,10 +221,6 @@ __device__ __forceinline__ void scatter_row(const Q1DiffParams &p, int32_t L, in
   }
 }
 
-// Owner groups: the TX lanes of one x line of owners (TX-1 of them own rows).
-constexpr int ROWBUF = 27 * TX;          // doubles per group: TX-1 rows of at most 27 entries,
-constexpr int ROWBUF_SPARE = 27 * (TX - 1);   // then a spare slot for the entries a row does not have
-
 // Orders LDS writes and reads between the lanes of one wavefront.
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -247,10 +257,11 @@ __device__ __forceinline__ RowMeta load_meta(const Q1DiffParams &p, int32_t L) {
 // ALIGN: store the range in aligned chunks of TX entries.  It removes the partial lines
 // inside a range (WRITE_SIZE 4.32 -> 3.81 GB at 256^3, the bytes of f and the values):
 // about 1% faster in both the FP32 and the FP64 kernels at 256^3.
-template <bool ALIGN>
+template <int TXV, bool ALIGN>
 __device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &m, const double (&acc)[27], double r,
                                           double *rowbuf, int grp) {
-  constexpr int GW = TX;   // lanes per group
+  constexpr int GW = TXV;   // lanes per group
+  constexpr int ROWBUF = Tile<TXV>::ROWBUF, ROWBUF_SPARE = Tile<TXV>::ROWBUF_SPARE;
   constexpr uint64_t GMASK = (1ull << GW) - 1;
   const int lane = threadIdx.x & 63, gl = lane & (GW - 1), gi = lane / GW;
   const bool act = m.L >= 0;
@@ -349,11 +360,13 @@ __device__ __forceinline__ RowMeta load_meta_flat(const Q1DiffParams &p, int32_t
 // [0] LID (-1: no row), [1] [2] low and high words of the row's first CRS
 // position, [3..9] the slot map words 0-6 with the row length in the top byte of
 // word 6 (byte 27 of the map, which carries no slot).
-constexpr int NOWN = TX * (TY - 1);
 constexpr int MWORDS = 10;
-constexpr int MSLOT = MWORDS * NOWN;
+template <int TXV>
+constexpr int mslot() { return MWORDS * Tile<TXV>::NOWN; }
 
+template <int TXV>
 __device__ __forceinline__ void put_meta(uint32_t *ms, int t, const RowMeta &m) {
+  constexpr int NOWN = Tile<TXV>::NOWN;
   ms[t] = static_cast<uint32_t>(m.L);
   ms[NOWN + t] = static_cast<uint32_t>(static_cast<uint64_t>(m.base));
   ms[2 * NOWN + t] = static_cast<uint32_t>(static_cast<uint64_t>(m.base) >> 32);
@@ -362,7 +375,9 @@ __device__ __forceinline__ void put_meta(uint32_t *ms, int t, const RowMeta &m) 
   ms[9 * NOWN + t] = (m.w[6] & 0x00ffffffu) | (static_cast<uint32_t>(m.end - m.base) << 24);
 }
 
+template <int TXV>
 __device__ __forceinline__ RowMeta get_meta(const uint32_t *ms, int t) {
+  constexpr int NOWN = Tile<TXV>::NOWN;
   RowMeta m;
   m.L = static_cast<int32_t>(ms[t]);
   m.base = static_cast<int64_t>(ms[NOWN + t] | static_cast<uint64_t>(ms[2 * NOWN + t]) << 32);
@@ -410,8 +425,9 @@ __device__ __forceinline__ void take_corner(const Real *s, int col, double (&acc
 // Residual of the FP32 form: the stage holds, per element column c, the x-pair sums
 // P_c[by + 2 bz] = R_c[1 + 2by + 4bz] + R_{c+1}[2by + 4bz] (FP64) of the element and its
 // +x neighbour, so the owner at corner (1,1) of column c adds two values per vertex layer.
-template <typename Real>
+template <int TXV, typename Real>
 __device__ __forceinline__ void take_pairs(const Real *s, int col, double &rc, double &rn) {
+  constexpr int TX = TXV;
   if constexpr (flux_residual<Real>()) {
     const double *ps = reinterpret_cast<const double *>(s + 36 * NE);
     rc += ps[1 * NE + col] + ps[0 * NE + col + TX];
@@ -454,9 +470,10 @@ __device__ __forceinline__ void load_face_lids(const Q1DiffParams &p, int64_t n0
 // owner wave to one layer of stores in flight.  The element waves therefore load
 // the owners' LIDs, row pointers and slot maps and publish them with the layer.
 // UNIT: the steady tangent with beta = 1 (the element matrix is stored unscaled).
-template <int KIND, typename Real, bool TRANS, bool UNIT>
+template <int TXV, int KIND, typename Real, bool TRANS, bool UNIT>
 __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by, int bz, Real *stage,
                                            double *rowbuf, uint32_t *meta) {
+  constexpr int TX = Tile<TXV>::TX, TY = Tile<TXV>::TY, NOWN = Tile<TXV>::NOWN, MSLOT = mslot<TXV>();
   constexpr int NSLOT = stage_slots<Real>();
   const bool is_elem = threadIdx.x < NE;
   const int t = is_elem ? threadIdx.x : threadIdx.x - NE;
@@ -575,7 +592,7 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
         for (int i = 0; i < (flux_residual<Real>() ? 4 : 8); ++i)
           reinterpret_cast<double *>(st + 36 * NE)[i * NE + t] = o.R[i];
       }
-      if (t < NOWN) put_meta(meta + sl * MSLOT, t, om);
+      if (t < NOWN) put_meta<TXV>(meta + sl * MSLOT, t, om);
       Lo = Lo_next;
       __syncthreads();   // layer ez is in the stage (two slots: the owners are also done with layer ez-1)
     }
@@ -598,14 +615,14 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
         const Real *st = stage + sl * (NSTAGE * NE);
         RowMeta m;
         if (owner) {
-          m = get_meta(meta + sl * MSLOT, t);
+          m = get_meta<TXV>(meta + sl * MSLOT, t);
           take_corner<1, 1>(st, t, acc, accn, rc, rn);
           take_corner<0, 1>(st, t + 1, acc, accn, rc, rn);
           take_corner<1, 0>(st, t + TX, acc, accn, rc, rn);
           take_corner<0, 0>(st, t + TX + 1, acc, accn, rc, rn);
-          take_pairs(st, t, rc, rn);
+          take_pairs<TXV>(st, t, rc, rn);
         }
-        if (e >= k0) emit_rows<TXA_Q1_ALIGNED_STORES>(p, m, acc, rc, rowbuf, t / TX);
+        if (e >= k0) emit_rows<TXV, TXA_Q1_ALIGNED_STORES>(p, m, acc, rc, rowbuf, t / TX);
 #pragma unroll
         for (int s = 0; s < 18; ++s) acc[s] = accn[s];
         rc = rn;
@@ -616,8 +633,8 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
     // the top vertex layer of the mesh has no element layer above it
     if (k1 == p.nz + 1) {
       RowMeta m;
-      if (owner) m = get_meta(meta + (NSLOT == 2 ? ((ez1 + 1) & 1) : 0) * MSLOT, t);
-      emit_rows<TXA_Q1_ALIGNED_STORES>(p, m, acc, rc, rowbuf, t / TX);
+      if (owner) m = get_meta<TXV>(meta + (NSLOT == 2 ? ((ez1 + 1) & 1) : 0) * MSLOT, t);
+      emit_rows<TXV, TXA_Q1_ALIGNED_STORES>(p, m, acc, rc, rowbuf, t / TX);
     }
   }
 }
@@ -625,10 +642,11 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
 
 template <int KIND, typename Real, bool TRANS, bool UNIT>
 __global__ __launch_bounds__(NT) void k_q1_hex8_diffusion(Q1DiffParams p) {
+  using T = Tile<kTX>;
   __shared__ __align__(16) Real stage[stage_slots<Real>() * NSTAGE * NE];
-  __shared__ double rowbuf[(TY - 1) * ROWBUF];
-  __shared__ uint32_t meta[stage_slots<Real>() * MSLOT];
-  spec_block<KIND, Real, TRANS, UNIT>(p, blockIdx.x, blockIdx.y, blockIdx.z, stage, rowbuf, meta);
+  __shared__ double rowbuf[(T::TY - 1) * T::ROWBUF];
+  __shared__ uint32_t meta[stage_slots<Real>() * mslot<kTX>()];
+  spec_block<kTX, KIND, Real, TRANS, UNIT>(p, blockIdx.x, blockIdx.y, blockIdx.z, stage, rowbuf, meta);
 }
 
 // ============================================================================
@@ -664,12 +682,13 @@ __device__ __forceinline__ void leave_queue(BatchQueue *q) {
 // One element kind per launch (FP64, transient or general-cell batches, and batches of one
 // kind): a kernel holding both bodies spills there.  FP32 steady batches of both kinds with
 // parallelepiped hex8 cells take k_q1_batch_mixed.
-template <int KIND, typename Real, bool TRANS>
+template <int TXV, int KIND, typename Real, bool TRANS>
 __global__ __launch_bounds__(NT) void k_q1_batch_spec(const Q1DiffParams *__restrict__ prm,
                                                       const int4 *__restrict__ items, int nitems, BatchQueue *q) {
+  using T = Tile<TXV>;
   __shared__ __align__(16) Real stage[stage_slots<Real>() * NSTAGE * NE];
-  __shared__ double rowbuf[(TY - 1) * ROWBUF];
-  __shared__ uint32_t meta[stage_slots<Real>() * MSLOT];
+  __shared__ double rowbuf[(T::TY - 1) * T::ROWBUF];
+  __shared__ uint32_t meta[stage_slots<Real>() * mslot<TXV>()];
   __shared__ int slot;
   for (;;) {
     const int it = fetch_item(q, &slot);
@@ -677,7 +696,7 @@ __global__ __launch_bounds__(NT) void k_q1_batch_spec(const Q1DiffParams *__rest
     const int4 w = items[it];
     const Q1DiffParams &p = prm[w.x];
     if (w.w * p.zc > p.nz) continue;   // slab past the mesh: nothing to do (uniform over the block)
-    spec_block<KIND, Real, TRANS, false>(p, w.y, w.z, w.w, stage, rowbuf, meta);
+    spec_block<TXV, KIND, Real, TRANS, false>(p, w.y, w.z, w.w, stage, rowbuf, meta);
     __syncthreads();
   }
   leave_queue(q);
@@ -687,12 +706,13 @@ __global__ __launch_bounds__(NT) void k_q1_batch_spec(const Q1DiffParams *__rest
 // meshes, tile x, tile y, slab) items over hex8 and tet4 meshes, longest first.
 constexpr int kMixedTet = 1 << 30;
 
-template <int HEXKIND, typename Real, bool TRANS>
+template <int TXV, int HEXKIND, typename Real, bool TRANS>
 __global__ __launch_bounds__(NT) void k_q1_batch_mixed(const Q1DiffParams *__restrict__ prm,
                                                        const int4 *__restrict__ items, int nitems, BatchQueue *q) {
+  using T = Tile<TXV>;
   __shared__ __align__(16) Real stage[stage_slots<Real>() * NSTAGE * NE];
-  __shared__ double rowbuf[(TY - 1) * ROWBUF];
-  __shared__ uint32_t meta[stage_slots<Real>() * MSLOT];
+  __shared__ double rowbuf[(T::TY - 1) * T::ROWBUF];
+  __shared__ uint32_t meta[stage_slots<Real>() * mslot<TXV>()];
   __shared__ int slot;
   for (;;) {
     const int it = fetch_item(q, &slot);
@@ -701,8 +721,8 @@ __global__ __launch_bounds__(NT) void k_q1_batch_mixed(const Q1DiffParams *__res
     const bool tet = (w.x & kMixedTet) != 0;
     const Q1DiffParams &p = prm[w.x & (kMixedTet - 1)];
     if (w.w * p.zc > p.nz) continue;   // slab past the mesh: nothing to do (uniform over the block)
-    if (tet) spec_block<TXA_TET4, Real, TRANS, false>(p, w.y, w.z, w.w, stage, rowbuf, meta);
-    else spec_block<HEXKIND, Real, TRANS, false>(p, w.y, w.z, w.w, stage, rowbuf, meta);
+    if (tet) spec_block<TXV, TXA_TET4, Real, TRANS, false>(p, w.y, w.z, w.w, stage, rowbuf, meta);
+    else spec_block<TXV, HEXKIND, Real, TRANS, false>(p, w.y, w.z, w.w, stage, rowbuf, meta);
     __syncthreads();
   }
   leave_queue(q);
@@ -792,8 +812,12 @@ Q1DiffParams make_params(const Mesh &m, const txa_assembly_args &a, int zc) {
   return p;
 }
 
-inline int tiles_x(const Mesh &m) { return (m.nx + 1 + TX - 2) / (TX - 1); }
-inline int tiles_y(const Mesh &m) { return (m.ny + 1 + TY - 2) / (TY - 1); }
+// tiles of width tx (height 256 / tx) over a brick's vertex columns
+inline int tiles_x(const Mesh &m, int tx = kTX) { return (m.nx + 1 + tx - 2) / (tx - 1); }
+inline int tiles_y(const Mesh &m, int tx = kTX) {
+  const int ty = NE / tx;
+  return (m.ny + 1 + ty - 2) / (ty - 1);
+}
 
 }  // namespace
 
@@ -837,6 +861,7 @@ struct Batch {
   int precision = TXA_FP64;
   bool transient = false;
   int slab = 0;   // vertex layers per work item (0: whole column)
+  int tx = kTX;   // tile width (vertex columns along x; the tile is tx x 256/tx)
   int grid = 0;   // persistent workgroups
   std::vector<int> zc;
   std::vector<int4> items_h;   // hex8 items, then tet4 items
@@ -900,8 +925,8 @@ void batch_build_items(Batch &b, int slab) {
     const int nch = (layers + zc - 1) / zc;
     std::vector<int4> &out = lists[m.topo == TXA_TET4 ? 1 : 0];
     for (int z = 0; z < nch; ++z)
-      for (int y = 0; y < tiles_y(m); ++y)
-        for (int x = 0; x < tiles_x(m); ++x) out.push_back(make_int4(static_cast<int>(i), x, y, z));
+      for (int y = 0; y < tiles_y(m, b.tx); ++y)
+        for (int x = 0; x < tiles_x(m, b.tx); ++x) out.push_back(make_int4(static_cast<int>(i), x, y, z));
   }
   auto cost = [&](const int4 &w) {
     const Mesh &m = *b.meshes[w.x];
@@ -973,16 +998,22 @@ void batch_fill(Batch &b, const txa_assembly_args *args) {
   }
 }
 
-template <int KIND>
-void launch_batch(const Batch &b, dim3 g, const int4 *items, int n, BatchQueue *q, hipStream_t s) {
+template <int TXV, int KIND>
+void launch_batch_tx(const Batch &b, dim3 g, const int4 *items, int n, BatchQueue *q, hipStream_t s) {
   if (b.precision == TXA_FP64) {
-    if (b.transient) hipLaunchKernelGGL((k_q1_batch_spec<KIND, double, true>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
-    else hipLaunchKernelGGL((k_q1_batch_spec<KIND, double, false>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
+    if (b.transient) hipLaunchKernelGGL((k_q1_batch_spec<TXV, KIND, double, true>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
+    else hipLaunchKernelGGL((k_q1_batch_spec<TXV, KIND, double, false>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
   } else {
-    if (b.transient) hipLaunchKernelGGL((k_q1_batch_spec<KIND, float, true>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
-    else hipLaunchKernelGGL((k_q1_batch_spec<KIND, float, false>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
+    if (b.transient) hipLaunchKernelGGL((k_q1_batch_spec<TXV, KIND, float, true>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
+    else hipLaunchKernelGGL((k_q1_batch_spec<TXV, KIND, float, false>), g, dim3(NT), 0, s, b.prm.p, items, n, q);
   }
   TXA_HIP(hipGetLastError());
+}
+
+template <int KIND>
+void launch_batch(const Batch &b, dim3 g, const int4 *items, int n, BatchQueue *q, hipStream_t s) {
+  if (b.tx == 16) launch_batch_tx<16, KIND>(b, g, items, n, q, s);
+  else launch_batch_tx<32, KIND>(b, g, items, n, q, s);
 }
 
 void batch_enqueue(Batch &b, hipStream_t s, int *err) {
@@ -1000,7 +1031,10 @@ void batch_enqueue(Batch &b, hipStream_t s, int *err) {
   if (b.mixed) {
     const int nall = static_cast<int>(b.items_h.size());
     const dim3 g(std::min(b.grid, nall));
-    hipLaunchKernelGGL((k_q1_batch_mixed<kHex8Affine, float, false>), g, dim3(NT), 0, s, b.prm.p, b.items.p, nall, b.q.p);
+    if (b.tx == 16)
+      hipLaunchKernelGGL((k_q1_batch_mixed<16, kHex8Affine, float, false>), g, dim3(NT), 0, s, b.prm.p, b.items.p, nall, b.q.p);
+    else
+      hipLaunchKernelGGL((k_q1_batch_mixed<32, kHex8Affine, float, false>), g, dim3(NT), 0, s, b.prm.p, b.items.p, nall, b.q.p);
     TXA_HIP(hipGetLastError());
   }
   for (int k = 0; k < 2 && !b.mixed; ++k) {
@@ -1087,8 +1121,9 @@ void batch_assemble(Batch &b, const txa_assembly_args *args, bool use_graph, hip
   b.launched = true;
 }
 
-// Autotuner: times the batch over slab heights (the analogue of Panzer's workset
-// size sweep) and persistent grid sizes, keeps the fastest.  out: best ms.
+// Autotuner: times the batch over tile shapes (32 x 8, 16 x 16), slab heights (the
+// analogue of Panzer's workset size sweep) and persistent grid sizes, keeps the
+// fastest.  out: best ms.
 double batch_tune(Batch &b, const txa_assembly_args *args, hipStream_t s, int reps, int *err) {
   int dev = 0, ncu = 0;
   TXA_HIP(hipGetDevice(&dev));
@@ -1099,26 +1134,31 @@ double batch_tune(Batch &b, const txa_assembly_args *args, hipStream_t s, int re
   TXA_HIP(hipEventCreate(&e0));
   TXA_HIP(hipEventCreate(&e1));
   double best = 1e30;
-  int best_slab = b.slab, best_grid = b.grid;
+  int best_slab = b.slab, best_grid = b.grid, best_tx = b.tx;
   if (b.launched) TXA_HIP(hipEventSynchronize(b.done));
   batch_fill(b, args);   // the argument blocks stay as they are for every timed launch
-  for (int sl : slabs) {
-    TXA_HIP(hipStreamSynchronize(s));
-    batch_build_items(b, sl);
-    for (int g : grids) {
-      b.grid = g;
-      batch_enqueue(b, s, err);   // warm-up
-      TXA_HIP(hipEventRecord(e0, s));
-      for (int r = 0; r < reps; ++r) batch_enqueue(b, s, err);
-      TXA_HIP(hipEventRecord(e1, s));
-      TXA_HIP(hipEventSynchronize(e1));
-      float ms = 0.f;
-      TXA_HIP(hipEventElapsedTime(&ms, e0, e1));
-      if (ms / reps < best) { best = ms / reps; best_slab = sl; best_grid = g; }
+  for (int tx : {32, 16}) {
+    b.tx = tx;
+    for (int sl : slabs) {
+      TXA_HIP(hipStreamSynchronize(s));
+      batch_build_items(b, sl);
+      for (int g : grids) {
+        b.grid = g;
+        batch_enqueue(b, s, err);   // warm-up
+        TXA_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; ++r) batch_enqueue(b, s, err);
+        TXA_HIP(hipEventRecord(e1, s));
+        TXA_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        TXA_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms / reps < best) { best = ms / reps; best_slab = sl; best_grid = g; best_tx = tx; }
+      }
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  TXA_HIP(hipStreamSynchronize(s));
+  b.tx = best_tx;
   batch_build_items(b, best_slab);
   b.grid = best_grid;
   TXA_HIP(hipEventRecord(b.done, s));
@@ -1126,11 +1166,24 @@ double batch_tune(Batch &b, const txa_assembly_args *args, hipStream_t s, int re
   return best;
 }
 
+// The work shape set by hand (what the autotuner would otherwise pick): tile width,
+// slab height and persistent grid.  No launch of the batch may be in flight.
+void batch_shape(Batch &b, int tile_x, int slab, int grid) {
+  TXA_REQUIRE(tile_x == 16 || tile_x == 32, TXA_ERR_INVALID, "tile width must be 16 or 32");
+  TXA_REQUIRE(slab >= 0 && grid >= 1, TXA_ERR_INVALID, "slab must be >= 0 (0: whole column) and grid >= 1");
+  if (b.launched) TXA_HIP(hipEventSynchronize(b.done));
+  b.tx = tile_x;
+  b.grid = grid;
+  batch_build_items(b, slab);
+}
+
 void batch_info(const Batch &b, int64_t *out) {
   out[0] = static_cast<int64_t>(b.meshes.size());
   out[1] = static_cast<int64_t>(b.items_h.size());
   out[2] = b.slab;
   out[3] = b.grid;
+  out[4] = b.tx;
+  out[5] = b.mixed ? 1 : 2;
 }
 
 const std::vector<int> &batch_mesh_ids(const Batch &b) { return b.ids; }
