@@ -18,9 +18,14 @@
 //   T[(k,a),(l,b)] = sum_q w_q |det J_q| dN_a/dx_k dN_b/dx_l        (3 NB x 3 NB),
 //   K[(a,i),(b,j)] = lambda T[(i,a),(j,b)] + mu T[(j,a),(i,b)] + delta_ij mu sum_k T[(k,a),(k,b)],
 // i.e. B^T D B for isotropic D written as one symmetric product G^T W G.  T is
-// computed in FP64 or FP32 on the VALU, or as T = O^T O with O = sqrt(w |det J|) G
-// by bf16 MFMA on a three-way split of each FP32 operand (x = x0 + x1 + x2, every
-// x_s in bf16) keeping the six cross products of order <= 2^-16, accumulated in FP32.
+// computed on the matrix cores: T = A^T G with A = w det J G in FP64
+// (v_mfma_f64_16x16x4f64) or FP32 (v_mfma_f32_16x16x4f32), or as T = O^T O with
+// O = sqrt(w |det J|) G by bf16 MFMA (v_mfma_f32_16x16x32_bf16) on a three-way split of
+// each FP32 operand (x = x0 + x1 + x2, every x_s in bf16) keeping the six cross products
+// of order <= 2^-16, accumulated in FP32.
+//
+// Bricks of parallelepiped hex8 cells skip both phases: k_elastic8_affine forms each
+// node's rows directly from closed-form cell blocks (one wave per node, no workspace).
 //
 // Assembly into the CRS is two-phase.  The element kernel writes each element's
 // Jacobian block (FP32, FP64 in FP64 mode) and residual (FP64) to a workspace,
