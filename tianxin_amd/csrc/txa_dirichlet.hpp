@@ -13,7 +13,7 @@ namespace txa {
 // G lanes per row (64/G rows per wave) and kDirU passes per wave whose loads (LID, row range,
 // the row's first G columns) are all issued before any store, so a wave has 64/G * kDirU rows
 // in flight.  G: the longest row rounded up to a power of two, 8 to 64.
-constexpr int kDirU = 4;
+constexpr int kDirU = 8;
 
 inline int dirichlet_lanes(int max_row) { return max_row <= 8 ? 8 : max_row <= 16 ? 16 : max_row <= 32 ? 32 : 64; }
 
