@@ -141,7 +141,8 @@ int txa_mesh_affine(txa_ctx ctx, int mesh_id, int *affine);
  * out[0] = rows per patch, out[1] = patches, out[2] = element instances (elements counted
  * once per patch they touch), out[3] = element colours, out[4] = longest row, out[5] = 1 when
  * the state is read in curve order (the LIDs do not follow the curve; a gather pass precedes
- * each assembly), 0 when it is read by LID.  out holds 6 entries.  All zero
+ * each assembly), 0 when it is read by LID, out[6] = 1 when every hex8 element is a
+ * parallelepiped (the one-metric element form).  out holds 7 entries.  All zero
  * when the mesh has none (bricks, several blocks, other topologies or fields, or more than
  * 64 colours): its assembly then takes the element kernels.  Introspection only; no
  * counterpart in the reference. */

@@ -19,7 +19,7 @@ import torch
 
 import oracle as O
 import tianxin_amd as txa
-from test_gpu_parity import TOL32, TOL64, dev, entry_err, rel_err
+from test_gpu_parity import TOL32, TOL64, dev, entry_err, rel_err, sheared
 from tianxin_amd import exodus, synth
 
 pytestmark = pytest.mark.gpu
@@ -151,6 +151,65 @@ def test_uploaded_fp32_form(engine, topo, transient):
         assert er <= TOL32 and ev_ <= TOL32
     finally:
         m.destroy()
+
+
+@pytest.mark.parametrize("shuffle", [False, True])
+@pytest.mark.parametrize("precision,transient", [(txa.FP64, False), (txa.FP64, True), (txa.FP32, False),
+                                                 (txa.FP32, True)])
+def test_uploaded_parallelepiped_cells(engine, shuffle, precision, transient):
+    """An uploaded hex8 mesh whose cells are all parallelepipeds (a sheared brick, in brick order
+    or shuffled and relabelled) takes the one-metric element form in the node patches; a
+    jittered one, or one vertex moved by 1e-9, does not.  Against the FP64 oracle: FP64 to
+    1e-12, the FP32 form R entrywise on the 1e-12 band and J normwise to 1e-6."""
+    n = 10
+    conn, ev, _ = O.brick_mesh(O.HEX8, n, n + 1, n + 2)
+    X = sheared(n, n + 1, n + 2)
+    if shuffle:
+        rng = np.random.default_rng(150)
+        perm = rng.permutation(X.shape[0])
+        conn, ev = perm[conn], perm[ev]
+        Xn = np.empty_like(X)
+        Xn[perm] = X
+        X = Xn
+        order = rng.permutation(conn.shape[0])
+        conn, ev = conn[order], ev[order]
+    m = engine.upload(txa.HEX8, conn, X)
+    try:
+        nd = m.build_dofmap(1)
+        nnz = m.build_graph()
+        pt = m.patches()
+        assert pt is not None and pt["parallelepiped_cells"] == 1
+        gids, owned, el = O.dofmap(conn, 1)
+        rp, col = O.graph(el, owned.size)
+        coef = synth.lognormal(151, conn.shape[0], 0.5)
+        x, xd = synth.uniform(152, nd, -1, 1), synth.uniform(153, nd, -1, 1)
+        kw = dict(alpha=1.5, beta=0.5, gamma2=0.0) if transient else dict(alpha=0.0, beta=1.0, gamma2=0.0)
+        A = O.Assembler(O.HEX8, txa.DIFFUSION, ev, X, el, coef, rp, col, rho=1.5, src=(1.0, 0.0, 0.0))
+        fr, vr = A.evaluate(x, xd if transient else None, None, transient=transient, mode=O.MODE_EXPLICIT, **kw)
+        f = torch.empty(nd, device="cuda", dtype=torch.float64)
+        v = torch.empty(nnz, device="cuda", dtype=torch.float64)
+        m.assemble(physics=txa.DIFFUSION, precision=precision, coef=dev(coef), x=dev(x),
+                   xdot=dev(xd) if transient else None, f=f, vals=v, rho=1.5, src=(1.0,), transient=transient, **kw)
+        torch.cuda.synchronize()
+        f, v = f.cpu().numpy(), v.cpu().numpy()
+        if precision == txa.FP64:
+            assert rel_err(f, fr) <= TOL64 and rel_err(v, vr) <= TOL64
+        else:
+            assert entry_err(f, fr, 1e-12) <= TOL32 and rel_err(v, vr) <= TOL32
+    finally:
+        m.destroy()
+    # not parallelepipeds: jittered cells, or one vertex moved by 1e-9
+    for Xb in (synth.brick_coords(n, n + 1, n + 2, jitter=0.1, seed=154), None):
+        if Xb is None:
+            Xb = sheared(n, n + 1, n + 2)
+            Xb[(n + 1) * 3 + 4, 2] += 1e-9
+        m = engine.upload(txa.HEX8, O.brick_mesh(O.HEX8, n, n + 1, n + 2)[0], Xb)
+        try:
+            m.build_dofmap(1)
+            m.build_graph()
+            assert m.patches()["parallelepiped_cells"] == 0
+        finally:
+            m.destroy()
 
 
 @pytest.mark.parametrize("topo", [txa.HEX8, txa.TET4])

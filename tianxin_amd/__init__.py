@@ -371,13 +371,13 @@ class BrickMesh:
     def patches(self):
         """Node patches of an uploaded one-field hex8 / tet4 mesh (after build_graph): rows per
         patch, patches, element instances, colours, longest row, whether the state is read in
-        curve order; None when the mesh has none."""
-        out = np.zeros(6, np.int64)
+        curve order, whether every hex8 element is a parallelepiped; None when the mesh has none."""
+        out = np.zeros(7, np.int64)
         self.eng.check(self.eng.lib.txa_mesh_patches(self.eng.h, self.id, _np(out)))
         if out[1] == 0:
             return None
-        return dict(zip(("rows_per_patch", "patches", "instances", "colours", "longest_row", "state_in_curve_order"),
-                        map(int, out)))
+        return dict(zip(("rows_per_patch", "patches", "instances", "colours", "longest_row", "state_in_curve_order",
+                         "parallelepiped_cells"), map(int, out)))
 
     def graph(self, with_cols=True):
         rp = np.zeros(self.ndofs + 1, np.int64)

@@ -229,6 +229,7 @@ int txa_mesh_patches(txa_ctx ctx, int mesh_id, int64_t *out) {
     out[3] = on ? m.patch_NC : 0;
     out[4] = on ? m.patch_RS : 0;
     out[5] = on && m.patch_bypos ? 1 : 0;
+    out[6] = on && m.patch_affine ? 1 : 0;
   });
 }
 

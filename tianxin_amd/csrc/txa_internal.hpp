@@ -119,6 +119,7 @@ struct Mesh {
   DevBuf<uint8_t> patch_icol;     // colour of each instance
   DevBuf<int32_t> patch_rows;     // [npatch*P] row at each curve position, -1 past the last row
   DevBuf<double> patch_rec;       // [rows][4] vertex x, y, z and the row's curve position, by LID
+  bool patch_affine = false;      // every hex8 element a parallelepiped: the one-metric element form
   bool patch_bypos = false;       // the LIDs do not follow the curve: state and coordinates read in curve order
   DevBuf<int32_t> patch_ivtx;     // [instances][nb] vertices of each instance's element: curve positions
                                   // (patch_bypos) or LIDs

@@ -427,7 +427,29 @@ constexpr int kMaxRowsPerThread = 4;   // patches of at most 4 x 256 rows
 
 // One workgroup per patch, its instances in chunks of one per lane.  A chunk's row positions
 // load while the previous chunk adds its rows; everything is read in curve order.
-template <int TOPO, typename Real, bool TRANS, bool UNIT>
+// Whether every element of an uploaded hex8 mesh is a parallelepiped: the test of the lattice
+// path's cell check (bilinear and trilinear Walsh coefficients of the vertex coordinates, in
+// binary vertex order, below 2^-47 of the coordinate magnitude).  Any element that is not
+// clears *flag.
+__global__ void k_elem_affine(int64_t nelem, const int64_t *__restrict__ conn, const double *__restrict__ xyz,
+                              int64_t nverts, int *flag) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= nelem) return;
+  double nonaff = 0.0, mag = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    double h[8];
+#pragma unroll
+    for (int v = 0; v < 8; ++v) h[hex_bin(v)] = xyz[d * nverts + conn[8 * e + v]];
+    walsh8(h);
+    nonaff += 0.125 * (fabs(h[3]) + fabs(h[5]) + fabs(h[6]) + fabs(h[7]));
+    mag += 0.125 * (fabs(h[0]) + fabs(h[1]) + fabs(h[2]) + fabs(h[4]));
+  }
+  if (!(nonaff <= 0x1p-47 * mag)) __hip_atomic_store(flag, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// AFF: every element is a parallelepiped (one metric per element, as the lattice kernels'
+// kind for such cells).
+template <int TOPO, typename Real, bool TRANS, bool UNIT, bool AFF>
 __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
   constexpr int NB = TOPO == TXA_HEX8 ? 8 : 4;
   extern __shared__ double sacc[];   // [P][RS] row entries, then [P] residual entries
@@ -478,7 +500,7 @@ __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
           udb[hex_bin(s)] = TRANS ? nx.ud[s] : 0.0;
         }
         Hex8Out<Real, TRANS> o;
-        hex8_element<Real, TRANS, UNIT, false>(Xb, ub, udb, nx.k, p.rhoc, p.src, p.alpha, p.beta, o, [] {});
+        hex8_element<Real, TRANS, UNIT, AFF>(Xb, ub, udb, nx.k, p.rhoc, p.src, p.alpha, p.beta, o, [] {});
 #pragma unroll
         for (int a = 0; a < 8; ++a) {
           Re[a] = o.R[hex_bin(a)];
@@ -540,13 +562,13 @@ __global__ __launch_bounds__(kPatchThreads) void k_patch(PatchParams p) {
   }
 }
 
-template <int TOPO, typename Real>
+template <int TOPO, typename Real, bool AFF>
 void launch_topo(const Mesh &m, const PatchParams &pp, bool tr, bool unit, hipStream_t s) {
   const dim3 grid(static_cast<unsigned>(m.npatch)), block(kPatchThreads);
   const size_t lds = patch_lds_bytes(pp.P, pp.RS);
-  if (tr) hipLaunchKernelGGL((k_patch<TOPO, Real, true, false>), grid, block, lds, s, pp);
-  else if (unit) hipLaunchKernelGGL((k_patch<TOPO, Real, false, true>), grid, block, lds, s, pp);
-  else hipLaunchKernelGGL((k_patch<TOPO, Real, false, false>), grid, block, lds, s, pp);
+  if (tr) hipLaunchKernelGGL((k_patch<TOPO, Real, true, false, AFF>), grid, block, lds, s, pp);
+  else if (unit) hipLaunchKernelGGL((k_patch<TOPO, Real, false, true, AFF>), grid, block, lds, s, pp);
+  else hipLaunchKernelGGL((k_patch<TOPO, Real, false, false, AFF>), grid, block, lds, s, pp);
   TXA_HIP(hipGetLastError());
 }
 
@@ -555,6 +577,7 @@ void launch_topo(const Mesh &m, const PatchParams &pp, bool tr, bool unit, hipSt
 void patch_build(Mesh &m, hipStream_t s) {
   m.npatch = 0;
   m.patch_P = 0;
+  m.patch_affine = false;
   const bool fits = !m.structured && m.nfields == 1 && m.blocks.size() == 1 &&
                     (m.topo == TXA_HEX8 || m.topo == TXA_TET4) && m.occ.p &&
                     m.nelem * (m.topo == TXA_HEX8 ? 8 : 4) < (int64_t(1) << 31);   // sort lengths are int
@@ -566,6 +589,19 @@ void patch_build(Mesh &m, hipStream_t s) {
   int P = kMaxRowsPerThread * kPatchThreads;
   while (P >= 32 && patch_lds_bytes(P, rs) > static_cast<size_t>(kPatchLdsBytes)) P /= 2;
   if (P < 32 || rs > 255) return;
+  if (m.topo == TXA_HEX8) {
+    DevBuf<int> flag;
+    flag.alloc(1);
+    const int one = 1;
+    TXA_HIP(hipMemcpyAsync(flag.p, &one, sizeof(int), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_elem_affine, dim3(setup_grid(nelem)), dim3(kSetupBlock), 0, s, nelem, m.conn.p, m.xyz.p,
+                       m.nverts, flag.p);
+    TXA_HIP(hipGetLastError());
+    int h = 0;
+    TXA_HIP(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    TXA_HIP(hipStreamSynchronize(s));
+    m.patch_affine = h != 0;
+  }
   // rows along the Morton curve of their vertices
   m.patch_rec.alloc(4 * nrows);
   hipLaunchKernelGGL(k_node_xyz, dim3(setup_grid(npos)), dim3(kSetupBlock), 0, s, npos, m.conn.p, m.elem_lid.p,
@@ -771,11 +807,16 @@ void launch_patch(const Mesh &m, const txa_assembly_args &a, hipStream_t s, void
   pp.npatch = m.npatch;
   const bool tr = a.transient != 0, unit = !tr && a.beta == 1.0;
   if (m.topo == TXA_HEX8) {
-    if (a.precision == TXA_FP32) launch_topo<TXA_HEX8, float>(m, pp, tr, unit, s);
-    else launch_topo<TXA_HEX8, double>(m, pp, tr, unit, s);
+    if (m.patch_affine) {
+      if (a.precision == TXA_FP32) launch_topo<TXA_HEX8, float, true>(m, pp, tr, unit, s);
+      else launch_topo<TXA_HEX8, double, true>(m, pp, tr, unit, s);
+    } else {
+      if (a.precision == TXA_FP32) launch_topo<TXA_HEX8, float, false>(m, pp, tr, unit, s);
+      else launch_topo<TXA_HEX8, double, false>(m, pp, tr, unit, s);
+    }
   } else {
-    if (a.precision == TXA_FP32) launch_topo<TXA_TET4, float>(m, pp, tr, unit, s);
-    else launch_topo<TXA_TET4, double>(m, pp, tr, unit, s);
+    if (a.precision == TXA_FP32) launch_topo<TXA_TET4, float, false>(m, pp, tr, unit, s);
+    else launch_topo<TXA_TET4, double, false>(m, pp, tr, unit, s);
   }
 }
 
