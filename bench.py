@@ -683,7 +683,8 @@ def bench_upload(args):
     """Uploaded-mesh path (txa_mesh_upload, the connectivity a ConnManager hands over):
     an n^3 hex8 brick (default 128) uploaded as unstructured connectivity, FP64 (and the
     FP32 form beside it), steady Poisson (k lognormal sigma 0.5 per element, f = 1,
-    u ~ U[-1, 1], u = 0 on x = 0).  Times the cells in brick order and in a seeded random order.  Checks: in brick order
+    u ~ U[-1, 1], u = 0 on x = 0).  Times the jittered cells (vertices moved by U[-0.1h, 0.1h]) in brick
+    order and in a seeded random order, and unjittered (parallelepiped) cells in brick order.  Checks: in brick order
     the upload numbers LIDs as the lattice mesh does, so f and J must equal the lattice
     FP64 path's to rounding; Poisson rows off the Dirichlet set sum to zero."""
     import numpy as np
@@ -698,10 +699,11 @@ def bench_upload(args):
     k = torch.as_tensor(synth.lognormal(5, n ** 3, 0.5), device="cuda")
     out, checks = {}, {}
     ref = None
-    for name, seed in (("brick_order", None), ("shuffled", 11)):
+    X_cube = synth.brick_coords(n, n, n)
+    for name, seed, XX in (("brick_order", None, X), ("shuffled", 11, X), ("parallelepiped_cells_brick_order", None, X_cube)):
         t0 = time.time()
         conn = synth.brick_hex8_conn(n, n, n, shuffle_seed=seed)
-        m = eng.upload(txa.HEX8, conn, X)
+        m = eng.upload(txa.HEX8, conn, XX)
         nd = m.build_dofmap(1)
         nnz = m.build_graph()
         owned, _ = m.dofmap()
@@ -757,7 +759,7 @@ def bench_upload(args):
         keep = torch.ones(nd, dtype=torch.bool, device="cuda")
         keep[bc.long()] = False
         checks[name] = {"row_sum_rel_max": float(rs[keep].abs().max() / v.abs().max())}
-        if seed is None:
+        if name == "brick_order":
             ref = (f.clone(), v.clone(), x, bc)
         m.destroy()
         del f, v
@@ -1111,30 +1113,37 @@ def summarize_upload(stats_dir, fetch_dir, write_dir, util_dir, bench_json):
              "--workload upload --steps 5 --warmup 1 --no-cpu --no-extra",
              cmd.format("FETCH_SIZE", "pmc_f"), cmd.format("WRITE_SIZE", "pmc_w"), cmd.format(util, "pmc_u"), "",
              f"Workload: {line['config']['workload']} ({ne} elements, {rows_} rows, {nnz} nnz), steady Poisson,",
-             "k lognormal sigma 0.5 per element, u = 0 on x = 0; the cells in brick order, then in a seeded",
-             "random order (the LIDs, numbered by first touch, follow the element order).  The assembly is",
+             "k lognormal sigma 0.5 per element, u = 0 on x = 0; jittered cells (vertices moved by",
+             "U[-0.1h, 0.1h]) in brick order, then in a seeded random order (the LIDs, numbered by first",
+             "touch, follow the element order), then unjittered cells in brick order (every element a",
+             "parallelepiped: the one-metric element form, decided once per mesh).  The assembly is",
              f"one launch of `k_patch` (one workgroup per patch of {pt['rows_per_patch']} rows along the Morton "
              "curve) and the", "Dirichlet rows.", "",
              f"Patches: {pt['patches']}, element instances {pt['instances']} ({pt['instances'] / ne:.2f} per element: "
              f"an element is integrated once per patch it touches), {pt['colours']} colours, rows of at most "
              f"{pt['longest_row']} entries.", "",
              "| order | form | ms per step (HIP events) | p50 / p99 ms | elements/s |", "|---|---|---:|---:|---:|"]
-    for name in ("brick_order", "shuffled"):
+    for name in [k for k in ("brick_order", "shuffled", "parallelepiped_cells_brick_order") if k in o]:
         c = o[name]
         lines.append(f"| {name.replace('_', ' ')} | FP64 | {c['ms_per_step']:.3f} | {c['p50']:.3f} / {c['p99']:.3f} | "
                      f"{c['elements_per_s']:.3g} |")
         lines.append(f"| {name.replace('_', ' ')} | FP32 | {c['fp32']['ms_per_step']:.3f} | | "
                      f"{ne / c['fp32']['ms_per_step'] * 1e3:.3g} |")
-    lines += ["", "Per-dispatch durations from the kernel trace (us); each order runs the FP64 form (1 + 5",
+    lines += ["", "Per-dispatch durations from the kernel trace (us); each case runs the FP64 form (1 + 5",
               "dispatches), then the FP32 form (1 + 5):", "",
               "| kernel | order | calls | mean | min | max |", "|---|---|---:|---:|---:|---:|"]
-    for k in ("k_patch<1, double, false, true>", "k_patch<1, float, false, true>"):
+    for k in ("k_patch<1, double, false, true, false>", "k_patch<1, float, false, true, false>"):
         d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in trace if _match(r["Kernel_Name"], k)]
         h = len(d) // 2
         for name, part in (("brick order", d[:h]), ("shuffled", d[h:])):
             if part:
                 lines.append(f"| `{k}` | {name} | {len(part)} | {np.mean(part):.1f} | {np.min(part):.1f} | "
                              f"{np.max(part):.1f} |")
+    for k in ("k_patch<1, double, false, true, true>", "k_patch<1, float, false, true, true>"):
+        d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in trace if _match(r["Kernel_Name"], k)]
+        if d:
+            lines.append(f"| `{k}` | parallelepiped cells, brick order | {len(d)} | {np.mean(d):.1f} | "
+                         f"{np.min(d):.1f} | {np.max(d):.1f} |")
     d = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3 for r in trace if "k_state_by_pos" in r["Kernel_Name"]]
     if d:
         lines.append(f"| `k_state_by_pos` (the state in curve order, before each assembly) | shuffled | {len(d)} | "
@@ -1142,7 +1151,7 @@ def summarize_upload(stats_dir, fetch_dir, write_dir, util_dir, bench_json):
     lines += ["", "The brick in its own order reads the state by LID (its rows touch 0.19 lines of 128 bytes",
               "per row along the curve); with shuffled elements the LIDs scatter (0.41 lines per row) and",
               "the state is gathered into curve order first (`txa_mesh_patches` reports the choice)."]
-    kern = "k_patch<1, double, false, true>"
+    kern = "k_patch<1, double, false, true, false>"
 
     def per_dispatch(d, counter):
         tot = {}
