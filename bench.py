@@ -1083,8 +1083,8 @@ def summarize_elastic(stats_dir, pmc_dir, bench_json):
                   "  cell's scaled cofactors and |det J| (a per-mesh table, `k_cell_geom`): Chat S Chat^T, the",
                   "  Newmark/Rayleigh combination, eight cell-ordered LDS adds, rows stored non-temporally.",
                   "  Timing probes (work skipped) put the LDS adds and the state loads' dependence on the",
-                  "  LIDs at 1% each and the row stores at about 0.3 ms: the VALU work at three waves per",
-                  "  SIMD sets the rest."]
+                  "  LIDs at 1% each and the row stores at about 0.3 ms: the VALU work and its dependences",
+                  "  (four waves per SIMD at 125 VGPRs) set the rest."]
     out = os.path.join(ROOT, "profiles", "elastic_hex8_128_kernel_stats.md")
     open(out, "w").write("\n".join(lines) + "\n")
     print(out)
