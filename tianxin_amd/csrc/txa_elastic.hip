@@ -660,6 +660,15 @@ struct GatherParams {
   double *f, *vals;
 };
 
+// One wave stores n consecutive FP64 values from LDS to dst, non-temporally (written once,
+// not re-read here), in chunks aligned to 128-byte lines: only the first and the last line
+// of the range are partial.
+__device__ __forceinline__ void row_store(const double *src, double *dst, int n, int lane) {
+  const int lead = static_cast<int>((reinterpret_cast<uintptr_t>(dst) >> 3) & 15);
+  for (int k = lane - lead; k < n; k += 64)
+    if (k >= 0) __builtin_nontemporal_store(src[k], dst + k);
+}
+
 __device__ __forceinline__ void gather_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -785,7 +794,7 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
     }
     gather_lds_sync();   // all additions done before the rows are read
     if (g.vals)
-      for (int k = lane; k < 3 * len; k += 64) __builtin_nontemporal_store(b[k], g.vals + base + k);
+      row_store(b, g.vals + base, 3 * len, lane);
     if (g.f && lane < 3) g.f[L0 + lane] = racc;
     gather_lds_sync();   // the buffer is reused by the next node
     Lc = Ln;
@@ -880,7 +889,7 @@ __global__ __launch_bounds__(256) void k_elastic_gather_u(GatherUParams g) {
     }
     gather_lds_sync();
     if (g.vals)
-      for (int k = lane; k < 3 * len; k += 64) __builtin_nontemporal_store(b[k], g.vals + base + k);
+      row_store(b, g.vals + base, 3 * len, lane);
     if (g.f && lane < 3) g.f[L0 + lane] = racc;
     gather_lds_sync();
   }
@@ -1183,8 +1192,7 @@ __global__ __launch_bounds__(256, 4) void k_elastic8_affine(Elastic8Params p) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    if (p.vals)
-      for (int k = lane; k < 3 * len; k += 64) __builtin_nontemporal_store(rows[w][k], &p.vals[base + k]);
+    if (p.vals) row_store(rows[w], p.vals + base, 3 * len, lane);
     if (p.f && lane < 3) p.f[L0 + lane] = rr[lane];
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
