@@ -35,6 +35,7 @@
 // run in one wave's program order): results are bitwise reproducible.
 // The row position of every (node, neighbour) pair comes from a per-node table
 // built with the graph.
+#include <map>
 #include <type_traits>
 
 #include "txa_internal.hpp"
@@ -47,6 +48,7 @@ constexpr int MODE_F64 = 0, MODE_F32 = 1, MODE_SPLIT = 2;
 struct ElasticParams {
   int nq;
   int64_t nelem;
+  int64_t e0, e1;               // elements [e0, e1) of this launch
   int nconn;
   const int64_t *conn;          // the first 8 entries of an element are its vertices
   const double *X, *Y, *Z;
@@ -250,12 +252,12 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
   int32_t gL = 0;
   double gX = 0.0, gu = 0.0, gv = 0.0, gac = 0.0, gE = 0.0;
   auto fetch_index = [&](int64_t e) {
-    const int64_t ec = e < p.nelem ? e : p.nelem - 1;
+    const int64_t ec = e < p.e1 ? e : p.e1 - 1;
     gn = p.conn[ec * p.nconn + min(t, 23) / 3];
     gL = p.elem_lid[ec * ND + min(t, ND - 1)];
   };
   auto fetch_data = [&](int64_t e) {
-    const int64_t ec = e < p.nelem ? e : p.nelem - 1;
+    const int64_t ec = e < p.e1 ? e : p.e1 - 1;
     const int d = t % 3;
     const double *P = d == 0 ? p.X : (d == 1 ? p.Y : p.Z);
     gX = P[gn];
@@ -273,13 +275,13 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
     }
     asm volatile("" : "+v"(gE));   // the coefficient is taken here too, not after the block store
   };
-  fetch_index(blockIdx.x);
-  fetch_data(blockIdx.x);
+  fetch_index(p.e0 + blockIdx.x);
+  fetch_data(p.e0 + blockIdx.x);
   put_data();
   double E = gE;
-  fetch_index(blockIdx.x + gridDim.x);
+  fetch_index(p.e0 + blockIdx.x + gridDim.x);
 
-  for (int64_t e = blockIdx.x; e < p.nelem; e += gridDim.x) {
+  for (int64_t e = p.e0 + blockIdx.x; e < p.e1; e += gridDim.x) {
     __syncthreads();   // this element's vertices and state are in LDS
     // ---- geometry (FP64)
     if (t < NQ) {
@@ -651,6 +653,7 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
 struct GatherParams {
   int nx, ny, nz;
   int64_t px, py, pz, npoints;
+  int64_t p0, p1;             // lattice points [p0, p1) of this launch
   const int32_t *point_lid;   // [points][3]
   const int64_t *rowptr;
   const uint16_t *nperm;
@@ -714,20 +717,20 @@ __global__ __launch_bounds__(256) void k_elastic_gather(GatherParams g) {
   const cint *plid = (const cint *)g.point_lid;
   const clong *prow = (const clong *)g.rowptr;
   const int64_t stride = static_cast<int64_t>(gridDim.x) * 4;
-  int64_t pt = static_cast<int64_t>(blockIdx.x) * 4 + w;
-  int32_t Lc = plid[min(pt, g.npoints - 1) * 3];
+  int64_t pt = g.p0 + static_cast<int64_t>(blockIdx.x) * 4 + w;
+  int32_t Lc = plid[min(pt, g.p1 - 1) * 3];
   int64_t bc = prow[Lc], ec = prow[Lc + 1];
   // lattice coordinates of the node, advanced by the stride's own coordinates with
   // carries: the divisions happen once, not per node
   const int64_t pxy = g.px * g.py;
   int64_t I = pt % g.px, Jy = (pt / g.px) % g.py, K = pt / pxy;
   const int64_t sI = stride % g.px, sJ = (stride / g.px) % g.py, sK = stride / pxy;
-  for (; pt < g.npoints; pt += stride) {
+  for (; pt < g.p1; pt += stride) {
     const int32_t L0 = Lc;
     const int64_t base = bc;
     // row length (at most S^3 neighbours x 3 fields; clamped so the LDS rows never overflow)
     const int len = min(static_cast<int>(ec - bc), MAXLEN);
-    const int32_t Ln = plid[min(pt + stride, g.npoints - 1) * 3];   // the next node's LID
+    const int32_t Ln = plid[min(pt + stride, g.p1 - 1) * 3];   // the next node's LID
     const int64_t node = L0 / 3;
     const uint16_t *np = g.nperm + node * g.nperm_w;
     for (int k = lane; k < 3 * len; k += 64) b[k] = 0.0;
@@ -1211,7 +1214,7 @@ void launch(const ElasticParams &p, hipStream_t s) {
   int per_cu = 0;
   TXA_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_elastic<NB, NQ, BS, MODE>, BS, 0));
   const int64_t want = static_cast<int64_t>(ncu) * std::max(per_cu, 1);
-  const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(p.nelem, want)));
+  const unsigned grid = static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(p.e1 - p.e0, want)));
   hipLaunchKernelGGL((k_elastic<NB, NQ, BS, MODE>), dim3(grid), dim3(BS), 0, s, p);
   TXA_HIP(hipGetLastError());
 }
@@ -1224,6 +1227,29 @@ size_t ws_round(size_t b) { return (b + kWsAlign - 1) / kWsAlign * kWsAlign; }
 // affine HEX8 bricks take the node-centric kernel (no workspace) in every precision
 bool affine_hex8(const Mesh &m) {
   return m.topo == TXA_HEX8 && m.affine && m.structured && m.nfields == 3 && m.npoints < (int64_t(1) << 31);
+}
+
+// Streams and events of the element / gather slab pipeline, one set per host thread and device
+// (a context is driven by one thread at a time; the events are recorded and waited on within
+// one call, so calls in sequence may reuse them).  Created on first use; kept until the
+// process ends.
+constexpr int kSlabs = 8;
+struct SlabPipe {
+  hipStream_t e = nullptr, g = nullptr;
+  hipEvent_t ev[kSlabs] = {}, ev0 = nullptr, evg = nullptr;
+};
+
+SlabPipe &slab_pipe(int dev) {
+  thread_local std::map<int, SlabPipe> pipes;
+  SlabPipe &sp = pipes[dev];
+  if (!sp.e) {
+    TXA_HIP(hipStreamCreateWithFlags(&sp.g, hipStreamNonBlocking));
+    for (int k = 0; k < kSlabs; ++k) TXA_HIP(hipEventCreateWithFlags(&sp.ev[k], hipEventDisableTiming));
+    TXA_HIP(hipEventCreateWithFlags(&sp.ev0, hipEventDisableTiming));
+    TXA_HIP(hipEventCreateWithFlags(&sp.evg, hipEventDisableTiming));
+    TXA_HIP(hipStreamCreateWithFlags(&sp.e, hipStreamNonBlocking));   // last: marks the set complete
+  }
+  return sp;
 }
 
 void ws_parts(const Mesh &m, int precision, bool want_f, bool want_vals, size_t *rbytes, size_t *jbytes) {
@@ -1351,6 +1377,8 @@ void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws
   ElasticParams p;
   p.nq = m.topo == TXA_HEX8 ? 8 : 27;
   p.nelem = m.nelem;
+  p.e0 = 0;
+  p.e1 = m.nelem;
   p.nconn = m.nconn;
   p.conn = m.conn.p;
   p.X = m.xyz.p; p.Y = m.xyz.p + m.nverts; p.Z = m.xyz.p + 2 * m.nverts;
@@ -1364,14 +1392,17 @@ void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws
   p.transient = a.transient;
   p.rbuf = rbytes ? reinterpret_cast<double *>(w) : nullptr;
   p.jbuf = jbytes ? static_cast<void *>(w + rbytes) : nullptr;
-  if (m.topo == TXA_HEX8) {
-    if (a.precision == TXA_FP32) launch<8, 8, 64, MODE_F32>(p, s);
-    else launch<8, 8, 64, MODE_F64>(p, s);
-  } else {
-    if (a.precision == TXA_SPLIT_BF16) launch<27, 27, 256, MODE_SPLIT>(p, s);
-    else if (a.precision == TXA_FP32) launch<27, 27, 256, MODE_F32>(p, s);
-    else launch<27, 27, 256, MODE_F64>(p, s);
-  }
+  auto elem_launch = [&](const ElasticParams &pp, hipStream_t st) {
+    if (m.topo == TXA_HEX8) {
+      if (a.precision == TXA_FP32) launch<8, 8, 64, MODE_F32>(pp, st);
+      else launch<8, 8, 64, MODE_F64>(pp, st);
+    } else {
+      if (a.precision == TXA_SPLIT_BF16) launch<27, 27, 256, MODE_SPLIT>(pp, st);
+      else if (a.precision == TXA_FP32) launch<27, 27, 256, MODE_F32>(pp, st);
+      else launch<27, 27, 256, MODE_F64>(pp, st);
+    }
+  };
+  if (!m.structured) elem_launch(p, s);   // bricks: element slabs interleaved with the gather below
   int dev = 0, ncu = 0;
   TXA_HIP(hipGetDevice(&dev));
   TXA_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1403,6 +1434,8 @@ void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws
   const int64_t st = m.topo == TXA_HEX8 ? 1 : 2;
   g.px = st * m.nx + 1; g.py = st * m.ny + 1; g.pz = st * m.nz + 1;
   g.npoints = m.npoints;
+  g.p0 = 0;
+  g.p1 = m.npoints;
   g.point_lid = m.point_lid.p;
   g.rowptr = m.rowptr.p;
   g.nperm = m.nperm.p;
@@ -1411,15 +1444,51 @@ void launch_elastic(Mesh &m, const txa_assembly_args &a, hipStream_t s, void *ws
   g.rbuf = p.rbuf;
   g.f = a.f;
   g.vals = a.vals;
-  const unsigned grid = static_cast<unsigned>(std::min<int64_t>((m.npoints + 3) / 4, static_cast<int64_t>(ncu) * 16));
-  if (m.topo == TXA_HEX8) {
-    if (wide) hipLaunchKernelGGL((k_elastic_gather<8, double>), dim3(grid), dim3(256), 0, s, g);
-    else hipLaunchKernelGGL((k_elastic_gather<8, float>), dim3(grid), dim3(256), 0, s, g);
-  } else {
-    if (wide) hipLaunchKernelGGL((k_elastic_gather<27, double>), dim3(grid), dim3(256), 0, s, g);
-    else hipLaunchKernelGGL((k_elastic_gather<27, float>), dim3(grid), dim3(256), 0, s, g);
+  auto gather_launch = [&](const GatherParams &gg, hipStream_t st) {
+    const unsigned grid = static_cast<unsigned>(std::min<int64_t>((gg.p1 - gg.p0 + 3) / 4, static_cast<int64_t>(ncu) * 16));
+    if (m.topo == TXA_HEX8) {
+      if (wide) hipLaunchKernelGGL((k_elastic_gather<8, double>), dim3(grid), dim3(256), 0, st, gg);
+      else hipLaunchKernelGGL((k_elastic_gather<8, float>), dim3(grid), dim3(256), 0, st, gg);
+    } else {
+      if (wide) hipLaunchKernelGGL((k_elastic_gather<27, double>), dim3(grid), dim3(256), 0, st, gg);
+      else hipLaunchKernelGGL((k_elastic_gather<27, float>), dim3(grid), dim3(256), 0, st, gg);
+    }
+    TXA_HIP(hipGetLastError());
+  };
+  // Slab pipeline: the elements in z slabs on one stream, the gather of each slab's node planes
+  // on another once that slab (and the one below) is done, so the element kernel (bound by
+  // its per-element latency and VALU) of slab k+1 runs beside the gather (bound by HBM) of
+  // slab k.  Node plane st*z1 at a slab's top needs the next slab: the next gather takes it.
+  // A stream under graph capture (or a brick one slab high) takes the two kernels in order
+  // on that stream: no streams or events are created inside a capture.
+  const int nslab = std::min(kSlabs, m.nz);
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  TXA_HIP(hipStreamIsCapturing(s, &cs));
+  if (cs != hipStreamCaptureStatusNone || nslab < 2) {
+    elem_launch(p, s);
+    gather_launch(g, s);
+    return;
   }
-  TXA_HIP(hipGetLastError());
+  SlabPipe &sp = slab_pipe(dev);
+  const int64_t layer = static_cast<int64_t>(m.nx) * m.ny, pxy = g.px * g.py;
+  TXA_HIP(hipEventRecord(sp.ev0, s));
+  TXA_HIP(hipStreamWaitEvent(sp.e, sp.ev0, 0));
+  TXA_HIP(hipStreamWaitEvent(sp.g, sp.ev0, 0));
+  for (int k = 0; k < nslab; ++k) {
+    const int64_t z0 = static_cast<int64_t>(m.nz) * k / nslab, z1 = static_cast<int64_t>(m.nz) * (k + 1) / nslab;
+    ElasticParams pk = p;
+    pk.e0 = z0 * layer;
+    pk.e1 = z1 * layer;
+    elem_launch(pk, sp.e);
+    TXA_HIP(hipEventRecord(sp.ev[k], sp.e));
+    TXA_HIP(hipStreamWaitEvent(sp.g, sp.ev[k], 0));
+    GatherParams gk = g;
+    gk.p0 = st * z0 * pxy;
+    gk.p1 = k == nslab - 1 ? m.npoints : st * z1 * pxy;
+    gather_launch(gk, sp.g);
+  }
+  TXA_HIP(hipEventRecord(sp.evg, sp.g));
+  TXA_HIP(hipStreamWaitEvent(s, sp.evg, 0));
 }
 
 }  // namespace txa
