@@ -569,6 +569,8 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
       if constexpr (MODE == MODE_SPLIT) return sm.w.T32[r][c];
       else return sm.w.v.T[r][c];
     };
+    const double cK = tr ? p.beta + p.alpha * p.rb : p.beta;
+    const double cM = tr ? p.alpha * p.ra + (tra ? p.gamma2 : 0.0) : 0.0;
 #pragma unroll
     for (int k = 0; k < NPASS; ++k) {
       const int a = ga + k * AP, b = gb;
@@ -581,22 +583,24 @@ __global__ __launch_bounds__(BS, MODE == MODE_SPLIT ? 4 : 1) void k_elastic(Elas
           for (int j = 0; j < 3; ++j) t9[i][j] = static_cast<double>(Tref(i * NB + a, j * NB + b));
         const double trs = t9[0][0] + t9[1][1] + t9[2][2];
         const double Mab = tr ? p.rho * static_cast<double>(sm.M[a][b]) : 0.0;
+        // with C = a_R M + b_R K:  J = (beta + alpha b_R) K + (alpha a_R + gamma2) M  and
+        // R = K (u + b_R v) + M (a_R v + a), node b's state combined once for its nine entries
+        double ut[3], mt[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          ut[j] = tr ? sm.u[3 * b + j] + p.rb * sm.v[3 * b + j] : sm.u[3 * b + j];
+          mt[j] = tr ? p.ra * sm.v[3 * b + j] + (tra ? sm.a[3 * b + j] : 0.0) : 0.0;
+        }
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
           for (int j = 0; j < 3; ++j) {
             const double Kv = lam * t9[i][j] + mu * t9[j][i] + (i == j ? mu * trs : 0.0);
-            const double Mv = i == j ? Mab : 0.0;
-            double jv = p.beta * Kv;
-            rr[i] += Kv * sm.u[3 * b + j];
-            if (tr) {
-              const double Cv = p.ra * Mv + p.rb * Kv;
-              rr[i] += Cv * sm.v[3 * b + j];
-              jv += p.alpha * Cv;
-              if (tra) {
-                rr[i] += Mv * sm.a[3 * b + j];
-                jv += p.gamma2 * Mv;
-              }
+            rr[i] += Kv * ut[j];
+            double jv = cK * Kv;
+            if (i == j) {
+              jv += cM * Mab;
+              rr[i] += Mab * mt[j];
             }
             Tref(i * NB + a, j * NB + b) = static_cast<TReal>(jv);
           }
@@ -1083,6 +1087,8 @@ __global__ __launch_bounds__(256, 4) void k_elastic8_affine(Elastic8Params p) {
   const bool tr = p.transient && p.xdot;
   const bool tra = tr && p.xddot;
   const double lamf = p.nu / ((1.0 + p.nu) * (1.0 - 2.0 * p.nu)), muf = 0.5 / (1.0 + p.nu);
+  const double cK = tr ? p.beta + p.alpha * p.rb : p.beta;
+  const double cM = tr ? p.alpha * p.ra + (tra ? p.gamma2 : 0.0) : 0.0;
   const int stride = static_cast<int>(gridDim.x) * 4;
   // Loads come in two rounds per node (the lane's vertex LID and the node's LID; then the
   // state by that LID, the node's row pointers and slot map, and the cell's geometry and
@@ -1150,22 +1156,24 @@ __global__ __launch_bounds__(256, 4) void k_elastic8_affine(Elastic8Params p) {
     const double Mab = tr ? p.rho * adet * mf : 0.0;
     double rr[3] = {0.0, 0.0, 0.0};
     double jb[9];
+    // with C = a_R M + b_R K:  J = (beta + alpha b_R) K + (alpha a_R + gamma2) M  and
+    // R = K (u + b_R v) + M (a_R v + a)
+    double ut[3], mt[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      ut[j] = tr ? ub[j] + p.rb * vb[j] : ub[j];
+      mt[j] = tr ? p.ra * vb[j] + (tra ? ab[j] : 0.0) : 0.0;
+    }
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const double Kv = lam * T[i][j] + mu * T[j][i] + (i == j ? mu * trs : 0.0);
-        const double Mv = i == j ? Mab : 0.0;
-        double jv = p.beta * Kv;
-        rr[i] += Kv * ub[j];
-        if (tr) {
-          const double Cv = p.ra * Mv + p.rb * Kv;
-          rr[i] += Cv * vb[j];
-          jv += p.alpha * Cv;
-          if (tra) {
-            rr[i] += Mv * ab[j];
-            jv += p.gamma2 * Mv;
-          }
+        rr[i] += Kv * ut[j];
+        double jv = cK * Kv;
+        if (i == j) {
+          jv += cM * Mab;
+          rr[i] += Mab * mt[j];
         }
         jb[3 * i + j] = jv;
       }
