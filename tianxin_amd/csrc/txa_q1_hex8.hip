@@ -50,18 +50,7 @@ constexpr int NSTAGE = 44;   // per element: 36 packed K, then 8 FP64 R (FP64) o
 template <typename Real>
 constexpr int stage_slots() { return sizeof(Real) == 4 ? 2 : 1; }
 
-#ifndef TXA_Q1_TARGET_BLOCKS
-#define TXA_Q1_TARGET_BLOCKS 2560   // z slabs are sized to give about this many workgroups
-#endif
-#ifndef TXA_Q1_IDENT_FAST
-#define TXA_Q1_IDENT_FAST 1   // rows whose slot map is the identity skip the per-entry decode
-#endif
-#ifndef TXA_Q1_NT_STORES
-#define TXA_Q1_NT_STORES 1    // non-temporal stores for the CRS values (written once, not re-read)
-#endif
-#ifndef TXA_Q1_ALIGNED_STORES
-#define TXA_Q1_ALIGNED_STORES 1   // row stores in 256-byte aligned chunks (no partial lines inside a range)
-#endif
+constexpr int kTargetBlocks = 2560;   // z slabs are sized to give about this many workgroups
 
 struct Q1DiffParams {
   int nx, ny, nz;
@@ -254,10 +243,10 @@ __device__ __forceinline__ RowMeta load_meta(const Q1DiffParams &p, int32_t L) {
   return m;
 }
 
-// ALIGN: store the range in aligned chunks of TX entries.  It removes the partial lines
+// The range is stored in chunks of TX entries aligned to TX entries.  It removes the partial lines
 // inside a range (WRITE_SIZE 4.32 -> 3.81 GB at 256^3, the bytes of f and the values):
 // about 1% faster in both the FP32 and the FP64 kernels at 256^3.
-template <int TXV, bool ALIGN>
+template <int TXV>
 __device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &m, const double (&acc)[27], double r,
                                           double *rowbuf, int grp) {
   constexpr int GW = TXV;   // lanes per group
@@ -280,15 +269,12 @@ __device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &
   const int64_t base0 = __shfl(base, 0, GW);
   const int64_t end_last = __shfl(end, nact > 0 ? nact - 1 : 0, GW);
   double *buf = rowbuf + grp * ROWBUF;   // dereferenced only by groups with a row: grp < TY-1
-#if TXA_Q1_IDENT_FAST
-  // interior rows: the 27 neighbours sit in the row in lattice order (slot s at s)
+  // interior rows: the 27 neighbours sit in the row in lattice order (slot s at s), so the
+  // per-entry decode of the slot map is skipped
   const bool ident = m.w[0] == 0x03020100u && m.w[1] == 0x07060504u && m.w[2] == 0x0b0a0908u &&
                      m.w[3] == 0x0f0e0d0cu && m.w[4] == 0x13121110u && m.w[5] == 0x17161514u &&
                      (m.w[6] & 0x00ffffffu) == 0x001a1918u;
   const bool fast = __all(!act || (ident && contiguous));   // uniform over the wave
-#else
-  const bool fast = false;
-#endif
   if (fast) {
     if (act) {
       const int64_t off = base - base0;
@@ -309,7 +295,7 @@ __device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &
   // entries of the range; lanes before the range's first entry idle in the first chunk,
   // so only the two end chunks of the range are partial
   const int total = static_cast<int>(end_last - base0);
-  const int lead = ALIGN ? static_cast<int>((reinterpret_cast<uintptr_t>(p.vals + base0) >> 3) & (GW - 1)) : 0;
+  const int lead = static_cast<int>((reinterpret_cast<uintptr_t>(p.vals + base0) >> 3) & (GW - 1));
   int nchw = contiguous ? (total + lead + GW - 1) / GW : 0;   // chunks of the group's range
 #pragma unroll
   for (int o = GW; o < 64; o <<= 1) nchw = max(nchw, __shfl_xor(nchw, o));   // uniform over the wave
@@ -327,11 +313,7 @@ __device__ __forceinline__ void emit_rows(const Q1DiffParams &p, const RowMeta &
 #pragma unroll
       for (int i = 0; i < 4; ++i)
         if (static_cast<unsigned>(j0 + GW * (c + i)) < static_cast<unsigned>(total)) {
-#if TXA_Q1_NT_STORES
-          __builtin_nontemporal_store(v[i], dst + GW * (c + i));
-#else
-          dst[GW * (c + i)] = v[i];
-#endif
+          __builtin_nontemporal_store(v[i], dst + GW * (c + i));   // written once, not re-read
         }
     }
   } else if (act) {
@@ -622,7 +604,7 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
           take_corner<0, 0>(st, t + TX + 1, acc, accn, rc, rn);
           take_pairs<TXV>(st, t, rc, rn);
         }
-        if (e >= k0) emit_rows<TXV, TXA_Q1_ALIGNED_STORES>(p, m, acc, rc, rowbuf, t / TX);
+        if (e >= k0) emit_rows<TXV>(p, m, acc, rc, rowbuf, t / TX);
 #pragma unroll
         for (int s = 0; s < 18; ++s) acc[s] = accn[s];
         rc = rn;
@@ -634,7 +616,7 @@ __device__ __forceinline__ void spec_block(const Q1DiffParams &p, int bx, int by
     if (k1 == p.nz + 1) {
       RowMeta m;
       if (owner) m = get_meta<TXV>(meta + (NSLOT == 2 ? ((ez1 + 1) & 1) : 0) * MSLOT, t);
-      emit_rows<TXV, TXA_Q1_ALIGNED_STORES>(p, m, acc, rc, rowbuf, t / TX);
+      emit_rows<TXV>(p, m, acc, rc, rowbuf, t / TX);
     }
   }
 }
@@ -841,7 +823,7 @@ void launch_q1_hex8_diffusion(Mesh &m, const txa_assembly_args &a, hipStream_t s
   // z slabs: enough workgroups to cover the 256 CUs several times over
   const int gx = tiles_x(m), gy = tiles_y(m);
   static_assert(NT == 512, "block shape");
-  int nchunk = (TXA_Q1_TARGET_BLOCKS + gx * gy - 1) / (gx * gy);
+  int nchunk = (kTargetBlocks + gx * gy - 1) / (gx * gy);
   nchunk = std::max(1, std::min(nchunk, (m.nz + 1 + 7) / 8));
   int zc = (m.nz + 1 + nchunk - 1) / nchunk;
   nchunk = (m.nz + 1 + zc - 1) / zc;
